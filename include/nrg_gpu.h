@@ -1,0 +1,121 @@
+/*
+ * nrg_gpu.h -- C ABI of the MI355X (gfx950) NRG hot-path library libnrg_gpu.so.
+ *
+ * The reference program (eminamitani/nrg-mpi, Fortran 90 + MPI + MKL) runs
+ * its hot path as argument-less subroutines that read and write module
+ * globals.  Each entry point below replaces the numerical body of one of
+ * them; the caller (a Fortran host through iso_c_binding, or any C caller)
+ * keeps the NRG loop, the (Q,Sz) block bookkeeping and the truncation, and
+ * passes the module arrays as plain pointers.
+ *
+ * Conventions
+ *   - integers are int32_t (Fortran default integer), offsets into the packed
+ *     eigenvector array are int64_t (the reference's int32 offsets overflow
+ *     for the large workloads);
+ *   - arrays are Fortran column-major; the packed records are the reference's
+ *     contiguous `memory(:)` arrays: SubspaceInfo (labels(nv), dimension,
+ *     count_eigenvector, start_input) and Basis (labels(nv), reference,
+ *     variation, operation), all 1-based values;
+ *   - every call returns 0 on success, a nonzero status otherwise, with a
+ *     message from nrg_gpu_last_error();
+ *   - one device, one host thread, not reentrant.  Heavy arrays
+ *     (eigenvectors, invariant matrices, history) stay resident in HBM
+ *     between calls; host copies are made only when an output pointer is
+ *     non-NULL.
+ */
+#ifndef NRG_GPU_H
+#define NRG_GPU_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- lifecycle -------------------------------------------------------- */
+/* Select the device and create the stream/context.  Replaces the MPI setup
+ * of main.F90:18-20 (one device instead of ranks). */
+int nrg_gpu_init(int device);
+int nrg_gpu_finalize(void);
+const char* nrg_gpu_last_error(void);
+
+/* ---- model constants --------------------------------------------------- */
+/* Static tables read by setupHamiltonian / read_initial_basis
+ * (module.F90:826-944, read_initial_basis.F90:160-302).
+ *   operation(nops, nv+nchain), coefficient_diagonalization_type(ncoef,4),
+ *   coefficient_diagonalization(ncoef), coefficient_invariant_matrix_type
+ *   (ncoef,3), coefficient_invariant_matrix(ncoef), conservationBlock(2,2),
+ *   coefficient_invariant_matrix_spectrum(nmat,nops) and
+ *   conservation_difference_spectrum(nmat,nv) (both may be NULL when the
+ *   spectrum is off). */
+int nrg_gpu_set_model(int nv, int nops, int nchain, int ncm, int ncoef,
+                      const int32_t* operation, const int32_t* coef_diag_type,
+                      const double* coef_diag, const int32_t* coef_inv_type,
+                      const double* coef_inv, const int32_t* conservation_block,
+                      int nmat, const double* coef_spec, const int32_t* cons_diff_spec);
+
+/* Initial invariant matrices of the impurity basis:
+ * invariant_matrix(nold,nold,ncm) (read_initial_basis.F90:163-173) and
+ * invariant_matrix_spectrum(nold,nold,nmat) (read_initial_basis.F90:271-282). */
+int nrg_gpu_set_invariant(int nold, const double* invariant_matrix);
+int nrg_gpu_set_invariant_spectrum(int nold, const double* invariant_matrix_spectrum);
+
+/* ---- diagonalization (diagonalization.F90:1-410) ------------------------
+ * Builds every block Hamiltonian of the iteration from the previous kept
+ * eigenvalues and the device-resident invariant matrix (:153-263), solves all
+ * blocks in one batched eigensolver (:266-269), chops |U| < 1e-14
+ * (:273-280), shifts the eigenvalues by their global minimum and zeroes
+ * those <= 1e-12 (:351-356).  `ismysubspace` has no counterpart: one device
+ * owns every block.
+ *   subspace_info  nsub*(nv+3)  (prepareBasis.F90:93-114)
+ *   basis_input    nfull*(nv+3) (prepareBasis.F90:41-59)
+ *   eigenvalue_old nold          eigenvalues of the previous kept basis
+ *   chain_diag     chain_diagonal(iteration+1, 1:nchain) after rescaling
+ *   chain_nondiag  chain_nondiagonal(iteration, 1:nchain) after rescaling
+ *   eigenvalue     out, nfull, block order (start_input offsets)
+ *   eigenvector    out (may be NULL), packed at count_eigenvector offsets,
+ *                  eigenvector_size = sum of dimension^2 */
+int nrg_gpu_diagonalize(int iteration, int nsub, const int32_t* subspace_info, int nfull,
+                        const int32_t* basis_input, int nold, const double* eigenvalue_old,
+                        const double* chain_diag, const double* chain_nondiag,
+                        double sqrt_lambda, double* eigenvalue, double* eigenvector,
+                        int64_t eigenvector_size);
+
+/* ---- invariant matrices ------------------------------------------------
+ * invariantMatrixDGEMM (invariantMatrixDGEMM.F90:1-394): new conduction
+ * invariant matrices of the kept basis from the eigenvectors of the last
+ * nrg_gpu_diagonalize call.  kept_per_block(nsub) is keeped_basis_number
+ * (:84-100).  The result stays on the device as the input of the next
+ * diagonalization; invariant_matrix_out (k,k,ncm) may be NULL. */
+int nrg_gpu_invariant(const int32_t* kept_per_block, double* invariant_matrix_out);
+
+/* invariantMatrixForSpectrumDGEMM (invariantMatrixForSpectrumDGEMM.F90:1-392). */
+int nrg_gpu_invariant_spectrum(const int32_t* kept_per_block, double* invariant_matrix_spectrum_out);
+
+/* ---- per-stage timing of the last calls (milliseconds) -----------------
+ * t[0] H assembly, t[1] eigensolver total, t[2] tridiagonalisation,
+ * t[3] divide and conquer, t[4] back-transform, t[5] invariant,
+ * t[6] spectral invariant, t[7] rho sweep, t[8] CFS.  Also returns the
+ * algorithmic FP64 flops of the last eigensolve (sum 10/3 n^3) in t[9]. */
+int nrg_gpu_stage_times(double* t, int n);
+
+/* ---- batched eigensolver on host arrays (kernel-level interface) -------
+ * Solves nblk packed symmetric blocks like one dsyev('V','U') each
+ * (diagonalization.F90:267): H packed column-major at offsets sum n^2,
+ * w at offsets sum n.  U receives the eigenvectors (no chop, no shift). */
+int nrg_gpu_syev_batched(int nblk, const int32_t* n, const double* H, double* w, double* U);
+
+/* ---- batched FP64 GEMM on host arrays (kernel-level interface) ---------
+ * C = alpha op(A) op(B) + beta C, column-major, op = N (0) or T (1): the
+ * dgemm call sites of the reference (e.g. invariantMatrixDGEMM.F90:300). */
+int nrg_gpu_dgemm(int transa, int transb, int m, int n, int k, double alpha,
+                  const double* A, int lda, const double* B, int ldb, double beta,
+                  double* C, int ldc);
+
+/* Test hook with no counterpart in the reference: make the next
+ * nrg_gpu_diagonalize of `iteration` fail with status 5 (error-path tests). */
+int nrg_gpu_inject_fault(int iteration);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NRG_GPU_H */

@@ -1,0 +1,332 @@
+// extern "C" entry points of libnrg_gpu.so (declared in include/nrg_gpu.h).
+#include "../../include/nrg_gpu.h"
+#include "assemble.h"
+#include "context.h"
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+namespace nrg {
+
+Context& ctx() {
+  static Context c;
+  return c;
+}
+
+static double wall_ms() {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+template <typename F>
+static int guarded(F&& f) {
+  Context& C = ctx();
+  try {
+    f(C);
+    C.last_error.clear();
+    return NRG_OK;
+  } catch (const Error& e) {
+    C.last_error = e.what();
+    return e.code;
+  } catch (const std::exception& e) {
+    C.last_error = e.what();
+    return NRG_ERR_STATE;
+  }
+}
+
+static void need_init(Context& C) {
+  if (!C.initialised) throw Error(NRG_ERR_STATE, "nrg_gpu_init has not been called");
+}
+
+// Parse subspace records into block sizes and packed offsets.
+static void set_blocks(Context& C, int nsub, const int32_t* sub, int nfull) {
+  const int rs = C.nv + 3;
+  C.nsub = nsub;
+  C.nfull = nfull;
+  C.sub.assign(sub, sub + (size_t)nsub * rs);
+  C.bn.resize(nsub);
+  C.boff.resize(nsub);
+  C.bvoff.resize(nsub);
+  int64_t o2 = 0, o1 = 0;
+  for (int b = 0; b < nsub; ++b) {
+    const int n = sub[b * rs + C.nv];
+    const int64_t cnt = sub[b * rs + C.nv + 1];
+    const int64_t si = sub[b * rs + C.nv + 2];
+    NRG_REQUIRE(n >= 1, "subspace dimension < 1");
+    NRG_REQUIRE(si - 1 == o1, "subspace start_input is not the running sum of dimensions");
+    // count_eigenvector is int32 in the reference and may have wrapped: the
+    // packed offset is recomputed in 64 bits and checked modulo 2^32
+    NRG_REQUIRE((uint32_t)(cnt - 1) == (uint32_t)o2, "subspace count_eigenvector is not the running sum of dimension^2");
+    C.bn[b] = n;
+    C.boff[b] = o2;
+    C.bvoff[b] = o1;
+    o2 += (int64_t)n * n;
+    o1 += n;
+  }
+  NRG_REQUIRE(o1 == nfull, "sum of subspace dimensions differs from numberOfBasis");
+}
+
+}  // namespace nrg
+
+using namespace nrg;
+
+extern "C" {
+
+int nrg_gpu_init(int device) {
+  return guarded([&](Context& C) {
+    if (C.initialised) return;
+    int nd = 0;
+    NRG_HIP(hipGetDeviceCount(&nd));
+    NRG_REQUIRE(device >= 0 && device < nd, "nrg_gpu_init: no such device");
+    NRG_HIP(hipSetDevice(device));
+    C.device = device;
+    NRG_HIP(hipStreamCreateWithFlags(&C.stream, hipStreamNonBlocking));
+    NRG_HIP(hipEventCreate(&C.ev0));
+    NRG_HIP(hipEventCreate(&C.ev1));
+    C.initialised = true;
+  });
+}
+
+int nrg_gpu_finalize(void) {
+  return guarded([&](Context& C) {
+    if (!C.initialised) return;
+    (void)hipStreamSynchronize(C.stream);
+    C.eig.release();
+    C.gemm.release();
+    C.d_op.release(); C.d_hop.release(); C.d_hop_coef.release();
+    C.d_basis_in.release(); C.d_H.release(); C.d_U.release(); C.d_w.release(); C.d_Eold.release();
+    C.d_inv.release(); C.d_inv_next.release(); C.d_spec.release(); C.d_spec_next.release();
+    C.d_tmp.release(); C.d_gidx.release(); C.d_gscale.release();
+    (void)hipEventDestroy(C.ev0);
+    (void)hipEventDestroy(C.ev1);
+    (void)hipStreamDestroy(C.stream);
+    C.stream = nullptr;
+    C.initialised = false;
+    C.kold = 0;
+    C.kold_spec = 0;
+  });
+}
+
+const char* nrg_gpu_last_error(void) { return ctx().last_error.c_str(); }
+
+int nrg_gpu_set_model(int nv, int nops, int nchain, int ncm, int ncoef, const int32_t* operation,
+                      const int32_t* coef_diag_type, const double* coef_diag,
+                      const int32_t* coef_inv_type, const double* coef_inv,
+                      const int32_t* conservation_block, int nmat, const double* coef_spec,
+                      const int32_t* cons_diff_spec) {
+  return guarded([&](Context& C) {
+    need_init(C);
+    NRG_REQUIRE(nv >= 1 && nops >= 1 && nchain >= 1 && ncm >= 1 && ncoef >= 0, "set_model: bad sizes");
+    NRG_REQUIRE(nchain <= 4, "set_model: at most 4 chains");
+    C.nv = nv; C.nops = nops; C.nchain = nchain; C.ncm = ncm; C.ncoef = ncoef; C.nmat = nmat;
+    const int nc = nv + nchain;
+    C.operation.resize((size_t)nops * nc);
+    for (int o = 0; o < nops; ++o)
+      for (int j = 0; j < nc; ++j) C.operation[o * nc + j] = operation[o + j * nops];
+    // hopping lookup: replay every coefficient row in order for each
+    // (op_i, op_j); the last match wins and a reversed match swaps refs
+    // (diagonalization.F90:195-213)
+    const int nn = nops * nops;
+    C.hop_valid.assign(nn, 0); C.hop_type.assign(nn, 0); C.hop_chan.assign(nn, 0);
+    C.hop_swap.assign(nn, 0); C.hop_coef.assign(nn, 0.0);
+    for (int oi = 0; oi < nops; ++oi)
+      for (int oj = 0; oj < nops; ++oj) {
+        const int idx = oi * nops + oj;
+        for (int ic = 0; ic < ncoef; ++ic) {
+          const int a = coef_diag_type[ic] - 1, b = coef_diag_type[ic + ncoef] - 1;
+          const int t = coef_diag_type[ic + 2 * ncoef] - 1, ch = coef_diag_type[ic + 3 * ncoef] - 1;
+          int sw;
+          if (oi == a && oj == b) sw = 0;
+          else if (oi == b && oj == a) sw = 1;
+          else continue;
+          NRG_REQUIRE(t >= 0 && t < ncm && ch >= 0 && ch < nchain, "set_model: bad hopping row");
+          C.hop_valid[idx] = 1; C.hop_type[idx] = t; C.hop_chan[idx] = ch; C.hop_swap[idx] = sw;
+          C.hop_coef[idx] = coef_diag[ic];
+        }
+      }
+    // invariant table, deduplicated on (op_a, op_b, t): the later row wins
+    // (invariantMatrixDGEMM.F90:248-253 overwrites the same slot)
+    C.inv_a.clear(); C.inv_b.clear(); C.inv_t.clear(); C.inv_c.clear();
+    for (int ic = 0; ic < ncoef; ++ic) {
+      const int a = coef_inv_type[ic] - 1, b = coef_inv_type[ic + ncoef] - 1, t = coef_inv_type[ic + 2 * ncoef] - 1;
+      NRG_REQUIRE(a >= 0 && a < nops && b >= 0 && b < nops && t >= 0 && t < ncm, "set_model: bad invariant row");
+      bool found = false;
+      for (size_t q = 0; q < C.inv_a.size(); ++q)
+        if (C.inv_a[q] == a && C.inv_b[q] == b && C.inv_t[q] == t) { C.inv_c[q] = coef_inv[ic]; found = true; }
+      if (!found) { C.inv_a.push_back(a); C.inv_b.push_back(b); C.inv_t.push_back(t); C.inv_c.push_back(coef_inv[ic]); }
+    }
+    C.cons = {conservation_block[0], conservation_block[2], conservation_block[1], conservation_block[3]};
+    C.coef_spec.clear();
+    C.cons_diff_spec.clear();
+    if (nmat > 0 && coef_spec && cons_diff_spec) {
+      NRG_REQUIRE(nmat <= ncm, "set_model: numberOfMatrix must not exceed numberOfConductionMatrix");
+      C.coef_spec.resize((size_t)nmat * nops);
+      for (int im = 0; im < nmat; ++im)
+        for (int o = 0; o < nops; ++o) C.coef_spec[im * nops + o] = coef_spec[im + o * nmat];
+      C.cons_diff_spec.resize((size_t)nmat * nv);
+      for (int im = 0; im < nmat; ++im)
+        for (int i = 0; i < nv; ++i) C.cons_diff_spec[im * nv + i] = cons_diff_spec[im + i * nmat];
+    }
+    C.d_op.upload(C.operation, C.stream);
+    std::vector<int32_t> hop(4 * nn);
+    for (int i = 0; i < nn; ++i) {
+      hop[i] = C.hop_valid[i]; hop[nn + i] = C.hop_type[i]; hop[2 * nn + i] = C.hop_chan[i]; hop[3 * nn + i] = C.hop_swap[i];
+    }
+    C.d_hop.upload(hop, C.stream);
+    C.d_hop_coef.upload(C.hop_coef, C.stream);
+    NRG_HIP(hipStreamSynchronize(C.stream));
+  });
+}
+
+int nrg_gpu_set_invariant(int nold, const double* inv) {
+  return guarded([&](Context& C) {
+    need_init(C);
+    NRG_REQUIRE(C.ncm > 0, "set_invariant: call nrg_gpu_set_model first");
+    C.d_inv.upload(inv, (size_t)nold * nold * C.ncm, C.stream);
+    C.kold = nold;
+    NRG_HIP(hipStreamSynchronize(C.stream));
+  });
+}
+
+int nrg_gpu_set_invariant_spectrum(int nold, const double* ims) {
+  return guarded([&](Context& C) {
+    need_init(C);
+    NRG_REQUIRE(C.nmat > 0, "set_invariant_spectrum: call nrg_gpu_set_model with spectral tables first");
+    C.d_spec.upload(ims, (size_t)nold * nold * C.nmat, C.stream);
+    C.kold_spec = nold;
+    NRG_HIP(hipStreamSynchronize(C.stream));
+  });
+}
+
+int nrg_gpu_diagonalize(int iteration, int nsub, const int32_t* subspace_info, int nfull,
+                        const int32_t* basis_input, int nold, const double* eigenvalue_old,
+                        const double* chain_diag, const double* chain_nondiag, double sqrt_lambda,
+                        double* eigenvalue, double* eigenvector, int64_t eigenvector_size) {
+  return guarded([&](Context& C) {
+    need_init(C);
+    if (iteration == C.fault_iteration) {
+      C.fault_iteration = -1000000;
+      throw Error(NRG_ERR_INJECTED, "injected fault in diagonalization at iteration " + std::to_string(iteration));
+    }
+    NRG_REQUIRE(C.nops > 0, "diagonalize: call nrg_gpu_set_model first");
+    NRG_REQUIRE(nold == C.kold, "diagonalize: eigenvalue_old size differs from the invariant-matrix basis");
+    set_blocks(C, nsub, subspace_info, nfull);
+    const int rs = C.nv + 3;
+    C.basis_in.assign(basis_input, basis_input + (size_t)nfull * rs);
+    for (int i = 0; i < nfull; ++i) {
+      const int ref = basis_input[i * rs + C.nv], op = basis_input[i * rs + C.nv + 2];
+      NRG_REQUIRE(ref >= 1 && ref <= nold && op >= 1 && op <= C.nops, "diagonalize: basis_input reference/operation out of range");
+    }
+    int64_t n2 = 0;
+    for (int b = 0; b < nsub; ++b) n2 += (int64_t)C.bn[b] * C.bn[b];
+    if (eigenvector) NRG_REQUIRE(eigenvector_size == n2, "diagonalize: eigenvector_size differs from sum of dimension^2");
+    C.d_basis_in.upload(C.basis_in, C.stream);
+    C.d_Eold.upload(eigenvalue_old, nold, C.stream);
+    C.d_H.reserve(n2);
+    C.d_U.reserve(n2);
+    C.d_w.reserve(nfull);
+    double t0 = wall_ms();
+    assemble(C, chain_diag, chain_nondiag, sqrt_lambda);
+    NRG_HIP(hipStreamSynchronize(C.stream));
+    double t1 = wall_ms();
+    C.timers.assemble = t1 - t0;
+    C.eig.solve(nsub, C.bn.data(), C.boff.data(), C.bvoff.data(), C.d_H.p, C.d_U.p, C.d_w.p, nullptr,
+                C.stream, &C.eig_stats);
+    NRG_HIP(hipStreamSynchronize(C.stream));
+    C.timers.eig = wall_ms() - t1;
+    C.timers.tridiag = C.eig_stats.ms_tridiag;
+    C.timers.dc = C.eig_stats.ms_dc;
+    C.timers.back = C.eig_stats.ms_back;
+    C.d_w.download(eigenvalue, nfull, C.stream);
+    if (eigenvector) C.d_U.download(eigenvector, n2, C.stream);
+    NRG_HIP(hipStreamSynchronize(C.stream));
+    // global shift and chop (diagonalization.F90:351-356)
+    double mn = eigenvalue[0];
+    for (int i = 1; i < nfull; ++i) mn = std::min(mn, eigenvalue[i]);
+    for (int i = 0; i < nfull; ++i) {
+      const double v = eigenvalue[i] - mn;
+      eigenvalue[i] = (v <= 1.0e-12) ? 0.0 : v;
+    }
+    for (int i = 0; i < nfull; ++i)
+      if (!std::isfinite(eigenvalue[i])) throw Error(NRG_ERR_NUMERIC, "diagonalize: non-finite eigenvalue");
+  });
+}
+
+int nrg_gpu_invariant(const int32_t* kept_per_block, double* out) {
+  return guarded([&](Context& C) {
+    need_init(C);
+    NRG_REQUIRE(C.nsub > 0, "invariant: no diagonalised iteration");
+    const double t0 = wall_ms();
+    invariant(C, kept_per_block, out);
+    C.timers.invariant = wall_ms() - t0;
+  });
+}
+
+int nrg_gpu_invariant_spectrum(const int32_t* kept_per_block, double* out) {
+  return guarded([&](Context& C) {
+    need_init(C);
+    NRG_REQUIRE(C.nsub > 0, "invariant_spectrum: no diagonalised iteration");
+    const double t0 = wall_ms();
+    invariant_spectrum(C, kept_per_block, out);
+    C.timers.spectral = wall_ms() - t0;
+  });
+}
+
+int nrg_gpu_stage_times(double* t, int n) {
+  return guarded([&](Context& C) {
+    const double v[10] = {C.timers.assemble, C.timers.eig, C.timers.tridiag, C.timers.dc, C.timers.back,
+                          C.timers.invariant, C.timers.spectral, C.timers.rho, C.timers.cfs,
+                          C.eig_stats.flops_alg};
+    for (int i = 0; i < n && i < 10; ++i) t[i] = v[i];
+  });
+}
+
+int nrg_gpu_syev_batched(int nblk, const int32_t* n, const double* H, double* w, double* U) {
+  return guarded([&](Context& C) {
+    need_init(C);
+    std::vector<int> nn(n, n + nblk);
+    std::vector<int64_t> ao(nblk), vo(nblk);
+    int64_t a = 0, v = 0;
+    for (int b = 0; b < nblk; ++b) {
+      NRG_REQUIRE(nn[b] >= 1, "syev_batched: block size < 1");
+      ao[b] = a; vo[b] = v;
+      a += (int64_t)nn[b] * nn[b];
+      v += nn[b];
+    }
+    DevBuf<double> dA, dZ, dw;
+    dA.upload(H, a, C.stream);
+    dZ.reserve(a);
+    dw.reserve(v);
+    C.eig.solve(nblk, nn.data(), ao.data(), vo.data(), dA.p, dZ.p, dw.p, nullptr, C.stream, &C.eig_stats);
+    dw.download(w, v, C.stream);
+    dZ.download(U, a, C.stream);
+    NRG_HIP(hipStreamSynchronize(C.stream));
+    dA.release(); dZ.release(); dw.release();
+  });
+}
+
+int nrg_gpu_dgemm(int transa, int transb, int m, int n, int k, double alpha, const double* A, int lda,
+                  const double* B, int ldb, double beta, double* C_, int ldc) {
+  return guarded([&](Context& C) {
+    need_init(C);
+    NRG_REQUIRE(m >= 0 && n >= 0 && k >= 0, "dgemm: negative size");
+    if (m == 0 || n == 0) return;
+    const int64_t na = (int64_t)lda * (transa ? m : k), nb = (int64_t)ldb * (transb ? k : n), nc = (int64_t)ldc * n;
+    DevBuf<double> dA, dB, dC;
+    dA.upload(A, std::max<int64_t>(na, 1), C.stream);
+    dB.upload(B, std::max<int64_t>(nb, 1), C.stream);
+    dC.upload(C_, nc, C.stream);
+    GemmBatch b;
+    b.add(gemm_task(m, n, k, transa, transb, 0, lda, 0, ldb, 0, ldc, alpha, beta));
+    C.gemm.run(b, dA.p, dB.p, dC.p, nullptr, nullptr, nullptr, C.stream);
+    dC.download(C_, nc, C.stream);
+    NRG_HIP(hipStreamSynchronize(C.stream));
+    dA.release(); dB.release(); dC.release();
+  });
+}
+
+int nrg_gpu_inject_fault(int iteration) {
+  return guarded([&](Context& C) { C.fault_iteration = iteration; });
+}
+
+}  // extern "C"

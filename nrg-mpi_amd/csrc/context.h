@@ -1,0 +1,60 @@
+// Library context: one device, one stream, persistent device buffers.
+#pragma once
+#include "common.h"
+#include "eig.h"
+#include "gemm.h"
+
+namespace nrg {
+
+struct Timers {
+  // per-stage milliseconds of the last call (host wall clock around synchronised stages)
+  double assemble = 0, eig = 0, tridiag = 0, dc = 0, back = 0, invariant = 0, spectral = 0;
+  double rho = 0, cfs = 0;
+};
+
+struct Context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  bool initialised = false;
+  std::string last_error;
+  BatchedEigensolver eig;
+  EigStats eig_stats;
+  GemmRunner gemm;
+  Timers timers;
+  int fault_iteration = -1000000;   // test hook: fail diagonalisation at this iteration
+
+  // ---- model constants (set once)
+  int nv = 0, nops = 0, nchain = 0, ncm = 0, ncoef = 0, nmat = 0;
+  std::vector<int32_t> operation;       // [nops][nv+nchain] row-major (op, column)
+  std::vector<int32_t> hop_valid, hop_type, hop_chan, hop_swap;  // [nops*nops]
+  std::vector<double> hop_coef;
+  std::vector<int32_t> inv_a, inv_b, inv_t;  // invariant table (deduplicated, last wins)
+  std::vector<double> inv_c;
+  std::vector<double> coef_spec;        // [nmat][nops]
+  std::vector<int32_t> cons;            // chargeStart, chargeEnd, spinStart, spinEnd
+  std::vector<int32_t> cons_diff_spec;  // [nmat][nv]
+  DevBuf<int32_t> d_op, d_hop;          // d_hop: [valid,type,chan,swap] x nops^2
+  DevBuf<double> d_hop_coef;
+
+  // ---- current iteration (device resident)
+  int nsub = 0, nfull = 0;
+  std::vector<int32_t> sub;             // nsub*(nv+3)
+  std::vector<int32_t> basis_in;        // nfull*(nv+3)
+  std::vector<int> bn;                  // block sizes
+  std::vector<int64_t> boff, bvoff;     // sum n^2 / sum n offsets
+  DevBuf<int32_t> d_basis_in;
+  DevBuf<double> d_H, d_U, d_w, d_Eold;
+  // invariant matrices of the kept basis (dense k x k x ncm / nmat, column-major)
+  int kold = 0;
+  DevBuf<double> d_inv, d_inv_next;
+  int kold_spec = 0;
+  DevBuf<double> d_spec, d_spec_next;
+  DevBuf<double> d_tmp;
+  DevBuf<int32_t> d_gidx;
+  DevBuf<double> d_gscale;
+};
+
+Context& ctx();
+
+}  // namespace nrg

@@ -1,0 +1,67 @@
+// Batched symmetric eigensolver for the (Q,Sz) blocks of one NRG iteration.
+//
+// Replaces one dsyev('V','U') call per block (diagonalization.F90:266-269)
+// by a single batched pipeline over all blocks of the iteration:
+//   1. blocked Householder tridiagonalisation (panel columns with a
+//      multi-workgroup SYMV, trailing SYR2K on the FP64 matrix cores);
+//   2. divide-and-conquer tridiagonal eigensolver (QL leaves, rank-one
+//      merges with deflation, bisection-solved secular equations,
+//      Gu-Eisenstat eigenvectors, merge products on the FP64 matrix cores);
+//   3. blocked WY back-transform of the eigenvectors (FP64 matrix cores).
+// Blocks are packed back to back: block b is an n_b x n_b column-major
+// matrix at element offset aoff[b] (= sum of n^2 of earlier blocks) and owns
+// n_b eigenvalues at voff[b] (= sum of n of earlier blocks), the layout of the
+// reference's eigenvector/eigenvalue arrays (prepareBasis.F90:97-110).
+#pragma once
+#include "common.h"
+#include "gemm.h"
+
+namespace nrg {
+
+struct EigStats {
+  double ms_tridiag = 0, ms_dc = 0, ms_back = 0;
+  int max_n = 0, nblk = 0, dc_levels = 0;
+  double flops_alg = 0;   // sum (10/3) n^3 over blocks
+};
+
+class BatchedEigensolver {
+ public:
+  // Solve all blocks.  On entry A holds the matrices (full storage); on exit
+  // Z holds eigenvectors (ascending eigenvalue order, one column each) and w
+  // the eigenvalues.  A is overwritten with Householder data.  A and Z are
+  // device buffers of sum n^2 doubles; w of sum n.  ncols_back[b] limits the
+  // back-transform to the first ncols_back[b] columns (n_b for all).
+  void solve(int nblk, const int* n, const int64_t* aoff, const int64_t* voff,
+             double* A, double* Z, double* w, const int* ncols_back, hipStream_t s,
+             EigStats* st = nullptr);
+  void release();
+
+ private:
+  void tridiag(hipStream_t s);
+  void divide_conquer(hipStream_t s);
+  void back_transform(const int* ncols_back, hipStream_t s);
+
+  int nblk_ = 0;
+  std::vector<int> n_;
+  std::vector<int64_t> aoff_, voff_;
+  std::vector<int> order_;          // blocks by n descending
+  int64_t tot_n_ = 0, tot_n2_ = 0;
+  double *A_ = nullptr, *Z_ = nullptr, *w_ = nullptr;
+
+  DevBuf<int32_t> d_n_, d_order_;
+  DevBuf<int64_t> d_aoff_, d_voff_;
+  DevBuf<double> d_d_, d_e_, d_tau_, d_y_, d_P_, d_part_;
+  DevBuf<int64_t> d_poff_;
+  // divide and conquer
+  DevBuf<double> d_ws_;
+  DevBuf<int32_t> d_iws_;
+  DevBuf<int32_t> d_tasks_;
+  DevBuf<int32_t> d_k_;
+  std::vector<int32_t> h_k_;
+  // back-transform
+  DevBuf<double> d_V_, d_T_, d_X_, d_X2_;
+  GemmRunner gemm_;
+  GemmBatch batch_;
+};
+
+}  // namespace nrg

@@ -1,0 +1,1002 @@
+// Batched symmetric eigensolver: see eig.h for the pipeline.
+//
+// Published methods followed (all code here is written from the mathematics):
+//  * blocked Householder tridiagonalisation with the panel recurrence of
+//    Dongarra, Sorensen & Hammarling (1989) (the LAPACK dsytrd/dlatrd
+//    scheme), lower storage;
+//  * Cuppen's divide and conquer (1981) with the deflation rules of Dongarra
+//    & Sorensen (1987) and the eigenvector formula of Gu & Eisenstat (1995);
+//  * compact WY accumulation of reflectors (Schreiber & Van Loan 1989).
+#include "eig.h"
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+
+namespace nrg {
+
+constexpr int NB = 32;     // tridiagonalisation panel width
+constexpr int RA = 256;    // rows per workgroup, column-update kernel
+constexpr int RB = 64;     // rows per workgroup, SYMV kernel
+constexpr int PSTRIDE = 2 * NB + 1;  // partial-sum record per SYMV chunk
+constexpr int LEAF = 32;   // divide-and-conquer leaf size
+constexpr int NBB = 32;    // back-transform panel width
+constexpr int MAXN = 16384;
+
+__device__ inline double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// ===========================================================================
+// 1. tridiagonalisation
+// ===========================================================================
+// Per-block panel buffer P (n x 4NB, column-major, ld n):
+//   [0,NB) V, [NB,2NB) W, [2NB,3NB) W copy, [3NB,4NB) V copy
+// so that the trailing update is one product [V W] * [W V]^T with K = 2NB.
+struct TrArgs {
+  const int32_t* n;
+  const int32_t* order;
+  const int64_t* aoff;
+  const int64_t* voff;
+  const int64_t* poff;     // [0,nblk): P offsets, [nblk,2nblk): partial offsets
+  double* A;
+  double* d;
+  double* e;
+  double* tau;
+  double* y;
+  double* P;
+  double* part;
+  int nblk;
+};
+
+__device__ inline int chunksB(int n, int c) { return (n - c - 1 + RB - 1) / RB; }
+__device__ inline int64_t part_norm_base(int n) { return (int64_t)((n + RB - 1) / RB + 1) * PSTRIDE; }
+
+// Column kernel (one workgroup per RA rows of one block):
+//   jp >= 0: finalise w of column c-1 (panel slot jp) for rows >= c;
+//   jj >= 0: update column c with the jj previous panel columns, write it
+//            back, record d[c] and the partial norm of rows >= c+2;
+//   always:  copy the Householder vector of column c-1 into A (rows >= c+1).
+__global__ __launch_bounds__(256) void k_tr_col(TrArgs a, int c, int jp, int jj, int vslot_prev) {
+  const int b = a.order[blockIdx.y];
+  const int n = a.n[b];
+  if (c >= n) return;
+  if (c + (int)blockIdx.x * RA >= n) return;   // no rows of this block here
+  const int64_t ld = n;
+  double* A = a.A + a.aoff[b];
+  double* P = a.P + a.poff[b];
+  double* part = a.part + a.poff[a.nblk + b];
+  const double* y = a.y + a.voff[b];
+  const int tid = threadIdx.x;
+  const int r = c + blockIdx.x * RA + tid;
+  __shared__ double su1[NB], su2[NB], sVc[NB], sWc[NB];
+  __shared__ double s_alpha, s_wc, s_red[4];
+
+  double w_r = 0.0;
+  if (jp >= 0) {
+    const int p = c - 1;
+    const double tp = a.tau[a.voff[b] + p];
+    const int nch = chunksB(n, p);
+    // sum the SYMV partials of column p (fixed order: deterministic)
+    for (int q = tid; q < 2 * jp + 1; q += 256) {
+      const int slot = (q < jp) ? q : (q < 2 * jp ? NB + (q - jp) : 2 * NB);
+      double s = 0.0;
+      for (int ch = 0; ch < nch; ++ch) s += part[(int64_t)ch * PSTRIDE + slot];
+      if (q < jp) su1[q] = s;
+      else if (q < 2 * jp) su2[q - jp] = s;
+      else s_alpha = s;  // y.v for now
+    }
+    __syncthreads();
+    if (tid == 0) {
+      double uu = 0.0;
+      for (int i = 0; i < jp; ++i) uu += su1[i] * su2[i];
+      // w_pre.v = tau (y.v - 2 u1.u2);  alpha = -tau/2 * w_pre.v
+      s_alpha = -0.5 * tp * tp * (s_alpha - 2.0 * uu);
+      // w at row c (needed by every workgroup for the column update)
+      double acc = y[c];
+      for (int i = 0; i < jp; ++i)
+        acc -= P[c + (NB + i) * ld] * su1[i] + P[c + i * ld] * su2[i];
+      s_wc = tp * acc + s_alpha * P[c + jp * ld];
+    }
+    __syncthreads();
+    if (r < n) {
+      double acc = y[r];
+      for (int i = 0; i < jp; ++i)
+        acc -= P[r + (NB + i) * ld] * su1[i] + P[r + i * ld] * su2[i];
+      w_r = tp * acc + s_alpha * P[r + jp * ld];
+      P[r + (NB + jp) * ld] = w_r;
+      P[r + (2 * NB + jp) * ld] = w_r;
+    }
+  }
+  // Householder vector of column c-1 into A (rows >= c+1)
+  if (c >= 1 && c - 1 <= n - 2 && r >= c + 1 && r < n) A[r + (c - 1) * ld] = P[r + vslot_prev * ld];
+
+  if (jj >= 0) {
+    for (int i = tid; i < jj; i += 256) {
+      sVc[i] = P[c + i * ld];
+      sWc[i] = (jp >= 0 && i == jp) ? s_wc : P[c + (NB + i) * ld];
+    }
+    __syncthreads();
+    double col = 0.0;
+    if (r < n) {
+      col = A[r + c * ld];
+      for (int i = 0; i < jj; ++i) {
+        const double Wri = (jp >= 0 && i == jp) ? w_r : P[r + (NB + i) * ld];
+        col -= P[r + i * ld] * sWc[i] + Wri * sVc[i];
+      }
+      A[r + c * ld] = col;
+      if (r == c) a.d[a.voff[b] + c] = col;
+    }
+    double sq = (r < n && r >= c + 2) ? col * col : 0.0;
+    sq = wave_sum(sq);
+    if ((tid & 63) == 0) s_red[tid >> 6] = sq;
+    __syncthreads();
+    if (tid == 0) {
+      double* pn = part + part_norm_base(n);
+      pn[blockIdx.x] = s_red[0] + s_red[1] + s_red[2] + s_red[3];
+    }
+  }
+}
+
+// Reflector + SYMV kernel (one workgroup per RB rows of one block).
+__global__ __launch_bounds__(256) void k_tr_symv(TrArgs a, int c, int jj) {
+  extern __shared__ double vs[];   // v[j], j in [c+1, n)
+  const int b = a.order[blockIdx.y];
+  const int n = a.n[b];
+  if (c > n - 2) return;
+  const int row0 = c + 1 + blockIdx.x * RB;
+  if (row0 >= n) return;
+  const int64_t ld = n;
+  const double* A = a.A + a.aoff[b];
+  double* P = a.P + a.poff[b];
+  double* part = a.part + a.poff[a.nblk + b];
+  const int tid = threadIdx.x;
+  __shared__ double s_par[4];
+  __shared__ double ysum[4][RB];
+  __shared__ double yv_s[RB];
+
+  // reflector from the updated column c (identical in every workgroup)
+  const int ncA = (n - c + RA - 1) / RA;
+  double xn2 = 0.0;
+  const double* pn = part + part_norm_base(n);
+  for (int q = 0; q < ncA; ++q) xn2 += pn[q];
+  const double alpha = A[(c + 1) + c * ld];
+  double beta, tau, scal;
+  if (xn2 == 0.0) {
+    beta = alpha; tau = 0.0; scal = 0.0;
+  } else {
+    beta = -copysign(sqrt(alpha * alpha + xn2), alpha);
+    tau = (beta - alpha) / beta;
+    scal = 1.0 / (alpha - beta);
+  }
+  if (blockIdx.x == 0 && tid == 0) {
+    a.e[a.voff[b] + c] = beta;
+    a.tau[a.voff[b] + c] = tau;
+  }
+  const int m = n - c - 1;
+  for (int j = tid; j < m; j += 256) vs[j] = (j == 0) ? 1.0 : A[(c + 1 + j) + c * ld] * scal;
+  __syncthreads();
+  const int rl = tid & (RB - 1);
+  const int g = tid >> 6;
+  const int r = row0 + rl;
+  if (g == 0 && r < n) {
+    P[r + jj * ld] = vs[r - c - 1];
+    P[r + (3 * NB + jj) * ld] = vs[r - c - 1];
+  }
+  // y_r = sum_j A[r, j] v[j] over the trailing block (4 column groups)
+  double acc = 0.0;
+  if (r < n) {
+    const double* Ar = A + r;
+    for (int j = g; j < m; j += 4) acc += Ar[(int64_t)(c + 1 + j) * ld] * vs[j];
+  }
+  ysum[g][rl] = acc;
+  __syncthreads();
+  double yr = 0.0, vr = 0.0;
+  if (r < n) {
+    yr = ysum[0][rl] + ysum[1][rl] + ysum[2][rl] + ysum[3][rl];
+    vr = vs[r - c - 1];
+    if (g == 0) a.y[a.voff[b] + r] = yr;
+  }
+  // chunk partials: u1_i = sum V[r,i] v_r, u2_i = sum W[r,i] v_r (i < jj), y.v
+  double* pc = part + (int64_t)blockIdx.x * PSTRIDE;
+  for (int q = g; q < 2 * jj + 1; q += 4) {
+    double val = 0.0;
+    if (r < n) {
+      if (q < jj) val = P[r + q * ld] * vr;
+      else if (q < 2 * jj) val = P[r + (NB + q - jj) * ld] * vr;
+      else val = yr * vr;
+    }
+    val = wave_sum(val);
+    if (rl == 0) {
+      const int slot = (q < jj) ? q : (q < 2 * jj ? NB + (q - jj) : 2 * NB);
+      pc[slot] = val;
+    }
+  }
+  (void)s_par; (void)yv_s;
+}
+
+// ===========================================================================
+// 2. divide and conquer
+// ===========================================================================
+// integer task records
+//   leaf:  [b, s, m]
+//   merge: [b, s, mid, e, wsoff_lo, wsoff_hi, iwsoff_lo, iwsoff_hi]
+struct DcArgs {
+  const int32_t* n;
+  const int64_t* aoff;
+  const int64_t* voff;
+  double* d;   // diagonal (torn)
+  const double* e;
+  double* w;   // eigenvalues of the current subproblems
+  double* Z;
+  double* ws;
+  int32_t* iws;
+  int32_t* kout;
+};
+
+__device__ inline int64_t join64(int32_t lo, int32_t hi) {
+  return (int64_t)(uint32_t)lo | ((int64_t)hi << 32);
+}
+
+__global__ void k_dc_tear(const int32_t* bnd, int nb_, const int32_t* n, const int64_t* voff,
+                          double* d, const double* e) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nb_) return;
+  const int b = bnd[2 * i], mid = bnd[2 * i + 1];
+  const int64_t o = voff[b];
+  const double r = fabs(e[o + mid - 1]);
+  d[o + mid - 1] -= r;
+  d[o + mid] -= r;
+}
+
+// QL with implicit shifts on a leaf (m <= LEAF), one wave per leaf.  Every
+// lane runs the identical scalar recurrence on the shared dd/ee arrays and
+// applies each plane rotation to its own row of the leaf eigenvector matrix.
+__global__ __launch_bounds__(64) void k_dc_leaf(DcArgs a, const int32_t* tasks, int ntask) {
+  const int t = blockIdx.x;
+  if (t >= ntask) return;
+  const int b = tasks[3 * t], s = tasks[3 * t + 1], m = tasks[3 * t + 2];
+  const int lane = threadIdx.x;
+  const int n = a.n[b];
+  const int64_t vo = a.voff[b];
+  __shared__ double dd[LEAF + 1], ee[LEAF + 1];
+  __shared__ double zl[LEAF][LEAF + 1];
+  if (lane < m) {
+    dd[lane] = a.d[vo + s + lane];
+    ee[lane] = (lane < m - 1) ? a.e[vo + s + lane] : 0.0;
+    for (int j = 0; j < m; ++j) zl[lane][j] = (j == lane) ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  const double eps = 2.220446049250313e-16;
+  for (int l = 0; l < m; ++l) {
+    int iter = 0;
+    while (true) {
+      int mm;
+      for (mm = l; mm < m - 1; ++mm) {
+        const double dsum = fabs(dd[mm]) + fabs(dd[mm + 1]);
+        if (fabs(ee[mm]) <= eps * dsum) break;
+      }
+      if (mm == l) break;
+      if (++iter > 90) break;
+      double g = (dd[l + 1] - dd[l]) / (2.0 * ee[l]);
+      double r = hypot(g, 1.0);
+      g = dd[mm] - dd[l] + ee[l] / (g + copysign(r, g));
+      double sn = 1.0, cs = 1.0, p = 0.0;
+      int i;
+      bool early = false;
+      for (i = mm - 1; i >= l; --i) {
+        double f = sn * ee[i];
+        double bb = cs * ee[i];
+        r = hypot(f, g);
+        __syncthreads();
+        if (lane == 0) ee[i + 1] = r;
+        __syncthreads();
+        if (r == 0.0) {
+          __syncthreads();
+          if (lane == 0) { dd[i + 1] -= p; ee[mm] = 0.0; }
+          __syncthreads();
+          early = true;
+          break;
+        }
+        sn = f / r;
+        cs = g / r;
+        g = dd[i + 1] - p;
+        r = (dd[i] - g) * sn + 2.0 * cs * bb;
+        p = sn * r;
+        __syncthreads();
+        if (lane == 0) dd[i + 1] = g + p;
+        __syncthreads();
+        g = cs * r - bb;
+        if (lane < m) {
+          const double zf = zl[lane][i + 1];
+          const double zi = zl[lane][i];
+          zl[lane][i + 1] = sn * zi + cs * zf;
+          zl[lane][i] = cs * zi - sn * zf;
+        }
+      }
+      if (early) continue;
+      __syncthreads();
+      if (lane == 0) { dd[l] -= p; ee[l] = g; ee[mm] = 0.0; }
+      __syncthreads();
+    }
+  }
+  // selection sort ascending, columns follow
+  for (int i = 0; i < m - 1; ++i) {
+    int kmin = i;
+    double vmin = dd[i];
+    for (int j = i + 1; j < m; ++j)
+      if (dd[j] < vmin) { vmin = dd[j]; kmin = j; }
+    __syncthreads();
+    if (kmin != i) {
+      if (lane == 0) { dd[kmin] = dd[i]; dd[i] = vmin; }
+      if (lane < m) {
+        const double tz = zl[lane][i];
+        zl[lane][i] = zl[lane][kmin];
+        zl[lane][kmin] = tz;
+      }
+    }
+    __syncthreads();
+  }
+  if (lane < m) {
+    a.w[vo + s + lane] = dd[lane];
+    double* Z = a.Z + a.aoff[b];
+    for (int j = 0; j < m; ++j) Z[(int64_t)(s + lane) + (int64_t)(s + j) * n] = zl[lane][j];
+  }
+}
+
+// merge workspace layout (doubles, per merge, m = e - s):
+//   ds[m] zs[m] dl[m] zl[m] dv[m] zh[m] tr[m] rc[m] rs[m] | Qcd[m*m] | U[m*m] | Qt[m*m]
+// ints: col[m] nd[m] df[m] org[m] rp[2m] | sorted-deflated order[m]
+__device__ inline int dws_vec(int m) { return 9 * m; }
+
+__global__ __launch_bounds__(256) void k_dc_merge_prep(DcArgs a, const int32_t* tasks, int ntask) {
+  const int t = blockIdx.x;
+  const int32_t* tk = tasks + 8 * t;
+  const int b = tk[0], s = tk[1], mid = tk[2], e = tk[3];
+  const int64_t wo = join64(tk[4], tk[5]);
+  const int64_t io = join64(tk[6], tk[7]);
+  const int m = e - s, m1 = mid - s, m2 = e - mid;
+  const int n = a.n[b];
+  const int64_t vo = a.voff[b];
+  double* Z = a.Z + a.aoff[b];
+  double* W = a.ws + wo;
+  int32_t* I = a.iws + io;
+  double *ds = W, *zs = W + m, *dl = W + 2 * m, *zl = W + 3 * m, *dv = W + 4 * m;
+  double *rc = W + 7 * m, *rsn = W + 8 * m;
+  double* Qcd = W + dws_vec(m);
+  int32_t *col = I, *nd = I + m, *df = I + 2 * m, *rp = I + 4 * m;
+  const int tid = threadIdx.x;
+  __shared__ double s_red[4];
+  __shared__ int s_k, s_nrot;
+  __shared__ double s_rho;
+
+  const double beta = a.e[vo + mid - 1];
+  const double rho = fabs(beta);
+  const double sg = (beta >= 0.0) ? 1.0 : -1.0;
+  const double* D1 = a.w + vo + s;
+  const double* D2 = a.w + vo + mid;
+  // merge ranks of the two ascending lists (ties: left first)
+  for (int i = tid; i < m; i += 256) {
+    double v, z;
+    int rank, cidx;
+    if (i < m1) {
+      v = D1[i];
+      z = Z[(int64_t)(mid - 1) + (int64_t)(s + i) * n];
+      int lo = 0, hi = m2;
+      while (lo < hi) { int md = (lo + hi) >> 1; if (D2[md] < v) lo = md + 1; else hi = md; }
+      rank = i + lo;
+      cidx = s + i;
+    } else {
+      const int j = i - m1;
+      v = D2[j];
+      z = sg * Z[(int64_t)mid + (int64_t)(mid + j) * n];
+      int lo = 0, hi = m1;
+      while (lo < hi) { int md = (lo + hi) >> 1; if (D1[md] <= v) lo = md + 1; else hi = md; }
+      rank = j + lo;
+      cidx = mid + j;
+    }
+    ds[rank] = v;
+    zs[rank] = z * 0.7071067811865476;  // z / sqrt(2): |z| = 1
+    col[rank] = cidx;
+  }
+  __syncthreads();
+  double mx = 0.0;
+  for (int i = tid; i < m; i += 256) mx = fmax(mx, fmax(fabs(ds[i]), fabs(zs[i])));
+  for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o, 64));
+  if ((tid & 63) == 0) s_red[tid >> 6] = mx;
+  __syncthreads();
+  if (tid == 0) {
+    const double rp2 = 2.0 * rho;
+    const double tol = 8.0 * 2.220446049250313e-16 * fmax(fmax(s_red[0], s_red[1]), fmax(s_red[2], s_red[3]));
+    s_rho = rp2;
+    // serial deflation scan in ascending order
+    int k = 0, nd_def = 0, nrot = 0, pj = -1;
+    for (int j = 0; j < m; ++j) {
+      if (rp2 * fabs(zs[j]) <= tol) { df[nd_def++] = j; continue; }
+      if (pj < 0) { pj = j; continue; }
+      double sv = zs[pj], cv = zs[j];
+      const double tt = hypot(cv, sv);
+      const double dt = ds[j] - ds[pj];
+      cv /= tt;
+      sv = -sv / tt;
+      if (fabs(dt * cv * sv) <= tol) {
+        zs[j] = tt;
+        zs[pj] = 0.0;
+        rp[2 * nrot] = pj;
+        rp[2 * nrot + 1] = j;
+        rc[nrot] = cv;
+        rsn[nrot] = sv;
+        ++nrot;
+        const double t2 = ds[pj] * cv * cv + ds[j] * sv * sv;
+        ds[j] = ds[pj] * sv * sv + ds[j] * cv * cv;
+        ds[pj] = t2;
+        df[nd_def++] = pj;
+        pj = j;
+      } else {
+        nd[k++] = pj;
+        pj = j;
+      }
+    }
+    if (pj >= 0) nd[k++] = pj;
+    s_k = k;
+    s_nrot = nrot;
+    a.kout[t] = k;
+  }
+  __syncthreads();
+  const int k = s_k, nrot = s_nrot;
+  // plane rotations on the merged columns (rows s..e-1)
+  if (nrot > 0) {
+    for (int row = tid; row < m; row += 256) {
+      double* Zr = Z + (s + row);
+      for (int q = 0; q < nrot; ++q) {
+        const int64_t cp = (int64_t)col[rp[2 * q]] * n, cj = (int64_t)col[rp[2 * q + 1]] * n;
+        const double x = Zr[cp], yv = Zr[cj];
+        Zr[cp] = rc[q] * x + rsn[q] * yv;
+        Zr[cj] = rc[q] * yv - rsn[q] * x;
+      }
+    }
+  }
+  for (int i = tid; i < k; i += 256) {
+    dl[i] = ds[nd[i]];
+    zl[i] = zs[nd[i]];
+  }
+  for (int i = tid; i < m - k; i += 256) dv[i] = ds[df[i]];
+  if (tid == 0) W[5 * m] = s_rho;  // zh[0] slot keeps rho until the z-hat pass
+  __syncthreads();
+  // gather columns: [non-deflated | deflated] into Qcd (m x m, ld m)
+  for (int64_t q = tid; q < (int64_t)m * m; q += 256) {
+    const int row = (int)(q % m), c = (int)(q / m);
+    const int srcc = (c < k) ? col[nd[c]] : col[df[c - k]];
+    Qcd[q] = Z[(int64_t)(s + row) + (int64_t)srcc * n];
+  }
+}
+
+// Secular equation by bisection on tau = lambda - d_origin (one lane per root).
+__global__ __launch_bounds__(64) void k_dc_secular(DcArgs a, const int32_t* tasks, const int64_t* rootbase,
+                                                   int ntask) {
+  const int t = blockIdx.y;
+  const int i = blockIdx.x * 64 + threadIdx.x;
+  const int k = a.kout[t];
+  if (i >= k) return;
+  const int32_t* tk = tasks + 8 * t;
+  const int m = tk[3] - tk[1];
+  const int64_t wo = join64(tk[4], tk[5]);
+  const int64_t io = join64(tk[6], tk[7]);
+  double* W = a.ws + wo;
+  const double *dl = W + 2 * m, *zl = W + 3 * m;
+  double* tr = W + 6 * m;
+  int32_t* org = a.iws + io + 3 * m;
+  const double rho = W[5 * m];
+  int o;
+  double lo, hi;
+  if (i < k - 1) {
+    const double gap = dl[i + 1] - dl[i];
+    const double half = 0.5 * gap;
+    double f = 1.0;
+    for (int j = 0; j < k; ++j) f += rho * zl[j] * zl[j] / ((dl[j] - dl[i]) - half);
+    if (f > 0.0) { o = i; lo = 0.0; hi = half; }
+    else { o = i + 1; lo = -(gap - half); hi = 0.0; }
+  } else {
+    o = k - 1; lo = 0.0; hi = rho;
+    // make sure g(hi) > 0 (sum z^2 = 1 up to rounding)
+    for (int it = 0; it < 8; ++it) {
+      double f = 1.0;
+      for (int j = 0; j < k; ++j) f += rho * zl[j] * zl[j] / ((dl[j] - dl[o]) - hi);
+      if (f > 0.0) break;
+      hi *= 2.0;
+    }
+  }
+  const double dorg = dl[o];
+  for (int it = 0; it < 2200; ++it) {
+    const double md = lo + 0.5 * (hi - lo);
+    if (md <= lo || md >= hi) break;
+    double f = 1.0;
+    for (int j = 0; j < k; ++j) f += rho * zl[j] * zl[j] / ((dl[j] - dorg) - md);
+    if (f > 0.0) hi = md; else lo = md;
+  }
+  tr[i] = (o == i) ? hi : lo;   // the bracket end away from the pole
+  org[i] = o;
+}
+
+// z-hat (Gu & Eisenstat): one lane per non-deflated pole.
+__global__ __launch_bounds__(64) void k_dc_zhat(DcArgs a, const int32_t* tasks, int ntask) {
+  const int t = blockIdx.y;
+  const int j = blockIdx.x * 64 + threadIdx.x;
+  const int k = a.kout[t];
+  if (j >= k) return;
+  const int32_t* tk = tasks + 8 * t;
+  const int m = tk[3] - tk[1];
+  double* W = a.ws + join64(tk[4], tk[5]);
+  const int32_t* org = a.iws + join64(tk[6], tk[7]) + 3 * m;
+  const double *dl = W + 2 * m, *zl = W + 3 * m, *tr = W + 6 * m;
+  double* zh = W + 7 * m;   // rc slot is free after the rotations
+  const double rho = W[5 * m];
+  const double dj = dl[j];
+  double p = (dl[org[k - 1]] - dj) + tr[k - 1];
+  for (int i = 0; i < j; ++i) p *= ((dl[org[i]] - dj) + tr[i]) / (dl[i] - dj);
+  for (int i = j; i < k - 1; ++i) p *= ((dl[org[i]] - dj) + tr[i]) / (dl[i + 1] - dj);
+  p /= rho;
+  zh[j] = copysign(sqrt(fabs(p)), zl[j]);
+}
+
+// eigenvectors of the rank-one problem: one wave per column.
+__global__ __launch_bounds__(64) void k_dc_vectors(DcArgs a, const int32_t* tasks, int ntask) {
+  const int t = blockIdx.y;
+  const int i = blockIdx.x;
+  const int k = a.kout[t];
+  if (i >= k) return;
+  const int32_t* tk = tasks + 8 * t;
+  const int m = tk[3] - tk[1];
+  double* W = a.ws + join64(tk[4], tk[5]);
+  const int32_t* org = a.iws + join64(tk[6], tk[7]) + 3 * m;
+  const double *dl = W + 2 * m, *tr = W + 6 * m, *zh = W + 7 * m;
+  double* U = W + dws_vec(m) + (int64_t)m * m;
+  const double dorg = dl[org[i]], ti = tr[i];
+  double ss = 0.0;
+  for (int j = threadIdx.x; j < k; j += 64) {
+    const double v = zh[j] / ((dl[j] - dorg) - ti);
+    U[j + (int64_t)i * k] = v;
+    ss += v * v;
+  }
+  ss = wave_sum(ss);
+  const double inv = 1.0 / sqrt(ss);
+  for (int j = threadIdx.x; j < k; j += 64) U[j + (int64_t)i * k] *= inv;
+}
+
+// final ordering of one merge: lambda (ascending) merged with the sorted
+// deflated values; columns from Qt (non-deflated) and Qcd (deflated).
+__global__ __launch_bounds__(256) void k_dc_finish(DcArgs a, const int32_t* tasks, int ntask) {
+  const int t = blockIdx.x;
+  const int32_t* tk = tasks + 8 * t;
+  const int b = tk[0], s = tk[1], e = tk[3];
+  const int m = e - s;
+  const int k = a.kout[t];
+  const int n = a.n[b];
+  const int64_t vo = a.voff[b];
+  double* W = a.ws + join64(tk[4], tk[5]);
+  int32_t* I = a.iws + join64(tk[6], tk[7]);
+  const int32_t* org = I + 3 * m;
+  int32_t* dord = I + 6 * m;
+  const double *dl = W + 2 * m, *dv = W + 4 * m, *tr = W + 6 * m;
+  double* sdv = W + 8 * m;  // sorted deflated values (rs slot is free now)
+  const double* Qcd = W + dws_vec(m);
+  const double* Qt = W + dws_vec(m) + 2 * (int64_t)m * m;
+  double* Z = a.Z + a.aoff[b];
+  const int md = m - k;
+  const int tid = threadIdx.x;
+  // rank sort of the deflated values (ties by index)
+  for (int i = tid; i < md; i += 256) {
+    const double v = dv[i];
+    int r = 0;
+    for (int j = 0; j < md; ++j) {
+      const double u = dv[j];
+      r += (u < v) || (u == v && j < i);
+    }
+    dord[r] = i;
+    sdv[r] = v;
+  }
+  __syncthreads();
+  // eigenvalues: lambda_i then deflated, merged by rank
+  for (int i = tid; i < m; i += 256) {
+    double v;
+    int fin, src;
+    if (i < k) {
+      v = dl[org[i]] + tr[i];
+      int lo = 0, hi = md;
+      while (lo < hi) { int mm = (lo + hi) >> 1; if (sdv[mm] < v) lo = mm + 1; else hi = mm; }
+      fin = i + lo;
+      src = -1 - i;
+    } else {
+      const int j = i - k;
+      v = sdv[j];
+      int lo = 0, hi = k;
+      while (lo < hi) {
+        int mm = (lo + hi) >> 1;
+        const double lv = dl[org[mm]] + tr[mm];
+        if (lv <= v) lo = mm + 1; else hi = mm;
+      }
+      fin = j + lo;
+      src = dord[j];
+    }
+    a.w[vo + s + fin] = v;
+    I[5 * m + fin] = src;   // rp slot reused: source column per final position
+  }
+  __syncthreads();
+  for (int64_t q = tid; q < (int64_t)m * m; q += 256) {
+    const int row = (int)(q % m), c = (int)(q / m);
+    const int src = I[5 * m + c];
+    const double v = (src < 0) ? Qt[row + (int64_t)(-1 - src) * m] : Qcd[row + (int64_t)(k + src) * m];
+    Z[(int64_t)(s + row) + (int64_t)(s + c) * n] = v;
+  }
+}
+
+// ===========================================================================
+// 3. back-transform
+// ===========================================================================
+struct BtArgs {
+  const int32_t* n;
+  const int32_t* order;
+  const int64_t* aoff;
+  const int64_t* voff;
+  const double* A;
+  const double* tau;
+  double* V;   // per block n x NBB (ld n)
+  double* T;   // per block 2 * NBB^2: [G | T]
+};
+
+// Explicit reflector panel V (rows c0+1..n-1) for panel p.
+__global__ void k_bt_buildv(BtArgs a, int p) {
+  const int b = a.order[blockIdx.y];
+  const int n = a.n[b];
+  const int c0 = p * NBB;
+  if (c0 >= n - 1) return;
+  const int nbp = min(NBB, n - 1 - c0);
+  const int mr = n - c0 - 1;
+  const double* A = a.A + a.aoff[b];
+  double* V = a.V + a.voff[b] * NBB;
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < mr * NBB; q += gridDim.x * blockDim.x) {
+    const int r = q % mr, i = q / mr;
+    const int row = c0 + 1 + r, c = c0 + i;
+    double v = 0.0;
+    if (i < nbp) {
+      if (row == c + 1) v = 1.0;
+      else if (row > c + 1) v = A[row + (int64_t)c * n];
+    }
+    V[r + (int64_t)i * n] = v;
+  }
+}
+
+// T of the compact WY form from the Gram matrix G = V^T V.
+__global__ void k_bt_tmat(BtArgs a, int p) {
+  const int b = a.order[blockIdx.x];
+  const int n = a.n[b];
+  const int c0 = p * NBB;
+  if (c0 >= n - 1) return;
+  const int nbp = min(NBB, n - 1 - c0);
+  const double* G = a.T + (int64_t)b * 2 * NBB * NBB;
+  double* T = a.T + (int64_t)b * 2 * NBB * NBB + NBB * NBB;
+  const double* tau = a.tau + a.voff[b] + c0;
+  __shared__ double Ts[NBB][NBB + 1];
+  __shared__ double tmp[NBB];
+  const int j = threadIdx.x;
+  for (int i = 0; i < NBB; ++i) Ts[j][i] = 0.0;
+  __syncthreads();
+  for (int i = 0; i < nbp; ++i) {
+    // tmp = -tau_i G(0:i, i)
+    if (j < i) tmp[j] = -tau[i] * G[j + i * NBB];
+    __syncthreads();
+    if (j < i) {
+      double acc = 0.0;
+      for (int l = j; l < i; ++l) acc += Ts[j][l] * tmp[l];
+      Ts[j][i] = acc;
+    }
+    if (j == i) Ts[i][i] = tau[i];
+    __syncthreads();
+  }
+  for (int i = 0; i < NBB; ++i) T[j + i * NBB] = Ts[j][i];
+}
+
+// ===========================================================================
+// chop |U| < 1e-14 (diagonalization.F90:273-280)
+// ===========================================================================
+__global__ void k_chop(double* Z, size_t n, double thr) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    if (fabs(Z[i]) < thr) Z[i] = 0.0;
+}
+
+// ===========================================================================
+// host driver
+// ===========================================================================
+static double now_ms() {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+void BatchedEigensolver::release() {
+  d_n_.release(); d_order_.release(); d_aoff_.release(); d_voff_.release();
+  d_d_.release(); d_e_.release(); d_tau_.release(); d_y_.release(); d_P_.release(); d_part_.release();
+  d_poff_.release(); d_ws_.release(); d_iws_.release(); d_tasks_.release(); d_k_.release();
+  d_V_.release(); d_T_.release(); d_X_.release(); d_X2_.release(); gemm_.release();
+}
+
+void BatchedEigensolver::solve(int nblk, const int* n, const int64_t* aoff, const int64_t* voff,
+                               double* A, double* Z, double* w, const int* ncols_back, hipStream_t s,
+                               EigStats* st) {
+  nblk_ = nblk;
+  n_.assign(n, n + nblk);
+  aoff_.assign(aoff, aoff + nblk);
+  voff_.assign(voff, voff + nblk);
+  A_ = A; Z_ = Z; w_ = w;
+  tot_n_ = 0; tot_n2_ = 0;
+  int nmax = 0;
+  for (int b = 0; b < nblk; ++b) {
+    NRG_REQUIRE(n_[b] >= 1 && n_[b] <= MAXN, "eigensolver: block size out of range");
+    tot_n_ += n_[b];
+    tot_n2_ += (int64_t)n_[b] * n_[b];
+    nmax = std::max(nmax, n_[b]);
+  }
+  if (nblk == 0) return;
+  order_.resize(nblk);
+  for (int b = 0; b < nblk; ++b) order_[b] = b;
+  std::stable_sort(order_.begin(), order_.end(), [&](int x, int y) { return n_[x] > n_[y]; });
+  d_n_.upload(n_.data(), nblk, s);
+  d_order_.upload(order_.data(), nblk, s);
+  d_aoff_.upload(aoff_.data(), nblk, s);
+  d_voff_.upload(voff_.data(), nblk, s);
+  d_d_.reserve(tot_n_);
+  d_e_.reserve(tot_n_);
+  d_tau_.reserve(tot_n_);
+  fill(d_e_.p, 0.0, tot_n_, s);
+  fill(d_tau_.p, 0.0, tot_n_, s);
+  double t0 = now_ms();
+  tridiag(s);
+  if (st) { NRG_HIP(hipStreamSynchronize(s)); st->ms_tridiag = now_ms() - t0; t0 = now_ms(); }
+  divide_conquer(s);
+  if (st) { NRG_HIP(hipStreamSynchronize(s)); st->ms_dc = now_ms() - t0; t0 = now_ms(); }
+  back_transform(ncols_back, s);
+  if (st) {
+    NRG_HIP(hipStreamSynchronize(s));
+    st->ms_back = now_ms() - t0;
+    st->max_n = nmax;
+    st->nblk = nblk;
+    double f = 0;
+    for (int b = 0; b < nblk; ++b) f += (10.0 / 3.0) * (double)n_[b] * n_[b] * n_[b];
+    st->flops_alg = f;
+  }
+}
+
+void BatchedEigensolver::tridiag(hipStream_t s) {
+  const int nblk = nblk_;
+  const int nmax = n_[order_[0]];
+  std::vector<int64_t> poff(2 * nblk);
+  int64_t pacc = 0;
+  for (int b = 0; b < nblk; ++b) {
+    poff[b] = voff_[b] * 4 * NB;
+    poff[nblk + b] = pacc;
+    pacc += (int64_t)((n_[b] + RB - 1) / RB + 1) * PSTRIDE + (n_[b] + RA - 1) / RA + 1;
+  }
+  d_poff_.upload(poff.data(), poff.size(), s);
+  d_y_.reserve(tot_n_);
+  d_P_.reserve(tot_n_ * 4 * NB);
+  d_part_.reserve(pacc);
+  TrArgs ta{d_n_.p, d_order_.p, d_aoff_.p, d_voff_.p, d_poff_.p, A_, d_d_.p, d_e_.p, d_tau_.p,
+            d_y_.p, d_P_.p, d_part_.p, nblk};
+  // number of blocks with n > x (order_ is by n descending)
+  auto active = [&](int x) {
+    int lo = 0, hi = nblk;
+    while (lo < hi) { int md = (lo + hi) / 2; if (n_[order_[md]] > x) lo = md + 1; else hi = md; }
+    return lo;
+  };
+  const size_t shm = (size_t)std::max(nmax, 1) * sizeof(double);
+  NRG_REQUIRE(shm <= 64 * 1024, "eigensolver: block too large for the SYMV stage (n > 8192)");
+  for (int c = 0; c < nmax; ++c) {
+    const int j0 = c - c % NB;
+    const int jj = c - j0;
+    const int actA = active(c);
+    const int gA = ceil_div(nmax - c, RA);
+    if (jj == 0 && c > 0) {
+      // close the previous panel: finalise its last w, trailing SYR2K
+      hipLaunchKernelGGL(k_tr_col, dim3(gA, actA), dim3(256), 0, s, ta, c, NB - 1, -1, NB - 1);
+      batch_.clear();
+      for (int q = 0; q < actA; ++q) {
+        const int b = order_[q];
+        const int nb_ = n_[b];
+        const int mt = nb_ - c;
+        if (mt <= 0) continue;
+        GemmTask t = gemm_task(mt, mt, 2 * NB, 0, 1,
+                               poff[b] + c, nb_,                       // [V W] rows c..
+                               poff[b] + c + (int64_t)2 * NB * nb_, nb_, // [W V] rows c..
+                               aoff_[b] + c + (int64_t)c * nb_, nb_, -1.0, 1.0);
+        batch_.add(t);
+      }
+      gemm_.run(batch_, d_P_.p, d_P_.p, A_, nullptr, nullptr, nullptr, s);
+      hipLaunchKernelGGL(k_tr_col, dim3(gA, actA), dim3(256), 0, s, ta, c, -1, 0, NB - 1);
+    } else {
+      const int jp = (jj >= 1) ? jj - 1 : -1;
+      const int vslot = (c >= 1) ? (c - 1) % NB : 0;
+      hipLaunchKernelGGL(k_tr_col, dim3(gA, actA), dim3(256), 0, s, ta, c, jp, jj, vslot);
+    }
+    const int actB = active(c + 1);
+    if (actB > 0) {
+      const int gB = ceil_div(nmax - c - 1, RB);
+      hipLaunchKernelGGL(k_tr_symv, dim3(gB, actB), dim3(256), shm, s, ta, c, jj);
+    }
+  }
+  NRG_HIP(hipGetLastError());
+}
+
+void BatchedEigensolver::divide_conquer(hipStream_t s) {
+  const int nblk = nblk_;
+  // copy the tridiagonal diagonal (tearing modifies it)
+  std::vector<int32_t> leaves, bnd;
+  std::vector<std::vector<int32_t>> levels;  // merge records per level
+  std::vector<int> L(nblk);
+  int maxL = 0;
+  for (int b = 0; b < nblk; ++b) {
+    const int n = n_[b];
+    int l = 0;
+    while ((n + (1 << l) - 1) >> l > LEAF) ++l;
+    L[b] = l;
+    maxL = std::max(maxL, l);
+    const int nl = 1 << l;
+    auto bd = [&](int i) { return (int)(((int64_t)i * n) >> l); };
+    for (int i = 0; i < nl; ++i) {
+      const int s0 = bd(i), s1 = bd(i + 1);
+      if (s1 > s0) { leaves.push_back(b); leaves.push_back(s0); leaves.push_back(s1 - s0); }
+      if (i > 0 && s0 > 0 && s0 < n) { bnd.push_back(b); bnd.push_back(s0); }
+    }
+  }
+  levels.resize(maxL);
+  int64_t wsmax = 0, iwsmax = 0;
+  for (int lev = 1; lev <= maxL; ++lev) {
+    auto& rec = levels[lev - 1];
+    int64_t wacc = 0, iacc = 0;
+    for (int b = 0; b < nblk; ++b) {
+      if (L[b] < lev) continue;
+      const int n = n_[b];
+      const int l = L[b];
+      auto bd = [&](int i) { return (int)(((int64_t)i * n) >> l); };
+      const int nm = 1 << (l - lev);
+      for (int j = 0; j < nm; ++j) {
+        const int s0 = bd(j << lev), mid = bd((j << lev) + (1 << (lev - 1))), e0 = bd((j + 1) << lev);
+        if (!(s0 < mid && mid < e0)) continue;
+        const int m = e0 - s0;
+        rec.push_back(b); rec.push_back(s0); rec.push_back(mid); rec.push_back(e0);
+        rec.push_back((int32_t)(wacc & 0xffffffff)); rec.push_back((int32_t)(wacc >> 32));
+        rec.push_back((int32_t)(iacc & 0xffffffff)); rec.push_back((int32_t)(iacc >> 32));
+        wacc += 9 * (int64_t)m + 3 * (int64_t)m * m + 8;
+        iacc += 8 * (int64_t)m + 8;
+      }
+    }
+    wsmax = std::max(wsmax, wacc);
+    iwsmax = std::max(iwsmax, iacc);
+  }
+  d_ws_.reserve(std::max<int64_t>(wsmax, 1));
+  d_iws_.reserve(std::max<int64_t>(iwsmax, 1));
+  // zero Z (off-diagonal blocks of the merge tree must be zero)
+  fill(Z_, 0.0, tot_n2_, s);
+  DcArgs da{d_n_.p, d_aoff_.p, d_voff_.p, d_d_.p, d_e_.p, w_, Z_, d_ws_.p, d_iws_.p, nullptr};
+  // tear
+  if (!bnd.empty()) {
+    d_tasks_.upload(bnd.data(), bnd.size(), s);
+    const int nb_ = (int)bnd.size() / 2;
+    hipLaunchKernelGGL(k_dc_tear, dim3(ceil_div(nb_, 128)), dim3(128), 0, s, d_tasks_.p, nb_, d_n_.p,
+                       d_voff_.p, d_d_.p, d_e_.p);
+  }
+  // leaves
+  {
+    DevBuf<int32_t>& tb = d_tasks_;
+    tb.upload(leaves.data(), leaves.size(), s);
+    const int nt = (int)leaves.size() / 3;
+    hipLaunchKernelGGL(k_dc_leaf, dim3(nt), dim3(64), 0, s, da, tb.p, nt);
+  }
+  for (int lev = 1; lev <= maxL; ++lev) {
+    auto& rec = levels[lev - 1];
+    const int nt = (int)rec.size() / 8;
+    if (nt == 0) continue;
+    d_tasks_.upload(rec.data(), rec.size(), s);
+    d_k_.reserve(nt);
+    da.kout = d_k_.p;
+    hipLaunchKernelGGL(k_dc_merge_prep, dim3(nt), dim3(256), 0, s, da, d_tasks_.p, nt);
+    h_k_.resize(nt);
+    d_k_.download(h_k_.data(), nt, s);
+    NRG_HIP(hipStreamSynchronize(s));
+    int kmax = 0;
+    for (int i = 0; i < nt; ++i) kmax = std::max(kmax, h_k_[i]);
+    if (kmax > 0) {
+      hipLaunchKernelGGL(k_dc_secular, dim3(ceil_div(kmax, 64), nt), dim3(64), 0, s, da, d_tasks_.p,
+                         (const int64_t*)nullptr, nt);
+      hipLaunchKernelGGL(k_dc_zhat, dim3(ceil_div(kmax, 64), nt), dim3(64), 0, s, da, d_tasks_.p, nt);
+      hipLaunchKernelGGL(k_dc_vectors, dim3(kmax, nt), dim3(64), 0, s, da, d_tasks_.p, nt);
+      batch_.clear();
+      for (int i = 0; i < nt; ++i) {
+        const int k = h_k_[i];
+        if (k == 0) continue;
+        const int m = rec[8 * i + 3] - rec[8 * i + 1];
+        const int64_t wo = (int64_t)(uint32_t)rec[8 * i + 4] | ((int64_t)rec[8 * i + 5] << 32);
+        const int64_t base = wo + 9 * (int64_t)m;
+        batch_.add(gemm_task(m, k, k, 0, 0, base, m, base + (int64_t)m * m, k,
+                             base + 2 * (int64_t)m * m, m, 1.0, 0.0));
+      }
+      gemm_.run(batch_, d_ws_.p, d_ws_.p, d_ws_.p, nullptr, nullptr, nullptr, s);
+    }
+    hipLaunchKernelGGL(k_dc_finish, dim3(nt), dim3(256), 0, s, da, d_tasks_.p, nt);
+  }
+  NRG_HIP(hipGetLastError());
+}
+
+void BatchedEigensolver::back_transform(const int* ncols_back, hipStream_t s) {
+  const int nblk = nblk_;
+  const int nmax = n_[order_[0]];
+  d_V_.reserve(tot_n_ * NBB);
+  d_T_.reserve((int64_t)nblk * 2 * NBB * NBB);
+  d_X_.reserve(tot_n_ * NBB);
+  d_X2_.reserve(tot_n_ * NBB);
+  BtArgs ba{d_n_.p, d_order_.p, d_aoff_.p, d_voff_.p, A_, d_tau_.p, d_V_.p, d_T_.p};
+  const int npan = (nmax - 1 + NBB - 1) / NBB;
+  for (int p = npan - 1; p >= 0; --p) {
+    const int c0 = p * NBB;
+    // active: n - 1 > c0  <=>  n > c0 + 1
+    int act = 0;
+    while (act < nblk && n_[order_[act]] > c0 + 1) ++act;
+    if (act == 0) continue;
+    hipLaunchKernelGGL(k_bt_buildv, dim3(std::max(1, ceil_div((int64_t)(nmax - c0) * NBB, 256 * 4)), act),
+                       dim3(256), 0, s, ba, p);
+    // G = V^T V
+    batch_.clear();
+    for (int q = 0; q < act; ++q) {
+      const int b = order_[q];
+      const int n = n_[b];
+      const int mr = n - c0 - 1;
+      const int nbp = std::min(NBB, n - 1 - c0);
+      batch_.add(gemm_task(nbp, nbp, mr, 1, 0, voff_[b] * NBB, n, voff_[b] * NBB, n,
+                           (int64_t)b * 2 * NBB * NBB, NBB, 1.0, 0.0));
+    }
+    gemm_.run(batch_, d_V_.p, d_V_.p, d_T_.p, nullptr, nullptr, nullptr, s);
+    hipLaunchKernelGGL(k_bt_tmat, dim3(act), dim3(NBB), 0, s, ba, p);
+    // X = V^T Z(c0+1:, :nc)
+    batch_.clear();
+    for (int q = 0; q < act; ++q) {
+      const int b = order_[q];
+      const int n = n_[b];
+      const int mr = n - c0 - 1;
+      const int nbp = std::min(NBB, n - 1 - c0);
+      const int nc = ncols_back ? ncols_back[b] : n;
+      if (nc <= 0) continue;
+      batch_.add(gemm_task(nbp, nc, mr, 1, 0, voff_[b] * NBB, n, aoff_[b] + c0 + 1, n,
+                           voff_[b] * NBB, NBB, 1.0, 0.0));
+    }
+    gemm_.run(batch_, d_V_.p, Z_, d_X_.p, nullptr, nullptr, nullptr, s);
+    // X2 = T X
+    batch_.clear();
+    for (int q = 0; q < act; ++q) {
+      const int b = order_[q];
+      const int n = n_[b];
+      const int nbp = std::min(NBB, n - 1 - c0);
+      const int nc = ncols_back ? ncols_back[b] : n;
+      if (nc <= 0) continue;
+      batch_.add(gemm_task(nbp, nc, nbp, 0, 0, (int64_t)b * 2 * NBB * NBB + NBB * NBB, NBB,
+                           voff_[b] * NBB, NBB, voff_[b] * NBB, NBB, 1.0, 0.0));
+    }
+    gemm_.run(batch_, d_T_.p, d_X_.p, d_X2_.p, nullptr, nullptr, nullptr, s);
+    // Z(c0+1:, :nc) -= V X2
+    batch_.clear();
+    for (int q = 0; q < act; ++q) {
+      const int b = order_[q];
+      const int n = n_[b];
+      const int mr = n - c0 - 1;
+      const int nbp = std::min(NBB, n - 1 - c0);
+      const int nc = ncols_back ? ncols_back[b] : n;
+      if (nc <= 0) continue;
+      batch_.add(gemm_task(mr, nc, nbp, 0, 0, voff_[b] * NBB, n, voff_[b] * NBB, NBB,
+                           aoff_[b] + c0 + 1, n, -1.0, 1.0));
+    }
+    gemm_.run(batch_, d_V_.p, d_X2_.p, Z_, nullptr, nullptr, nullptr, s);
+  }
+  {
+    unsigned g = (unsigned)std::min<int64_t>((tot_n2_ + 255) / 256, 16384);
+    hipLaunchKernelGGL(k_chop, dim3(g), dim3(256), 0, s, Z_, (size_t)tot_n2_, 1e-14);
+  }
+  NRG_HIP(hipGetLastError());
+}
+
+}  // namespace nrg

@@ -1,0 +1,61 @@
+// Batched, variable-size FP64 GEMM on the CDNA4 FP64 matrix cores.
+//
+//   C_t = alpha_t * opA_t * opB_t + beta_t * C_t      for every task t
+//
+// Matrices are column-major.  Each stored operand may carry a row-gather
+// index list (stored row r of the operand is fetched from row idx[r]) and the
+// inner dimension may carry a per-k scale applied to opB.  This covers the
+// plain products of the eigensolver (trailing updates, divide-and-conquer
+// merges, back-transform) and the gathered products of the invariant-matrix,
+// spectral and reduced-density contractions, whose coefficient matrices are
+// signed partial permutations or per-operator dense sub-blocks.
+#pragma once
+#include "common.h"
+
+namespace nrg {
+
+struct GemmTask {
+  int m, n, k;
+  int lda, ldb, ldc;
+  int transA, transB;       // 0: N, 1: T (of the stored matrix)
+  int64_t offA, offB, offC; // element offsets into the base pointers
+  int64_t offGA, offGB;     // offsets into the gather arrays, -1 = identity
+  int64_t offS;             // offset into the k-scale array, -1 = none
+  double alpha, beta;
+};
+
+struct GemmBatch {
+  std::vector<GemmTask> tasks;
+  void add(const GemmTask& t) {
+    if (t.m > 0 && t.n > 0) tasks.push_back(t);
+  }
+  void clear() { tasks.clear(); }
+};
+
+// Runs all tasks of the batch in one launch.  A/B/C/gather/scale base
+// pointers are shared by all tasks; tasks address them through offsets.
+// Buffers for the task table live in the caller-owned workspace.
+struct GemmRunner {
+  DevBuf<GemmTask> d_tasks;
+  DevBuf<int32_t> d_tile_start;
+  std::vector<int32_t> h_tile_start;
+  void run(const GemmBatch& b, const double* A, const double* B, double* C,
+           const int32_t* gatherA, const int32_t* gatherB, const double* kscale,
+           hipStream_t s);
+  void release() { d_tasks.release(); d_tile_start.release(); }
+};
+
+inline GemmTask gemm_task(int m, int n, int k, int transA, int transB,
+                          int64_t offA, int lda, int64_t offB, int ldb,
+                          int64_t offC, int ldc, double alpha, double beta) {
+  GemmTask t;
+  t.m = m; t.n = n; t.k = k;
+  t.lda = lda; t.ldb = ldb; t.ldc = ldc;
+  t.transA = transA; t.transB = transB;
+  t.offA = offA; t.offB = offB; t.offC = offC;
+  t.offGA = -1; t.offGB = -1; t.offS = -1;
+  t.alpha = alpha; t.beta = beta;
+  return t;
+}
+
+}  // namespace nrg

@@ -1,0 +1,124 @@
+"""ctypes binding of libnrg_gpu.so (the C ABI declared in include/nrg_gpu.h).
+
+The library is loaded from the package's in-tree ``lib/`` directory.  There is
+no fallback: if the shared library is missing or the device is absent the
+calls raise.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.dirname(HERE)
+LIB_PATH = os.path.join(PKG, "lib", "libnrg_gpu.so")
+
+_lib = None
+
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_f64p = ctypes.POINTER(ctypes.c_double)
+
+SIGNATURES = {
+    "nrg_gpu_init": (ctypes.c_int, [ctypes.c_int]),
+    "nrg_gpu_finalize": (ctypes.c_int, []),
+    "nrg_gpu_last_error": (ctypes.c_char_p, []),
+    "nrg_gpu_set_model": (ctypes.c_int, [ctypes.c_int] * 5 + [_i32p, _i32p, _f64p, _i32p, _f64p, _i32p,
+                                                              ctypes.c_int, _f64p, _i32p]),
+    "nrg_gpu_set_invariant": (ctypes.c_int, [ctypes.c_int, _f64p]),
+    "nrg_gpu_set_invariant_spectrum": (ctypes.c_int, [ctypes.c_int, _f64p]),
+    "nrg_gpu_diagonalize": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _i32p, ctypes.c_int, _i32p, ctypes.c_int,
+                                           _f64p, _f64p, _f64p, ctypes.c_double, _f64p, _f64p, ctypes.c_int64]),
+    "nrg_gpu_invariant": (ctypes.c_int, [_i32p, _f64p]),
+    "nrg_gpu_invariant_spectrum": (ctypes.c_int, [_i32p, _f64p]),
+    "nrg_gpu_stage_times": (ctypes.c_int, [_f64p, ctypes.c_int]),
+    "nrg_gpu_syev_batched": (ctypes.c_int, [ctypes.c_int, _i32p, _f64p, _f64p, _f64p]),
+    "nrg_gpu_dgemm": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                     ctypes.c_double, _f64p, ctypes.c_int, _f64p, ctypes.c_int, ctypes.c_double,
+                                     _f64p, ctypes.c_int]),
+    "nrg_gpu_inject_fault": (ctypes.c_int, [ctypes.c_int]),
+}
+
+
+class NrgGpuError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+def load(path: Optional[str] = None):
+    """Load the shared library (raises if it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise FileNotFoundError(f"{p} not built (run __graft_entry__.build() or make -C nrg-mpi_amd)")
+    lib = ctypes.CDLL(p)
+    for name, (res, args) in SIGNATURES.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _p(a, t):
+    if a is None:
+        return None
+    return a.ctypes.data_as(t)
+
+
+def _check(rc):
+    if rc != 0:
+        raise NrgGpuError(rc, _lib.nrg_gpu_last_error().decode())
+
+
+def init(device: int = 0):
+    _check(load().nrg_gpu_init(device))
+
+
+def finalize():
+    _check(load().nrg_gpu_finalize())
+
+
+def dgemm(A, B, C=None, transa=False, transb=False, alpha=1.0, beta=0.0):
+    A = np.asfortranarray(A, dtype=np.float64)
+    B = np.asfortranarray(B, dtype=np.float64)
+    m = A.shape[1] if transa else A.shape[0]
+    k = A.shape[0] if transa else A.shape[1]
+    n = B.shape[0] if transb else B.shape[1]
+    C = np.zeros((m, n), order="F") if C is None else np.asfortranarray(C, dtype=np.float64).copy(order="F")
+    _check(load().nrg_gpu_dgemm(int(transa), int(transb), m, n, k, alpha, _p(A, _f64p), max(A.shape[0], 1),
+                                _p(B, _f64p), max(B.shape[0], 1), beta, _p(C, _f64p), max(m, 1)))
+    return C
+
+
+def syev_batched(blocks):
+    """Eigen-decompose a list of symmetric matrices in one batched call.
+
+    Returns lists (w_b ascending, U_b) like numpy.linalg.eigh per block.
+    """
+    n = np.array([b.shape[0] for b in blocks], dtype=np.int32)
+    H = np.concatenate([np.asfortranarray(b, dtype=np.float64).ravel(order="F") for b in blocks])
+    w = np.zeros(int(n.sum()))
+    U = np.zeros_like(H)
+    _check(load().nrg_gpu_syev_batched(len(blocks), _p(n, _i32p), _p(H, _f64p), _p(w, _f64p), _p(U, _f64p)))
+    ws, Us = [], []
+    o1 = o2 = 0
+    for nb in n:
+        nb = int(nb)
+        ws.append(w[o1:o1 + nb])
+        Us.append(U[o2:o2 + nb * nb].reshape(nb, nb, order="F"))
+        o1 += nb
+        o2 += nb * nb
+    return ws, Us
+
+
+def stage_times():
+    t = np.zeros(10)
+    _check(load().nrg_gpu_stage_times(_p(t, _f64p), 10))
+    keys = ["assemble", "eig", "tridiag", "dc", "back", "invariant", "spectral", "rho", "cfs", "eig_flops"]
+    return dict(zip(keys, t.tolist()))
