@@ -111,6 +111,18 @@ int nrg_gpu_dgemm(int transa, int transb, int m, int n, int k, double alpha,
                   const double* A, int lda, const double* B, int ldb, double beta,
                   double* C, int ldc);
 
+/* Per-kernel-family HIP-event timing (no counterpart in the reference; the
+ * reference's count.f90 timers measure host wall clock only).  enable != 0
+ * clears and starts accumulation; nrg_gpu_kernel_stats writes, for family
+ * f < nfam, out[4f..4f+3] = {milliseconds, launches, algorithmic bytes,
+ * algorithmic flops}.  Families: 0 H assembly, 1 tridiagonal column update,
+ * 2 tridiagonal SYMV, 3 tridiagonal trailing GEMM, 4 D&C leaves, 5 D&C merge
+ * set-up, 6 D&C secular equations, 7 D&C vectors, 8 D&C GEMM, 9 D&C
+ * reorder, 10 back-transform small kernels, 11 back-transform GEMM,
+ * 12 invariant GEMM, 13 spectral invariant GEMM, 14 other. */
+int nrg_gpu_profile(int enable);
+int nrg_gpu_kernel_stats(double* out, int nfam);
+
 /* Test hook with no counterpart in the reference: make the next
  * nrg_gpu_diagonalize of `iteration` fail with status 5 (error-path tests). */
 int nrg_gpu_inject_fault(int iteration);

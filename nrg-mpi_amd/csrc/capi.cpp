@@ -14,6 +14,11 @@ Context& ctx() {
   return c;
 }
 
+Profiler& profiler() {
+  static Profiler p;
+  return p;
+}
+
 static double wall_ms() {
   return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
@@ -23,6 +28,10 @@ static int guarded(F&& f) {
   Context& C = ctx();
   try {
     f(C);
+    if (profiler().on && C.stream) {
+      NRG_HIP(hipStreamSynchronize(C.stream));
+      profiler().drain();
+    }
     C.last_error.clear();
     return NRG_OK;
   } catch (const Error& e) {
@@ -327,6 +336,27 @@ int nrg_gpu_dgemm(int transa, int transb, int m, int n, int k, double alpha, con
 
 int nrg_gpu_inject_fault(int iteration) {
   return guarded([&](Context& C) { C.fault_iteration = iteration; });
+}
+
+int nrg_gpu_profile(int enable) {
+  return guarded([&](Context& C) {
+    if (C.stream) NRG_HIP(hipStreamSynchronize(C.stream));
+    profiler().drain();
+    profiler().reset();
+    profiler().on = (enable != 0);
+  });
+}
+
+int nrg_gpu_kernel_stats(double* out, int nfam) {
+  return guarded([&](Context&) {
+    for (int f = 0; f < nfam && f < KF_COUNT; ++f) {
+      const KStat& k = profiler().st[f];
+      out[4 * f + 0] = k.ms;
+      out[4 * f + 1] = (double)k.launches;
+      out[4 * f + 2] = k.bytes;
+      out[4 * f + 3] = k.flops;
+    }
+  });
 }
 
 }  // extern "C"

@@ -8,6 +8,7 @@
 //    & Sorensen (1987) and the eigenvector formula of Gu & Eisenstat (1995);
 //  * compact WY accumulation of reflectors (Schreiber & Van Loan 1989).
 #include "eig.h"
+#include "profiler.h"
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -787,6 +788,11 @@ void BatchedEigensolver::tridiag(hipStream_t s) {
     return lo;
   };
   const size_t shm = (size_t)std::max(nmax, 1) * sizeof(double);
+  auto colbytes = [&](int c, int act) {
+    double b = 0;
+    for (int q = 0; q < act; ++q) b += 8.0 * 4.0 * (n_[order_[q]] - c) * (1 + NB / 8);
+    return b;
+  };
   NRG_REQUIRE(shm <= 64 * 1024, "eigensolver: block too large for the SYMV stage (n > 8192)");
   for (int c = 0; c < nmax; ++c) {
     const int j0 = c - c % NB;
@@ -795,7 +801,11 @@ void BatchedEigensolver::tridiag(hipStream_t s) {
     const int gA = ceil_div(nmax - c, RA);
     if (jj == 0 && c > 0) {
       // close the previous panel: finalise its last w, trailing SYR2K
-      hipLaunchKernelGGL(k_tr_col, dim3(gA, actA), dim3(256), 0, s, ta, c, NB - 1, -1, NB - 1);
+      {
+        const int pid = profiler().begin(s);
+        hipLaunchKernelGGL(k_tr_col, dim3(gA, actA), dim3(256), 0, s, ta, c, NB - 1, -1, NB - 1);
+        profiler().end(pid, s, KF_TR_COL, colbytes(c, actA), 0);
+      }
       batch_.clear();
       for (int q = 0; q < actA; ++q) {
         const int b = order_[q];
@@ -808,17 +818,30 @@ void BatchedEigensolver::tridiag(hipStream_t s) {
                                aoff_[b] + c + (int64_t)c * nb_, nb_, -1.0, 1.0);
         batch_.add(t);
       }
-      gemm_.run(batch_, d_P_.p, d_P_.p, A_, nullptr, nullptr, nullptr, s);
+      gemm_.run(batch_, d_P_.p, d_P_.p, A_, nullptr, nullptr, nullptr, s, KF_GEMM_TRIDIAG);
+      const int pid = profiler().begin(s);
       hipLaunchKernelGGL(k_tr_col, dim3(gA, actA), dim3(256), 0, s, ta, c, -1, 0, NB - 1);
+      profiler().end(pid, s, KF_TR_COL, colbytes(c, actA), 0);
     } else {
       const int jp = (jj >= 1) ? jj - 1 : -1;
       const int vslot = (c >= 1) ? (c - 1) % NB : 0;
+      const int pid = profiler().begin(s);
       hipLaunchKernelGGL(k_tr_col, dim3(gA, actA), dim3(256), 0, s, ta, c, jp, jj, vslot);
+      profiler().end(pid, s, KF_TR_COL, colbytes(c, actA), 0);
     }
     const int actB = active(c + 1);
     if (actB > 0) {
       const int gB = ceil_div(nmax - c - 1, RB);
+      // algorithmic traffic: the trailing (n-c-1)^2 block once, plus the vectors
+      double bytes = 0, flops = 0;
+      for (int q = 0; q < actB; ++q) {
+        const double m = n_[order_[q]] - c - 1;
+        bytes += 8.0 * (m * m + 6.0 * m);
+        flops += 2.0 * m * m;
+      }
+      const int pid = profiler().begin(s);
       hipLaunchKernelGGL(k_tr_symv, dim3(gB, actB), dim3(256), shm, s, ta, c, jj);
+      profiler().end(pid, s, KF_TR_SYMV, bytes, flops);
     }
   }
   NRG_HIP(hipGetLastError());
@@ -887,7 +910,9 @@ void BatchedEigensolver::divide_conquer(hipStream_t s) {
     DevBuf<int32_t>& tb = d_tasks_;
     tb.upload(leaves.data(), leaves.size(), s);
     const int nt = (int)leaves.size() / 3;
+    const int pid = profiler().begin(s);
     hipLaunchKernelGGL(k_dc_leaf, dim3(nt), dim3(64), 0, s, da, tb.p, nt);
+    profiler().end(pid, s, KF_DC_LEAF, 0, 0);
   }
   for (int lev = 1; lev <= maxL; ++lev) {
     auto& rec = levels[lev - 1];
@@ -896,17 +921,25 @@ void BatchedEigensolver::divide_conquer(hipStream_t s) {
     d_tasks_.upload(rec.data(), rec.size(), s);
     d_k_.reserve(nt);
     da.kout = d_k_.p;
-    hipLaunchKernelGGL(k_dc_merge_prep, dim3(nt), dim3(256), 0, s, da, d_tasks_.p, nt);
+    {
+      const int pid = profiler().begin(s);
+      hipLaunchKernelGGL(k_dc_merge_prep, dim3(nt), dim3(256), 0, s, da, d_tasks_.p, nt);
+      profiler().end(pid, s, KF_DC_PREP, 0, 0);
+    }
     h_k_.resize(nt);
     d_k_.download(h_k_.data(), nt, s);
     NRG_HIP(hipStreamSynchronize(s));
     int kmax = 0;
     for (int i = 0; i < nt; ++i) kmax = std::max(kmax, h_k_[i]);
     if (kmax > 0) {
+      int pid = profiler().begin(s);
       hipLaunchKernelGGL(k_dc_secular, dim3(ceil_div(kmax, 64), nt), dim3(64), 0, s, da, d_tasks_.p,
                          (const int64_t*)nullptr, nt);
+      profiler().end(pid, s, KF_DC_SECULAR, 0, 0);
+      pid = profiler().begin(s);
       hipLaunchKernelGGL(k_dc_zhat, dim3(ceil_div(kmax, 64), nt), dim3(64), 0, s, da, d_tasks_.p, nt);
       hipLaunchKernelGGL(k_dc_vectors, dim3(kmax, nt), dim3(64), 0, s, da, d_tasks_.p, nt);
+      profiler().end(pid, s, KF_DC_VECTORS, 0, 0);
       batch_.clear();
       for (int i = 0; i < nt; ++i) {
         const int k = h_k_[i];
@@ -917,9 +950,11 @@ void BatchedEigensolver::divide_conquer(hipStream_t s) {
         batch_.add(gemm_task(m, k, k, 0, 0, base, m, base + (int64_t)m * m, k,
                              base + 2 * (int64_t)m * m, m, 1.0, 0.0));
       }
-      gemm_.run(batch_, d_ws_.p, d_ws_.p, d_ws_.p, nullptr, nullptr, nullptr, s);
+      gemm_.run(batch_, d_ws_.p, d_ws_.p, d_ws_.p, nullptr, nullptr, nullptr, s, KF_GEMM_DC);
     }
+    const int pid = profiler().begin(s);
     hipLaunchKernelGGL(k_dc_finish, dim3(nt), dim3(256), 0, s, da, d_tasks_.p, nt);
+    profiler().end(pid, s, KF_DC_FINISH, 0, 0);
   }
   NRG_HIP(hipGetLastError());
 }
@@ -951,7 +986,7 @@ void BatchedEigensolver::back_transform(const int* ncols_back, hipStream_t s) {
       batch_.add(gemm_task(nbp, nbp, mr, 1, 0, voff_[b] * NBB, n, voff_[b] * NBB, n,
                            (int64_t)b * 2 * NBB * NBB, NBB, 1.0, 0.0));
     }
-    gemm_.run(batch_, d_V_.p, d_V_.p, d_T_.p, nullptr, nullptr, nullptr, s);
+    gemm_.run(batch_, d_V_.p, d_V_.p, d_T_.p, nullptr, nullptr, nullptr, s, KF_GEMM_BACK);
     hipLaunchKernelGGL(k_bt_tmat, dim3(act), dim3(NBB), 0, s, ba, p);
     // X = V^T Z(c0+1:, :nc)
     batch_.clear();
@@ -965,7 +1000,7 @@ void BatchedEigensolver::back_transform(const int* ncols_back, hipStream_t s) {
       batch_.add(gemm_task(nbp, nc, mr, 1, 0, voff_[b] * NBB, n, aoff_[b] + c0 + 1, n,
                            voff_[b] * NBB, NBB, 1.0, 0.0));
     }
-    gemm_.run(batch_, d_V_.p, Z_, d_X_.p, nullptr, nullptr, nullptr, s);
+    gemm_.run(batch_, d_V_.p, Z_, d_X_.p, nullptr, nullptr, nullptr, s, KF_GEMM_BACK);
     // X2 = T X
     batch_.clear();
     for (int q = 0; q < act; ++q) {
@@ -977,7 +1012,7 @@ void BatchedEigensolver::back_transform(const int* ncols_back, hipStream_t s) {
       batch_.add(gemm_task(nbp, nc, nbp, 0, 0, (int64_t)b * 2 * NBB * NBB + NBB * NBB, NBB,
                            voff_[b] * NBB, NBB, voff_[b] * NBB, NBB, 1.0, 0.0));
     }
-    gemm_.run(batch_, d_T_.p, d_X_.p, d_X2_.p, nullptr, nullptr, nullptr, s);
+    gemm_.run(batch_, d_T_.p, d_X_.p, d_X2_.p, nullptr, nullptr, nullptr, s, KF_GEMM_BACK);
     // Z(c0+1:, :nc) -= V X2
     batch_.clear();
     for (int q = 0; q < act; ++q) {
@@ -990,7 +1025,7 @@ void BatchedEigensolver::back_transform(const int* ncols_back, hipStream_t s) {
       batch_.add(gemm_task(mr, nc, nbp, 0, 0, voff_[b] * NBB, n, voff_[b] * NBB, NBB,
                            aoff_[b] + c0 + 1, n, -1.0, 1.0));
     }
-    gemm_.run(batch_, d_V_.p, d_X2_.p, Z_, nullptr, nullptr, nullptr, s);
+    gemm_.run(batch_, d_V_.p, d_X2_.p, Z_, nullptr, nullptr, nullptr, s, KF_GEMM_BACK);
   }
   {
     unsigned g = (unsigned)std::min<int64_t>((tot_n2_ + 255) / 256, 16384);

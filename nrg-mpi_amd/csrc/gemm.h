@@ -11,6 +11,7 @@
 // signed partial permutations or per-operator dense sub-blocks.
 #pragma once
 #include "common.h"
+#include "profiler.h"
 
 namespace nrg {
 
@@ -41,7 +42,7 @@ struct GemmRunner {
   std::vector<int32_t> h_tile_start;
   void run(const GemmBatch& b, const double* A, const double* B, double* C,
            const int32_t* gatherA, const int32_t* gatherB, const double* kscale,
-           hipStream_t s);
+           hipStream_t s, int family = KF_OTHER);
   void release() { d_tasks.release(); d_tile_start.release(); }
 };
 

@@ -103,12 +103,13 @@ __global__ __launch_bounds__(256) void k_gemm_f64(
     }
     __syncthreads();
   }
-  // ---- epilogue: D[i][j] lives in lane j + 16*(i/4), register i%4
+  // ---- epilogue: D[i][j] lives in lane j + 16*(i%4), register i/4
+  // (lane layout of v_mfma_f64_16x16x4f64 checked on gfx950 by test_dgemm)
   double* Cb = C + t.offC;
   for (int mi = 0; mi < 2; ++mi)
     for (int ni = 0; ni < 2; ++ni)
       for (int r = 0; r < 4; ++r) {
-        const int gi = m0 + wm * 32 + mi * 16 + 4 * (lane >> 4) + r;
+        const int gi = m0 + wm * 32 + mi * 16 + (lane >> 4) + 4 * r;
         const int gj = n0 + wn * 32 + ni * 16 + (lane & 15);
         if (gi < t.m && gj < t.n) {
           double* c = Cb + (int64_t)gi + (int64_t)gj * t.ldc;
@@ -121,22 +122,27 @@ __global__ __launch_bounds__(256) void k_gemm_f64(
 
 void GemmRunner::run(const GemmBatch& b, const double* A, const double* B, double* C,
                      const int32_t* gatherA, const int32_t* gatherB, const double* kscale,
-                     hipStream_t s) {
+                     hipStream_t s, int family) {
   const int nt = (int)b.tasks.size();
   if (nt == 0) return;
   h_tile_start.resize(nt);
   long long total = 0;
+  double flops = 0, bytes = 0;
   for (int i = 0; i < nt; ++i) {
     const GemmTask& t = b.tasks[i];
     NRG_REQUIRE(t.m > 0 && t.n > 0 && t.k >= 0, "gemm: bad task shape");
+    flops += 2.0 * t.m * t.n * t.k;
+    bytes += 8.0 * ((double)t.m * t.k + (double)t.k * t.n + (t.beta != 0.0 ? 2.0 : 1.0) * t.m * t.n);
     h_tile_start[i] = (int32_t)total;
     total += (long long)ceil_div(t.m, TM) * ceil_div(t.n, TN);
   }
   NRG_REQUIRE(total < (1LL << 31), "gemm: too many tiles");
   d_tasks.upload(b.tasks, s);
   d_tile_start.upload(h_tile_start, s);
+  const int pid = profiler().begin(s);
   hipLaunchKernelGGL(k_gemm_f64, dim3((unsigned)total), dim3(256), 0, s, d_tasks.p,
                      d_tile_start.p, nt, A, B, C, gatherA, gatherB, kscale);
+  profiler().end(pid, s, family, bytes, flops);
   NRG_HIP(hipGetLastError());
 }
 

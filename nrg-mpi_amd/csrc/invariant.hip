@@ -121,7 +121,7 @@ void invariant(Context& C, const int32_t* kept, double* out_host) {
   if (!batch.tasks.empty()) {
     C.d_gidx.upload(gidx, C.stream);
     C.d_gscale.upload(gsc, C.stream);
-    C.gemm.run(batch, C.d_U.p, C.d_U.p, C.d_inv_next.p, C.d_gidx.p, C.d_gidx.p, C.d_gscale.p, C.stream);
+    C.gemm.run(batch, C.d_U.p, C.d_U.p, C.d_inv_next.p, C.d_gidx.p, C.d_gidx.p, C.d_gscale.p, C.stream, KF_GEMM_INV);
   }
   std::swap(C.d_inv, C.d_inv_next);
   C.kold = (int)k;
@@ -203,8 +203,8 @@ void invariant_spectrum(Context& C, const int32_t* kept, double* out_host) {
   if (!bm.tasks.empty()) {
     C.d_tmp.reserve(xacc);
     C.d_gidx.upload(gidx, C.stream);
-    C.gemm.run(bx, C.d_spec.p, C.d_U.p, C.d_tmp.p, nullptr, C.d_gidx.p, nullptr, C.stream);
-    C.gemm.run(bm, C.d_tmp.p, C.d_U.p, C.d_spec_next.p, nullptr, C.d_gidx.p, nullptr, C.stream);
+    C.gemm.run(bx, C.d_spec.p, C.d_U.p, C.d_tmp.p, nullptr, C.d_gidx.p, nullptr, C.stream, KF_GEMM_SPEC);
+    C.gemm.run(bm, C.d_tmp.p, C.d_U.p, C.d_spec_next.p, nullptr, C.d_gidx.p, nullptr, C.stream, KF_GEMM_SPEC);
   }
   std::swap(C.d_spec, C.d_spec_next);
   C.kold_spec = (int)k;

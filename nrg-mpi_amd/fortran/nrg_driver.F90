@@ -1,0 +1,174 @@
+! bind(C) entry points of libnrg_host.so: the driver surface used by the
+! Python tests and the benchmark (and by any C caller).  They wrap the
+! Fortran host's NRG loop; the program nrg_amd uses the same routines.
+module nrg_driver
+  use iso_c_binding
+  use nrg_gpu_api
+  use nrg_host
+  implicit none
+contains
+
+  function c_string(p, n) result(s)
+    type(c_ptr), value :: p
+    integer(c_int), value :: n
+    character(len=n) :: s
+    character(kind=c_char), pointer :: cs(:)
+    integer :: i
+    call c_f_pointer(p, cs, [n])
+    do i = 1, n
+       s(i:i) = cs(i)
+    end do
+  end function c_string
+
+  ! Read the three input files (no device needed).
+  integer(c_int) function nrg_host_read(dir, n) bind(C, name="nrg_host_read")
+    type(c_ptr), value :: dir
+    integer(c_int), value :: n
+    integer :: ierr
+    call read_inputs(c_string(dir, n), ierr)
+    nrg_host_read = ierr
+  end function nrg_host_read
+
+  ! Read the inputs, initialise the device and upload the model.
+  !   sort_mode 0: exact replay of the reference's exchange sort, 1: canonical
+  integer(c_int) function nrg_host_open(dir, n, device, smode, keepvec) bind(C, name="nrg_host_open")
+    type(c_ptr), value :: dir
+    integer(c_int), value :: n, device, smode, keepvec
+    integer :: ierr
+    sort_mode = smode
+    keep_vectors = (keepvec /= 0)
+    call setup(c_string(dir, n), int(device), ierr)
+    nrg_host_open = ierr
+  end function nrg_host_open
+
+  ! One NRG iteration (nrgIteration.F90:1-31).
+  integer(c_int) function nrg_host_iterate(it) bind(C, name="nrg_host_iterate")
+    integer(c_int), value :: it
+    integer :: ierr
+    call nrg_iteration(int(it), ierr)
+    nrg_host_iterate = ierr
+  end function nrg_host_iterate
+
+  ! prepareBasis alone on the current basis_output (CPU only, for tests).
+  integer(c_int) function nrg_host_prepare(smode) bind(C, name="nrg_host_prepare")
+    integer(c_int), value :: smode
+    integer :: ierr
+    sort_mode = smode
+    call prepare_basis(ierr)
+    nrg_host_prepare = ierr
+  end function nrg_host_prepare
+
+  ! Replace the current basis_output / eigenvalues (CPU tests of the host
+  ! bookkeeping): basis(nv+3, n), e(n), nold = n.
+  integer(c_int) function nrg_host_set_basis_output(n, basis, e) bind(C, name="nrg_host_set_basis_output")
+    integer(c_int), value :: n
+    integer(c_int), intent(in) :: basis(*)
+    real(c_double), intent(in) :: e(*)
+    integer :: i
+    if (allocated(basis_out)) deallocate(basis_out)
+    if (allocated(eigval)) deallocate(eigval)
+    allocate(basis_out(nv + 3, n), eigval(n))
+    do i = 1, n
+       basis_out(:, i) = basis((i - 1) * (nv + 3) + 1:i * (nv + 3))
+       eigval(i) = e(i)
+    end do
+    nold = n
+    nrg_host_set_basis_output = 0
+  end function nrg_host_set_basis_output
+
+  ! Truncation and reordering alone (CPU tests): eigenvalues e(nfull) in
+  ! block order of the current subspace table; returns the kept count.
+  integer(c_int) function nrg_host_truncate(it, n, e) bind(C, name="nrg_host_truncate")
+    integer(c_int), value :: it, n
+    real(c_double), intent(in) :: e(*)
+    integer :: b, j, s0
+    if (allocated(eigval)) deallocate(eigval)
+    allocate(eigval(n))
+    eigval = e(1:n)
+    nfull = n
+    if (allocated(basis_out)) deallocate(basis_out)
+    allocate(basis_out(nv + 3, nfull))
+    do b = 1, nsub
+       s0 = subspace(nv + 3, b)
+       do j = 1, subspace(nv + 1, b)
+          basis_out(1:nv, s0 + j - 1) = subspace(1:nv, b)
+          basis_out(nv + 1, s0 + j - 1) = j
+          basis_out(nv + 2:nv + 3, s0 + j - 1) = 0
+       end do
+    end do
+    call calc_thermo()
+    call truncate(int(it))
+    call kept_per_block()
+    nrg_host_truncate = nkept
+  end function nrg_host_truncate
+
+  ! info(1:16): iteration, nsub, nfull, nkept, start_trunc, nold, i_first,
+  ! n_iter, flag_spectrum, nv, nops, ncm, nmat, nkeep, flag_trunc, nchain
+  integer(c_int) function nrg_host_info(info) bind(C, name="nrg_host_info")
+    integer(c_int), intent(out) :: info(16)
+    info = 0
+    info(1) = cur_iteration; info(2) = nsub; info(3) = nfull; info(4) = nkept
+    info(5) = start_trunc; info(6) = nold; info(7) = i_first; info(8) = n_iter
+    info(9) = merge(1, 0, flag_spectrum); info(10) = nv; info(11) = nops; info(12) = ncm
+    info(13) = nmat; info(14) = nkeep; info(15) = merge(1, 0, flag_trunc); info(16) = nchain
+    nrg_host_info = 0
+  end function nrg_host_info
+
+  integer(c_int) function nrg_host_get_eigenvalues(out, n) bind(C, name="nrg_host_get_eigenvalues")
+    real(c_double), intent(out) :: out(*)
+    integer(c_int), value :: n
+    integer :: m
+    m = min(int(n), size(eigval))
+    out(1:m) = eigval(1:m)
+    nrg_host_get_eigenvalues = m
+  end function nrg_host_get_eigenvalues
+
+  integer(c_int) function nrg_host_get_ints(which, out, n) bind(C, name="nrg_host_get_ints")
+    ! which: 1 subspace, 2 basis_input, 3 basis_output, 4 kept per block
+    integer(c_int), value :: which, n
+    integer(c_int), intent(out) :: out(*)
+    integer :: m
+    m = 0
+    select case (which)
+    case (1)
+       m = min(int(n), size(subspace))
+       out(1:m) = reshape(subspace, [size(subspace)])
+    case (2)
+       m = min(int(n), size(basis_in))
+       out(1:m) = reshape(basis_in, [size(basis_in)])
+    case (3)
+       m = min(int(n), size(basis_out))
+       out(1:m) = reshape(basis_out, [size(basis_out)])
+    case (4)
+       m = min(int(n), size(kept_blk))
+       out(1:m) = kept_blk(1:m)
+    end select
+    nrg_host_get_ints = m
+  end function nrg_host_get_ints
+
+  integer(c_int) function nrg_host_get_eigenvectors(out, n) bind(C, name="nrg_host_get_eigenvectors")
+    real(c_double), intent(out) :: out(*)
+    integer(c_int64_t), value :: n
+    integer(8) :: m
+    m = 0
+    if (allocated(eigvec)) then
+       m = min(n, int(size(eigvec, kind=8), c_int64_t))
+       out(1:m) = eigvec(1:m)
+    end if
+    nrg_host_get_eigenvectors = int(min(m, 2147483647_8), c_int)
+  end function nrg_host_get_eigenvectors
+
+  ! t(1:8): prepare, diagonalization, postprocess, invariant, iteration (s),
+  ! thermo X, S, C of the last iteration; t(9) truncation margin
+  integer(c_int) function nrg_host_get_times(t) bind(C, name="nrg_host_get_times")
+    real(c_double), intent(out) :: t(9)
+    t(1) = t_prepare; t(2) = t_diag; t(3) = t_post; t(4) = t_invariant; t(5) = t_iter
+    t(6) = thermo_x; t(7) = thermo_s; t(8) = thermo_c; t(9) = trunc_margin
+    nrg_host_get_times = 0
+  end function nrg_host_get_times
+
+  integer(c_int) function nrg_host_close() bind(C, name="nrg_host_close")
+    nrg_host_close = nrg_gpu_finalize()
+  end function nrg_host_close
+
+end module nrg_driver

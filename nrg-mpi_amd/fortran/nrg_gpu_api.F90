@@ -1,0 +1,85 @@
+! iso_c_binding interfaces of libnrg_gpu.so (include/nrg_gpu.h).
+module nrg_gpu_api
+  use iso_c_binding
+  implicit none
+  interface
+     integer(c_int) function nrg_gpu_init(device) bind(C, name="nrg_gpu_init")
+       import :: c_int
+       integer(c_int), value :: device
+     end function
+     integer(c_int) function nrg_gpu_finalize() bind(C, name="nrg_gpu_finalize")
+       import :: c_int
+     end function
+     type(c_ptr) function nrg_gpu_last_error() bind(C, name="nrg_gpu_last_error")
+       import :: c_ptr
+     end function
+     integer(c_int) function nrg_gpu_set_model(nv, nops, nchain, ncm, ncoef, operation, coef_diag_type, &
+          coef_diag, coef_inv_type, coef_inv, conservation_block, nmat, coef_spec, cons_diff_spec) &
+          bind(C, name="nrg_gpu_set_model")
+       import :: c_int, c_double, c_ptr
+       integer(c_int), value :: nv, nops, nchain, ncm, ncoef, nmat
+       integer(c_int), intent(in) :: operation(*), coef_diag_type(*), coef_inv_type(*), conservation_block(*)
+       real(c_double), intent(in) :: coef_diag(*), coef_inv(*)
+       type(c_ptr), value :: coef_spec, cons_diff_spec
+     end function
+     integer(c_int) function nrg_gpu_set_invariant(nold, inv) bind(C, name="nrg_gpu_set_invariant")
+       import :: c_int, c_double
+       integer(c_int), value :: nold
+       real(c_double), intent(in) :: inv(*)
+     end function
+     integer(c_int) function nrg_gpu_set_invariant_spectrum(nold, inv) bind(C, name="nrg_gpu_set_invariant_spectrum")
+       import :: c_int, c_double
+       integer(c_int), value :: nold
+       real(c_double), intent(in) :: inv(*)
+     end function
+     integer(c_int) function nrg_gpu_diagonalize(iteration, nsub, subspace_info, nfull, basis_input, nold, &
+          eigenvalue_old, chain_diag, chain_nondiag, sqrt_lambda, eigenvalue, eigenvector, eigenvector_size) &
+          bind(C, name="nrg_gpu_diagonalize")
+       import :: c_int, c_double, c_ptr, c_int64_t
+       integer(c_int), value :: iteration, nsub, nfull, nold
+       integer(c_int), intent(in) :: subspace_info(*), basis_input(*)
+       real(c_double), intent(in) :: eigenvalue_old(*), chain_diag(*), chain_nondiag(*)
+       real(c_double), value :: sqrt_lambda
+       real(c_double), intent(out) :: eigenvalue(*)
+       type(c_ptr), value :: eigenvector
+       integer(c_int64_t), value :: eigenvector_size
+     end function
+     integer(c_int) function nrg_gpu_invariant(kept, out) bind(C, name="nrg_gpu_invariant")
+       import :: c_int, c_ptr
+       integer(c_int), intent(in) :: kept(*)
+       type(c_ptr), value :: out
+     end function
+     integer(c_int) function nrg_gpu_invariant_spectrum(kept, out) bind(C, name="nrg_gpu_invariant_spectrum")
+       import :: c_int, c_ptr
+       integer(c_int), intent(in) :: kept(*)
+       type(c_ptr), value :: out
+     end function
+     integer(c_int) function nrg_gpu_stage_times(t, n) bind(C, name="nrg_gpu_stage_times")
+       import :: c_int, c_double
+       real(c_double), intent(out) :: t(*)
+       integer(c_int), value :: n
+     end function
+  end interface
+
+contains
+
+  ! Message of the last library error as a Fortran string.
+  function gpu_error_text() result(msg)
+    character(len=:), allocatable :: msg
+    type(c_ptr) :: p
+    character(kind=c_char), pointer :: cs(:)
+    integer :: n
+    p = nrg_gpu_last_error()
+    msg = ""
+    if (.not. c_associated(p)) return
+    call c_f_pointer(p, cs, [4096])
+    n = 0
+    do while (n < 4096)
+       if (cs(n + 1) == c_null_char) exit
+       n = n + 1
+    end do
+    allocate(character(len=n) :: msg)
+    msg = transfer(cs(1:n), msg)
+  end function gpu_error_text
+
+end module nrg_gpu_api

@@ -1,0 +1,706 @@
+! Fortran host of the MI355X NRG hot path.
+!
+! Keeps the reference's call surface and bookkeeping on the host:
+!   nrgIteration(it) = prepareBasis -> diagonalization -> postprocess
+! (nrgIteration.F90:18-28) with the numerical bodies of diagonalization,
+! invariantMatrixDGEMM and invariantMatrixForSpectrumDGEMM on the GPU
+! through the C ABI of libnrg_gpu.so.  MPI and loadbalance.F90 have no
+! counterpart: one device owns every block.
+!
+! Data layout mirrors the reference's packed records (module.F90:122-604):
+!   subspace(nv+3, nsub)  labels, dimension, count_eigenvector, start_input
+!   basis_in(nv+3, nfull) labels, reference, variation, operation
+!   basis_out(nv+3, nfull)
+! so that the arrays cross the C ABI without conversion.
+module nrg_host
+  use iso_c_binding
+  use nrg_gpu_api
+  implicit none
+
+  ! thresholds (module.F90:5-7, 629-630, 652)
+  double precision, parameter :: numerical_threshold = 1.0d-12
+  double precision, parameter :: degeneracy_threshold = 1.0d-8
+  double precision, parameter :: truncation_threshold = 1.0d-6
+  double precision, parameter :: truncation_limit_scale = 1.2d0
+  double precision, parameter :: betabar = real(0.8, kind(1.0)) * 1.0d0
+
+  ! ---- model (namelists + initial.dat)
+  integer :: n_iter, i_first, nchain, nv, nmat, ncm, nb0, ncoef, nops
+  integer :: cons_block(2, 2)
+  logical :: flag_spectrum = .false.
+  double precision :: band_width, ztrick, lambda
+  integer :: scaling_type, nkeep
+  integer, allocatable :: operation(:, :)
+  integer, allocatable :: coef_inv_type(:, :), coef_diag_type(:, :)
+  double precision, allocatable :: coef_inv(:), coef_diag(:)
+  double precision, allocatable :: chain_diag(:, :), chain_nondiag(:, :)
+  double precision, allocatable, target :: coef_spec(:, :), inv0(:, :, :), spec0(:, :, :)
+  integer, allocatable, target :: cons_diff_spec(:, :), pos_op(:, :), neg_op(:, :)
+
+  ! ---- iteration state
+  integer :: nold = 0, nsub = 0, nfull = 0, nkept = 0
+  integer :: start_trunc, cur_iteration
+  logical :: flag_trunc = .false.
+  integer, allocatable, target :: basis_out(:, :), basis_in(:, :), subspace(:, :), kept_blk(:)
+  double precision, allocatable, target :: eigval(:), eigval_old(:)
+  double precision :: trunc_margin = -1.0d0
+  double precision :: thermo_x, thermo_s, thermo_c
+
+  ! ---- options
+  integer :: sort_mode = 0          ! 0: exact replay of the reference sort, 1: canonical
+  logical :: keep_vectors = .false. ! copy eigenvectors to the host (spectra / tests)
+  double precision, allocatable, target :: eigvec(:)
+
+  ! ---- timing (seconds)
+  double precision :: t_prepare = 0, t_diag = 0, t_post = 0, t_invariant = 0, t_iter = 0
+
+contains
+
+  ! =====================================================================
+  ! input (module.F90:656-688, 826-944; read_initial_basis.F90:146-302)
+  ! =====================================================================
+  ! namelist /hamiltonianInfo/ (module.F90:846-861); the locals carry the
+  ! namelist names and hide the module variables of the same name
+  subroutine read_hamiltonian_nml(path, ierr, o_iter, o_nchain, o_nv, o_cons, o_nmat, o_ncm, o_nb, &
+       o_ncoef, o_nops, o_i0, o_spectrum, o_bcs)
+    character(len=*), intent(in) :: path
+    integer, intent(out) :: ierr, o_iter, o_nchain, o_nv, o_cons(2, 2), o_nmat, o_ncm, o_nb, o_ncoef, o_nops, o_i0
+    logical, intent(out) :: o_spectrum, o_bcs
+    integer :: numberOfIteration, numberOfChain, numberOfVariation, conservationBlock(2, 2)
+    integer :: numberOfMatrix, numberOfConductionMatrix, numberOfBasis, numberOfCoefficient
+    integer :: numberOfCoefficientImpurity, numberOfOperation, numberOfBCSCoefficient, indexOfFirstIteration
+    logical :: flag_spectrum, flag_local, flag_BCS
+    integer :: u
+    namelist /hamiltonianInfo/ numberOfIteration, numberOfChain, numberOfVariation, conservationBlock, &
+         numberOfMatrix, numberOfConductionMatrix, numberOfBasis, numberOfCoefficient, numberOfBCSCoefficient, &
+         numberOfCoefficientImpurity, numberOfOperation, indexOfFirstIteration, flag_spectrum, flag_local, flag_BCS
+    numberOfCoefficientImpurity = 0
+    numberOfBCSCoefficient = 0
+    flag_BCS = .false.
+    flag_local = .false.
+    flag_spectrum = .false.
+    open(newunit=u, file=path, status="old", action="read", iostat=ierr)
+    if (ierr /= 0) return
+    read(u, nml=hamiltonianInfo, iostat=ierr)
+    close(u)
+    if (ierr /= 0) return
+    o_iter = numberOfIteration; o_nchain = numberOfChain; o_nv = numberOfVariation; o_cons = conservationBlock
+    o_nmat = numberOfMatrix; o_ncm = numberOfConductionMatrix; o_nb = numberOfBasis; o_ncoef = numberOfCoefficient
+    o_nops = numberOfOperation; o_i0 = indexOfFirstIteration; o_spectrum = flag_spectrum; o_bcs = flag_BCS
+  end subroutine read_hamiltonian_nml
+
+  ! namelist /nrg/ (module.F90:665-675)
+  subroutine read_parameter_nml(path, ierr, o_band, o_z, o_lambda, o_scaling, o_nkeep)
+    character(len=*), intent(in) :: path
+    integer, intent(out) :: ierr, o_scaling, o_nkeep
+    double precision, intent(out) :: o_band, o_z, o_lambda
+    double precision :: bandWidth, ztrick, Lambda
+    integer :: scalingType, truncation, u
+    logical :: cfsnrg, noimpurity, loadRestart, saveRestart
+    namelist /nrg/ bandWidth, ztrick, Lambda, scalingType, truncation, cfsnrg, noimpurity, loadRestart, saveRestart
+    cfsnrg = .false.; noimpurity = .false.; loadRestart = .false.; saveRestart = .false.
+    open(newunit=u, file=path, status="old", action="read", iostat=ierr)
+    if (ierr /= 0) return
+    read(u, nml=nrg, iostat=ierr)
+    close(u)
+    if (ierr /= 0) return
+    o_band = bandWidth; o_z = ztrick; o_lambda = Lambda; o_scaling = scalingType; o_nkeep = truncation
+  end subroutine read_parameter_nml
+
+  ! =====================================================================
+  ! input (module.F90:656-688, 826-944; read_initial_basis.F90:146-302)
+  ! =====================================================================
+  subroutine read_inputs(dir, ierr)
+    character(len=*), intent(in) :: dir
+    integer, intent(out) :: ierr
+    integer :: u, i, j, k, c
+    character(len=64) :: hdr
+    double precision :: f
+    logical :: bcs
+
+    call read_hamiltonian_nml(dir // "/hamiltonianInfo.dat", ierr, n_iter, nchain, nv, cons_block, nmat, ncm, &
+         nb0, ncoef, nops, i_first, flag_spectrum, bcs)
+    if (ierr /= 0) return
+    call read_parameter_nml(dir // "/parameter.txt", ierr, band_width, ztrick, lambda, scaling_type, nkeep)
+    if (ierr /= 0) return
+    if (bcs) then
+       ierr = 2   ! BCS chains are outside the accelerated path
+       return
+    end if
+
+    if (allocated(operation)) deallocate(operation, coef_inv_type, coef_diag_type, coef_inv, coef_diag, &
+         chain_diag, chain_nondiag, inv0)
+    if (allocated(basis_out)) deallocate(basis_out)
+    if (allocated(eigval)) deallocate(eigval)
+    allocate(operation(nops, nv + nchain), coef_inv_type(ncoef, 3), coef_diag_type(ncoef, 4))
+    allocate(coef_inv(ncoef), coef_diag(ncoef))
+    allocate(chain_diag(0:n_iter + 1, nchain), chain_nondiag(i_first:n_iter + 1, nchain))
+    allocate(inv0(nb0, nb0, ncm), basis_out(nv + 3, nb0), eigval(nb0))
+
+    open(newunit=u, file=dir // "/initial.dat", status="old", action="read", iostat=ierr)
+    if (ierr /= 0) return
+    read(u, *) hdr
+    do i = 1, nb0
+       read(u, *) basis_out(1:nv, i), basis_out(nv + 1, i), eigval(i)
+       basis_out(nv + 2:nv + 3, i) = 0
+    end do
+    read(u, *) hdr
+    do c = 1, ncm
+       do j = 1, nb0
+          do k = 1, nb0
+             read(u, *) inv0(k, j, c)
+          end do
+       end do
+    end do
+    read(u, *) hdr
+    do i = 1, ncoef
+       read(u, *) coef_inv_type(i, 1:3), coef_inv(i)
+    end do
+    read(u, *) hdr
+    do i = 1, ncoef
+       read(u, *) coef_diag_type(i, 1:4), coef_diag(i)
+    end do
+    read(u, *) hdr
+    do i = 1, nops
+       read(u, *) operation(i, 1:nv + nchain)
+    end do
+    read(u, *) hdr
+    do c = 1, nchain
+       read(u, *) hdr
+       do j = 0, n_iter + 1
+          read(u, *) chain_diag(j, c)
+       end do
+    end do
+    do c = 1, nchain
+       read(u, *) hdr
+       do j = i_first, n_iter + 1
+          read(u, *) chain_nondiag(j, c)
+       end do
+    end do
+    ! rescaling of indices >= 0 (read_initial_basis.F90:222-261)
+    do c = 1, nchain
+       do j = 0, n_iter + 1
+          select case (scaling_type)
+          case (0)
+             chain_diag(j, c) = chain_diag(j, c) * lambda**(dble(j - 1) * 0.5d0)
+             chain_nondiag(j, c) = chain_nondiag(j, c) * lambda**(dble(j) * 0.5d0)
+          case (2)
+             f = log(lambda) / band_width / (lambda - 1.0d0)
+             chain_diag(j, c) = chain_diag(j, c) * lambda**(dble(j - 1) * 0.5d0 + ztrick) * f
+             chain_nondiag(j, c) = chain_nondiag(j, c) * lambda**(dble(j) * 0.5d0 + ztrick) * f
+          case default
+             f = 1.0d0 / band_width * 2.0d0 / (1.0d0 + 1.0d0 / lambda)
+             chain_diag(j, c) = chain_diag(j, c) * lambda**(dble(j - 1) * 0.5d0 + ztrick - 1.0d0) * f
+             chain_nondiag(j, c) = chain_nondiag(j, c) * lambda**(dble(j) * 0.5d0 + ztrick - 1.0d0) * f
+          end select
+       end do
+    end do
+    if (flag_spectrum) then
+       if (allocated(coef_spec)) deallocate(coef_spec, spec0, cons_diff_spec, pos_op, neg_op)
+       allocate(coef_spec(nmat, nops), spec0(nb0, nb0, nmat), cons_diff_spec(nmat, nv))
+       allocate(pos_op(nmat, nv), neg_op(nmat, nv))
+       read(u, *) hdr
+       do i = 1, nmat
+          do j = 1, nops
+             read(u, *) coef_spec(i, j)
+          end do
+       end do
+       read(u, *) hdr
+       do i = 1, nmat
+          do j = 1, nb0
+             do k = 1, nb0
+                read(u, *) spec0(k, j, i)
+             end do
+          end do
+       end do
+       read(u, *) hdr
+       do i = 1, nmat
+          read(u, *) cons_diff_spec(i, 1:nv)
+       end do
+       read(u, *) hdr
+       do i = 1, nmat
+          read(u, *) pos_op(i, 1:nv)
+       end do
+       read(u, *) hdr
+       do i = 1, nmat
+          read(u, *) neg_op(i, 1:nv)
+       end do
+    end if
+    close(u)
+    nold = nb0
+    start_trunc = n_iter + 1
+    flag_trunc = .false.
+  end subroutine read_inputs
+
+  double precision function energy_scale(n)
+    ! module.F90:690-711
+    integer, intent(in) :: n
+    select case (scaling_type)
+    case (0)
+       energy_scale = lambda**(-dble(n) * 0.5d0)
+    case (2)
+       energy_scale = lambda**(-dble(n) * 0.5d0 - ztrick) * band_width * (lambda - 1.0d0) / log(lambda)
+    case default
+       energy_scale = band_width * 0.5d0 * (1.0d0 + 1.0d0 / lambda) * lambda**(-dble(n) * 0.5d0 - ztrick + 1.0d0)
+    end select
+  end function energy_scale
+
+  ! =====================================================================
+  ! sorts with the reference's permutations
+  ! =====================================================================
+  ! Replay of the exchange sort of module.F90:537-543: for n, for m > n, if
+  ! key(n) > key(m) swap.  Ties (equal labels and reference) keep the order
+  ! that the swap history gives them, which this loop reproduces exactly.
+  subroutine exchange_sort(n, key, perm)
+    integer, intent(in) :: n
+    integer(8), intent(inout) :: key(n)
+    integer, intent(inout) :: perm(n)
+    integer :: i, m, pi, pm
+    integer(8) :: cur, km
+    do i = 1, n - 1
+       cur = key(i)
+       pi = perm(i)
+       do m = i + 1, n
+          km = key(m)
+          if (km < cur) then
+             key(m) = cur
+             pm = perm(m)
+             perm(m) = pi
+             cur = km
+             pi = pm
+          end if
+       end do
+       key(i) = cur
+       perm(i) = pi
+    end do
+  end subroutine exchange_sort
+
+  ! Stable merge sort of (key, perm) by key (the permutation of an insertion
+  ! sort with strict comparisons, module.F90:16-44).
+  recursive subroutine merge_sort_i8(n, key, perm, wk, wp)
+    integer, intent(in) :: n
+    integer(8), intent(inout) :: key(n)
+    integer, intent(inout) :: perm(n)
+    integer(8), intent(inout) :: wk(n)
+    integer, intent(inout) :: wp(n)
+    integer :: h, i, j, k
+    if (n <= 16) then
+       call insertion_i8(n, key, perm)
+       return
+    end if
+    h = n / 2
+    call merge_sort_i8(h, key(1:h), perm(1:h), wk(1:h), wp(1:h))
+    call merge_sort_i8(n - h, key(h + 1:n), perm(h + 1:n), wk(h + 1:n), wp(h + 1:n))
+    i = 1; j = h + 1; k = 1
+    do while (i <= h .and. j <= n)
+       if (key(j) < key(i)) then
+          wk(k) = key(j); wp(k) = perm(j); j = j + 1
+       else
+          wk(k) = key(i); wp(k) = perm(i); i = i + 1
+       end if
+       k = k + 1
+    end do
+    do while (i <= h)
+       wk(k) = key(i); wp(k) = perm(i); i = i + 1; k = k + 1
+    end do
+    do while (j <= n)
+       wk(k) = key(j); wp(k) = perm(j); j = j + 1; k = k + 1
+    end do
+    key = wk
+    perm = wp
+  end subroutine merge_sort_i8
+
+  subroutine insertion_i8(n, key, perm)
+    integer, intent(in) :: n
+    integer(8), intent(inout) :: key(n)
+    integer, intent(inout) :: perm(n)
+    integer :: i, j, tp
+    integer(8) :: tk
+    do i = 2, n
+       tk = key(i); tp = perm(i)
+       j = i - 1
+       do while (j >= 1)
+          if (key(j) <= tk) exit
+          key(j + 1) = key(j); perm(j + 1) = perm(j)
+          j = j - 1
+       end do
+       key(j + 1) = tk; perm(j + 1) = tp
+    end do
+  end subroutine insertion_i8
+
+  recursive subroutine merge_sort_r8(n, val, perm, wv, wp)
+    integer, intent(in) :: n
+    double precision, intent(inout) :: val(n)
+    integer, intent(inout) :: perm(n)
+    double precision, intent(inout) :: wv(n)
+    integer, intent(inout) :: wp(n)
+    integer :: h, i, j, k, tp
+    double precision :: tv
+    if (n <= 16) then
+       do i = 2, n
+          tv = val(i); tp = perm(i)
+          j = i - 1
+          do while (j >= 1)
+             if (.not. (tv < val(j))) exit
+             val(j + 1) = val(j); perm(j + 1) = perm(j)
+             j = j - 1
+          end do
+          val(j + 1) = tv; perm(j + 1) = tp
+       end do
+       return
+    end if
+    h = n / 2
+    call merge_sort_r8(h, val(1:h), perm(1:h), wv(1:h), wp(1:h))
+    call merge_sort_r8(n - h, val(h + 1:n), perm(h + 1:n), wv(h + 1:n), wp(h + 1:n))
+    i = 1; j = h + 1; k = 1
+    do while (i <= h .and. j <= n)
+       if (val(j) < val(i)) then
+          wv(k) = val(j); wp(k) = perm(j); j = j + 1
+       else
+          wv(k) = val(i); wp(k) = perm(i); i = i + 1
+       end if
+       k = k + 1
+    end do
+    do while (i <= h)
+       wv(k) = val(i); wp(k) = perm(i); i = i + 1; k = k + 1
+    end do
+    do while (j <= n)
+       wv(k) = val(j); wp(k) = perm(j); j = j + 1; k = k + 1
+    end do
+    val = wv
+    perm = wp
+  end subroutine merge_sort_r8
+
+  ! =====================================================================
+  ! prepareBasis (prepareBasis.F90:41-114)
+  ! =====================================================================
+  subroutine prepare_basis(ierr)
+    integer, intent(out) :: ierr
+    integer :: i, j, p, c, b, cnt
+    integer(8), allocatable :: key(:), wk(:)
+    integer, allocatable :: perm(:), wp(:), tmp(:, :)
+    integer(8) :: kk
+    integer :: lab
+    ierr = 0
+    nfull = nold * nops
+    if (allocated(basis_in)) deallocate(basis_in)
+    allocate(basis_in(nv + 3, nfull), key(nfull), perm(nfull))
+    p = 0
+    do i = 1, nold
+       do j = 1, nops
+          p = p + 1
+          basis_in(1:nv, p) = basis_out(1:nv, i) - operation(j, 1:nv)
+          basis_in(nv + 1, p) = i
+          basis_in(nv + 2, p) = basis_out(nv + 1, i)
+          basis_in(nv + 3, p) = j
+          kk = 0
+          do c = 1, nv
+             lab = basis_in(c, p)
+             if (abs(lab) >= 512) then
+                ierr = 3
+                return
+             end if
+             kk = kk * 1024_8 + int(lab + 512, 8)
+          end do
+          key(p) = kk * 16777216_8 + int(i, 8)
+          perm(p) = p
+       end do
+    end do
+    if (nold >= 16777216) then
+       ierr = 3
+       return
+    end if
+    if (sort_mode == 0) then
+       call exchange_sort(nfull, key, perm)
+    else
+       allocate(wk(nfull), wp(nfull))
+       call merge_sort_i8(nfull, key, perm, wk, wp)
+       deallocate(wk, wp)
+    end if
+    allocate(tmp(nv + 3, nfull))
+    tmp = basis_in(:, perm)
+    call move_alloc(tmp, basis_in)
+    ! block table
+    cnt = 1
+    do i = 2, nfull
+       if (any(basis_in(1:nv, i) /= basis_in(1:nv, i - 1))) cnt = cnt + 1
+    end do
+    nsub = cnt
+    if (allocated(subspace)) deallocate(subspace)
+    allocate(subspace(nv + 3, nsub))
+    b = 0
+    do i = 1, nfull
+       if (i == 1) then
+          b = 1
+          subspace(1:nv, b) = basis_in(1:nv, i)
+          subspace(nv + 1, b) = 1
+          subspace(nv + 3, b) = i
+       else if (any(basis_in(1:nv, i) /= basis_in(1:nv, i - 1))) then
+          b = b + 1
+          subspace(1:nv, b) = basis_in(1:nv, i)
+          subspace(nv + 1, b) = 1
+          subspace(nv + 3, b) = i
+       else
+          subspace(nv + 1, b) = subspace(nv + 1, b) + 1
+       end if
+    end do
+    ! count_eigenvector: running sum of dimension^2 (int32, as the reference)
+    kk = 1
+    do b = 1, nsub
+       subspace(nv + 2, b) = int(kk)          ! may wrap beyond 2^31 like the reference
+       kk = kk + int(subspace(nv + 1, b), 8)**2
+    end do
+    deallocate(key, perm)
+  end subroutine prepare_basis
+
+  ! =====================================================================
+  ! diagonalization (diagonalization.F90:1-410) on the GPU
+  ! =====================================================================
+  subroutine diagonalization(it, ierr)
+    integer, intent(in) :: it
+    integer, intent(out) :: ierr
+    double precision :: cd(nchain), cnd(nchain)
+    integer(8) :: n2
+    integer :: b, j, s0
+    type(c_ptr) :: pvec
+    do b = 1, nchain
+       cd(b) = chain_diag(it + 1, b)
+       cnd(b) = chain_nondiag(it, b)
+    end do
+    n2 = 0
+    do b = 1, nsub
+       n2 = n2 + int(subspace(nv + 1, b), 8)**2
+    end do
+    if (allocated(eigval_old)) deallocate(eigval_old)
+    allocate(eigval_old(nold))
+    eigval_old = eigval(1:nold)
+    if (allocated(eigval)) deallocate(eigval)
+    allocate(eigval(nfull))
+    pvec = c_null_ptr
+    if (keep_vectors) then
+       if (allocated(eigvec)) deallocate(eigvec)
+       allocate(eigvec(n2))
+       pvec = c_loc(eigvec)
+    end if
+    ierr = nrg_gpu_diagonalize(int(it, c_int), int(nsub, c_int), subspace, int(nfull, c_int), basis_in, &
+         int(nold, c_int), eigval_old, cd, cnd, sqrt(lambda), eigval, pvec, int(n2, c_int64_t))
+    if (ierr /= 0) return
+    ! basis_output <- (block labels, column index) (diagonalization.F90:395-407)
+    if (allocated(basis_out)) deallocate(basis_out)
+    allocate(basis_out(nv + 3, nfull))
+    do b = 1, nsub
+       s0 = subspace(nv + 3, b)
+       do j = 1, subspace(nv + 1, b)
+          basis_out(1:nv, s0 + j - 1) = subspace(1:nv, b)
+          basis_out(nv + 1, s0 + j - 1) = j
+          basis_out(nv + 2:nv + 3, s0 + j - 1) = 0
+       end do
+    end do
+  end subroutine diagonalization
+
+  ! =====================================================================
+  ! thermodynamics (module.F90:1931-1982), before truncation
+  ! =====================================================================
+  subroutine calc_thermo()
+    integer :: i, j
+    double precision :: be, ex, zn, en, sz2, sz, cn, rs
+    integer :: szt
+    zn = 0; en = 0; sz2 = 0; sz = 0; cn = 0
+    do i = 1, nfull
+       be = betabar * eigval(i)
+       ex = exp(-be)
+       zn = zn + ex
+       en = en + be * ex
+       szt = 0
+       do j = cons_block(2, 1), cons_block(2, 2)
+          szt = szt + basis_out(j, i)
+       end do
+       rs = dble(szt) * 0.5d0
+       sz2 = sz2 + ex * rs * rs
+       sz = sz + ex * rs
+    end do
+    do i = 1, nfull
+       be = betabar * eigval(i)
+       ex = exp(-be)
+       cn = cn + (be - en / zn) * (be - en / zn) * ex
+    end do
+    thermo_c = cn / zn
+    thermo_s = en / zn + log(zn)
+    thermo_x = sz2 / zn - (sz / zn) * (sz / zn)
+  end subroutine calc_thermo
+
+  ! =====================================================================
+  ! postprocess (postprocess.F90:71-207): sort, truncate, reorder
+  ! =====================================================================
+  subroutine truncate(it)
+    integer, intent(in) :: it
+    integer, allocatable :: idx(:), wp(:), kidx(:), order(:), tmp(:, :)
+    double precision, allocatable :: es(:), wv(:), e2(:)
+    integer(8), allocatable :: kk(:), wk(:)
+    integer :: i, limit, cutoff, nb
+    double precision :: ecut
+    allocate(idx(nfull), wp(nfull), es(nfull), wv(nfull))
+    do i = 1, nfull
+       idx(i) = i
+    end do
+    es = eigval
+    call merge_sort_r8(nfull, es, idx, wv, wp)
+    nb = nfull
+    trunc_margin = -1.0d0
+    if (it /= i_first .and. it /= n_iter .and. nfull > nkeep) then
+       if (start_trunc > it) then
+          start_trunc = it
+          flag_trunc = .true.
+       end if
+       limit = int(nkeep * truncation_limit_scale)
+       ecut = es(nkeep) + truncation_threshold
+       cutoff = nkeep
+       i = nkeep + 1
+       do while (i <= nfull)
+          if (es(i) <= ecut) then
+             cutoff = cutoff + 1
+          else
+             exit
+          end if
+          i = i + 1
+       end do
+       do while (cutoff > limit)
+          ecut = ecut - 0.001d0
+          do i = cutoff, 1, -1
+             if (es(i) > ecut) then
+                cutoff = cutoff - 1
+             else
+                exit
+             end if
+          end do
+       end do
+       nb = cutoff
+       trunc_margin = minval(abs(es - ecut))
+    end if
+    ! kept prefix back to index order (sortByIndex on 1..nb)
+    allocate(kidx(nb), order(nfull), kk(nb), wk(nb))
+    do i = 1, nb
+       kk(i) = idx(i)
+       kidx(i) = idx(i)
+    end do
+    if (nb > 0) call merge_sort_i8(nb, kk, kidx, wk, wp(1:nb))
+    order(1:nb) = kidx
+    order(nb + 1:nfull) = idx(nb + 1:nfull)
+    allocate(e2(nfull))
+    e2 = eigval(order)
+    eigval = e2
+    if (nb < nfull) then
+       allocate(tmp(nv + 3, nfull))
+       tmp = basis_out(:, order)
+       call move_alloc(tmp, basis_out)
+    end if
+    nkept = nb
+  end subroutine truncate
+
+  ! kept states per block (invariantMatrixDGEMM.F90:84-100): blocks are
+  ! matched by labels against the kept prefix of basis_output
+  subroutine kept_per_block()
+    integer :: b, i
+    if (allocated(kept_blk)) deallocate(kept_blk)
+    allocate(kept_blk(nsub))
+    kept_blk = 0
+    ! the kept prefix is block-ordered: walk both lists once
+    b = 1
+    do i = 1, nkept
+       do while (b <= nsub)
+          if (all(subspace(1:nv, b) == basis_out(1:nv, i))) exit
+          b = b + 1
+       end do
+       if (b > nsub) then
+          ! fall back to a full search (never expected)
+          do b = 1, nsub
+             if (all(subspace(1:nv, b) == basis_out(1:nv, i))) exit
+          end do
+       end if
+       kept_blk(b) = kept_blk(b) + 1
+    end do
+  end subroutine kept_per_block
+
+  subroutine postprocess(it, ierr)
+    integer, intent(in) :: it
+    integer, intent(out) :: ierr
+    double precision :: t0, t1
+    ierr = 0
+    call calc_thermo()
+    call truncate(it)
+    call kept_per_block()
+    if (it < n_iter) then
+       call cpu_time_wall(t0)
+       ierr = nrg_gpu_invariant(kept_blk, c_null_ptr)
+       if (ierr /= 0) return
+       if (flag_spectrum) then
+          ierr = nrg_gpu_invariant_spectrum(kept_blk, c_null_ptr)
+          if (ierr /= 0) return
+       end if
+       call cpu_time_wall(t1)
+       t_invariant = t1 - t0
+    end if
+    nold = nkept
+  end subroutine postprocess
+
+  subroutine cpu_time_wall(t)
+    double precision, intent(out) :: t
+    integer(8) :: c, r
+    call system_clock(c, r)
+    t = dble(c) / dble(r)
+  end subroutine cpu_time_wall
+
+  ! =====================================================================
+  ! nrgIteration (nrgIteration.F90:1-31)
+  ! =====================================================================
+  subroutine nrg_iteration(it, ierr)
+    integer, intent(in) :: it
+    integer, intent(out) :: ierr
+    double precision :: t0, t1, t2, t3
+    cur_iteration = it
+    call cpu_time_wall(t0)
+    call prepare_basis(ierr)
+    if (ierr /= 0) return
+    call cpu_time_wall(t1)
+    call diagonalization(it, ierr)
+    if (ierr /= 0) return
+    call cpu_time_wall(t2)
+    call postprocess(it, ierr)
+    if (ierr /= 0) return
+    call cpu_time_wall(t3)
+    t_prepare = t1 - t0
+    t_diag = t2 - t1
+    t_post = t3 - t2
+    t_iter = t3 - t0
+  end subroutine nrg_iteration
+
+  ! =====================================================================
+  ! setup: inputs + device
+  ! =====================================================================
+  subroutine setup(dir, device, ierr)
+    character(len=*), intent(in) :: dir
+    integer, intent(in) :: device
+    integer, intent(out) :: ierr
+    call read_inputs(dir, ierr)
+    if (ierr /= 0) then
+       ierr = 100 + ierr
+       return
+    end if
+    ierr = nrg_gpu_init(int(device, c_int))
+    if (ierr /= 0) return
+    if (flag_spectrum) then
+       ierr = nrg_gpu_set_model(int(nv, c_int), int(nops, c_int), int(nchain, c_int), int(ncm, c_int), &
+            int(ncoef, c_int), operation, coef_diag_type, coef_diag, coef_inv_type, coef_inv, cons_block, &
+            int(nmat, c_int), c_loc(coef_spec), c_loc(cons_diff_spec))
+    else
+       ierr = nrg_gpu_set_model(int(nv, c_int), int(nops, c_int), int(nchain, c_int), int(ncm, c_int), &
+            int(ncoef, c_int), operation, coef_diag_type, coef_diag, coef_inv_type, coef_inv, cons_block, &
+            0_c_int, c_null_ptr, c_null_ptr)
+    end if
+    if (ierr /= 0) return
+    ierr = nrg_gpu_set_invariant(int(nb0, c_int), inv0)
+    if (ierr /= 0) return
+    if (flag_spectrum) ierr = nrg_gpu_set_invariant_spectrum(int(nb0, c_int), spec0)
+  end subroutine setup
+
+end module nrg_host

@@ -39,7 +39,13 @@ SIGNATURES = {
                                      ctypes.c_double, _f64p, ctypes.c_int, _f64p, ctypes.c_int, ctypes.c_double,
                                      _f64p, ctypes.c_int]),
     "nrg_gpu_inject_fault": (ctypes.c_int, [ctypes.c_int]),
+    "nrg_gpu_profile": (ctypes.c_int, [ctypes.c_int]),
+    "nrg_gpu_kernel_stats": (ctypes.c_int, [_f64p, ctypes.c_int]),
 }
+
+KERNEL_FAMILIES = ["assemble", "tridiag_col", "tridiag_symv", "tridiag_gemm", "dc_leaf", "dc_prep",
+                   "dc_secular", "dc_vectors", "dc_gemm", "dc_finish", "back_small", "back_gemm",
+                   "invariant_gemm", "spectral_gemm", "other"]
 
 
 class NrgGpuError(RuntimeError):
@@ -115,6 +121,19 @@ def syev_batched(blocks):
         o1 += nb
         o2 += nb * nb
     return ws, Us
+
+
+def profile(enable: bool):
+    _check(load().nrg_gpu_profile(int(enable)))
+
+
+def kernel_stats():
+    n = len(KERNEL_FAMILIES)
+    a = np.zeros(4 * n)
+    _check(load().nrg_gpu_kernel_stats(_p(a, _f64p), n))
+    a = a.reshape(n, 4)
+    return {f: dict(ms=a[i, 0], launches=int(a[i, 1]), bytes=a[i, 2], flops=a[i, 3])
+            for i, f in enumerate(KERNEL_FAMILIES)}
 
 
 def stage_times():

@@ -72,12 +72,17 @@ def fixture(name, workdir, wall, nrank):
     for d in its:
         k = f"it{d['it']}"
         out[k + "_subspace"] = d["subspace"].astype(np.int32)
+        nv_ = d["nv"]
+        kl = d["basis_output"][:d["nkept"], :nv_]
+        out[k + "_kept_blk"] = np.array([int(np.all(kl == row[:nv_], axis=1).sum()) for row in d["subspace"]],
+                                        dtype=np.int32)
         out[k + "_sha_basis_input"] = np.array(h(d["basis_input"]))
         out[k + "_sha_basis_output"] = np.array(h(d["basis_output"]))
         E = d["eigenvalue"]
         out[k + "_E_sum"] = np.array(E.sum())
         if small:
             out[k + "_eigenvalue"] = E
+            out[k + "_basis_output"] = d["basis_output"].astype(np.int16)
             if d["it"] <= 0:
                 out[k + "_basis_input"] = d["basis_input"].astype(np.int32)
         else:
@@ -121,6 +126,14 @@ def fixture(name, workdir, wall, nrank):
 
 def main(argv):
     names = argv[1:] or list(RUNS)
+    if names and names[0] == "--from-dirs":
+        # re-make fixtures from existing run directories: name=dir:seconds:ranks
+        for spec in names[1:]:
+            name, rest = spec.split("=")
+            wd, wall, nr = rest.split(":")
+            p = fixture(name, wd, float(wall), int(nr))
+            print(name, p, os.path.getsize(p))
+        return
     for name in names:
         wd = os.path.join("/tmp", "nrg_ref_" + name)
         os.makedirs(wd, exist_ok=True)
