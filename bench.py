@@ -1,0 +1,183 @@
+#!/usr/bin/env python3
+"""Benchmark: NRG iterations per second on one MI355X.
+
+Workload: two-orbital Anderson model with Kanamori interaction (U=0.3,
+J=0.05, Gamma=0.01 per orbital, two Wilson chains, Lambda=2, local site of
+dimension 16), Nkeep=4000.  One step is one NRG iteration of the forward
+pass: prepareBasis + block-Hamiltonian assembly + batched FP64 eigensolve of
+all (Q,Sz) blocks + truncation + invariant-matrix contractions, driven by the
+Fortran host (nrg-mpi_amd/lib/libnrg_host.so) through the HIP library.
+Warmup iterations run the growth phase (-1, 0, 1, ...); the timed steps are
+steady-state truncating iterations (~64k states, ~90 blocks up to ~4700).
+
+Prints one JSON line (see DESIGN.md for the fields).
+"""
+import argparse
+import json
+import os
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "nrg-mpi_amd"))
+
+METRIC = "NRG iter/s at Nkeep=4000, batched+split-prec vs plain FP64; % MFMA peak"
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFS = 78.6
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=6)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="two_orbital_nk4000_it45")
+    ap.add_argument("--sort", choices=["exact", "canonical"], default="exact")
+    ap.add_argument("--no-profile", action="store_true", help="skip the HIP-event kernel timing")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    return ap.parse_args()
+
+
+def cpu_baseline(block_sizes, budget_s):
+    """Reference-equivalent CPU eigensolve (LAPACK dsyev, the routine the
+    reference calls per block, diagonalization.F90:267) on one host core, on a
+    bounded sample of the iteration's blocks; the remaining blocks are
+    extrapolated with the n^3 cost of the largest measured block."""
+    os.environ.setdefault("OPENBLAS_NUM_THREADS", "1")
+    import numpy as np
+    from scipy.linalg import eigh
+    try:
+        from threadpoolctl import threadpool_limits
+        lim = threadpool_limits(1)
+    except Exception:  # pragma: no cover
+        lim = None
+    rng = np.random.default_rng(0)
+    sizes = sorted(int(n) for n in block_sizes)
+    t_meas, n_meas, last_rate = 0.0, [], None
+    for n in sizes:
+        if t_meas > budget_s:
+            break
+        a = rng.standard_normal((n, n))
+        a = a + a.T
+        t0 = time.perf_counter()
+        eigh(a, driver="ev", overwrite_a=True, check_finite=False)
+        dt = time.perf_counter() - t0
+        t_meas += dt
+        n_meas.append(n)
+        if n >= 64:
+            last_rate = dt / float(n) ** 3
+    rest = sizes[len(n_meas):]
+    t_rest = sum(last_rate * float(n) ** 3 for n in rest) if rest else 0.0
+    total = t_meas + t_rest
+    if lim is not None:
+        lim.unregister() if hasattr(lim, "unregister") else None
+    sample = (f"dsyev (scipy/OpenBLAS, 1 thread) on {len(n_meas)} of {len(sizes)} blocks of one steady-state "
+              f"iteration (n <= {max(n_meas) if n_meas else 0}, {t_meas:.1f} s measured), the other "
+              f"{len(rest)} blocks extrapolated by n^3; eigensolve stage only, host bookkeeping excluded")
+    return dict(value=1.0 / total, unit="iter/s", cores=1, kind="port", sample=sample,
+                seconds_per_iteration=total)
+
+
+def main():
+    a = parse()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if a.gpus != 1 or world != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} (WORLD_SIZE={world}) is not "
+              f"supported: the NRG hot path is benchmarked on a single MI355X (multi-GPU sharding is "
+              f"design-only, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    from nrgamd import gpu, models
+    from nrgamd.host import Host
+
+    spec = models.workload(a.workload)
+    need = a.warmup + a.steps
+    if need > spec.niter:
+        raise SystemExit(f"warmup+steps={need} exceeds the {spec.niter} iterations of {a.workload}")
+    tmp = tempfile.mkdtemp(prefix="nrgbench_")
+    models.write_inputs(spec, tmp)
+    h = Host(tmp, device=0, sort_mode=0 if a.sort == "exact" else 1)
+    its = list(range(spec.first_iteration, spec.first_iteration + need))
+    for it in its[:a.warmup]:
+        h.iterate(it)
+        print(f"warmup iteration {it}: nfull={h.i['nfull']} nsub={h.i['nsub']} kept={h.i['nkept']}",
+              file=sys.stderr, flush=True)
+    if not a.no_profile:
+        gpu.profile(True)
+    stages = {}
+    flops_eig = 0.0
+    t0 = time.perf_counter()
+    for it in its[a.warmup:]:
+        h.iterate(it)
+        t = h.times()
+        st = gpu.stage_times()
+        flops_eig += st["eig_flops"]
+        for k in ("prepare", "diagonalization", "postprocess", "invariant"):
+            stages[k] = stages.get(k, 0.0) + t[k]
+        for k in ("assemble", "eig", "tridiag", "dc", "back"):
+            stages["gpu_" + k] = stages.get("gpu_" + k, 0.0) + st[k] * 1e-3
+        print(f"timed iteration {it}: {t['iteration']:.3f} s  (prepare {t['prepare']:.3f}, diag "
+              f"{t['diagonalization']:.3f} [tri {st['tridiag']:.0f} ms, dc {st['dc']:.0f} ms, back "
+              f"{st['back']:.0f} ms], post {t['postprocess']:.3f}) nfull={h.i['nfull']} nsub={h.i['nsub']}",
+              file=sys.stderr, flush=True)
+    elapsed = time.perf_counter() - t0
+    ks = gpu.kernel_stats() if not a.no_profile else {}
+    if not a.no_profile:
+        gpu.profile(False)
+    sizes = h.subspace()[:, h.i["nv"]].tolist()
+    nfull, nsub, nmax = h.i["nfull"], h.i["nsub"], int(max(sizes))
+    h.close()
+
+    value = a.steps / elapsed
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "iter/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": 1e3 * elapsed / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": a.workload,
+                   "model": "two-orbital Anderson, Kanamori U=0.3 J=0.05, Gamma=0.01/orbital, 2 Wilson chains, "
+                            "Lambda=2, D=1, local dim 16, labels (Q,2Sz)",
+                   "nkeep": spec.nkeep, "timed_iterations": [its[a.warmup], its[-1]],
+                   "states_per_iteration": nfull, "blocks_per_iteration": nsub, "largest_block": nmax,
+                   "sort": a.sort, "form": "batched FP64 (plain precision)"},
+        "stages_s_per_step": {k: v / a.steps for k, v in stages.items()},
+        "eig_alg_tflops": flops_eig / max(stages.get("gpu_eig", 1e-9), 1e-9) / 1e12,
+    }
+    if ks:
+        fam = max(ks, key=lambda f: ks[f]["ms"])
+        k = ks[fam]
+        if fam in ("tridiag_symv", "tridiag_col", "assemble"):
+            ach = k["bytes"] / (k["ms"] * 1e-3) / 1e9
+            roof = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": ach / HBM_PEAK_GBS, "traffic": None}
+        else:
+            ach = k["flops"] / (k["ms"] * 1e-3) / 1e12
+            roof = {"bound": "mfma", "achieved": ach, "peak": FP64_PEAK_TFS, "unit": "TFLOP/s",
+                    "frac": ach / FP64_PEAK_TFS, "traffic": None}
+        roof["kernel_family"] = fam
+        roof["launches"] = k["launches"]
+        roof["avg_launch_us"] = 1e3 * k["ms"] / max(k["launches"], 1)
+        out["roofline"] = roof
+        out["kernel_ms_per_step"] = {f: v["ms"] / a.steps for f, v in ks.items() if v["launches"]}
+    else:
+        out["roofline"] = None
+    if not a.no_cpu_baseline:
+        cb = cpu_baseline(sizes, a.cpu_seconds)
+        out["cpu_baseline"] = cb
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()
