@@ -17,10 +17,11 @@ namespace nrg {
 
 constexpr int NB = 32;     // tridiagonalisation panel width
 constexpr int RA = 256;    // rows per workgroup, column-update kernel
-constexpr int RB = 64;     // rows per workgroup, SYMV kernel
+constexpr int RB = 32;     // rows (output entries) per workgroup, SYMV kernel
 constexpr int PSTRIDE = 2 * NB + 1;  // partial-sum record per SYMV chunk
 constexpr int LEAF = 32;   // divide-and-conquer leaf size
-constexpr int NBB = 32;    // back-transform panel width
+constexpr int DC_LDS_M = 4000;  // merges up to this size run the deflation scan in LDS
+constexpr int NBB = 128;   // back-transform panel width (Z is streamed once per panel)
 constexpr int MAXN = 16384;
 
 __device__ inline double wave_sum(double v) {
@@ -78,14 +79,21 @@ __global__ __launch_bounds__(256) void k_tr_col(TrArgs a, int c, int jp, int jj,
     const int p = c - 1;
     const double tp = a.tau[a.voff[b] + p];
     const int nch = chunksB(n, p);
-    // sum the SYMV partials of column p (fixed order: deterministic)
-    for (int q = tid; q < 2 * jp + 1; q += 256) {
-      const int slot = (q < jp) ? q : (q < 2 * jp ? NB + (q - jp) : 2 * NB);
-      double s = 0.0;
-      for (int ch = 0; ch < nch; ++ch) s += part[(int64_t)ch * PSTRIDE + slot];
-      if (q < jp) su1[q] = s;
-      else if (q < 2 * jp) su2[q - jp] = s;
-      else s_alpha = s;  // y.v for now
+    // sum the SYMV partials of column p: one wave per slot, lanes over
+    // chunks, fixed reduction tree (deterministic)
+    {
+      const int lane = tid & 63, wv = tid >> 6;
+      for (int q = wv; q < 2 * jp + 1; q += 4) {
+        const int slot = (q < jp) ? q : (q < 2 * jp ? NB + (q - jp) : 2 * NB);
+        double s = 0.0;
+        for (int ch = lane; ch < nch; ch += 64) s += part[(int64_t)ch * PSTRIDE + slot];
+        s = wave_sum(s);
+        if (lane == 0) {
+          if (q < jp) su1[q] = s;
+          else if (q < 2 * jp) su2[q - jp] = s;
+          else s_alpha = s;  // y.v for now
+        }
+      }
     }
     __syncthreads();
     if (tid == 0) {
@@ -158,9 +166,13 @@ __global__ __launch_bounds__(256) void k_tr_symv(TrArgs a, int c, int jj) {
 
   // reflector from the updated column c (identical in every workgroup)
   const int ncA = (n - c + RA - 1) / RA;
-  double xn2 = 0.0;
   const double* pn = part + part_norm_base(n);
-  for (int q = 0; q < ncA; ++q) xn2 += pn[q];
+  double xn2 = 0.0;
+  {
+    double sl = 0.0;
+    for (int q = (tid & 63); q < ncA; q += 64) sl += pn[q];
+    xn2 = wave_sum(sl);   // every wave forms the same sum
+  }
   const double alpha = A[(c + 1) + c * ld];
   double beta, tau, scal;
   if (xn2 == 0.0) {
@@ -177,43 +189,65 @@ __global__ __launch_bounds__(256) void k_tr_symv(TrArgs a, int c, int jj) {
   const int m = n - c - 1;
   for (int j = tid; j < m; j += 256) vs[j] = (j == 0) ? 1.0 : A[(c + 1 + j) + c * ld] * scal;
   __syncthreads();
-  const int rl = tid & (RB - 1);
-  const int g = tid >> 6;
-  const int r = row0 + rl;
-  if (g == 0 && r < n) {
-    P[r + jj * ld] = vs[r - c - 1];
-    P[r + (3 * NB + jj) * ld] = vs[r - c - 1];
-  }
-  // y_r = sum_j A[r, j] v[j] over the trailing block (4 column groups)
-  double acc = 0.0;
-  if (r < n) {
-    const double* Ar = A + r;
-    for (int j = g; j < m; j += 4) acc += Ar[(int64_t)(c + 1 + j) * ld] * vs[j];
-  }
-  ysum[g][rl] = acc;
-  __syncthreads();
-  double yr = 0.0, vr = 0.0;
-  if (r < n) {
-    yr = ysum[0][rl] + ysum[1][rl] + ysum[2][rl] + ysum[3][rl];
-    vr = vs[r - c - 1];
-    if (g == 0) a.y[a.voff[b] + r] = yr;
-  }
-  // chunk partials: u1_i = sum V[r,i] v_r, u2_i = sum W[r,i] v_r (i < jj), y.v
-  double* pc = part + (int64_t)blockIdx.x * PSTRIDE;
-  for (int q = g; q < 2 * jj + 1; q += 4) {
-    double val = 0.0;
+  // y_r = sum_j A[j, r] v[j] (the trailing block is symmetric, so column r
+  // is read instead of row r: one wave streams one contiguous column with
+  // eight independent loads in flight per lane)
+  const int lane = tid & 63;
+  const int wv = tid >> 6;
+  const double* Ab = A + (c + 1);
+  for (int q = 0; q < RB / 4; ++q) {
+    const int rl = wv * (RB / 4) + q;
+    const int r = row0 + rl;
+    double acc = 0.0;
     if (r < n) {
-      if (q < jj) val = P[r + q * ld] * vr;
-      else if (q < 2 * jj) val = P[r + (NB + q - jj) * ld] * vr;
-      else val = yr * vr;
+      const double* col = Ab + (int64_t)r * ld;
+      double s0 = 0, s1 = 0, s2 = 0, s3 = 0, s4 = 0, s5 = 0, s6 = 0, s7 = 0;
+      int j = lane;
+      for (; j + 448 < m; j += 512) {
+        s0 += col[j] * vs[j];
+        s1 += col[j + 64] * vs[j + 64];
+        s2 += col[j + 128] * vs[j + 128];
+        s3 += col[j + 192] * vs[j + 192];
+        s4 += col[j + 256] * vs[j + 256];
+        s5 += col[j + 320] * vs[j + 320];
+        s6 += col[j + 384] * vs[j + 384];
+        s7 += col[j + 448] * vs[j + 448];
+      }
+      for (; j < m; j += 64) s0 += col[j] * vs[j];
+      acc = wave_sum(((s0 + s1) + (s2 + s3)) + ((s4 + s5) + (s6 + s7)));
     }
-    val = wave_sum(val);
-    if (rl == 0) {
-      const int slot = (q < jj) ? q : (q < 2 * jj ? NB + (q - jj) : 2 * NB);
-      pc[slot] = val;
-    }
+    if (lane == 0) ysum[0][rl] = acc;
   }
-  (void)s_par; (void)yv_s;
+  __syncthreads();
+  // own rows: y, v into the panel, chunk partials
+  //   u1_i = sum V[r,i] v_r, u2_i = sum W[r,i] v_r (i < jj), y.v
+  const int rl = tid & (RB - 1);
+  const int r = row0 + rl;
+  double yr = 0.0, vr = 0.0;
+  if (tid < RB && r < n) {
+    yr = ysum[0][rl];
+    vr = vs[r - c - 1];
+    a.y[a.voff[b] + r] = yr;
+    P[r + jj * ld] = vr;
+    P[r + (3 * NB + jj) * ld] = vr;
+    yv_s[rl] = yr * vr;
+  } else if (tid < RB) {
+    yv_s[rl] = 0.0;
+  }
+  __syncthreads();
+  double* pc = part + (int64_t)blockIdx.x * PSTRIDE;
+  for (int q = tid; q < 2 * jj + 1; q += 256) {
+    double val = 0.0;
+    if (q == 2 * jj) {
+      for (int x = 0; x < RB; ++x) val += yv_s[x];
+    } else {
+      const int col = (q < jj) ? q : NB + (q - jj);
+      for (int x = 0; x < RB && row0 + x < n; ++x) val += P[(row0 + x) + col * ld] * vs[row0 + x - c - 1];
+    }
+    const int slot = (q < jj) ? q : (q < 2 * jj ? NB + (q - jj) : 2 * NB);
+    pc[slot] = val;
+  }
+  (void)s_par;
 }
 
 // ===========================================================================
@@ -367,6 +401,7 @@ __global__ __launch_bounds__(256) void k_dc_merge_prep(DcArgs a, const int32_t* 
   double* Qcd = W + dws_vec(m);
   int32_t *col = I, *nd = I + m, *df = I + 2 * m, *rp = I + 4 * m;
   const int tid = threadIdx.x;
+  extern __shared__ double dc_shm[];
   __shared__ double s_red[4];
   __shared__ int s_k, s_nrot;
   __shared__ double s_rho;
@@ -405,6 +440,12 @@ __global__ __launch_bounds__(256) void k_dc_merge_prep(DcArgs a, const int32_t* 
   for (int i = tid; i < m; i += 256) mx = fmax(mx, fmax(fabs(ds[i]), fabs(zs[i])));
   for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o, 64));
   if ((tid & 63) == 0) s_red[tid >> 6] = mx;
+  // stage the sorted poles and weights in LDS for the serial scan
+  const bool use_lds = (m <= DC_LDS_M);
+  double* sds = use_lds ? dc_shm : ds;
+  double* szs = use_lds ? dc_shm + m : zs;
+  if (use_lds)
+    for (int i = tid; i < m; i += 256) { sds[i] = ds[i]; szs[i] = zs[i]; }
   __syncthreads();
   if (tid == 0) {
     const double rp2 = 2.0 * rho;
@@ -413,24 +454,24 @@ __global__ __launch_bounds__(256) void k_dc_merge_prep(DcArgs a, const int32_t* 
     // serial deflation scan in ascending order
     int k = 0, nd_def = 0, nrot = 0, pj = -1;
     for (int j = 0; j < m; ++j) {
-      if (rp2 * fabs(zs[j]) <= tol) { df[nd_def++] = j; continue; }
+      if (rp2 * fabs(szs[j]) <= tol) { df[nd_def++] = j; continue; }
       if (pj < 0) { pj = j; continue; }
-      double sv = zs[pj], cv = zs[j];
+      double sv = szs[pj], cv = szs[j];
       const double tt = hypot(cv, sv);
-      const double dt = ds[j] - ds[pj];
+      const double dt = sds[j] - sds[pj];
       cv /= tt;
       sv = -sv / tt;
       if (fabs(dt * cv * sv) <= tol) {
-        zs[j] = tt;
-        zs[pj] = 0.0;
+        szs[j] = tt;
+        szs[pj] = 0.0;
         rp[2 * nrot] = pj;
         rp[2 * nrot + 1] = j;
         rc[nrot] = cv;
         rsn[nrot] = sv;
         ++nrot;
-        const double t2 = ds[pj] * cv * cv + ds[j] * sv * sv;
-        ds[j] = ds[pj] * sv * sv + ds[j] * cv * cv;
-        ds[pj] = t2;
+        const double t2 = sds[pj] * cv * cv + sds[j] * sv * sv;
+        sds[j] = sds[pj] * sv * sv + sds[j] * cv * cv;
+        sds[pj] = t2;
         df[nd_def++] = pj;
         pj = j;
       } else {
@@ -442,49 +483,80 @@ __global__ __launch_bounds__(256) void k_dc_merge_prep(DcArgs a, const int32_t* 
     s_k = k;
     s_nrot = nrot;
     a.kout[t] = k;
+    I[7 * m] = nrot;
   }
   __syncthreads();
-  const int k = s_k, nrot = s_nrot;
-  // plane rotations on the merged columns (rows s..e-1)
-  if (nrot > 0) {
-    for (int row = tid; row < m; row += 256) {
-      double* Zr = Z + (s + row);
-      for (int q = 0; q < nrot; ++q) {
-        const int64_t cp = (int64_t)col[rp[2 * q]] * n, cj = (int64_t)col[rp[2 * q + 1]] * n;
-        const double x = Zr[cp], yv = Zr[cj];
-        Zr[cp] = rc[q] * x + rsn[q] * yv;
-        Zr[cj] = rc[q] * yv - rsn[q] * x;
-      }
+  const int k = s_k;
+  for (int i = tid; i < k; i += 256) {
+    dl[i] = sds[nd[i]];
+    zl[i] = szs[nd[i]];
+  }
+  for (int i = tid; i < m - k; i += 256) dv[i] = sds[df[i]];
+  if (tid == 0) W[5 * m] = s_rho;  // zh[0] slot keeps rho until the z-hat pass
+  (void)Qcd;
+}
+
+// Apply the deflation rotations to the merged columns and gather them as
+// [non-deflated | deflated] into Qcd (m x m, ld m).  Grid: (row chunks of 64,
+// merges); each workgroup owns 64 rows of one merge.
+__global__ __launch_bounds__(256) void k_dc_rotgather(DcArgs a, const int32_t* tasks, int ntask) {
+  const int t = blockIdx.y;
+  const int32_t* tk = tasks + 8 * t;
+  const int b = tk[0], s = tk[1], e = tk[3];
+  const int m = e - s;
+  const int row0 = blockIdx.x * 64;
+  if (row0 >= m) return;
+  const int n = a.n[b];
+  const int k = a.kout[t];
+  double* Z = a.Z + a.aoff[b];
+  double* W = a.ws + join64(tk[4], tk[5]);
+  const int32_t* I = a.iws + join64(tk[6], tk[7]);
+  const int32_t *col = I, *nd = I + m, *df = I + 2 * m, *rp = I + 4 * m;
+  const double *rc = W + 7 * m, *rsn = W + 8 * m;
+  double* Qcd = W + dws_vec(m);
+  const int nrot = I[7 * m];
+  const int tid = threadIdx.x;
+  const int rl = tid & 63, g = tid >> 6;
+  const int row = row0 + rl;
+  if (nrot > 0 && g == 0 && row < m) {
+    double* Zr = Z + (s + row);
+    for (int q = 0; q < nrot; ++q) {
+      const int64_t cp = (int64_t)col[rp[2 * q]] * n, cj = (int64_t)col[rp[2 * q + 1]] * n;
+      const double x = Zr[cp], yv = Zr[cj];
+      Zr[cp] = rc[q] * x + rsn[q] * yv;
+      Zr[cj] = rc[q] * yv - rsn[q] * x;
     }
   }
-  for (int i = tid; i < k; i += 256) {
-    dl[i] = ds[nd[i]];
-    zl[i] = zs[nd[i]];
-  }
-  for (int i = tid; i < m - k; i += 256) dv[i] = ds[df[i]];
-  if (tid == 0) W[5 * m] = s_rho;  // zh[0] slot keeps rho until the z-hat pass
   __syncthreads();
-  // gather columns: [non-deflated | deflated] into Qcd (m x m, ld m)
-  for (int64_t q = tid; q < (int64_t)m * m; q += 256) {
-    const int row = (int)(q % m), c = (int)(q / m);
+  if (row >= m) return;
+  for (int c = g; c < m; c += 4) {
     const int srcc = (c < k) ? col[nd[c]] : col[df[c - k]];
-    Qcd[q] = Z[(int64_t)(s + row) + (int64_t)srcc * n];
+    Qcd[row + (int64_t)c * m] = Z[(int64_t)(s + row) + (int64_t)srcc * n];
   }
 }
 
-// Secular equation by bisection on tau = lambda - d_origin (one lane per root).
+// Secular equation for tau = lambda - d_origin (one lane per root); poles and
+// weights are staged in LDS when they fit.
 __global__ __launch_bounds__(64) void k_dc_secular(DcArgs a, const int32_t* tasks, const int64_t* rootbase,
                                                    int ntask) {
+  extern __shared__ double sec_shm[];
   const int t = blockIdx.y;
   const int i = blockIdx.x * 64 + threadIdx.x;
   const int k = a.kout[t];
-  if (i >= k) return;
+  if (blockIdx.x * 64 >= k) return;
   const int32_t* tk = tasks + 8 * t;
   const int m = tk[3] - tk[1];
   const int64_t wo = join64(tk[4], tk[5]);
   const int64_t io = join64(tk[6], tk[7]);
   double* W = a.ws + wo;
   const double *dl = W + 2 * m, *zl = W + 3 * m;
+  if (k <= DC_LDS_M) {
+    for (int j = threadIdx.x; j < k; j += 64) { sec_shm[j] = dl[j]; sec_shm[k + j] = zl[j]; }
+    __syncthreads();
+    dl = sec_shm;
+    zl = sec_shm + k;
+  }
+  if (i >= k) return;
   double* tr = W + 6 * m;
   int32_t* org = a.iws + io + 3 * m;
   const double rho = W[5 * m];
@@ -507,15 +579,64 @@ __global__ __launch_bounds__(64) void k_dc_secular(DcArgs a, const int32_t* task
       hi *= 2.0;
     }
   }
+  // Safeguarded rational iteration (Bunch, Nielsen & Sorensen 1978): the
+  // sums over poles left and right of the root are each replaced by one pole
+  // term matching value and slope at tau; the two-pole model is solved in
+  // closed form.  Steps leaving the bracket [lo, hi] fall back to bisection.
   const double dorg = dl[o];
-  for (int it = 0; it < 2200; ++it) {
-    const double md = lo + 0.5 * (hi - lo);
-    if (md <= lo || md >= hi) break;
-    double f = 1.0;
-    for (int j = 0; j < k; ++j) f += rho * zl[j] * zl[j] / ((dl[j] - dorg) - md);
-    if (f > 0.0) hi = md; else lo = md;
+  const double p = dl[i] - dorg;                       // left pole (shifted)
+  const bool last = (i == k - 1);
+  const double q = last ? 0.0 : dl[i + 1] - dorg;      // right pole (shifted)
+  const double eps = 2.220446049250313e-16;
+  double tau = lo + 0.5 * (hi - lo);
+  for (int it = 0; it < 200; ++it) {
+    double psi = 0, dpsi = 0, phi = 0, dphi = 0;
+    for (int j = 0; j <= i; ++j) {
+      const double r = 1.0 / ((dl[j] - dorg) - tau);
+      const double w2 = rho * zl[j] * zl[j];
+      psi += w2 * r;
+      dpsi += w2 * r * r;
+    }
+    for (int j = i + 1; j < k; ++j) {
+      const double r = 1.0 / ((dl[j] - dorg) - tau);
+      const double w2 = rho * zl[j] * zl[j];
+      phi += w2 * r;
+      dphi += w2 * r * r;
+    }
+    const double g = 1.0 + psi + phi;
+    if (g == 0.0) break;
+    if (g > 0.0) hi = tau; else lo = tau;
+    // model: C + B1/(p-x) + B2/(q-x) = 0, u = p - x
+    const double B1 = dpsi * (p - tau) * (p - tau);
+    double C = 1.0 + psi - B1 / (p - tau);
+    double xnew;
+    if (last) {
+      C += phi;
+      xnew = (C != 0.0) ? p + B1 / C : 0.5 * (lo + hi);
+    } else {
+      const double B2 = dphi * (q - tau) * (q - tau);
+      C += phi - B2 / (q - tau);
+      const double G = q - p;
+      const double qa = C, qb = C * G + B1 + B2, qc = B1 * G;
+      double u;
+      if (qa == 0.0) {
+        u = -qc / qb;
+      } else {
+        const double disc = fmax(qb * qb - 4.0 * qa * qc, 0.0);
+        const double qq = -0.5 * (qb + copysign(sqrt(disc), qb));
+        const double u1 = qq / qa, u2 = (qq != 0.0) ? qc / qq : u1;
+        u = (u1 < 0.0 && u1 > -G) ? u1 : u2;
+      }
+      xnew = p - u;
+    }
+    if (!(xnew > lo && xnew < hi)) xnew = lo + 0.5 * (hi - lo);
+    if (xnew <= lo || xnew >= hi) break;          // bracket exhausted
+    const double step = fabs(xnew - tau);
+    tau = xnew;
+    if (step <= 2.0 * eps * fabs(tau)) break;
   }
-  tr[i] = (o == i) ? hi : lo;   // the bracket end away from the pole
+  if (tau == 0.0) tau = (o == i) ? hi : lo;     // never sit on the pole
+  tr[i] = tau;
   org[i] = o;
 }
 
@@ -622,9 +743,27 @@ __global__ __launch_bounds__(256) void k_dc_finish(DcArgs a, const int32_t* task
     a.w[vo + s + fin] = v;
     I[5 * m + fin] = src;   // rp slot reused: source column per final position
   }
-  __syncthreads();
-  for (int64_t q = tid; q < (int64_t)m * m; q += 256) {
-    const int row = (int)(q % m), c = (int)(q / m);
+  (void)Qcd; (void)Qt; (void)Z; (void)n;
+}
+
+// Write the merged eigenvectors in final (ascending) order into the merge's
+// square region of Z.  Grid: (row chunks of 64, merges).
+__global__ __launch_bounds__(256) void k_dc_scatter(DcArgs a, const int32_t* tasks, int ntask) {
+  const int t = blockIdx.y;
+  const int32_t* tk = tasks + 8 * t;
+  const int b = tk[0], s = tk[1], e = tk[3];
+  const int m = e - s;
+  const int row = blockIdx.x * 64 + (threadIdx.x & 63);
+  if (row >= m) return;
+  const int g = threadIdx.x >> 6;
+  const int k = a.kout[t];
+  const int n = a.n[b];
+  double* Z = a.Z + a.aoff[b];
+  const double* W = a.ws + join64(tk[4], tk[5]);
+  const int32_t* I = a.iws + join64(tk[6], tk[7]);
+  const double* Qcd = W + dws_vec(m);
+  const double* Qt = W + dws_vec(m) + 2 * (int64_t)m * m;
+  for (int c = g; c < m; c += 4) {
     const int src = I[5 * m + c];
     const double v = (src < 0) ? Qt[row + (int64_t)(-1 - src) * m] : Qcd[row + (int64_t)(k + src) * m];
     Z[(int64_t)(s + row) + (int64_t)(s + c) * n] = v;
@@ -677,24 +816,21 @@ __global__ void k_bt_tmat(BtArgs a, int p) {
   const double* G = a.T + (int64_t)b * 2 * NBB * NBB;
   double* T = a.T + (int64_t)b * 2 * NBB * NBB + NBB * NBB;
   const double* tau = a.tau + a.voff[b] + c0;
-  __shared__ double Ts[NBB][NBB + 1];
   __shared__ double tmp[NBB];
-  const int j = threadIdx.x;
-  for (int i = 0; i < NBB; ++i) Ts[j][i] = 0.0;
-  __syncthreads();
+  const int j = threadIdx.x;   // thread j owns row j of T (upper triangular)
+  for (int i = 0; i < NBB; ++i) T[j + i * NBB] = 0.0;
   for (int i = 0; i < nbp; ++i) {
-    // tmp = -tau_i G(0:i, i)
+    // T(0:i, i) = T(0:i, 0:i) * (-tau_i G(0:i, i)),  T(i, i) = tau_i
     if (j < i) tmp[j] = -tau[i] * G[j + i * NBB];
     __syncthreads();
     if (j < i) {
       double acc = 0.0;
-      for (int l = j; l < i; ++l) acc += Ts[j][l] * tmp[l];
-      Ts[j][i] = acc;
+      for (int l = j; l < i; ++l) acc += T[j + l * NBB] * tmp[l];
+      T[j + i * NBB] = acc;
     }
-    if (j == i) Ts[i][i] = tau[i];
+    if (j == i) T[i + i * NBB] = tau[i];
     __syncthreads();
   }
-  for (int i = 0; i < NBB; ++i) T[j + i * NBB] = Ts[j][i];
 }
 
 // ===========================================================================
@@ -921,9 +1057,13 @@ void BatchedEigensolver::divide_conquer(hipStream_t s) {
     d_tasks_.upload(rec.data(), rec.size(), s);
     d_k_.reserve(nt);
     da.kout = d_k_.p;
+    int mmax = 0;
+    for (int i = 0; i < nt; ++i) mmax = std::max(mmax, rec[8 * i + 3] - rec[8 * i + 1]);
     {
       const int pid = profiler().begin(s);
-      hipLaunchKernelGGL(k_dc_merge_prep, dim3(nt), dim3(256), 0, s, da, d_tasks_.p, nt);
+      const size_t shm = 2 * sizeof(double) * (size_t)std::min(mmax, DC_LDS_M);
+      hipLaunchKernelGGL(k_dc_merge_prep, dim3(nt), dim3(256), shm, s, da, d_tasks_.p, nt);
+      hipLaunchKernelGGL(k_dc_rotgather, dim3(ceil_div(mmax, 64), nt), dim3(256), 0, s, da, d_tasks_.p, nt);
       profiler().end(pid, s, KF_DC_PREP, 0, 0);
     }
     h_k_.resize(nt);
@@ -933,7 +1073,8 @@ void BatchedEigensolver::divide_conquer(hipStream_t s) {
     for (int i = 0; i < nt; ++i) kmax = std::max(kmax, h_k_[i]);
     if (kmax > 0) {
       int pid = profiler().begin(s);
-      hipLaunchKernelGGL(k_dc_secular, dim3(ceil_div(kmax, 64), nt), dim3(64), 0, s, da, d_tasks_.p,
+      hipLaunchKernelGGL(k_dc_secular, dim3(ceil_div(kmax, 64), nt), dim3(64),
+                         2 * sizeof(double) * (size_t)std::min(kmax, DC_LDS_M), s, da, d_tasks_.p,
                          (const int64_t*)nullptr, nt);
       profiler().end(pid, s, KF_DC_SECULAR, 0, 0);
       pid = profiler().begin(s);
@@ -954,6 +1095,7 @@ void BatchedEigensolver::divide_conquer(hipStream_t s) {
     }
     const int pid = profiler().begin(s);
     hipLaunchKernelGGL(k_dc_finish, dim3(nt), dim3(256), 0, s, da, d_tasks_.p, nt);
+    hipLaunchKernelGGL(k_dc_scatter, dim3(ceil_div(mmax, 64), nt), dim3(256), 0, s, da, d_tasks_.p, nt);
     profiler().end(pid, s, KF_DC_FINISH, 0, 0);
   }
   NRG_HIP(hipGetLastError());
