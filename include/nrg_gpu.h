@@ -91,6 +91,53 @@ int nrg_gpu_invariant(const int32_t* kept_per_block, double* invariant_matrix_ou
 /* invariantMatrixForSpectrumDGEMM (invariantMatrixForSpectrumDGEMM.F90:1-392). */
 int nrg_gpu_invariant_spectrum(const int32_t* kept_per_block, double* invariant_matrix_spectrum_out);
 
+/* ---- CFS spectra: history, reduced density, spikes -----------------------
+ * Label shifts of the spectral operators, positive/negative_spectrum_operation
+ * (nmat,nv) read with the model (read_initial_basis.F90:168-182). */
+int nrg_gpu_set_spectrum_operators(int nmat, const int32_t* pos_op, const int32_t* neg_op);
+
+/* Stock of one iteration >= startTrancation (postprocess.F90:221-374 and
+ * appendRestore, module.F90:1345-1570): the library keeps the kept columns of
+ * every block, the block table, the product-basis records and the energies
+ * of the last nrg_gpu_diagonalize call in HBM, and forms the spectral
+ * products that spikeCFS (or, with last != 0, spikeZeroTemperature) of this
+ * iteration will weight, while the previous spectral operator is resident.
+ * Call after the truncation and before nrg_gpu_invariant_spectrum.  A stock
+ * of an iteration not above the last stocked one starts a new history. */
+int nrg_gpu_stock(int iteration, const int32_t* kept_per_block, int last);
+
+/* initializeReducedDensity (initializeReducedDensity.F90:8-81): rho of the
+ * last iteration = 1/g on its g ground states (E < 1e-8). */
+int nrg_gpu_rho_init(int last_iteration);
+
+/* One shell of reducedDensityMatrixCalculation (reducedDensityMatrixDGEMM.F90:
+ * 62-321): rho(iteration) from rho(iteration+1); *trace receives Tr rho
+ * (printed by the reference at :364-367). */
+int nrg_gpu_rho_shell(int iteration, double* trace);
+
+/* Copy rho(iteration) to the host in the reference's reducedDensityMatrix
+ * layout: one k_b x k_b row-major block per stocked block in block order
+ * (:308-314).  size = sum k_b^2 (checked). */
+int nrg_gpu_rho_get(int iteration, double* rho, int64_t size);
+
+/* Bins of binnedSpikeNegative/Positive (main.F90:142-156): clear them for
+ * numberOfBin bins per type. */
+int nrg_gpu_spectrum_setup(int nbin);
+
+/* spikeCFS (calculateSpectrumDGEMM.F90:217-646) for iteration < last and
+ * spikeZeroTemperature (:1-215) for the last iteration, both as
+ * cfsSpectrumCalculation calls them (module.F90:1683-1706).  scale =
+ * scale(iteration) (module.F90:690-711); spikes are binned on the device. */
+int nrg_gpu_spike_cfs(int iteration, double scale);
+int nrg_gpu_spike_zero_temperature(int iteration, double scale);
+
+/* Copy the bin weights: negative and positive, each (nbin*nmat) with bin i
+ * of type t at i + t*nbin (the weight column of binnedSpike*, main.F90:146). */
+int nrg_gpu_spectrum_bins(double* negative, double* positive, int64_t size);
+
+/* Free the stocked history (no counterpart; the reference frees at exit). */
+int nrg_gpu_history_clear(void);
+
 /* ---- per-stage timing of the last calls (milliseconds) -----------------
  * t[0] H assembly, t[1] eigensolver total, t[2] tridiagonalisation,
  * t[3] divide and conquer, t[4] back-transform, t[5] invariant,
@@ -119,7 +166,9 @@ int nrg_gpu_dgemm(int transa, int transb, int m, int n, int k, double alpha,
  * 2 tridiagonal SYMV, 3 tridiagonal trailing GEMM, 4 D&C leaves, 5 D&C merge
  * set-up, 6 D&C secular equations, 7 D&C vectors, 8 D&C GEMM, 9 D&C
  * reorder, 10 back-transform small kernels, 11 back-transform GEMM,
- * 12 invariant GEMM, 13 spectral invariant GEMM, 14 other. */
+ * 12 invariant GEMM, 13 spectral invariant GEMM, 14 CFS spectral products,
+ * 15 CFS rho weighting GEMM, 16 CFS binning, 17 rho GEMM, 18 rho row
+ * gather, 19 other. */
 int nrg_gpu_profile(int enable);
 int nrg_gpu_kernel_stats(double* out, int nfam);
 

@@ -1,6 +1,7 @@
 // extern "C" entry points of libnrg_gpu.so (declared in include/nrg_gpu.h).
 #include "../../include/nrg_gpu.h"
 #include "assemble.h"
+#include "cfs.h"
 #include "context.h"
 #include <algorithm>
 #include <chrono>
@@ -100,6 +101,7 @@ int nrg_gpu_finalize(void) {
   return guarded([&](Context& C) {
     if (!C.initialised) return;
     (void)hipStreamSynchronize(C.stream);
+    cfs().release();
     C.eig.release();
     C.gemm.release();
     C.d_op.release(); C.d_hop.release(); C.d_hop_coef.release();
@@ -258,6 +260,116 @@ int nrg_gpu_diagonalize(int iteration, int nsub, const int32_t* subspace_info, i
     }
     for (int i = 0; i < nfull; ++i)
       if (!std::isfinite(eigenvalue[i])) throw Error(NRG_ERR_NUMERIC, "diagonalize: non-finite eigenvalue");
+    C.E_block.assign(eigenvalue, eigenvalue + nfull);
+  });
+}
+
+int nrg_gpu_set_spectrum_operators(int nmat, const int32_t* pos_op, const int32_t* neg_op) {
+  return guarded([&](Context& C) {
+    need_init(C);
+    NRG_REQUIRE(C.nv > 0 && nmat == C.nmat && nmat > 0, "set_spectrum_operators: nmat differs from the model");
+    NRG_REQUIRE(pos_op && neg_op, "set_spectrum_operators: NULL table");
+    C.pos_op.resize((size_t)nmat * C.nv);
+    C.neg_op.resize((size_t)nmat * C.nv);
+    for (int im = 0; im < nmat; ++im)
+      for (int i = 0; i < C.nv; ++i) {
+        C.pos_op[im * C.nv + i] = pos_op[im + i * nmat];
+        C.neg_op[im * C.nv + i] = neg_op[im + i * nmat];
+      }
+  });
+}
+
+int nrg_gpu_stock(int iteration, const int32_t* kept_per_block, int last) {
+  return guarded([&](Context& C) {
+    need_init(C);
+    NRG_REQUIRE(kept_per_block != nullptr, "stock: NULL kept_per_block");
+    const double t0 = wall_ms();
+    stock(C, iteration, kept_per_block, last);
+    C.timers.cfs = wall_ms() - t0;
+  });
+}
+
+int nrg_gpu_rho_init(int last_iteration) {
+  return guarded([&](Context& C) {
+    need_init(C);
+    rho_init(C, last_iteration);
+  });
+}
+
+int nrg_gpu_rho_shell(int iteration, double* trace) {
+  return guarded([&](Context& C) {
+    need_init(C);
+    const double t0 = wall_ms();
+    const double t = rho_shell(C, iteration);
+    if (trace) *trace = t;
+    C.timers.rho = wall_ms() - t0;
+  });
+}
+
+int nrg_gpu_rho_get(int iteration, double* rho, int64_t size) {
+  return guarded([&](Context& C) {
+    need_init(C);
+    HistEntry* h = cfs().at(iteration);
+    NRG_REQUIRE(h && h->rho_ready, "rho_get: no reduced density for this iteration");
+    int64_t tot = 0;
+    for (int b = 0; b < h->nsub; ++b) tot += (int64_t)h->kept[b] * h->kept[b];
+    NRG_REQUIRE(size == tot, "rho_get: size differs from the sum of kept^2");
+    std::vector<double> col(tot);
+    h->rho.download(col.data(), tot, C.stream);
+    NRG_HIP(hipStreamSynchronize(C.stream));
+    for (int b = 0; b < h->nsub; ++b) {
+      const int k = h->kept[b];
+      const double* s = col.data() + h->roff[b];
+      double* d = rho + h->roff[b];
+      for (int i = 0; i < k; ++i)
+        for (int j = 0; j < k; ++j) d[(int64_t)i * k + j] = s[i + (int64_t)j * k];
+    }
+  });
+}
+
+int nrg_gpu_spectrum_setup(int nbin) {
+  return guarded([&](Context& C) {
+    need_init(C);
+    spectrum_setup(C, nbin);
+  });
+}
+
+int nrg_gpu_spike_cfs(int iteration, double scale) {
+  return guarded([&](Context& C) {
+    need_init(C);
+    const double t0 = wall_ms();
+    spikes(C, iteration, scale, 0);
+    C.timers.cfs = wall_ms() - t0;
+  });
+}
+
+int nrg_gpu_spike_zero_temperature(int iteration, double scale) {
+  return guarded([&](Context& C) {
+    need_init(C);
+    const double t0 = wall_ms();
+    spikes(C, iteration, scale, 1);
+    C.timers.cfs = wall_ms() - t0;
+  });
+}
+
+int nrg_gpu_spectrum_bins(double* negative, double* positive, int64_t size) {
+  return guarded([&](Context& C) {
+    need_init(C);
+    Cfs& F = cfs();
+    NRG_REQUIRE(F.nbin > 0, "spectrum_bins: call nrg_gpu_spectrum_setup first");
+    const int64_t n = (int64_t)F.nbin * C.nmat;
+    NRG_REQUIRE(size == n, "spectrum_bins: size differs from nbin * nmat");
+    if (negative) NRG_HIP(hipMemcpyAsync(negative, F.bins.p, n * sizeof(double), hipMemcpyDeviceToHost, C.stream));
+    if (positive)
+      NRG_HIP(hipMemcpyAsync(positive, F.bins.p + n, n * sizeof(double), hipMemcpyDeviceToHost, C.stream));
+    NRG_HIP(hipStreamSynchronize(C.stream));
+  });
+}
+
+int nrg_gpu_history_clear(void) {
+  return guarded([&](Context& C) {
+    if (C.stream) NRG_HIP(hipStreamSynchronize(C.stream));
+    cfs().clear();
   });
 }
 

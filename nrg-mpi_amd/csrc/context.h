@@ -34,6 +34,7 @@ struct Context {
   std::vector<double> coef_spec;        // [nmat][nops]
   std::vector<int32_t> cons;            // chargeStart, chargeEnd, spinStart, spinEnd
   std::vector<int32_t> cons_diff_spec;  // [nmat][nv]
+  std::vector<int32_t> pos_op, neg_op;  // [nmat][nv] label shifts of the spectral operators
   DevBuf<int32_t> d_op, d_hop;          // d_hop: [valid,type,chan,swap] x nops^2
   DevBuf<double> d_hop_coef;
 
@@ -43,6 +44,7 @@ struct Context {
   std::vector<int32_t> basis_in;        // nfull*(nv+3)
   std::vector<int> bn;                  // block sizes
   std::vector<int64_t> boff, bvoff;     // sum n^2 / sum n offsets
+  std::vector<double> E_block;          // shifted eigenvalues in block order (as returned)
   DevBuf<int32_t> d_basis_in;
   DevBuf<double> d_H, d_U, d_w, d_Eold;
   // invariant matrices of the kept basis (dense k x k x ncm / nmat, column-major)

@@ -17,36 +17,14 @@
 // X_o = c(o) S_sub^T U_R[b_o,:] is a strided GEMM and the result
 // U_L[a,:]^T X is one gathered GEMM over the stacked rows.
 #include "assemble.h"
-#include <unordered_map>
+#include "blockrows.h"
 
 namespace nrg {
 
 namespace {
 
-// rows of block `blk` (0-based within the block) grouped by operation, in
-// increasing variation; rows_by_op[o][var-1] = row.
-struct BlockRows {
-  std::vector<std::vector<int32_t>> by_op;
-  std::vector<std::vector<int32_t>> ref_by_op;  // parent (old kept) index, 1-based
-};
-
 BlockRows block_rows(const Context& C, int blk) {
-  BlockRows br;
-  br.by_op.assign(C.nops, {});
-  br.ref_by_op.assign(C.nops, {});
-  const int rs = C.nv + 3;
-  const int64_t s0 = C.bvoff[blk];
-  for (int r = 0; r < C.bn[blk]; ++r) {
-    const int32_t* rec = &C.basis_in[(s0 + r) * rs];
-    const int op = rec[C.nv + 2] - 1;
-    const int var = rec[C.nv + 1];
-    NRG_REQUIRE(op >= 0 && op < C.nops, "invariant: bad operation index");
-    NRG_REQUIRE(var == (int)br.by_op[op].size() + 1,
-                "invariant: rows of one operation are not in variation order");
-    br.by_op[op].push_back(r);
-    br.ref_by_op[op].push_back(rec[C.nv]);
-  }
-  return br;
+  return nrg::block_rows(C.basis_in.data(), C.bvoff[blk], C.bn[blk], C.nv, C.nops);
 }
 
 std::vector<int64_t> kept_starts(const int32_t* kept, int nsub, int64_t& k) {

@@ -24,6 +24,11 @@ enum KFamily : int {
   KF_GEMM_BACK,
   KF_GEMM_INV,
   KF_GEMM_SPEC,
+  KF_GEMM_CFSW,
+  KF_GEMM_CFSX,
+  KF_CFS_BIN,
+  KF_GEMM_RHO,
+  KF_RHO_GATHER,
   KF_OTHER,
   KF_COUNT
 };

@@ -167,6 +167,48 @@ contains
     nrg_host_get_times = 0
   end function nrg_host_get_times
 
+  ! CFS-NRG spectrum phase after the last iteration (main.F90:185-224):
+  ! reduced density sweep, CFS spikes, bins copied to the host.
+  integer(c_int) function nrg_host_spectrum(keep_history) bind(C, name="nrg_host_spectrum")
+    integer(c_int), value :: keep_history
+    integer :: ierr
+    call spectrum_phase(ierr, keep_history /= 0)
+    nrg_host_spectrum = ierr
+  end function nrg_host_spectrum
+
+  ! Bin weights of binnedSpikeNegative/Positive (nbin*nmat each); returns
+  ! numberOfBin (0 before the spectrum phase).
+  integer(c_int) function nrg_host_get_bins(negative, positive, n) bind(C, name="nrg_host_get_bins")
+    integer(c_int), value :: n
+    real(c_double), intent(out) :: negative(*), positive(*)
+    integer :: m
+    nrg_host_get_bins = 0
+    if (.not. allocated(binned_pos)) return
+    m = min(int(n), size(binned_pos))
+    negative(1:m) = binned_neg(1:m)
+    positive(1:m) = binned_pos(1:m)
+    nrg_host_get_bins = number_of_bin
+  end function nrg_host_get_bins
+
+  ! Tr rho of the shells i_first..n_iter (zero outside the swept range).
+  integer(c_int) function nrg_host_get_rho_traces(out, n) bind(C, name="nrg_host_get_rho_traces")
+    integer(c_int), value :: n
+    real(c_double), intent(out) :: out(*)
+    integer :: m
+    nrg_host_get_rho_traces = 0
+    if (.not. allocated(rho_trace)) return
+    m = min(int(n), size(rho_trace))
+    out(1:m) = rho_trace(i_first:i_first + m - 1)
+    nrg_host_get_rho_traces = m
+  end function nrg_host_get_rho_traces
+
+  ! t(1:3): stock of the last iteration, rho sweep, CFS spikes (s)
+  integer(c_int) function nrg_host_get_spectrum_times(t) bind(C, name="nrg_host_get_spectrum_times")
+    real(c_double), intent(out) :: t(3)
+    t(1) = t_stock; t(2) = t_rho; t(3) = t_cfs
+    nrg_host_get_spectrum_times = 0
+  end function nrg_host_get_spectrum_times
+
   integer(c_int) function nrg_host_close() bind(C, name="nrg_host_close")
     nrg_host_close = nrg_gpu_finalize()
   end function nrg_host_close

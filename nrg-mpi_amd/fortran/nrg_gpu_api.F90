@@ -54,6 +54,49 @@ module nrg_gpu_api
        integer(c_int), intent(in) :: kept(*)
        type(c_ptr), value :: out
      end function
+     integer(c_int) function nrg_gpu_set_spectrum_operators(nmat, pos_op, neg_op) &
+          bind(C, name="nrg_gpu_set_spectrum_operators")
+       import :: c_int
+       integer(c_int), value :: nmat
+       integer(c_int), intent(in) :: pos_op(*), neg_op(*)
+     end function
+     integer(c_int) function nrg_gpu_stock(iteration, kept, last) bind(C, name="nrg_gpu_stock")
+       import :: c_int
+       integer(c_int), value :: iteration, last
+       integer(c_int), intent(in) :: kept(*)
+     end function
+     integer(c_int) function nrg_gpu_rho_init(last_iteration) bind(C, name="nrg_gpu_rho_init")
+       import :: c_int
+       integer(c_int), value :: last_iteration
+     end function
+     integer(c_int) function nrg_gpu_rho_shell(iteration, trace) bind(C, name="nrg_gpu_rho_shell")
+       import :: c_int, c_double
+       integer(c_int), value :: iteration
+       real(c_double), intent(out) :: trace
+     end function
+     integer(c_int) function nrg_gpu_spectrum_setup(nbin) bind(C, name="nrg_gpu_spectrum_setup")
+       import :: c_int
+       integer(c_int), value :: nbin
+     end function
+     integer(c_int) function nrg_gpu_spike_cfs(iteration, scale) bind(C, name="nrg_gpu_spike_cfs")
+       import :: c_int, c_double
+       integer(c_int), value :: iteration
+       real(c_double), value :: scale
+     end function
+     integer(c_int) function nrg_gpu_spike_zero_temperature(iteration, scale) &
+          bind(C, name="nrg_gpu_spike_zero_temperature")
+       import :: c_int, c_double
+       integer(c_int), value :: iteration
+       real(c_double), value :: scale
+     end function
+     integer(c_int) function nrg_gpu_spectrum_bins(negative, positive, n) bind(C, name="nrg_gpu_spectrum_bins")
+       import :: c_int, c_double, c_int64_t
+       real(c_double), intent(out) :: negative(*), positive(*)
+       integer(c_int64_t), value :: n
+     end function
+     integer(c_int) function nrg_gpu_history_clear() bind(C, name="nrg_gpu_history_clear")
+       import :: c_int
+     end function
      integer(c_int) function nrg_gpu_stage_times(t, n) bind(C, name="nrg_gpu_stage_times")
        import :: c_int, c_double
        real(c_double), intent(out) :: t(*)

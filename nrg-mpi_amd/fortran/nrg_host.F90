@@ -51,8 +51,16 @@ module nrg_host
   logical :: keep_vectors = .false. ! copy eigenvectors to the host (spectra / tests)
   double precision, allocatable, target :: eigvec(:)
 
+  ! ---- CFS spectra (main.F90:141-156): bins of types 1..nmat, weight column
+  integer, parameter :: mesh_of_bin = 1000
+  double precision, parameter :: log10_emin = -15.0d0
+  integer :: number_of_bin = 0
+  double precision, allocatable, target :: binned_pos(:), binned_neg(:)
+  double precision, allocatable :: rho_trace(:)   ! Tr rho per shell, indexed by iteration
+
   ! ---- timing (seconds)
   double precision :: t_prepare = 0, t_diag = 0, t_post = 0, t_invariant = 0, t_iter = 0
+  double precision :: t_stock = 0, t_rho = 0, t_cfs = 0
 
 contains
 
@@ -749,6 +757,16 @@ contains
     call calc_thermo()
     call truncate(it)
     call kept_per_block()
+    ! stock of the truncated iterations (postprocess.F90:213-374); the
+    ! library forms this iteration's spectral products now, while the
+    ! previous spectral operator is still resident
+    if (flag_spectrum .and. it >= start_trunc) then
+       call cpu_time_wall(t0)
+       ierr = nrg_gpu_stock(int(it, c_int), kept_blk, merge(1_c_int, 0_c_int, it == n_iter))
+       if (ierr /= 0) return
+       call cpu_time_wall(t1)
+       t_stock = t1 - t0
+    end if
     if (it < n_iter) then
        call cpu_time_wall(t0)
        ierr = nrg_gpu_invariant(kept_blk, c_null_ptr)
@@ -795,12 +813,82 @@ contains
   end subroutine nrg_iteration
 
   ! =====================================================================
+  ! CFS-NRG spectrum phase (main.F90:185-224), after the forward loop
+  ! =====================================================================
+  ! initializeReducedDensity (initializeReducedDensity.F90:8-81)
+  subroutine initialize_reduced_density(ierr)
+    integer, intent(out) :: ierr
+    ierr = nrg_gpu_rho_init(int(n_iter, c_int))
+  end subroutine initialize_reduced_density
+
+  ! reducedDensityMatrixCalculation (reducedDensityMatrixDGEMM.F90:2-377):
+  ! shells n_iter-1 down to startTrancation
+  subroutine reduced_density_matrix_calculation(ierr)
+    integer, intent(out) :: ierr
+    integer :: it
+    real(c_double) :: tr
+    ierr = 0
+    if (allocated(rho_trace)) deallocate(rho_trace)
+    allocate(rho_trace(i_first:n_iter))
+    rho_trace = 0.0d0
+    if (flag_trunc) rho_trace(n_iter) = 1.0d0
+    do it = n_iter - 1, start_trunc, -1
+       ierr = nrg_gpu_rho_shell(int(it, c_int), tr)
+       if (ierr /= 0) return
+       rho_trace(it) = tr
+    end do
+  end subroutine reduced_density_matrix_calculation
+
+  ! cfsSpectrumCalculation (module.F90:1683-1706)
+  subroutine cfs_spectrum_calculation(it, ierr)
+    integer, intent(in) :: it
+    integer, intent(out) :: ierr
+    if (it == n_iter) then
+       ierr = nrg_gpu_spike_zero_temperature(int(it, c_int), energy_scale(it))
+    else
+       ierr = nrg_gpu_spike_cfs(int(it, c_int), energy_scale(it))
+    end if
+  end subroutine cfs_spectrum_calculation
+
+  ! keep_history: leave the stocked history (rho, products) on the device
+  ! for inspection instead of freeing it
+  subroutine spectrum_phase(ierr, keep_history)
+    integer, intent(out) :: ierr
+    logical, intent(in) :: keep_history
+    integer :: it
+    double precision :: t0, t1, t2
+    ierr = 0
+    if (.not. flag_spectrum) return
+    call cpu_time_wall(t0)
+    if (flag_trunc) then
+       call initialize_reduced_density(ierr)
+       if (ierr /= 0) return
+    end if
+    call reduced_density_matrix_calculation(ierr)
+    if (ierr /= 0) return
+    call cpu_time_wall(t1)
+    do it = start_trunc, n_iter
+       call cfs_spectrum_calculation(it, ierr)
+       if (ierr /= 0) return
+    end do
+    if (allocated(binned_pos)) deallocate(binned_pos, binned_neg)
+    allocate(binned_pos(number_of_bin * nmat), binned_neg(number_of_bin * nmat))
+    ierr = nrg_gpu_spectrum_bins(binned_neg, binned_pos, int(number_of_bin * nmat, c_int64_t))
+    if (ierr /= 0) return
+    if (.not. keep_history) ierr = nrg_gpu_history_clear()
+    call cpu_time_wall(t2)
+    t_rho = t1 - t0
+    t_cfs = t2 - t1
+  end subroutine spectrum_phase
+
+  ! =====================================================================
   ! setup: inputs + device
   ! =====================================================================
   subroutine setup(dir, device, ierr)
     character(len=*), intent(in) :: dir
     integer, intent(in) :: device
     integer, intent(out) :: ierr
+    double precision :: emax, log10emax
     call read_inputs(dir, ierr)
     if (ierr /= 0) then
        ierr = 100 + ierr
@@ -820,7 +908,18 @@ contains
     if (ierr /= 0) return
     ierr = nrg_gpu_set_invariant(int(nb0, c_int), inv0)
     if (ierr /= 0) return
-    if (flag_spectrum) ierr = nrg_gpu_set_invariant_spectrum(int(nb0, c_int), spec0)
+    if (flag_spectrum) then
+       ierr = nrg_gpu_set_invariant_spectrum(int(nb0, c_int), spec0)
+       if (ierr /= 0) return
+       ierr = nrg_gpu_set_spectrum_operators(int(nmat, c_int), pos_op, neg_op)
+       if (ierr /= 0) return
+       ! bins (main.F90:141-156): Emax = scale(-1), 1000 bins per decade from 1e-15
+       emax = energy_scale(-1)
+       log10emax = dble(int(log10(emax))) + 1.0d0
+       number_of_bin = int(log10emax - log10_emin) * mesh_of_bin + 1
+       ierr = nrg_gpu_spectrum_setup(int(number_of_bin, c_int))
+       if (ierr /= 0) return
+    end if
   end subroutine setup
 
 end module nrg_host

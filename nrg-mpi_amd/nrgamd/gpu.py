@@ -41,11 +41,22 @@ SIGNATURES = {
     "nrg_gpu_inject_fault": (ctypes.c_int, [ctypes.c_int]),
     "nrg_gpu_profile": (ctypes.c_int, [ctypes.c_int]),
     "nrg_gpu_kernel_stats": (ctypes.c_int, [_f64p, ctypes.c_int]),
+    "nrg_gpu_set_spectrum_operators": (ctypes.c_int, [ctypes.c_int, _i32p, _i32p]),
+    "nrg_gpu_stock": (ctypes.c_int, [ctypes.c_int, _i32p, ctypes.c_int]),
+    "nrg_gpu_rho_init": (ctypes.c_int, [ctypes.c_int]),
+    "nrg_gpu_rho_shell": (ctypes.c_int, [ctypes.c_int, _f64p]),
+    "nrg_gpu_rho_get": (ctypes.c_int, [ctypes.c_int, _f64p, ctypes.c_int64]),
+    "nrg_gpu_spectrum_setup": (ctypes.c_int, [ctypes.c_int]),
+    "nrg_gpu_spike_cfs": (ctypes.c_int, [ctypes.c_int, ctypes.c_double]),
+    "nrg_gpu_spike_zero_temperature": (ctypes.c_int, [ctypes.c_int, ctypes.c_double]),
+    "nrg_gpu_spectrum_bins": (ctypes.c_int, [_f64p, _f64p, ctypes.c_int64]),
+    "nrg_gpu_history_clear": (ctypes.c_int, []),
 }
 
 KERNEL_FAMILIES = ["assemble", "tridiag_col", "tridiag_symv", "tridiag_gemm", "dc_leaf", "dc_prep",
                    "dc_secular", "dc_vectors", "dc_gemm", "dc_finish", "back_small", "back_gemm",
-                   "invariant_gemm", "spectral_gemm", "other"]
+                   "invariant_gemm", "spectral_gemm", "cfs_products", "cfs_rho_weight", "cfs_binning",
+                   "rho_gemm", "rho_gather", "other"]
 
 
 class NrgGpuError(RuntimeError):
@@ -121,6 +132,20 @@ def syev_batched(blocks):
         o1 += nb
         o2 += nb * nb
     return ws, Us
+
+
+def rho_get(iteration: int, kept_per_block):
+    """Reduced density matrix of a swept iteration: list of k_b x k_b blocks
+    (stocked blocks in block order; row-major as the reference stores it)."""
+    kb = [int(k) for k in kept_per_block]
+    tot = sum(k * k for k in kb)
+    a = np.zeros(max(tot, 1))
+    _check(load().nrg_gpu_rho_get(iteration, _p(a, _f64p), tot))
+    out, o = [], 0
+    for k in kb:
+        out.append(a[o:o + k * k].reshape(k, k))
+        o += k * k
+    return out
 
 
 def profile(enable: bool):

@@ -35,6 +35,10 @@ SIGNATURES = {
     "nrg_host_get_eigenvectors": (ctypes.c_int, [_f64p, ctypes.c_int64]),
     "nrg_host_get_times": (ctypes.c_int, [_f64p]),
     "nrg_host_close": (ctypes.c_int, []),
+    "nrg_host_spectrum": (ctypes.c_int, [ctypes.c_int]),
+    "nrg_host_get_bins": (ctypes.c_int, [_f64p, _f64p, ctypes.c_int]),
+    "nrg_host_get_rho_traces": (ctypes.c_int, [_f64p, ctypes.c_int]),
+    "nrg_host_get_spectrum_times": (ctypes.c_int, [_f64p]),
 }
 
 
@@ -144,6 +148,38 @@ class Host:
         self.lib.nrg_host_get_times(t.ctypes.data_as(_f64p))
         keys = ["prepare", "diagonalization", "postprocess", "invariant", "iteration", "Xn", "Sn", "Cn", "trunc_margin"]
         return dict(zip(keys, t.tolist()))
+
+    def run_forward(self):
+        for it in self.iterations():
+            self.iterate(it)
+
+    def spectrum(self, keep_history: bool = False):
+        """CFS spectrum phase after the last iteration: rho sweep + spikes.
+        keep_history leaves rho and the products on the device (for
+        nrgamd.gpu.rho_get); otherwise the history is freed."""
+        _check(self.lib.nrg_host_spectrum(int(keep_history)), "nrg_host_spectrum")
+
+    def bins(self):
+        """(negative, positive) bin weights, each [nmat, numberOfBin]."""
+        i = self.info()
+        cap = 64 * 1024 * max(i["nmat"], 1)
+        neg, pos = np.zeros(cap), np.zeros(cap)
+        nbin = self.lib.nrg_host_get_bins(neg.ctypes.data_as(_f64p), pos.ctypes.data_as(_f64p), cap)
+        n = nbin * i["nmat"]
+        return neg[:n].reshape(i["nmat"], nbin), pos[:n].reshape(i["nmat"], nbin)
+
+    def rho_traces(self):
+        """{iteration: Tr rho} for the swept shells and the last iteration."""
+        i = self.info()
+        n = i["n_iter"] - i["i_first"] + 1
+        t = np.zeros(n)
+        self.lib.nrg_host_get_rho_traces(t.ctypes.data_as(_f64p), n)
+        return {i["i_first"] + j: t[j] for j in range(n) if i["start_trunc"] <= i["i_first"] + j}
+
+    def spectrum_times(self):
+        t = np.zeros(3)
+        self.lib.nrg_host_get_spectrum_times(t.ctypes.data_as(_f64p))
+        return dict(zip(["stock", "rho", "cfs"], t.tolist()))
 
     def close(self):
         return self.lib.nrg_host_close()
