@@ -756,19 +756,37 @@ class NRG:
                     self.bin_spikes(cont, sgn * E[s0:s0 + len(w)] * sc, w * w / pf, im)
 
     # -- calculateSpectrumDGEMM.F90:296-643 -------------------------------------
-    def spike_cfs(self, rec):
+    def spike_cfs(self, rec, per_state: bool = False):
+        """Spikes of every discarded state l (positions nkept+1..nfull).
+
+        per_state=True walks the discarded states one at a time in the
+        reference's order (``:296-302, :363``).  The default groups the
+        discarded states of one block L and evaluates the same products for
+        all of them at once (the columns of U_L[:, lvar]); the two orders
+        agree to rounding and are checked against each other."""
         m = self.m
         nv = m.nv
         E = rec.eigenvalue
         sc = m.scale(rec.it)
         full_it = {tuple(e["labels"]): e for e in self.full if e["it"] == rec.it}
         kept_lab = rec.basis_output[:rec.nkept, :nv]
-        for il in range(rec.nkept, len(E)):
-            lvar = int(rec.basis_output[il, nv])
+        disc = np.arange(rec.nkept, len(E))
+        if per_state:
+            groups = [(int(il),) for il in disc]
+        else:
+            labs = rec.basis_output[disc, :nv]
+            keys = [tuple(x) for x in labs]
+            order = {}
+            for j, k in enumerate(keys):
+                order.setdefault(k, []).append(int(disc[j]))
+            groups = [tuple(v) for v in order.values()]
+        for grp in groups:
+            il = np.array(grp)
+            lvar = rec.basis_output[il, nv].astype(np.int64)
             El = E[il]
-            lab = rec.basis_output[il, :nv]
+            lab = rec.basis_output[il[0], :nv]
             bl = self._block_of(rec, lab)
-            ul = rec.U[bl][:, lvar - 1:lvar]
+            ul = rec.U[bl][:, lvar - 1]
             for sgn, ops, cont in ((-1, m.pos_op, self.bins_neg), (+1, m.neg_op, self.bins_pos)):
                 for im in range(m.nmat):
                     klab = lab + ops[im]
@@ -780,16 +798,16 @@ class NRG:
                     start = int(np.argmax(np.all(kept_lab == klab, axis=1)))
                     uk = rec.U[bk][:, :kk]
                     if sgn < 0:
-                        w = self._spectral_element(rec, bk, bl, im, uk, ul)[:, 0]
+                        w = self._spectral_element(rec, bk, bl, im, uk, ul)          # (kk, nl)
                     else:
-                        w = self._spectral_element(rec, bl, bk, im, ul, uk)[0, :]
+                        w = self._spectral_element(rec, bl, bk, im, ul, uk).T        # (kk, nl)
                     rho = self.rho[(rec.it, tuple(klab))]
                     refs = rec.basis_output[start:start + kk, nv] - 1
                     # weight(k1) = w(k1) * sum_k2 w(k2) rho(k2, k1)
                     x = rho[np.ix_(refs, refs)].T @ w
                     Ek = E[start:start + kk]
-                    om = (Ek - El) * sc if sgn < 0 else (El - Ek) * sc
-                    self.bin_spikes(cont, om, w * x, im)
+                    om = (Ek[:, None] - El[None, :]) * sc if sgn < 0 else (El[None, :] - Ek[:, None]) * sc
+                    self.bin_spikes(cont, om.ravel(), (w * x).ravel(), im)
 
     def run_spectrum(self):
         self.setup_bins()

@@ -38,7 +38,12 @@ RUNS = {
     # (-1..4): the first four match a 45-iteration run exactly, the cost of a
     # steady-state iteration on the reference CPU path is ~minutes
     "two_orbital_nk4000_it45": (dict(niter=6), -100, 8),
+    # two-orbital model with spectra at a size the reference finishes in
+    # minutes: four spectral types, orbitally degenerate ground states
+    "two_orbital_nk300_it20_spec": (dict(nkeep=300, niter=20, spectrum=True), -100, 8),
 }
+# fixture name -> workload it is derived from (default: the same name)
+BASE = {"two_orbital_nk300_it20_spec": "two_orbital_nk4000_it45"}
 
 
 def h(a):
@@ -47,7 +52,7 @@ def h(a):
 
 def run_reference(name, workdir):
     ov, full, nrank = RUNS[name]
-    spec = models.workload(name, **ov)
+    spec = models.workload(BASE.get(name, name), **ov)
     models.write_inputs(spec, workdir)
     subprocess.check_call(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
     exe = os.path.join(ROOT, "oracle", "_ref", "nrg_ref")

@@ -1,8 +1,10 @@
 """CFS spectrum phase of the GPU path (stock -> rho sweep -> spikes -> bins)
-against the reference run of the SIAM workload (Nkeep=256, 40 iterations,
-spectra on): binned spectral weights A(omega) of both types and signs, the
-sum rule, Tr rho per shell, and the gauge-invariant content of every
-reduced-density block (trace and spectrum)."""
+against reference runs with spectra on: the SIAM workload (Nkeep=256, 40
+iterations, two spectral types) and the two-orbital Kanamori model (Nkeep=300,
+20 iterations, four spectral types, spin-triplet ground state).  Compared:
+binned spectral weights A(omega) of both types and signs, the sum rule,
+Tr rho per shell, and the gauge-invariant content of every reduced-density
+block (trace and spectrum)."""
 import os
 
 import numpy as np
@@ -15,14 +17,21 @@ GOLD = os.path.join(os.path.dirname(__file__), "golden")
 BIN_TOL = 1e-8      # absolute, relative to the largest bin weight
 RHO_TOL = 1e-10     # absolute, on rho entries (eigenvalues <= 1)
 
+CASES = {
+    # fixture -> (workload, overrides)
+    "siam_nk256_it40": ("siam_nk256_it40", {}),
+    "two_orbital_nk300_it20_spec": ("two_orbital_nk4000_it45", dict(nkeep=300, niter=20, spectrum=True)),
+}
 
-def test_siam_nk256_cfs_spectrum(tmp_path):
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_cfs_spectrum(tmp_path, name):
     from nrgamd import gpu
     from nrgamd.host import Host
-    name = "siam_nk256_it40"
     fx = np.load(os.path.join(GOLD, f"ref_{name}.npz"))
     d = str(tmp_path / name)
-    models.write_inputs(models.workload(name), d)
+    base, ov = CASES[name]
+    models.write_inputs(models.workload(base, **ov), d)
     h = Host(d, device=0)
     try:
         print(f"[{name}] forward", flush=True)
