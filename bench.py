@@ -10,7 +10,11 @@ Fortran host (nrg-mpi_amd/lib/libnrg_host.so) through the HIP library.
 Warmup iterations run the growth phase (-1, 0, 1, ...); the timed steps are
 steady-state truncating iterations (~64k states, ~90 blocks up to ~4700).
 
-Prints one JSON line (see DESIGN.md for the fields).
+Prints one JSON line: metric, value (iter/s), ms_per_step, config, the
+per-stage seconds, `roofline` of the dominant kernel family (algorithmic
+bytes or flops over its HIP-event time, against 8 TB/s HBM or 78.6 TFLOP/s
+FP64 MFMA) and `cpu_baseline` (LAPACK dsyev on one host core over a sample
+of one iteration's blocks).
 """
 import argparse
 import json
@@ -84,9 +88,9 @@ def main():
     a = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if a.gpus != 1 or world != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} (WORLD_SIZE={world}) is not "
-              f"supported: the NRG hot path is benchmarked on a single MI355X (multi-GPU sharding is "
-              f"design-only, see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py: runs on one GPU only; --gpus {a.gpus} (WORLD_SIZE={world}) is not supported: "
+              f"the NRG hot path is measured on a single MI355X and block sharding across GPUs is not "
+              f"implemented", file=sys.stderr)
         sys.exit(2)
     import numpy as np
     from nrgamd import gpu, models

@@ -39,7 +39,7 @@ RUNS = {
     # steady-state iteration on the reference CPU path is ~minutes
     "two_orbital_nk4000_it45": (dict(niter=6), -100, 8),
     # two-orbital model with spectra at a size the reference finishes in
-    # minutes: four spectral types, orbitally degenerate ground states
+    # seconds: four spectral types, spin-triplet ground state
     "two_orbital_nk300_it20_spec": (dict(nkeep=300, niter=20, spectrum=True), -100, 8),
 }
 # fixture name -> workload it is derived from (default: the same name)
