@@ -41,6 +41,8 @@ def parse():
     ap.add_argument("--no-profile", action="store_true", help="skip the HIP-event kernel timing")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-plain", dest="plain", action="store_false",
+                    help="skip the one-iteration plain (one block per pass) comparison")
     return ap.parse_args()
 
 
@@ -130,6 +132,19 @@ def main():
     ks = gpu.kernel_stats() if not a.no_profile else {}
     if not a.no_profile:
         gpu.profile(False)
+    plain = None
+    if a.plain and its[-1] + 1 <= spec.last_iteration:
+        # the same path with one block per eigensolver pass (the per-block
+        # schedule of the reference's dsyev loop), one iteration
+        gpu.load().nrg_gpu_set_batch_limit(1)
+        tp = time.perf_counter()
+        h.iterate(its[-1] + 1)
+        tp = time.perf_counter() - tp
+        gpu.load().nrg_gpu_set_batch_limit(0)
+        st = gpu.stage_times()
+        plain = {"value": 1.0 / tp, "unit": "iter/s", "iteration": its[-1] + 1,
+                 "eig_s": st["eig"] * 1e-3, "form": "plain FP64: one block per eigensolver pass"}
+        print(f"plain iteration {its[-1] + 1}: {tp:.3f} s", file=sys.stderr, flush=True)
     sizes = h.subspace()[:, h.i["nv"]].tolist()
     nfull, nsub, nmax = h.i["nfull"], h.i["nsub"], int(max(sizes))
     h.close()
@@ -156,6 +171,8 @@ def main():
                    "sort": a.sort, "form": "batched FP64 (plain precision)"},
         "stages_s_per_step": {k: v / a.steps for k, v in stages.items()},
         "eig_alg_tflops": flops_eig / max(stages.get("gpu_eig", 1e-9), 1e-9) / 1e12,
+        "plain_fp64": plain,
+        "batched_over_plain": (value / plain["value"]) if plain else None,
     }
     if ks:
         fam = max(ks, key=lambda f: ks[f]["ms"])

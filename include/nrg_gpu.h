@@ -172,6 +172,12 @@ int nrg_gpu_dgemm(int transa, int transb, int m, int n, int k, double alpha,
 int nrg_gpu_profile(int enable);
 int nrg_gpu_kernel_stats(double* out, int nfam);
 
+/* Scheduling option with no counterpart in the reference: solve the blocks
+ * of an iteration in groups of at most max_blocks (0 = all in one batched
+ * pass, the default; 1 = one block at a time, the per-block schedule of the
+ * reference's dsyev loop, used as the plain-FP64 comparison). */
+int nrg_gpu_set_batch_limit(int max_blocks);
+
 /* Test hook with no counterpart in the reference: make the next
  * nrg_gpu_diagonalize of `iteration` fail with status 5 (error-path tests). */
 int nrg_gpu_inject_fault(int iteration);

@@ -241,8 +241,32 @@ int nrg_gpu_diagonalize(int iteration, int nsub, const int32_t* subspace_info, i
     NRG_HIP(hipStreamSynchronize(C.stream));
     double t1 = wall_ms();
     C.timers.assemble = t1 - t0;
-    C.eig.solve(nsub, C.bn.data(), C.boff.data(), C.bvoff.data(), C.d_H.p, C.d_U.p, C.d_w.p, nullptr,
-                C.stream, &C.eig_stats);
+    if (C.batch_limit <= 0 || C.batch_limit >= nsub) {
+      C.eig.solve(nsub, C.bn.data(), C.boff.data(), C.bvoff.data(), C.d_H.p, C.d_U.p, C.d_w.p, nullptr,
+                  C.stream, &C.eig_stats);
+    } else {
+      // groups of at most batch_limit blocks, one solve each (batch_limit 1:
+      // one block at a time, the reference's per-block dsyev schedule)
+      EigStats sum;
+      for (int b0 = 0; b0 < nsub; b0 += C.batch_limit) {
+        const int nb = std::min(C.batch_limit, nsub - b0);
+        // offsets relative to the group's first block (the solver sizes its
+        // workspaces by the group)
+        std::vector<int64_t> ao(nb), vo(nb);
+        for (int q = 0; q < nb; ++q) {
+          ao[q] = C.boff[b0 + q] - C.boff[b0];
+          vo[q] = C.bvoff[b0 + q] - C.bvoff[b0];
+        }
+        EigStats st;
+        C.eig.solve(nb, C.bn.data() + b0, ao.data(), vo.data(), C.d_H.p + C.boff[b0], C.d_U.p + C.boff[b0],
+                    C.d_w.p + C.bvoff[b0], nullptr, C.stream, &st);
+        sum.ms_tridiag += st.ms_tridiag; sum.ms_dc += st.ms_dc; sum.ms_back += st.ms_back;
+        sum.flops_alg += st.flops_alg;
+        sum.max_n = std::max(sum.max_n, st.max_n);
+        sum.nblk += st.nblk;
+      }
+      C.eig_stats = sum;
+    }
     NRG_HIP(hipStreamSynchronize(C.stream));
     C.timers.eig = wall_ms() - t1;
     C.timers.tridiag = C.eig_stats.ms_tridiag;
@@ -443,6 +467,13 @@ int nrg_gpu_dgemm(int transa, int transb, int m, int n, int k, double alpha, con
     dC.download(C_, nc, C.stream);
     NRG_HIP(hipStreamSynchronize(C.stream));
     dA.release(); dB.release(); dC.release();
+  });
+}
+
+int nrg_gpu_set_batch_limit(int max_blocks) {
+  return guarded([&](Context& C) {
+    NRG_REQUIRE(max_blocks >= 0, "set_batch_limit: negative limit");
+    C.batch_limit = max_blocks;
   });
 }
 

@@ -23,6 +23,7 @@ struct Context {
   GemmRunner gemm;
   Timers timers;
   int fault_iteration = -1000000;   // test hook: fail diagonalisation at this iteration
+  int batch_limit = 0;              // max blocks per eigensolver launch sequence (0: all)
 
   // ---- model constants (set once)
   int nv = 0, nops = 0, nchain = 0, ncm = 0, ncoef = 0, nmat = 0;

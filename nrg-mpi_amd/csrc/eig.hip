@@ -12,13 +12,14 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 
 namespace nrg {
 
 constexpr int NB = 32;     // tridiagonalisation panel width
-constexpr int RA = 256;    // rows per workgroup, column-update kernel
-constexpr int RB = 32;     // rows (output entries) per workgroup, SYMV kernel
-constexpr int PSTRIDE = 2 * NB + 1;  // partial-sum record per SYMV chunk
+constexpr int RA = 64;     // rows per workgroup, column-update kernel
+constexpr int CG = 16;     // narrowest SYMV panel (16 waves x 1 column); sizes the partial buffers
+constexpr int USTRIDE = 2 * NB + 1;  // column-kernel record: V^T x, W^T x, norm
 constexpr int LEAF = 32;   // divide-and-conquer leaf size
 constexpr int DC_LDS_M = 4000;  // merges up to this size run the deflation scan in LDS
 constexpr int NBB = 128;   // back-transform panel width (Z is streamed once per panel)
@@ -35,81 +36,129 @@ __device__ inline double wave_sum(double v) {
 // Per-block panel buffer P (n x 4NB, column-major, ld n):
 //   [0,NB) V, [NB,2NB) W, [2NB,3NB) W copy, [3NB,4NB) V copy
 // so that the trailing update is one product [V W] * [W V]^T with K = 2NB.
+//
+// One column c takes two launches.  The column kernel (64 rows per
+// workgroup) finishes w of column c-1, updates column c with the panel so
+// far and records, per workgroup, the partial norm of the column below the
+// subdiagonal and the partial products V^T x, W^T x with that column x.  The
+// SYMV kernel forms the reflector v (v_0 = 1, v_i = x_i * scal), its
+// workgroup 0 completes u1 = V^T v and u2 = W^T v, and all workgroups
+// compute y = A v of the trailing block from its lower triangle only:
+// y_i = sum_{j<=i} A(i,j) v_j + sum_{j>i} A(j,i) v_j.  Workgroup p takes the
+// PW columns [PW p, PW p + PW) (PW = 16 or 64: 16 waves, 1 or 4 columns each),
+// streams each column's lower part
+// contiguously, keeps the column dot in registers and adds the row parts
+// A(i,j) v_j into an LDS vector with LDS atomics; that vector is written as
+// py[p][i] (rows i >= PW p) and y_i = sum_{p <= i/PW} py[p][i] is completed
+// in the next column kernel (16 waves share the sum, fixed order), as is
+// y.v from one partial per SYMV workgroup.  Every reduction has a fixed
+// order: results are reproducible bit for bit.
 struct TrArgs {
   const int32_t* n;
   const int32_t* order;
   const int64_t* aoff;
   const int64_t* voff;
-  const int64_t* poff;     // [0,nblk): P offsets, [nblk,2nblk): partial offsets
+  const int64_t* poff;     // [0,nblk): P, [nblk,2nblk): records, [2nblk,3nblk): py
   double* A;
   double* d;
   double* e;
   double* tau;
-  double* y;
+  double* py;              // per block: ceil(n/CG) partial vectors of length n
   double* P;
-  double* part;
+  double* part;            // per block: column records, SYMV records, u1/u2
   int nblk;
 };
 
-__device__ inline int chunksB(int n, int c) { return (n - c - 1 + RB - 1) / RB; }
-__device__ inline int64_t part_norm_base(int n) { return (int64_t)((n + RB - 1) / RB + 1) * PSTRIDE; }
+// record layout inside a block's `part` region
+__host__ __device__ inline int64_t colrec_size(int n) { return (int64_t)((n + RA - 1) / RA + 1) * USTRIDE; }
+__host__ __device__ inline int64_t symrec_size(int n) { return (int64_t)(n + CG - 1) / CG + 1; }
+__host__ __device__ inline int64_t part_size(int n) { return colrec_size(n) + symrec_size(n) + 2 * NB; }
 
-// Column kernel (one workgroup per RA rows of one block):
+// Column kernel.  Workgroup = RA = 64 rows (r = c + 64 blockIdx.x + lane),
+// 16 waves:
 //   jp >= 0: finalise w of column c-1 (panel slot jp) for rows >= c;
+//   always:  copy the Householder vector of column c-1 into A (rows >= c+1);
 //   jj >= 0: update column c with the jj previous panel columns, write it
-//            back, record d[c] and the partial norm of rows >= c+2;
-//   always:  copy the Householder vector of column c-1 into A (rows >= c+1).
-__global__ __launch_bounds__(256) void k_tr_col(TrArgs a, int c, int jp, int jj, int vslot_prev) {
+//            back, record d[c], the partial norm of rows >= c+2 and the
+//            partial products sum V[r,q] x_r, sum W[r,q] x_r (q < jj).
+__global__ __launch_bounds__(1024) void k_tr_col(TrArgs a, int c, int jp, int jj, int vslot_prev, int pw) {
   const int b = a.order[blockIdx.y];
   const int n = a.n[b];
   if (c >= n) return;
-  if (c + (int)blockIdx.x * RA >= n) return;   // no rows of this block here
+  const int r0 = c + (int)blockIdx.x * RA;
+  if (r0 >= n) return;   // no rows of this block here
   const int64_t ld = n;
   double* A = a.A + a.aoff[b];
   double* P = a.P + a.poff[b];
-  double* part = a.part + a.poff[a.nblk + b];
-  const double* y = a.y + a.voff[b];
+  double* colrec = a.part + a.poff[a.nblk + b];
+  const double* symrec = colrec + colrec_size(n);
+  const double* ubuf = symrec + symrec_size(n);
+  const double* py = a.py + a.poff[2 * a.nblk + b];
   const int tid = threadIdx.x;
-  const int r = c + blockIdx.x * RA + tid;
+  const int lane = tid & 63, wv = tid >> 6;
+  const int r = r0 + lane;
   __shared__ double su1[NB], su2[NB], sVc[NB], sWc[NB];
-  __shared__ double s_alpha, s_wc, s_red[4];
+  __shared__ double s_y[16][64];
+  __shared__ double s_red[16];
+  __shared__ double s_w[64], s_x[64];
+  __shared__ double s_alpha, s_wc;
 
   double w_r = 0.0;
   if (jp >= 0) {
     const int p = c - 1;
     const double tp = a.tau[a.voff[b] + p];
-    const int nch = chunksB(n, p);
-    // sum the SYMV partials of column p: one wave per slot, lanes over
-    // chunks, fixed reduction tree (deterministic)
+    const int npan = (n - c + pw - 1) / pw;   // SYMV workgroups of column p (m = n - c)
+    // y of row r: the 16 waves split the partial vectors py[q][r-c], q <= (r-c)/pw
     {
-      const int lane = tid & 63, wv = tid >> 6;
-      for (int q = wv; q < 2 * jp + 1; q += 4) {
-        const int slot = (q < jp) ? q : (q < 2 * jp ? NB + (q - jp) : 2 * NB);
-        double s = 0.0;
-        for (int ch = lane; ch < nch; ch += 64) s += part[(int64_t)ch * PSTRIDE + slot];
-        s = wave_sum(s);
-        if (lane == 0) {
-          if (q < jp) su1[q] = s;
-          else if (q < 2 * jp) su2[q - jp] = s;
-          else s_alpha = s;  // y.v for now
+      double t0 = 0.0, t1 = 0.0, t2 = 0.0, t3 = 0.0;
+      if (r < n) {
+        const int i = r - c;
+        const int nq = i / pw + 1;
+        const double* col = py + i;
+        int q = wv;
+        for (; q + 48 < nq; q += 64) {
+          t0 += col[(int64_t)q * n];
+          t1 += col[(int64_t)(q + 16) * n];
+          t2 += col[(int64_t)(q + 32) * n];
+          t3 += col[(int64_t)(q + 48) * n];
         }
+        for (; q < nq; q += 16) t0 += col[(int64_t)q * n];
+      }
+      s_y[wv][lane] = (t0 + t1) + (t2 + t3);
+    }
+    // y.v: one partial per SYMV workgroup
+    {
+      double s = 0.0;
+      for (int q = tid; q < npan; q += 1024) s += symrec[q];
+      s = wave_sum(s);
+      if (lane == 0) s_red[wv] = s;
+    }
+    // u1 = V^T v, u2 = W^T v of column p (completed by the SYMV kernel)
+    if (tid < jp) su1[tid] = ubuf[tid];
+    else if (tid >= 64 && tid - 64 < jp) su2[tid - 64] = ubuf[NB + tid - 64];
+    __syncthreads();
+    if (wv == 0) {
+      // w_pre.v = tau (y.v - 2 u1.u2);  alpha = -tau/2 * w_pre.v
+      // and w at row c (every workgroup needs it for the column update)
+      double uu = 0.0, cc = 0.0;
+      if (lane < jp) {
+        uu = su1[lane] * su2[lane];
+        cc = P[c + (NB + lane) * ld] * su1[lane] + P[c + lane * ld] * su2[lane];
+      }
+      uu = wave_sum(uu);
+      cc = wave_sum(cc);
+      if (lane == 0) {
+        double yv = 0.0;
+        for (int w = 0; w < 16; ++w) yv += s_red[w];
+        const double al = -0.5 * tp * tp * (yv - 2.0 * uu);
+        s_wc = tp * (py[0] - cc) + al * P[c + jp * ld];
+        s_alpha = al;
       }
     }
     __syncthreads();
-    if (tid == 0) {
-      double uu = 0.0;
-      for (int i = 0; i < jp; ++i) uu += su1[i] * su2[i];
-      // w_pre.v = tau (y.v - 2 u1.u2);  alpha = -tau/2 * w_pre.v
-      s_alpha = -0.5 * tp * tp * (s_alpha - 2.0 * uu);
-      // w at row c (needed by every workgroup for the column update)
-      double acc = y[c];
-      for (int i = 0; i < jp; ++i)
-        acc -= P[c + (NB + i) * ld] * su1[i] + P[c + i * ld] * su2[i];
-      s_wc = tp * acc + s_alpha * P[c + jp * ld];
-    }
-    __syncthreads();
-    if (r < n) {
-      double acc = y[r];
+    if (wv == 0 && r < n) {
+      double acc = 0.0;
+      for (int w = 0; w < 16; ++w) acc += s_y[w][lane];
       for (int i = 0; i < jp; ++i)
         acc -= P[r + (NB + i) * ld] * su1[i] + P[r + i * ld] * su2[i];
       w_r = tp * acc + s_alpha * P[r + jp * ld];
@@ -118,59 +167,81 @@ __global__ __launch_bounds__(256) void k_tr_col(TrArgs a, int c, int jp, int jj,
     }
   }
   // Householder vector of column c-1 into A (rows >= c+1)
-  if (c >= 1 && c - 1 <= n - 2 && r >= c + 1 && r < n) A[r + (c - 1) * ld] = P[r + vslot_prev * ld];
+  if (wv == 0 && c >= 1 && c - 1 <= n - 2 && r >= c + 1 && r < n) A[r + (c - 1) * ld] = P[r + vslot_prev * ld];
 
   if (jj >= 0) {
-    for (int i = tid; i < jj; i += 256) {
-      sVc[i] = P[c + i * ld];
-      sWc[i] = (jp >= 0 && i == jp) ? s_wc : P[c + (NB + i) * ld];
+    if (tid < jj) {
+      sVc[tid] = P[c + tid * ld];
+      sWc[tid] = (jp >= 0 && tid == jp) ? s_wc : P[c + (NB + tid) * ld];
     }
     __syncthreads();
-    double col = 0.0;
-    if (r < n) {
-      col = A[r + c * ld];
-      for (int i = 0; i < jj; ++i) {
-        const double Wri = (jp >= 0 && i == jp) ? w_r : P[r + (NB + i) * ld];
-        col -= P[r + i * ld] * sWc[i] + Wri * sVc[i];
+    if (wv == 0) {
+      double col = 0.0;
+      if (r < n) {
+        col = A[r + c * ld];
+        for (int i = 0; i < jj; ++i) {
+          const double Wri = (jp >= 0 && i == jp) ? w_r : P[r + (NB + i) * ld];
+          col -= P[r + i * ld] * sWc[i] + Wri * sVc[i];
+        }
+        A[r + c * ld] = col;
+        if (r == c) a.d[a.voff[b] + c] = col;
       }
-      A[r + c * ld] = col;
-      if (r == c) a.d[a.voff[b] + c] = col;
+      const double x = (r < n && r >= c + 2) ? col : 0.0;
+      s_x[lane] = x;
+      s_w[lane] = w_r;
+      const double sq = wave_sum(x * x);
+      if (lane == 0) colrec[(int64_t)blockIdx.x * USTRIDE + 2 * NB] = sq;
     }
-    double sq = (r < n && r >= c + 2) ? col * col : 0.0;
-    sq = wave_sum(sq);
-    if ((tid & 63) == 0) s_red[tid >> 6] = sq;
     __syncthreads();
-    if (tid == 0) {
-      double* pn = part + part_norm_base(n);
-      pn[blockIdx.x] = s_red[0] + s_red[1] + s_red[2] + s_red[3];
+    // partial V^T x (slots q < jj) and W^T x (slots NB + q): the waves take
+    // every 16th product, lanes over the 64 rows
+    for (int k = wv; k < 2 * jj; k += 16) {
+      double t = 0.0;
+      if (r < n && r >= c + 2) {
+        double vw;
+        if (k < jj) vw = P[r + k * ld];
+        else if (jp >= 0 && k - jj == jp) vw = s_w[lane];
+        else vw = P[r + (NB + (k - jj)) * ld];
+        t = vw * s_x[lane];
+      }
+      t = wave_sum(t);
+      if (lane == 0) colrec[(int64_t)blockIdx.x * USTRIDE + (k < jj ? k : NB + (k - jj))] = t;
     }
   }
 }
 
-// Reflector + SYMV kernel (one workgroup per RB rows of one block).
-__global__ __launch_bounds__(256) void k_tr_symv(TrArgs a, int c, int jj) {
-  extern __shared__ double vs[];   // v[j], j in [c+1, n)
+// Reflector + lower-triangle SYMV kernel, one workgroup (16 waves) per
+// PW = 16 CPW columns of the trailing block of one block.
+template <int CPW, int NW>
+__global__ __launch_bounds__(NW * 64) void k_tr_symv(TrArgs a, int c, int jj) {
+  constexpr int PW = CPW * NW;
+  extern __shared__ double ax[];   // row parts, rows [j0, m)
   const int b = a.order[blockIdx.y];
   const int n = a.n[b];
   if (c > n - 2) return;
-  const int row0 = c + 1 + blockIdx.x * RB;
-  if (row0 >= n) return;
+  const int m = n - c - 1;
+  const int pan = blockIdx.x;
+  const int j0 = pan * PW;
+  if (j0 >= m) return;
   const int64_t ld = n;
   const double* A = a.A + a.aoff[b];
   double* P = a.P + a.poff[b];
-  double* part = a.part + a.poff[a.nblk + b];
+  const double* colrec = a.part + a.poff[a.nblk + b];
+  double* symrec = a.part + a.poff[a.nblk + b] + colrec_size(n);
+  double* ubuf = symrec + symrec_size(n);
+  double* py = a.py + a.poff[2 * a.nblk + b] + (int64_t)pan * n;
   const int tid = threadIdx.x;
-  __shared__ double s_par[4];
-  __shared__ double ysum[4][RB];
-  __shared__ double yv_s[RB];
+  const int lane = tid & 63;
+  const int wv = tid >> 6;
+  __shared__ double s_red[NW];
+  __shared__ double s_u[NW][64];
 
   // reflector from the updated column c (identical in every workgroup)
-  const int ncA = (n - c + RA - 1) / RA;
-  const double* pn = part + part_norm_base(n);
-  double xn2 = 0.0;
+  const int ncol = (n - c + RA - 1) / RA;   // column-kernel records of column c
+  double xn2;
   {
     double sl = 0.0;
-    for (int q = (tid & 63); q < ncA; q += 64) sl += pn[q];
+    for (int q = lane; q < ncol; q += 64) sl += colrec[(int64_t)q * USTRIDE + 2 * NB];
     xn2 = wave_sum(sl);   // every wave forms the same sum
   }
   const double alpha = A[(c + 1) + c * ld];
@@ -182,72 +253,85 @@ __global__ __launch_bounds__(256) void k_tr_symv(TrArgs a, int c, int jj) {
     tau = (beta - alpha) / beta;
     scal = 1.0 / (alpha - beta);
   }
-  if (blockIdx.x == 0 && tid == 0) {
-    a.e[a.voff[b] + c] = beta;
-    a.tau[a.voff[b] + c] = tau;
-  }
-  const int m = n - c - 1;
-  for (int j = tid; j < m; j += 256) vs[j] = (j == 0) ? 1.0 : A[(c + 1 + j) + c * ld] * scal;
-  __syncthreads();
-  // y_r = sum_j A[j, r] v[j] (the trailing block is symmetric, so column r
-  // is read instead of row r: one wave streams one contiguous column with
-  // eight independent loads in flight per lane)
-  const int lane = tid & 63;
-  const int wv = tid >> 6;
-  const double* Ab = A + (c + 1);
-  for (int q = 0; q < RB / 4; ++q) {
-    const int rl = wv * (RB / 4) + q;
-    const int r = row0 + rl;
-    double acc = 0.0;
-    if (r < n) {
-      const double* col = Ab + (int64_t)r * ld;
-      double s0 = 0, s1 = 0, s2 = 0, s3 = 0, s4 = 0, s5 = 0, s6 = 0, s7 = 0;
-      int j = lane;
-      for (; j + 448 < m; j += 512) {
-        s0 += col[j] * vs[j];
-        s1 += col[j + 64] * vs[j + 64];
-        s2 += col[j + 128] * vs[j + 128];
-        s3 += col[j + 192] * vs[j + 192];
-        s4 += col[j + 256] * vs[j + 256];
-        s5 += col[j + 320] * vs[j + 320];
-        s6 += col[j + 384] * vs[j + 384];
-        s7 += col[j + 448] * vs[j + 448];
-      }
-      for (; j < m; j += 64) s0 += col[j] * vs[j];
-      acc = wave_sum(((s0 + s1) + (s2 + s3)) + ((s4 + s5) + (s6 + s7)));
+  // v_i = 1 (i = 0), A(c+1+i, c) * scal (i > 0): read from the column
+  const double* vcol = A + (c + 1) + (int64_t)c * ld;
+  auto vv = [&](int i) { return (i == 0) ? 1.0 : vcol[i] * scal; };
+  if (pan == 0) {
+    if (tid == 0) {
+      a.e[a.voff[b] + c] = beta;
+      a.tau[a.voff[b] + c] = tau;
     }
-    if (lane == 0) ysum[0][rl] = acc;
-  }
-  __syncthreads();
-  // own rows: y, v into the panel, chunk partials
-  //   u1_i = sum V[r,i] v_r, u2_i = sum W[r,i] v_r (i < jj), y.v
-  const int rl = tid & (RB - 1);
-  const int r = row0 + rl;
-  double yr = 0.0, vr = 0.0;
-  if (tid < RB && r < n) {
-    yr = ysum[0][rl];
-    vr = vs[r - c - 1];
-    a.y[a.voff[b] + r] = yr;
-    P[r + jj * ld] = vr;
-    P[r + (3 * NB + jj) * ld] = vr;
-    yv_s[rl] = yr * vr;
-  } else if (tid < RB) {
-    yv_s[rl] = 0.0;
-  }
-  __syncthreads();
-  double* pc = part + (int64_t)blockIdx.x * PSTRIDE;
-  for (int q = tid; q < 2 * jj + 1; q += 256) {
-    double val = 0.0;
-    if (q == 2 * jj) {
-      for (int x = 0; x < RB; ++x) val += yv_s[x];
-    } else {
-      const int col = (q < jj) ? q : NB + (q - jj);
-      for (int x = 0; x < RB && row0 + x < n; ++x) val += P[(row0 + x) + col * ld] * vs[row0 + x - c - 1];
+    // u1_q = V[c+1,q] + scal sum_{r>=c+2} V[r,q] x_r (same for W): the
+    // waves split the records, lane = slot, fixed order
+    double s = 0.0;
+    if (lane < 2 * jj) {
+      const int slot = (lane < jj) ? lane : NB + (lane - jj);
+      for (int q = wv; q < ncol; q += NW) s += colrec[(int64_t)q * USTRIDE + slot];
     }
-    const int slot = (q < jj) ? q : (q < 2 * jj ? NB + (q - jj) : 2 * NB);
-    pc[slot] = val;
+    s_u[wv][lane] = s;
   }
-  (void)s_par;
+  for (int i = j0 + tid; i < m; i += NW * 64) ax[i - j0] = 0.0;
+  __syncthreads();
+  if (pan == 0 && tid < 2 * jj) {
+    const int col = (tid < jj) ? tid : NB + (tid - jj);
+    double s = 0.0;
+    for (int w = 0; w < NW; ++w) s += s_u[w][tid];
+    ubuf[col] = P[(c + 1) + (int64_t)col * ld] + scal * s;
+  }
+
+  const double* At = A + (c + 1) + (int64_t)(c + 1) * ld;   // At[i + j*ld] = A(c+1+i, c+1+j)
+  for (int q = 0; q < CPW; ++q) {
+    const int j = j0 + wv * CPW + q;
+    if (j >= m) break;
+    const double* col = At + (int64_t)j * ld;
+    const double vj = vv(j);
+    // the hot loop uses vcol[i]*scal for v_i; only row 0 (column 0, lane 0)
+    // has v_0 = 1 instead, corrected after the loop
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int i = j + lane;
+    for (; i + 192 < m; i += 256) {
+      const double x0 = col[i], x1 = col[i + 64], x2 = col[i + 128], x3 = col[i + 192];
+      s0 += x0 * (vcol[i] * scal);
+      s1 += x1 * (vcol[i + 64] * scal);
+      s2 += x2 * (vcol[i + 128] * scal);
+      s3 += x3 * (vcol[i + 192] * scal);
+      if (i > j) atomicAdd(&ax[i - j0], x0 * vj);
+      atomicAdd(&ax[i + 64 - j0], x1 * vj);
+      atomicAdd(&ax[i + 128 - j0], x2 * vj);
+      atomicAdd(&ax[i + 192 - j0], x3 * vj);
+    }
+    for (; i < m; i += 64) {
+      const double x = col[i];
+      s0 += x * (vcol[i] * scal);
+      if (i > j) atomicAdd(&ax[i - j0], x * vj);
+    }
+    if (j == 0 && lane == 0) s0 += col[0] * (1.0 - vcol[0] * scal);
+    const double t = wave_sum((s0 + s1) + (s2 + s3));
+    if (lane == 0) atomicAdd(&ax[j - j0], t);
+  }
+  __syncthreads();
+  // row parts out, y.v partial, v into the panel buffer (panel rows)
+  double yv = 0.0;
+  for (int i = j0 + tid; i < m; i += NW * 64) {
+    const double t = ax[i - j0];
+    py[i] = t;
+    yv += t * vv(i);
+  }
+  if (tid < PW && j0 + tid < m) {
+    const int i = j0 + tid;
+    const int r = c + 1 + i;
+    const double vi = vv(i);
+    P[r + jj * ld] = vi;
+    P[r + (3 * NB + jj) * ld] = vi;
+  }
+  yv = wave_sum(yv);
+  if (lane == 0) s_red[wv] = yv;
+  __syncthreads();
+  if (tid == 0) {
+    double t = 0.0;
+    for (int w = 0; w < NW; ++w) t += s_red[w];
+    symrec[pan] = t;
+  }
 }
 
 // ===========================================================================
@@ -904,15 +988,17 @@ void BatchedEigensolver::solve(int nblk, const int* n, const int64_t* aoff, cons
 void BatchedEigensolver::tridiag(hipStream_t s) {
   const int nblk = nblk_;
   const int nmax = n_[order_[0]];
-  std::vector<int64_t> poff(2 * nblk);
-  int64_t pacc = 0;
+  std::vector<int64_t> poff(3 * nblk);
+  int64_t pacc = 0, yacc = 0;
   for (int b = 0; b < nblk; ++b) {
     poff[b] = voff_[b] * 4 * NB;
     poff[nblk + b] = pacc;
-    pacc += (int64_t)((n_[b] + RB - 1) / RB + 1) * PSTRIDE + (n_[b] + RA - 1) / RA + 1;
+    pacc += part_size(n_[b]);
+    poff[2 * nblk + b] = yacc;
+    yacc += (int64_t)((n_[b] + CG - 1) / CG) * n_[b];
   }
   d_poff_.upload(poff.data(), poff.size(), s);
-  d_y_.reserve(tot_n_);
+  d_y_.reserve(std::max<int64_t>(yacc, 1));
   d_P_.reserve(tot_n_ * 4 * NB);
   d_part_.reserve(pacc);
   TrArgs ta{d_n_.p, d_order_.p, d_aoff_.p, d_voff_.p, d_poff_.p, A_, d_d_.p, d_e_.p, d_tau_.p,
@@ -924,9 +1010,13 @@ void BatchedEigensolver::tridiag(hipStream_t s) {
     return lo;
   };
   const size_t shm = (size_t)std::max(nmax, 1) * sizeof(double);
+  int pw_prev = CG;   // panel width of the previous column's SYMV launch
   auto colbytes = [&](int c, int act) {
     double b = 0;
-    for (int q = 0; q < act; ++q) b += 8.0 * 4.0 * (n_[order_[q]] - c) * (1 + NB / 8);
+    for (int q = 0; q < act; ++q) {
+      const double m = n_[order_[q]] - c;
+      b += 8.0 * (4.0 * m * (1 + NB / 8) + m * m / (2.0 * pw_prev));   // column, panel rows, y partials
+    }
     return b;
   };
   NRG_REQUIRE(shm <= 64 * 1024, "eigensolver: block too large for the SYMV stage (n > 8192)");
@@ -939,7 +1029,7 @@ void BatchedEigensolver::tridiag(hipStream_t s) {
       // close the previous panel: finalise its last w, trailing SYR2K
       {
         const int pid = profiler().begin(s);
-        hipLaunchKernelGGL(k_tr_col, dim3(gA, actA), dim3(256), 0, s, ta, c, NB - 1, -1, NB - 1);
+        hipLaunchKernelGGL(k_tr_col, dim3(gA, actA), dim3(1024), 0, s, ta, c, NB - 1, -1, NB - 1, pw_prev);
         profiler().end(pid, s, KF_TR_COL, colbytes(c, actA), 0);
       }
       batch_.clear();
@@ -952,31 +1042,38 @@ void BatchedEigensolver::tridiag(hipStream_t s) {
                                poff[b] + c, nb_,                       // [V W] rows c..
                                poff[b] + c + (int64_t)2 * NB * nb_, nb_, // [W V] rows c..
                                aoff_[b] + c + (int64_t)c * nb_, nb_, -1.0, 1.0);
+        t.lower = 1;   // only the lower triangle is read from here on
         batch_.add(t);
       }
       gemm_.run(batch_, d_P_.p, d_P_.p, A_, nullptr, nullptr, nullptr, s, KF_GEMM_TRIDIAG);
       const int pid = profiler().begin(s);
-      hipLaunchKernelGGL(k_tr_col, dim3(gA, actA), dim3(256), 0, s, ta, c, -1, 0, NB - 1);
+      hipLaunchKernelGGL(k_tr_col, dim3(gA, actA), dim3(1024), 0, s, ta, c, -1, 0, NB - 1, pw_prev);
       profiler().end(pid, s, KF_TR_COL, colbytes(c, actA), 0);
     } else {
       const int jp = (jj >= 1) ? jj - 1 : -1;
       const int vslot = (c >= 1) ? (c - 1) % NB : 0;
       const int pid = profiler().begin(s);
-      hipLaunchKernelGGL(k_tr_col, dim3(gA, actA), dim3(256), 0, s, ta, c, jp, jj, vslot);
+      hipLaunchKernelGGL(k_tr_col, dim3(gA, actA), dim3(1024), 0, s, ta, c, jp, jj, vslot, pw_prev);
       profiler().end(pid, s, KF_TR_COL, colbytes(c, actA), 0);
     }
     const int actB = active(c + 1);
     if (actB > 0) {
-      const int gB = ceil_div(nmax - c - 1, RB);
-      // algorithmic traffic: the trailing (n-c-1)^2 block once, plus the vectors
+      // algorithmic traffic: the lower triangle of the symmetric trailing
+      // (n-c-1)^2 block once, plus the vectors
       double bytes = 0, flops = 0;
       for (int q = 0; q < actB; ++q) {
         const double m = n_[order_[q]] - c - 1;
-        bytes += 8.0 * (m * m + 6.0 * m);
+        bytes += 8.0 * (0.5 * m * (m + 1.0) + 6.0 * m);
         flops += 2.0 * m * m;
       }
+      // 16-column panels: 8 waves x 2 columns while the launch is
+      // bandwidth-bound, 16 waves x 1 column once it is latency-bound
+      const int pw = CG;
+      const int gB = ceil_div(nmax - c - 1, pw);
       const int pid = profiler().begin(s);
-      hipLaunchKernelGGL(k_tr_symv, dim3(gB, actB), dim3(256), shm, s, ta, c, jj);
+      if (bytes >= 64.0e6) hipLaunchKernelGGL((k_tr_symv<2, 8>), dim3(gB, actB), dim3(512), shm, s, ta, c, jj);
+      else hipLaunchKernelGGL((k_tr_symv<1, 16>), dim3(gB, actB), dim3(1024), shm, s, ta, c, jj);
+      pw_prev = pw;
       profiler().end(pid, s, KF_TR_SYMV, bytes, flops);
     }
   }

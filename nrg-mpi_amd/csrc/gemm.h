@@ -23,6 +23,8 @@ struct GemmTask {
   int64_t offGA, offGB;     // offsets into the gather arrays, -1 = identity
   int64_t offS;             // offset into the k-scale array, -1 = none
   double alpha, beta;
+  int lower;                // 1: square C, only tiles on or below the diagonal
+  int pad;
 };
 
 struct GemmBatch {
@@ -56,6 +58,7 @@ inline GemmTask gemm_task(int m, int n, int k, int transA, int transB,
   t.offA = offA; t.offB = offB; t.offC = offC;
   t.offGA = -1; t.offGB = -1; t.offS = -1;
   t.alpha = alpha; t.beta = beta;
+  t.lower = 0; t.pad = 0;
   return t;
 }
 

@@ -32,8 +32,18 @@ __global__ __launch_bounds__(256) void k_gemm_f64(
   const GemmTask t = tasks[lo];
   const int tiles_m = (t.m + TM - 1) / TM;
   const int local = bid - tile_start[lo];
-  const int m0 = (local % tiles_m) * TM;
-  const int n0 = (local / tiles_m) * TN;
+  int m0, n0;
+  if (t.lower) {
+    // tiles (ti, tj), tj <= ti, enumerated row by row: local = ti(ti+1)/2 + tj
+    int ti = (int)((sqrt(8.0 * (double)local + 1.0) - 1.0) * 0.5);
+    while ((int64_t)ti * (ti + 1) / 2 > local) --ti;
+    while ((int64_t)(ti + 1) * (ti + 2) / 2 <= local) ++ti;
+    m0 = ti * TM;
+    n0 = (local - ti * (ti + 1) / 2) * TN;
+  } else {
+    m0 = (local % tiles_m) * TM;
+    n0 = (local / tiles_m) * TN;
+  }
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
@@ -131,10 +141,13 @@ void GemmRunner::run(const GemmBatch& b, const double* A, const double* B, doubl
   for (int i = 0; i < nt; ++i) {
     const GemmTask& t = b.tasks[i];
     NRG_REQUIRE(t.m > 0 && t.n > 0 && t.k >= 0, "gemm: bad task shape");
-    flops += 2.0 * t.m * t.n * t.k;
-    bytes += 8.0 * ((double)t.m * t.k + (double)t.k * t.n + (t.beta != 0.0 ? 2.0 : 1.0) * t.m * t.n);
+    NRG_REQUIRE(!t.lower || t.m == t.n, "gemm: a lower-triangle task must be square");
+    const double half = t.lower ? 0.5 : 1.0;
+    flops += half * 2.0 * t.m * t.n * t.k;
+    bytes += 8.0 * ((double)t.m * t.k + (double)t.k * t.n + half * (t.beta != 0.0 ? 2.0 : 1.0) * t.m * t.n);
     h_tile_start[i] = (int32_t)total;
-    total += (long long)ceil_div(t.m, TM) * ceil_div(t.n, TN);
+    const long long tm = ceil_div(t.m, TM);
+    total += t.lower ? tm * (tm + 1) / 2 : tm * ceil_div(t.n, TN);
   }
   NRG_REQUIRE(total < (1LL << 31), "gemm: too many tiles");
   d_tasks.upload(b.tasks, s);

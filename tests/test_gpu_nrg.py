@@ -69,6 +69,16 @@ def test_two_orbital_nk4000_first_iterations(tmp_path):
     run_and_compare(tmp_path, "two_orbital_nk4000_it45", {"niter": 6})
 
 
+def test_one_block_per_pass_schedule(tmp_path):
+    # the plain schedule (one block per eigensolver pass) gives the same results
+    from nrgamd import gpu
+    gpu.load().nrg_gpu_set_batch_limit(1)
+    try:
+        run_and_compare(tmp_path, "siam_nk256_it40", {}, last=8)
+    finally:
+        gpu.load().nrg_gpu_set_batch_limit(0)
+
+
 def test_two_orbital_canonical_sort(tmp_path):
     # canonical tie order: identical gauge-invariant results
     run_and_compare(tmp_path, "two_orbital_nk4000_it45", {"niter": 6}, last=2, sort_mode=1)
