@@ -48,7 +48,7 @@ class BatchedEigensolver {
   int64_t tot_n_ = 0, tot_n2_ = 0;
   double *A_ = nullptr, *Z_ = nullptr, *w_ = nullptr;
 
-  DevBuf<int32_t> d_n_, d_order_;
+  DevBuf<int32_t> d_n_, d_order_, d_ns_;   // sizes, size order, sizes in that order
   DevBuf<int64_t> d_aoff_, d_voff_;
   DevBuf<double> d_d_, d_e_, d_tau_, d_y_, d_P_, d_part_;
   DevBuf<int64_t> d_poff_;

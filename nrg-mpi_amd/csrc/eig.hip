@@ -56,6 +56,7 @@ __device__ inline double wave_sum(double v) {
 struct TrArgs {
   const int32_t* n;
   const int32_t* order;
+  const int32_t* ns;       // n[order[q]]: sizes in decreasing order
   const int64_t* aoff;
   const int64_t* voff;
   const int64_t* poff;     // [0,nblk): P, [nblk,2nblk): records, [2nblk,3nblk): py
@@ -69,6 +70,34 @@ struct TrArgs {
   int nblk;
 };
 
+// Flattened launch over the active blocks (slots q < act of the size order):
+// slot q owns tiles(q) consecutive workgroups.  Maps workgroup L to its slot
+// and tile with a wave-level prefix scan; every wave does it (no barrier).
+template <class F>
+__device__ inline bool flat_map(int L, int act, F tiles, int& q_out, int& t_out) {
+  const int lane = threadIdx.x & 63;
+  int base = 0;
+  for (int q0 = 0; q0 < act; q0 += 64) {
+    const int q = q0 + lane;
+    const int t = (q < act) ? tiles(q) : 0;
+    int s = t;
+    for (int o = 1; o < 64; o <<= 1) {
+      const int u = __shfl_up(s, o, 64);
+      if (lane >= o) s += u;
+    }
+    const unsigned long long hit = __ballot(base + s > L);
+    if (hit) {
+      const int f = __ffsll((unsigned long long)hit) - 1;
+      const int sf = __shfl(s, f, 64), tf = __shfl(t, f, 64);
+      q_out = q0 + f;
+      t_out = L - (base + sf - tf);
+      return true;
+    }
+    base += __shfl(s, 63, 64);
+  }
+  return false;
+}
+
 // record layout inside a block's `part` region
 __host__ __device__ inline int64_t colrec_size(int n) { return (int64_t)((n + RA - 1) / RA + 1) * USTRIDE; }
 __host__ __device__ inline int64_t symrec_size(int n) { return (int64_t)(n + CG - 1) / CG + 1; }
@@ -81,12 +110,12 @@ __host__ __device__ inline int64_t part_size(int n) { return colrec_size(n) + sy
 //   jj >= 0: update column c with the jj previous panel columns, write it
 //            back, record d[c], the partial norm of rows >= c+2 and the
 //            partial products sum V[r,q] x_r, sum W[r,q] x_r (q < jj).
-__global__ __launch_bounds__(1024) void k_tr_col(TrArgs a, int c, int jp, int jj, int vslot_prev, int pw) {
-  const int b = a.order[blockIdx.y];
-  const int n = a.n[b];
-  if (c >= n) return;
-  const int r0 = c + (int)blockIdx.x * RA;
-  if (r0 >= n) return;   // no rows of this block here
+__global__ __launch_bounds__(1024) void k_tr_col(TrArgs a, int c, int jp, int jj, int vslot_prev, int pw, int act) {
+  int qs, tile;
+  if (!flat_map((int)blockIdx.x, act, [&](int q) { return (a.ns[q] - c + RA - 1) / RA; }, qs, tile)) return;
+  const int b = a.order[qs];
+  const int n = a.ns[qs];
+  const int r0 = c + tile * RA;
   const int64_t ld = n;
   double* A = a.A + a.aoff[b];
   double* P = a.P + a.poff[b];
@@ -97,92 +126,104 @@ __global__ __launch_bounds__(1024) void k_tr_col(TrArgs a, int c, int jp, int jj
   const int tid = threadIdx.x;
   const int lane = tid & 63, wv = tid >> 6;
   const int r = r0 + lane;
+  __shared__ double sV[NB][RA], sW[NB][RA];   // panel columns of this workgroup's rows
   __shared__ double su1[NB], su2[NB], sVc[NB], sWc[NB];
   __shared__ double s_y[16][64];
   __shared__ double s_red[16];
   __shared__ double s_w[64], s_x[64];
-  __shared__ double s_alpha, s_wc;
 
-  double w_r = 0.0;
+  // ---- every global read in one round: panel rows, row c, the SYMV
+  // partials of column c-1 (y, y.v, u1, u2), the column and the reflector
+  const int nvc = max(jp + 1, jj);          // V columns used (w reads V[:, jp])
+  const int nwc = (jp >= 0) ? jp : max(jj, 0);  // W columns read (W[:, jp] is formed here)
+#pragma unroll 1
+  for (int e = tid; e < nvc * RA; e += 1024) {
+    const int col = e / RA, row = e - col * RA, rr = r0 + row;
+    sV[col][row] = (rr < n) ? P[rr + (int64_t)col * ld] : 0.0;
+  }
+#pragma unroll 1
+  for (int e = tid; e < nwc * RA; e += 1024) {
+    const int col = e / RA, row = e - col * RA, rr = r0 + row;
+    sW[col][row] = (rr < n) ? P[rr + (int64_t)(NB + col) * ld] : 0.0;
+  }
+  if (tid < nvc) sVc[tid] = P[c + (int64_t)tid * ld];
+  else if (tid >= 64 && tid - 64 < nwc) sWc[tid - 64] = P[c + (int64_t)(NB + tid - 64) * ld];
+  const bool copy = (c >= 1 && c - 1 <= n - 2 && r >= c + 1 && r < n);
+  double a_rc = 0.0, vsl = 0.0, y_c = 0.0, tp = 0.0;
+  if (wv == 0) {
+    if (jj >= 0 && r < n) a_rc = A[r + c * ld];
+    if (copy) vsl = P[r + vslot_prev * ld];
+    if (jp >= 0) {
+      y_c = py[0];
+      tp = a.tau[a.voff[b] + c - 1];
+    }
+  }
   if (jp >= 0) {
-    const int p = c - 1;
-    const double tp = a.tau[a.voff[b] + p];
-    const int npan = (n - c + pw - 1) / pw;   // SYMV workgroups of column p (m = n - c)
+    const int npan = (n - c + pw - 1) / pw;   // SYMV workgroups of column c-1 (m = n - c)
     // y of row r: the 16 waves split the partial vectors py[q][r-c], q <= (r-c)/pw
-    {
-      double t0 = 0.0, t1 = 0.0, t2 = 0.0, t3 = 0.0;
-      if (r < n) {
-        const int i = r - c;
-        const int nq = i / pw + 1;
-        const double* col = py + i;
-        int q = wv;
-        for (; q + 48 < nq; q += 64) {
-          t0 += col[(int64_t)q * n];
-          t1 += col[(int64_t)(q + 16) * n];
-          t2 += col[(int64_t)(q + 32) * n];
-          t3 += col[(int64_t)(q + 48) * n];
-        }
-        for (; q < nq; q += 16) t0 += col[(int64_t)q * n];
+    double t0 = 0.0, t1 = 0.0, t2 = 0.0, t3 = 0.0;
+    if (r < n) {
+      const int i = r - c;
+      const int nq = i / pw + 1;
+      const double* col = py + i;
+      int q = wv;
+      for (; q + 48 < nq; q += 64) {
+        t0 += col[(int64_t)q * n];
+        t1 += col[(int64_t)(q + 16) * n];
+        t2 += col[(int64_t)(q + 32) * n];
+        t3 += col[(int64_t)(q + 48) * n];
       }
-      s_y[wv][lane] = (t0 + t1) + (t2 + t3);
+      for (; q < nq; q += 16) t0 += col[(int64_t)q * n];
     }
+    s_y[wv][lane] = (t0 + t1) + (t2 + t3);
     // y.v: one partial per SYMV workgroup
-    {
-      double s = 0.0;
-      for (int q = tid; q < npan; q += 1024) s += symrec[q];
-      s = wave_sum(s);
-      if (lane == 0) s_red[wv] = s;
-    }
-    // u1 = V^T v, u2 = W^T v of column p (completed by the SYMV kernel)
-    if (tid < jp) su1[tid] = ubuf[tid];
-    else if (tid >= 64 && tid - 64 < jp) su2[tid - 64] = ubuf[NB + tid - 64];
-    __syncthreads();
-    if (wv == 0) {
-      // w_pre.v = tau (y.v - 2 u1.u2);  alpha = -tau/2 * w_pre.v
-      // and w at row c (every workgroup needs it for the column update)
+    double s = 0.0;
+    for (int q = tid; q < npan; q += 1024) s += symrec[q];
+    s = wave_sum(s);
+    if (lane == 0) s_red[wv] = s;
+    // u1 = V^T v, u2 = W^T v of column c-1 (completed by the SYMV kernel)
+    if (tid >= 128 && tid - 128 < jp) su1[tid - 128] = ubuf[tid - 128];
+    else if (tid >= 192 && tid - 192 < jp) su2[tid - 192] = ubuf[NB + tid - 192];
+  }
+  __syncthreads();
+
+  // ---- wave 0: w of column c-1, the Householder vector, column c
+  if (wv == 0) {
+    double w_r = 0.0, wc = 0.0;
+    if (jp >= 0) {
+      // w_pre.v = tau (y.v - 2 u1.u2);  alpha = -tau/2 * w_pre.v; w at row c
       double uu = 0.0, cc = 0.0;
       if (lane < jp) {
         uu = su1[lane] * su2[lane];
-        cc = P[c + (NB + lane) * ld] * su1[lane] + P[c + lane * ld] * su2[lane];
+        cc = sWc[lane] * su1[lane] + sVc[lane] * su2[lane];
       }
       uu = wave_sum(uu);
       cc = wave_sum(cc);
-      if (lane == 0) {
-        double yv = 0.0;
-        for (int w = 0; w < 16; ++w) yv += s_red[w];
-        const double al = -0.5 * tp * tp * (yv - 2.0 * uu);
-        s_wc = tp * (py[0] - cc) + al * P[c + jp * ld];
-        s_alpha = al;
-      }
-    }
-    __syncthreads();
-    if (wv == 0 && r < n) {
+      double yv = 0.0;
+      for (int w = 0; w < 16; ++w) yv += s_red[w];
+      const double al = -0.5 * tp * tp * (yv - 2.0 * uu);
+      wc = tp * (y_c - cc) + al * sVc[jp];
       double acc = 0.0;
       for (int w = 0; w < 16; ++w) acc += s_y[w][lane];
-      for (int i = 0; i < jp; ++i)
-        acc -= P[r + (NB + i) * ld] * su1[i] + P[r + i * ld] * su2[i];
-      w_r = tp * acc + s_alpha * P[r + jp * ld];
-      P[r + (NB + jp) * ld] = w_r;
-      P[r + (2 * NB + jp) * ld] = w_r;
-    }
-  }
-  // Householder vector of column c-1 into A (rows >= c+1)
-  if (wv == 0 && c >= 1 && c - 1 <= n - 2 && r >= c + 1 && r < n) A[r + (c - 1) * ld] = P[r + vslot_prev * ld];
-
-  if (jj >= 0) {
-    if (tid < jj) {
-      sVc[tid] = P[c + tid * ld];
-      sWc[tid] = (jp >= 0 && tid == jp) ? s_wc : P[c + (NB + tid) * ld];
-    }
-    __syncthreads();
-    if (wv == 0) {
-      double col = 0.0;
+#pragma unroll 2
+      for (int i = 0; i < jp; ++i) acc -= sW[i][lane] * su1[i] + sV[i][lane] * su2[i];
+      w_r = tp * acc + al * sV[jp][lane];
       if (r < n) {
-        col = A[r + c * ld];
-        for (int i = 0; i < jj; ++i) {
-          const double Wri = (jp >= 0 && i == jp) ? w_r : P[r + (NB + i) * ld];
-          col -= P[r + i * ld] * sWc[i] + Wri * sVc[i];
-        }
+        P[r + (NB + jp) * ld] = w_r;
+        P[r + (2 * NB + jp) * ld] = w_r;
+      }
+    }
+    if (copy) A[r + (c - 1) * ld] = vsl;
+    if (jj >= 0) {
+      double col = a_rc;
+#pragma unroll 2
+      for (int i = 0; i < jj; ++i) {
+        const bool cur = (i == jp);
+        const double Wri = cur ? w_r : sW[i][lane];
+        const double Wci = cur ? wc : sWc[i];
+        col -= sV[i][lane] * Wci + Wri * sVc[i];
+      }
+      if (r < n) {
         A[r + c * ld] = col;
         if (r == c) a.d[a.voff[b] + c] = col;
       }
@@ -190,39 +231,36 @@ __global__ __launch_bounds__(1024) void k_tr_col(TrArgs a, int c, int jp, int jj
       s_x[lane] = x;
       s_w[lane] = w_r;
       const double sq = wave_sum(x * x);
-      if (lane == 0) colrec[(int64_t)blockIdx.x * USTRIDE + 2 * NB] = sq;
+      if (lane == 0) colrec[(int64_t)tile * USTRIDE + 2 * NB] = sq;
     }
-    __syncthreads();
-    // partial V^T x (slots q < jj) and W^T x (slots NB + q): the waves take
-    // every 16th product, lanes over the 64 rows
-    for (int k = wv; k < 2 * jj; k += 16) {
-      double t = 0.0;
-      if (r < n && r >= c + 2) {
-        double vw;
-        if (k < jj) vw = P[r + k * ld];
-        else if (jp >= 0 && k - jj == jp) vw = s_w[lane];
-        else vw = P[r + (NB + (k - jj)) * ld];
-        t = vw * s_x[lane];
-      }
-      t = wave_sum(t);
-      if (lane == 0) colrec[(int64_t)blockIdx.x * USTRIDE + (k < jj ? k : NB + (k - jj))] = t;
-    }
+  }
+  if (jj < 0) return;
+  __syncthreads();
+  // ---- partial V^T x (slots q < jj) and W^T x (slots NB + q): the waves
+  // take every 16th product, lanes over the 64 rows
+  for (int k = wv; k < 2 * jj; k += 16) {
+    double vw;
+    if (k < jj) vw = sV[k][lane];
+    else if (k - jj == jp) vw = s_w[lane];
+    else vw = sW[k - jj][lane];
+    double t = vw * s_x[lane];
+    t = wave_sum(t);
+    if (lane == 0) colrec[(int64_t)tile * USTRIDE + (k < jj ? k : NB + (k - jj))] = t;
   }
 }
 
 // Reflector + lower-triangle SYMV kernel, one workgroup (16 waves) per
 // PW = 16 CPW columns of the trailing block of one block.
 template <int CPW, int NW>
-__global__ __launch_bounds__(NW * 64) void k_tr_symv(TrArgs a, int c, int jj) {
+__global__ __launch_bounds__(NW * 64) void k_tr_symv(TrArgs a, int c, int jj, int act) {
   constexpr int PW = CPW * NW;
   extern __shared__ double ax[];   // row parts, rows [j0, m)
-  const int b = a.order[blockIdx.y];
-  const int n = a.n[b];
-  if (c > n - 2) return;
+  int qs, pan;
+  if (!flat_map((int)blockIdx.x, act, [&](int q) { return (a.ns[q] - c - 1 + PW - 1) / PW; }, qs, pan)) return;
+  const int b = a.order[qs];
+  const int n = a.ns[qs];
   const int m = n - c - 1;
-  const int pan = blockIdx.x;
   const int j0 = pan * PW;
-  if (j0 >= m) return;
   const int64_t ld = n;
   const double* A = a.A + a.aoff[b];
   double* P = a.P + a.poff[b];
@@ -933,7 +971,7 @@ static double now_ms() {
 }
 
 void BatchedEigensolver::release() {
-  d_n_.release(); d_order_.release(); d_aoff_.release(); d_voff_.release();
+  d_n_.release(); d_order_.release(); d_ns_.release(); d_aoff_.release(); d_voff_.release();
   d_d_.release(); d_e_.release(); d_tau_.release(); d_y_.release(); d_P_.release(); d_part_.release();
   d_poff_.release(); d_ws_.release(); d_iws_.release(); d_tasks_.release(); d_k_.release();
   d_V_.release(); d_T_.release(); d_X_.release(); d_X2_.release(); gemm_.release();
@@ -961,6 +999,11 @@ void BatchedEigensolver::solve(int nblk, const int* n, const int64_t* aoff, cons
   std::stable_sort(order_.begin(), order_.end(), [&](int x, int y) { return n_[x] > n_[y]; });
   d_n_.upload(n_.data(), nblk, s);
   d_order_.upload(order_.data(), nblk, s);
+  {
+    std::vector<int32_t> ns(nblk);
+    for (int q = 0; q < nblk; ++q) ns[q] = n_[order_[q]];
+    d_ns_.upload(ns.data(), nblk, s);
+  }
   d_aoff_.upload(aoff_.data(), nblk, s);
   d_voff_.upload(voff_.data(), nblk, s);
   d_d_.reserve(tot_n_);
@@ -1001,7 +1044,7 @@ void BatchedEigensolver::tridiag(hipStream_t s) {
   d_y_.reserve(std::max<int64_t>(yacc, 1));
   d_P_.reserve(tot_n_ * 4 * NB);
   d_part_.reserve(pacc);
-  TrArgs ta{d_n_.p, d_order_.p, d_aoff_.p, d_voff_.p, d_poff_.p, A_, d_d_.p, d_e_.p, d_tau_.p,
+  TrArgs ta{d_n_.p, d_order_.p, d_ns_.p, d_aoff_.p, d_voff_.p, d_poff_.p, A_, d_d_.p, d_e_.p, d_tau_.p,
             d_y_.p, d_P_.p, d_part_.p, nblk};
   // number of blocks with n > x (order_ is by n descending)
   auto active = [&](int x) {
@@ -1024,12 +1067,13 @@ void BatchedEigensolver::tridiag(hipStream_t s) {
     const int j0 = c - c % NB;
     const int jj = c - j0;
     const int actA = active(c);
-    const int gA = ceil_div(nmax - c, RA);
+    int gA = 0;   // flattened grid: the row tiles of the active blocks
+    for (int q = 0; q < actA; ++q) gA += ceil_div(n_[order_[q]] - c, RA);
     if (jj == 0 && c > 0) {
       // close the previous panel: finalise its last w, trailing SYR2K
       {
         const int pid = profiler().begin(s);
-        hipLaunchKernelGGL(k_tr_col, dim3(gA, actA), dim3(1024), 0, s, ta, c, NB - 1, -1, NB - 1, pw_prev);
+        hipLaunchKernelGGL(k_tr_col, dim3(gA), dim3(1024), 0, s, ta, c, NB - 1, -1, NB - 1, pw_prev, actA);
         profiler().end(pid, s, KF_TR_COL, colbytes(c, actA), 0);
       }
       batch_.clear();
@@ -1047,13 +1091,13 @@ void BatchedEigensolver::tridiag(hipStream_t s) {
       }
       gemm_.run(batch_, d_P_.p, d_P_.p, A_, nullptr, nullptr, nullptr, s, KF_GEMM_TRIDIAG);
       const int pid = profiler().begin(s);
-      hipLaunchKernelGGL(k_tr_col, dim3(gA, actA), dim3(1024), 0, s, ta, c, -1, 0, NB - 1, pw_prev);
+      hipLaunchKernelGGL(k_tr_col, dim3(gA), dim3(1024), 0, s, ta, c, -1, 0, NB - 1, pw_prev, actA);
       profiler().end(pid, s, KF_TR_COL, colbytes(c, actA), 0);
     } else {
       const int jp = (jj >= 1) ? jj - 1 : -1;
       const int vslot = (c >= 1) ? (c - 1) % NB : 0;
       const int pid = profiler().begin(s);
-      hipLaunchKernelGGL(k_tr_col, dim3(gA, actA), dim3(1024), 0, s, ta, c, jp, jj, vslot, pw_prev);
+      hipLaunchKernelGGL(k_tr_col, dim3(gA), dim3(1024), 0, s, ta, c, jp, jj, vslot, pw_prev, actA);
       profiler().end(pid, s, KF_TR_COL, colbytes(c, actA), 0);
     }
     const int actB = active(c + 1);
@@ -1069,10 +1113,11 @@ void BatchedEigensolver::tridiag(hipStream_t s) {
       // 16-column panels: 8 waves x 2 columns while the launch is
       // bandwidth-bound, 16 waves x 1 column once it is latency-bound
       const int pw = CG;
-      const int gB = ceil_div(nmax - c - 1, pw);
+      int gB = 0;
+      for (int q = 0; q < actB; ++q) gB += ceil_div(n_[order_[q]] - c - 1, pw);
       const int pid = profiler().begin(s);
-      if (bytes >= 64.0e6) hipLaunchKernelGGL((k_tr_symv<2, 8>), dim3(gB, actB), dim3(512), shm, s, ta, c, jj);
-      else hipLaunchKernelGGL((k_tr_symv<1, 16>), dim3(gB, actB), dim3(1024), shm, s, ta, c, jj);
+      if (bytes >= 64.0e6) hipLaunchKernelGGL((k_tr_symv<2, 8>), dim3(gB), dim3(512), shm, s, ta, c, jj, actB);
+      else hipLaunchKernelGGL((k_tr_symv<1, 16>), dim3(gB), dim3(1024), shm, s, ta, c, jj, actB);
       pw_prev = pw;
       profiler().end(pid, s, KF_TR_SYMV, bytes, flops);
     }

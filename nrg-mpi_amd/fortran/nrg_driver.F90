@@ -102,6 +102,31 @@ contains
     nrg_host_truncate = nkept
   end function nrg_host_truncate
 
+  ! The basis sort on caller keys (CPU tests of the exact tie order):
+  ! mode 0 the O(N log N) replay, 1 the swap-by-swap exchange sort; perm(n)
+  ! receives the 1-based source position of each sorted element.
+  integer(c_int) function nrg_host_sort_keys(n, key, perm, mode) bind(C, name="nrg_host_sort_keys")
+    integer(c_int), value :: n, mode
+    integer(c_int64_t), intent(inout) :: key(*)
+    integer(c_int), intent(out) :: perm(*)
+    integer(8), allocatable :: k(:)
+    integer, allocatable :: p(:)
+    integer :: i
+    allocate(k(n), p(n))
+    k = key(1:n)
+    do i = 1, n
+       p(i) = i
+    end do
+    if (mode == 0) then
+       call exchange_sort_lazy(int(n), k, p)
+    else
+       call exchange_sort(int(n), k, p)
+    end if
+    key(1:n) = k
+    perm(1:n) = p
+    nrg_host_sort_keys = 0
+  end function nrg_host_sort_keys
+
   ! info(1:16): iteration, nsub, nfull, nkept, start_trunc, nold, i_first,
   ! n_iter, flag_spectrum, nv, nops, ncm, nmat, nkeep, flag_trunc, nchain
   integer(c_int) function nrg_host_info(info) bind(C, name="nrg_host_info")

@@ -283,125 +283,117 @@ contains
     end do
   end subroutine exchange_sort
 
-  ! The same exchange sort, fused and pipelined.  Step i of the outer loop
-  ! only reads and writes positions >= i, and once it has passed position m it
-  ! never touches m again.  Hence (1) `nbat` consecutive steps can be carried
-  ! through one pass over the array, each position being compared against
-  ! the running holders of steps s0, s0+1, ... in order (a position that is
-  ! itself a step index becomes that step's first holder), and (2) batch b+1
-  ! may process position m as soon as batch b is beyond it.  Batches are dealt
-  ! round-robin to OpenMP threads, each publishing its progress every `chunk`
-  ! positions.  The result is bit-identical to the serial loop.
-  subroutine exchange_sort_parallel(n, key, perm)
-    use omp_lib
+  ! The same exchange sort in O(N log N).  Fuse all steps: element m passes
+  ! the holders of steps 1..m-1 in order (swapping where it is smaller) and
+  ! becomes the holder of step m.  The holders stay sorted, so a passing
+  ! element x of key v lands right after the holders of key v, and each swap
+  ! hands the first holder of the next key group to the slot of the first
+  ! holder of the group after it: every group of keys above v rotates its
+  ! first member to its end, and the carried element (first member of the
+  ! top group) becomes the new last holder, which rotates the top group too.
+  ! Groups are kept per distinct key (rank); the rotations of all groups
+  ! above v are one range update on a Fenwick tree over ranks, applied to a
+  ! group only when it is next touched.  The final order is the groups in
+  ! key order, each in its rotated order: identical to the swap-by-swap
+  ! loop above, tie order included.
+  subroutine exchange_sort_lazy(n, key, perm)
     integer, intent(in) :: n
     integer(8), intent(inout) :: key(n)
     integer, intent(inout) :: perm(n)
-    integer, parameter :: chunk = 4096
-    integer, parameter :: nbat = 512
-    integer, parameter :: pad = 16      ! one progress counter per 128-byte line
-    integer(8), allocatable :: prog(:)
-    integer :: nthr, nbatch
-    ! the fused passes are cheap; beyond a few threads the hand-off of
-    ! progress dominates
-    nthr = min(omp_get_max_threads(), 4)
-    if (n < 4 * chunk) then
-       call exchange_sort(n, key, perm)
-       return
-    end if
-    nbatch = (n - 1 + nbat - 1) / nbat
-    allocate(prog(0:pad * nthr - 1))
-    prog = -1_8
-    !$omp parallel num_threads(nthr) default(shared)
-    call pipeline_worker(omp_get_thread_num(), omp_get_num_threads())
-    !$omp end parallel
-    deallocate(prog)
+    integer(8), allocatable :: sk(:), wk(:), k2(:)
+    integer, allocatable :: idx(:), wp(:), rnk(:), gofs(:), glen(:), gbase(:), fw(:), gel(:), order(:), p2(:)
+    integer :: i, r, nr, top, m, rot, L, j, p, q, t
+    if (n <= 1) return
+    allocate(sk(n), wk(n), idx(n), wp(n), rnk(n))
+    sk = key
+    do i = 1, n
+       idx(i) = i
+    end do
+    call merge_sort_i8(n, sk, idx, wk, wp)
+    nr = 0
+    do i = 1, n
+       if (i == 1) then
+          nr = 1
+       else if (sk(i) /= sk(i - 1)) then
+          nr = nr + 1
+       end if
+       rnk(idx(i)) = nr
+    end do
+    deallocate(sk, wk, idx, wp)
+    ! group storage: members of rank r at gel(gofs(r)+1 : gofs(r)+glen(r))
+    allocate(gofs(nr + 1), glen(nr), gbase(nr), fw(nr), gel(n), order(n))
+    gofs = 0
+    do i = 1, n
+       gofs(rnk(i) + 1) = gofs(rnk(i) + 1) + 1
+    end do
+    do r = 2, nr + 1
+       gofs(r) = gofs(r) + gofs(r - 1)
+    end do
+    glen = 0
+    gbase = 0
+    fw = 0
+    top = 0
+    do m = 1, n
+       r = rnk(m)
+       q = fen_query(r)
+       L = glen(r)
+       if (L > 1) then
+          rot = mod(q - gbase(r), L)
+          if (rot /= 0) call rotate_left(gel(gofs(r) + 1:gofs(r) + L), L, rot)
+       end if
+       gbase(r) = q
+       glen(r) = L + 1
+       gel(gofs(r) + L + 1) = m
+       if (r < top) call fen_add(r + 1)
+       if (r > top) top = r
+    end do
+    p = 0
+    do r = 1, nr
+       L = glen(r)
+       rot = mod(fen_query(r) - gbase(r), L)
+       do j = 0, L - 1
+          p = p + 1
+          order(p) = gel(gofs(r) + 1 + mod(j + rot, L))
+       end do
+    end do
+    allocate(k2(n), p2(n))
+    do i = 1, n
+       k2(i) = key(order(i))
+       p2(i) = perm(order(i))
+    end do
+    key = k2
+    perm = p2
   contains
-    subroutine pipeline_worker(t, nt)
-      integer, intent(in) :: t, nt
-      integer :: b, s0, s1, ns, m, mend, s, pred, pm, tp, nh, lo, hi
-      integer(8) :: km, need, seen, stride, tk
-      integer(8) :: cur(nbat)
-      integer :: pcur(nbat)
-      stride = int(n, 8) + 2_8
-      pred = mod(t - 1 + nt, nt)
-      b = t
-      do while (b < nbatch)
-         s0 = b * nbat + 1
-         s1 = min(s0 + nbat - 1, n - 1)
-         ns = s1 - s0 + 1
-         m = s0
-         do while (m <= n)
-            mend = min(n, m + chunk - 1)
-            if (b > 0) then
-               ! batch b-1 must be beyond mend
-               need = int(b - 1, 8) * stride + int(mend, 8)
-               do
-                  !$omp atomic read seq_cst
-                  seen = prog(pad * pred)
-                  if (seen >= need) exit
-               end do
-            end if
-            do while (m <= mend)
-               km = key(m)
-               ! the holders cur(1:nh) stay sorted ascending (each step keeps
-               ! the smaller of what it holds and what passes), so a key not
-               ! below the last holder passes every step unchanged; otherwise
-               ! the swaps start at the first holder strictly above it
-               nh = min(ns, m - s0)
-               if (nh > 0) then
-                  if (km < cur(nh)) then
-                     pm = perm(m)
-                     lo = 1
-                     hi = nh
-                     do while (lo < hi)
-                        s = (lo + hi) / 2
-                        if (cur(s) > km) then
-                           hi = s
-                        else
-                           lo = s + 1
-                        end if
-                     end do
-                     do s = lo, nh
-                        if (km < cur(s)) then
-                           tk = cur(s); cur(s) = km; km = tk
-                           tp = pcur(s); pcur(s) = pm; pm = tp
-                        end if
-                     end do
-                     if (m > s1) then
-                        key(m) = km
-                        perm(m) = pm
-                     end if
-                  else if (m <= s1) then
-                     pm = perm(m)
-                  end if
-               else
-                  pm = perm(m)
-               end if
-               if (m <= s1) then
-                  ! position m is step m of this batch: it becomes that step's
-                  ! first holder (after the earlier steps of the batch passed it)
-                  cur(m - s0 + 1) = km
-                  pcur(m - s0 + 1) = pm
-               end if
-               m = m + 1
-            end do
-            if (mend == n) then
-               do s = 1, ns
-                  key(s0 + s - 1) = cur(s)
-                  perm(s0 + s - 1) = pcur(s)
-               end do
-            end if
-            !$omp atomic write seq_cst
-            prog(pad * t) = int(b, 8) * stride + int(mend, 8)
-         end do
-         b = b + nt
+    ! +1 on every rank >= i (difference form)
+    subroutine fen_add(i0)
+      integer, intent(in) :: i0
+      integer :: ii
+      ii = i0
+      do while (ii <= nr)
+         fw(ii) = fw(ii) + 1
+         ii = ii + iand(ii, -ii)
       end do
-      ! a thread whose batches are exhausted still lets its successor pass
-      !$omp atomic write seq_cst
-      prog(pad * t) = int(nbatch, 8) * stride + int(n, 8)
-    end subroutine pipeline_worker
-  end subroutine exchange_sort_parallel
+    end subroutine fen_add
+    ! rotations applied so far to rank i0
+    integer function fen_query(i0)
+      integer, intent(in) :: i0
+      integer :: ii
+      fen_query = 0
+      ii = i0
+      do while (ii > 0)
+         fen_query = fen_query + fw(ii)
+         ii = ii - iand(ii, -ii)
+      end do
+    end function fen_query
+    subroutine rotate_left(a, la, k)
+      integer, intent(in) :: la, k
+      integer, intent(inout) :: a(la)
+      integer :: tmp(la)
+      tmp = a
+      a(1:la - k) = tmp(k + 1:la)
+      a(la - k + 1:la) = tmp(1:k)
+    end subroutine rotate_left
+  end subroutine exchange_sort_lazy
 
   ! Stable merge sort of (key, perm) by key (the permutation of an insertion
   ! sort with strict comparisons, module.F90:16-44).
@@ -539,7 +531,7 @@ contains
        return
     end if
     if (sort_mode == 0) then
-       call exchange_sort_parallel(nfull, key, perm)
+       call exchange_sort_lazy(nfull, key, perm)
     else
        allocate(wk(nfull), wp(nfull))
        call merge_sort_i8(nfull, key, perm, wk, wp)

@@ -39,7 +39,18 @@ SIGNATURES = {
     "nrg_host_get_bins": (ctypes.c_int, [_f64p, _f64p, ctypes.c_int]),
     "nrg_host_get_rho_traces": (ctypes.c_int, [_f64p, ctypes.c_int]),
     "nrg_host_get_spectrum_times": (ctypes.c_int, [_f64p]),
+    "nrg_host_sort_keys": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_int64), _i32p, ctypes.c_int]),
 }
+
+
+def sort_keys(keys, mode=0):
+    """The host's basis sort on int64 keys: (sorted keys, 0-based source
+    positions).  mode 0: O(N log N) replay, 1: swap-by-swap exchange sort."""
+    k = np.ascontiguousarray(keys, dtype=np.int64).copy()
+    p = np.zeros(len(k), dtype=np.int32)
+    load().nrg_host_sort_keys(len(k), k.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), p.ctypes.data_as(_i32p),
+                              mode)
+    return k, p.astype(np.int64) - 1
 
 
 class HostError(RuntimeError):

@@ -107,3 +107,20 @@ def test_truncation_matches_reference(tmp_path):
         np.testing.assert_array_equal(h.kept_per_block(), fx[f"it{it}_kept_blk"])
         np.testing.assert_array_equal(h.eigenvalues(), E)
         assert sha(h.basis_output()) == str(fx[f"it{it}_sha_basis_output"])
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_lazy_sort_replays_exchange_sort(seed):
+    """The O(N log N) replay of the reference's exchange sort gives the same
+    permutation, tie order included, as the swap-by-swap loop and as the
+    oracle's independent restatement, on keys with many ties."""
+    import nrg_oracle as O
+    from nrgamd import host
+    rng = np.random.default_rng(seed)
+    n = int(rng.integers(2, 400))
+    keys = rng.integers(0, max(2, n // 3), n).astype(np.int64)
+    k0, p0 = host.sort_keys(keys, mode=0)
+    k1, p1 = host.sort_keys(keys, mode=1)
+    np.testing.assert_array_equal(k0, np.sort(keys))
+    np.testing.assert_array_equal(p0, p1)
+    np.testing.assert_array_equal(p0, O.exchange_sort_permutation(keys))
