@@ -39,6 +39,9 @@ def parse():
     ap.add_argument("--workload", default="two_orbital_nk4000_it45")
     ap.add_argument("--sort", choices=["exact", "canonical"], default="exact")
     ap.add_argument("--no-profile", action="store_true", help="skip the HIP-event kernel timing")
+    ap.add_argument("--profile-families", default="tridiag_symv",
+                    help="comma-separated kernel families whose launches carry HIP timing events in the "
+                         "timed steps (default: the dominant one, tridiag_symv; 'all' for every family)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-plain", dest="plain", action="store_false",
@@ -111,7 +114,8 @@ def main():
         print(f"warmup iteration {it}: nfull={h.i['nfull']} nsub={h.i['nsub']} kept={h.i['nkept']}",
               file=sys.stderr, flush=True)
     if not a.no_profile:
-        gpu.profile(True)
+        fams = None if a.profile_families == "all" else [f for f in a.profile_families.split(",") if f]
+        gpu.profile(True, families=fams)
     stages = {}
     flops_eig = 0.0
     t0 = time.perf_counter()

@@ -159,8 +159,11 @@ int nrg_gpu_dgemm(int transa, int transb, int m, int n, int k, double alpha,
                   double* C, int ldc);
 
 /* Per-kernel-family HIP-event timing (no counterpart in the reference; the
- * reference's count.f90 timers measure host wall clock only).  enable != 0
- * clears and starts accumulation; nrg_gpu_kernel_stats writes, for family
+ * reference's count.f90 timers measure host wall clock only).  enable = 0
+ * stops, -1 times every family, any other value is a bitmask of the
+ * families to time (bit f = family f); each call clears the accumulators.
+ * A timed launch carries its start/stop events in the dispatch itself
+ * (hipExtLaunchKernel).  nrg_gpu_kernel_stats writes, for family
  * f < nfam, out[4f..4f+3] = {milliseconds, launches, algorithmic bytes,
  * algorithmic flops}.  Families: 0 H assembly, 1 tridiagonal column update,
  * 2 tridiagonal SYMV, 3 tridiagonal trailing GEMM, 4 D&C leaves, 5 D&C merge

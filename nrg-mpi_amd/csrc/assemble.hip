@@ -107,9 +107,7 @@ void assemble(Context& C, const double* chain_diag, const double* chain_nondiag,
   a.H = C.d_H.p;
   double bytes = 0;
   for (int b = 0; b < nblk; ++b) bytes += 8.0 * (double)C.bn[b] * C.bn[b];
-  const int pid = profiler().begin(C.stream);
-  hipLaunchKernelGGL(k_assemble, dim3((unsigned)tot), dim3(256), 0, C.stream, a);
-  profiler().end(pid, C.stream, KF_ASSEMBLE, bytes, 0);
+  plaunch(KF_ASSEMBLE, bytes, 0, k_assemble, dim3((unsigned)tot), dim3(256), 0, C.stream, a);
   NRG_HIP(hipGetLastError());
 }
 

@@ -487,6 +487,7 @@ int nrg_gpu_profile(int enable) {
     profiler().drain();
     profiler().reset();
     profiler().on = (enable != 0);
+    profiler().mask = (enable == -1) ? 0xffffffffu : (uint32_t)enable;
   });
 }
 

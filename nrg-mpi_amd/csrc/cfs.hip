@@ -428,10 +428,9 @@ double rho_shell(Context& C, int iteration) {
     F.gidx.upload(gidx, C.stream);
     DevBuf<GatherTask> dt;
     dt.upload(gt, C.stream);
-    const int pid = profiler().begin(C.stream);
-    hipLaunchKernelGGL(k_gather_rows, dim3(grid_for(gtot)), dim3(256), 0, C.stream, dt.p, (int)gt.size(), gtot,
-                       p->U.p, F.gidx.p, F.ybuf.p);
-    profiler().end(pid, C.stream, KF_RHO_GATHER, 16.0 * gtot, 0.0);
+    plaunch(KF_RHO_GATHER, 16.0 * gtot, 0.0, k_gather_rows, dim3(grid_for(gtot)), dim3(256), 0, C.stream,
+            (const GatherTask*)dt.p, (int)gt.size(), gtot, (const double*)p->U.p, (const int32_t*)F.gidx.p,
+            F.ybuf.p);
     NRG_HIP(hipGetLastError());
     // Z (xbuf) = Vs (ybuf) rho_P ; rho_B = Z Vs^T
     C.gemm.run(b1, F.ybuf.p, p->rho.p, F.xbuf.p, nullptr, nullptr, nullptr, C.stream, KF_GEMM_RHO);
@@ -507,10 +506,9 @@ void spikes(Context& C, int iteration, double scale, int zero_temperature) {
   }
   DevBuf<SpikeDev> d;
   d.upload(sd, C.stream);
-  const int pid = profiler().begin(C.stream);
-  hipLaunchKernelGGL(k_bin_spikes, dim3(grid_for(tot)), dim3(256), 0, C.stream, d.p, (int)sd.size(), tot, h->W.p,
-                     zt ? h->W.p : F.xbuf.p, h->Ed.p, F.bins.p, F.nbin);
-  profiler().end(pid, C.stream, KF_CFS_BIN, (zt ? 24.0 : 32.0) * tot, 0.0);
+  plaunch(KF_CFS_BIN, (zt ? 24.0 : 32.0) * tot, 0.0, k_bin_spikes, dim3(grid_for(tot)), dim3(256), 0, C.stream,
+          (const SpikeDev*)d.p, (int)sd.size(), tot, (const double*)h->W.p,
+          (const double*)(zt ? h->W.p : F.xbuf.p), (const double*)h->Ed.p, F.bins.p, F.nbin);
   NRG_HIP(hipGetLastError());
   NRG_HIP(hipStreamSynchronize(C.stream));
   d.release();

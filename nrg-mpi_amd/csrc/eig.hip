@@ -1072,9 +1072,8 @@ void BatchedEigensolver::tridiag(hipStream_t s) {
     if (jj == 0 && c > 0) {
       // close the previous panel: finalise its last w, trailing SYR2K
       {
-        const int pid = profiler().begin(s);
-        hipLaunchKernelGGL(k_tr_col, dim3(gA), dim3(1024), 0, s, ta, c, NB - 1, -1, NB - 1, pw_prev, actA);
-        profiler().end(pid, s, KF_TR_COL, colbytes(c, actA), 0);
+        plaunch(KF_TR_COL, colbytes(c, actA), 0, k_tr_col, dim3(gA), dim3(1024), 0, s, ta, c, NB - 1, -1, NB - 1,
+                pw_prev, actA);
       }
       batch_.clear();
       for (int q = 0; q < actA; ++q) {
@@ -1090,15 +1089,13 @@ void BatchedEigensolver::tridiag(hipStream_t s) {
         batch_.add(t);
       }
       gemm_.run(batch_, d_P_.p, d_P_.p, A_, nullptr, nullptr, nullptr, s, KF_GEMM_TRIDIAG);
-      const int pid = profiler().begin(s);
-      hipLaunchKernelGGL(k_tr_col, dim3(gA), dim3(1024), 0, s, ta, c, -1, 0, NB - 1, pw_prev, actA);
-      profiler().end(pid, s, KF_TR_COL, colbytes(c, actA), 0);
+      plaunch(KF_TR_COL, colbytes(c, actA), 0, k_tr_col, dim3(gA), dim3(1024), 0, s, ta, c, -1, 0, NB - 1, pw_prev,
+              actA);
     } else {
       const int jp = (jj >= 1) ? jj - 1 : -1;
       const int vslot = (c >= 1) ? (c - 1) % NB : 0;
-      const int pid = profiler().begin(s);
-      hipLaunchKernelGGL(k_tr_col, dim3(gA), dim3(1024), 0, s, ta, c, jp, jj, vslot, pw_prev, actA);
-      profiler().end(pid, s, KF_TR_COL, colbytes(c, actA), 0);
+      plaunch(KF_TR_COL, colbytes(c, actA), 0, k_tr_col, dim3(gA), dim3(1024), 0, s, ta, c, jp, jj, vslot, pw_prev,
+              actA);
     }
     const int actB = active(c + 1);
     if (actB > 0) {
@@ -1115,11 +1112,11 @@ void BatchedEigensolver::tridiag(hipStream_t s) {
       const int pw = CG;
       int gB = 0;
       for (int q = 0; q < actB; ++q) gB += ceil_div(n_[order_[q]] - c - 1, pw);
-      const int pid = profiler().begin(s);
-      if (bytes >= 64.0e6) hipLaunchKernelGGL((k_tr_symv<2, 8>), dim3(gB), dim3(512), shm, s, ta, c, jj, actB);
-      else hipLaunchKernelGGL((k_tr_symv<1, 16>), dim3(gB), dim3(1024), shm, s, ta, c, jj, actB);
+      if (bytes >= 64.0e6)
+        plaunch(KF_TR_SYMV, bytes, flops, k_tr_symv<2, 8>, dim3(gB), dim3(512), (uint32_t)shm, s, ta, c, jj, actB);
+      else
+        plaunch(KF_TR_SYMV, bytes, flops, k_tr_symv<1, 16>, dim3(gB), dim3(1024), (uint32_t)shm, s, ta, c, jj, actB);
       pw_prev = pw;
-      profiler().end(pid, s, KF_TR_SYMV, bytes, flops);
     }
   }
   NRG_HIP(hipGetLastError());
@@ -1188,9 +1185,7 @@ void BatchedEigensolver::divide_conquer(hipStream_t s) {
     DevBuf<int32_t>& tb = d_tasks_;
     tb.upload(leaves.data(), leaves.size(), s);
     const int nt = (int)leaves.size() / 3;
-    const int pid = profiler().begin(s);
-    hipLaunchKernelGGL(k_dc_leaf, dim3(nt), dim3(64), 0, s, da, tb.p, nt);
-    profiler().end(pid, s, KF_DC_LEAF, 0, 0);
+    plaunch(KF_DC_LEAF, 0, 0, k_dc_leaf, dim3(nt), dim3(64), 0, s, da, (const int32_t*)tb.p, nt);
   }
   for (int lev = 1; lev <= maxL; ++lev) {
     auto& rec = levels[lev - 1];
@@ -1202,11 +1197,10 @@ void BatchedEigensolver::divide_conquer(hipStream_t s) {
     int mmax = 0;
     for (int i = 0; i < nt; ++i) mmax = std::max(mmax, rec[8 * i + 3] - rec[8 * i + 1]);
     {
-      const int pid = profiler().begin(s);
       const size_t shm = 2 * sizeof(double) * (size_t)std::min(mmax, DC_LDS_M);
-      hipLaunchKernelGGL(k_dc_merge_prep, dim3(nt), dim3(256), shm, s, da, d_tasks_.p, nt);
-      hipLaunchKernelGGL(k_dc_rotgather, dim3(ceil_div(mmax, 64), nt), dim3(256), 0, s, da, d_tasks_.p, nt);
-      profiler().end(pid, s, KF_DC_PREP, 0, 0);
+      const int32_t* tk = d_tasks_.p;
+      plaunch(KF_DC_PREP, 0, 0, k_dc_merge_prep, dim3(nt), dim3(256), (uint32_t)shm, s, da, tk, nt);
+      plaunch(KF_DC_PREP, 0, 0, k_dc_rotgather, dim3(ceil_div(mmax, 64), nt), dim3(256), 0, s, da, tk, nt);
     }
     h_k_.resize(nt);
     d_k_.download(h_k_.data(), nt, s);
@@ -1214,15 +1208,12 @@ void BatchedEigensolver::divide_conquer(hipStream_t s) {
     int kmax = 0;
     for (int i = 0; i < nt; ++i) kmax = std::max(kmax, h_k_[i]);
     if (kmax > 0) {
-      int pid = profiler().begin(s);
-      hipLaunchKernelGGL(k_dc_secular, dim3(ceil_div(kmax, 64), nt), dim3(64),
-                         2 * sizeof(double) * (size_t)std::min(kmax, DC_LDS_M), s, da, d_tasks_.p,
-                         (const int64_t*)nullptr, nt);
-      profiler().end(pid, s, KF_DC_SECULAR, 0, 0);
-      pid = profiler().begin(s);
-      hipLaunchKernelGGL(k_dc_zhat, dim3(ceil_div(kmax, 64), nt), dim3(64), 0, s, da, d_tasks_.p, nt);
-      hipLaunchKernelGGL(k_dc_vectors, dim3(kmax, nt), dim3(64), 0, s, da, d_tasks_.p, nt);
-      profiler().end(pid, s, KF_DC_VECTORS, 0, 0);
+      const int32_t* tk = d_tasks_.p;
+      plaunch(KF_DC_SECULAR, 0, 0, k_dc_secular, dim3(ceil_div(kmax, 64), nt), dim3(64),
+              (uint32_t)(2 * sizeof(double) * (size_t)std::min(kmax, DC_LDS_M)), s, da, tk,
+              (const int64_t*)nullptr, nt);
+      plaunch(KF_DC_VECTORS, 0, 0, k_dc_zhat, dim3(ceil_div(kmax, 64), nt), dim3(64), 0, s, da, tk, nt);
+      plaunch(KF_DC_VECTORS, 0, 0, k_dc_vectors, dim3(kmax, nt), dim3(64), 0, s, da, tk, nt);
       batch_.clear();
       for (int i = 0; i < nt; ++i) {
         const int k = h_k_[i];
@@ -1235,10 +1226,9 @@ void BatchedEigensolver::divide_conquer(hipStream_t s) {
       }
       gemm_.run(batch_, d_ws_.p, d_ws_.p, d_ws_.p, nullptr, nullptr, nullptr, s, KF_GEMM_DC);
     }
-    const int pid = profiler().begin(s);
-    hipLaunchKernelGGL(k_dc_finish, dim3(nt), dim3(256), 0, s, da, d_tasks_.p, nt);
-    hipLaunchKernelGGL(k_dc_scatter, dim3(ceil_div(mmax, 64), nt), dim3(256), 0, s, da, d_tasks_.p, nt);
-    profiler().end(pid, s, KF_DC_FINISH, 0, 0);
+    const int32_t* tk = d_tasks_.p;
+    plaunch(KF_DC_FINISH, 0, 0, k_dc_finish, dim3(nt), dim3(256), 0, s, da, tk, nt);
+    plaunch(KF_DC_FINISH, 0, 0, k_dc_scatter, dim3(ceil_div(mmax, 64), nt), dim3(256), 0, s, da, tk, nt);
   }
   NRG_HIP(hipGetLastError());
 }

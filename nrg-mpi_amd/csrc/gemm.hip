@@ -130,6 +130,145 @@ __global__ __launch_bounds__(256) void k_gemm_f64(
       }
 }
 
+// ---------------------------------------------------------------------------
+// Fast path for products without gathers or k-scales (the eigensolver's
+// trailing updates, merges and back-transform): the task table is searched
+// in LDS, the K loop runs in 32-deep slices and the next slice's operands
+// are fetched into registers while the matrix cores work on the current one.
+constexpr int TK2 = 32;
+constexpr int MAXT_LDS = 4096;   // tasks whose tile starts are searched in LDS
+
+__device__ inline int find_task(const int32_t* __restrict__ tile_start, int ntasks, int bid, int32_t* s_ts) {
+  if (ntasks <= MAXT_LDS) {
+    for (int i = threadIdx.x; i < ntasks; i += 256) s_ts[i] = tile_start[i];
+    __syncthreads();
+    int lo = 0, hi = ntasks - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (s_ts[mid] <= bid) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+  }
+  int lo = 0, hi = ntasks - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (tile_start[mid] <= bid) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+template <int TA, int TB>
+__global__ __launch_bounds__(256) void k_gemm_f64_fast(
+    const GemmTask* __restrict__ tasks, const int32_t* __restrict__ tile_start, int ntasks,
+    const double* __restrict__ A, const double* __restrict__ B, double* __restrict__ C) {
+  __shared__ double As[TK2 * LDS_STRIDE];
+  __shared__ double Bs[TK2 * LDS_STRIDE];
+  __shared__ int32_t s_ts[MAXT_LDS];
+  const int bid = blockIdx.x;
+  const int ti = find_task(tile_start, ntasks, bid, s_ts);
+  const GemmTask t = tasks[ti];
+  const int tiles_m = (t.m + TM - 1) / TM;
+  const int local = bid - tile_start[ti];
+  int m0, n0;
+  if (t.lower) {
+    int tr = (int)((sqrt(8.0 * (double)local + 1.0) - 1.0) * 0.5);
+    while ((int64_t)tr * (tr + 1) / 2 > local) --tr;
+    while ((int64_t)(tr + 1) * (tr + 2) / 2 <= local) ++tr;
+    m0 = tr * TM;
+    n0 = (local - tr * (tr + 1) / 2) * TN;
+  } else {
+    m0 = (local % tiles_m) * TM;
+    n0 = (local / tiles_m) * TN;
+  }
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+  const double* Ab = A + t.offA;
+  const double* Bb = B + t.offB;
+
+  // element e = tid + 256 u of the 64 x 32 slice of opA / 32 x 64 of opB
+  // (index order chosen so a wave reads contiguous stored memory)
+  double ra[8], rb[8];
+  auto fetch = [&](int k0) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int e = tid + 256 * u;
+      int i, kk;
+      if (!TA) { i = e & 63; kk = e >> 6; } else { kk = e & 31; i = e >> 5; }
+      const int gi = m0 + i, gk = k0 + kk;
+      double v = 0.0;
+      if (gi < t.m && gk < t.k)
+        v = !TA ? Ab[(int64_t)gi + (int64_t)gk * t.lda] : Ab[(int64_t)gk + (int64_t)gi * t.lda];
+      ra[u] = v;
+      int j, kb;
+      if (!TB) { kb = e & 31; j = e >> 5; } else { j = e & 63; kb = e >> 6; }
+      const int gj = n0 + j, gk2 = k0 + kb;
+      double w = 0.0;
+      if (gj < t.n && gk2 < t.k)
+        w = !TB ? Bb[(int64_t)gk2 + (int64_t)gj * t.ldb] : Bb[(int64_t)gj + (int64_t)gk2 * t.ldb];
+      rb[u] = w;
+    }
+  };
+  auto store = [&]() {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int e = tid + 256 * u;
+      int i, kk;
+      if (!TA) { i = e & 63; kk = e >> 6; } else { kk = e & 31; i = e >> 5; }
+      As[kk * LDS_STRIDE + i] = ra[u];
+      int j, kb;
+      if (!TB) { kb = e & 31; j = e >> 5; } else { j = e & 63; kb = e >> 6; }
+      Bs[kb * LDS_STRIDE + j] = rb[u];
+    }
+  };
+
+  d4 acc[2][2];
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
+
+  if (t.k > 0) {
+    fetch(0);
+    store();
+    __syncthreads();
+  }
+  for (int k0 = 0; k0 < t.k; k0 += TK2) {
+    const bool more = (k0 + TK2 < t.k);
+    if (more) fetch(k0 + TK2);   // in flight while this slice is multiplied
+#pragma unroll
+    for (int s = 0; s < TK2 / 4; ++s) {
+      const int kk = s * 4 + (lane >> 4);
+      const int li = lane & 15;
+      const double a0 = As[kk * LDS_STRIDE + wm * 32 + li];
+      const double a1 = As[kk * LDS_STRIDE + wm * 32 + 16 + li];
+      const double b0 = Bs[kk * LDS_STRIDE + wn * 32 + li];
+      const double b1 = Bs[kk * LDS_STRIDE + wn * 32 + 16 + li];
+      acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+      acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+      acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+      acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+    }
+    if (more) {
+      __syncthreads();
+      store();
+      __syncthreads();
+    }
+  }
+  double* Cb = C + t.offC;
+  for (int mi = 0; mi < 2; ++mi)
+    for (int ni = 0; ni < 2; ++ni)
+      for (int r = 0; r < 4; ++r) {
+        const int gi = m0 + wm * 32 + mi * 16 + (lane >> 4) + 4 * r;
+        const int gj = n0 + wn * 32 + ni * 16 + (lane & 15);
+        if (gi < t.m && gj < t.n) {
+          double* c = Cb + (int64_t)gi + (int64_t)gj * t.ldc;
+          double v = t.alpha * acc[mi][ni][r];
+          if (t.beta != 0.0) v += t.beta * (*c);
+          *c = v;
+        }
+      }
+}
+
 void GemmRunner::run(const GemmBatch& b, const double* A, const double* B, double* C,
                      const int32_t* gatherA, const int32_t* gatherB, const double* kscale,
                      hipStream_t s, int family) {
@@ -152,10 +291,21 @@ void GemmRunner::run(const GemmBatch& b, const double* A, const double* B, doubl
   NRG_REQUIRE(total < (1LL << 31), "gemm: too many tiles");
   d_tasks.upload(b.tasks, s);
   d_tile_start.upload(h_tile_start, s);
-  const int pid = profiler().begin(s);
-  hipLaunchKernelGGL(k_gemm_f64, dim3((unsigned)total), dim3(256), 0, s, d_tasks.p,
-                     d_tile_start.p, nt, A, B, C, gatherA, gatherB, kscale);
-  profiler().end(pid, s, family, bytes, flops);
+  bool uniform = true;
+  for (int i = 1; i < nt && uniform; ++i)
+    uniform = (b.tasks[i].transA == b.tasks[0].transA && b.tasks[i].transB == b.tasks[0].transB);
+  const int tv = b.tasks[0].transA * 2 + b.tasks[0].transB;
+  const dim3 g((unsigned)total), blk(256);
+  const GemmTask* tk = d_tasks.p;
+  const int32_t* ts = d_tile_start.p;
+  if (!gatherA && !gatherB && !kscale && uniform) {
+    if (tv == 0) plaunch(family, bytes, flops, k_gemm_f64_fast<0, 0>, g, blk, 0, s, tk, ts, nt, A, B, C);
+    else if (tv == 1) plaunch(family, bytes, flops, k_gemm_f64_fast<0, 1>, g, blk, 0, s, tk, ts, nt, A, B, C);
+    else if (tv == 2) plaunch(family, bytes, flops, k_gemm_f64_fast<1, 0>, g, blk, 0, s, tk, ts, nt, A, B, C);
+    else plaunch(family, bytes, flops, k_gemm_f64_fast<1, 1>, g, blk, 0, s, tk, ts, nt, A, B, C);
+  } else {
+    plaunch(family, bytes, flops, k_gemm_f64, g, blk, 0, s, tk, ts, nt, A, B, C, gatherA, gatherB, kscale);
+  }
   NRG_HIP(hipGetLastError());
 }
 

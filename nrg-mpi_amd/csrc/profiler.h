@@ -1,10 +1,12 @@
-// Optional per-kernel-family timing with HIP events recorded on the
-// library's stream around each launch.  Off by default; when on, each API
-// call drains the recorded events at its end (after its own synchronisation)
-// and accumulates duration, launch count and the algorithmic bytes/flops the
-// launch sites declare.
+// Optional per-kernel-family timing.  A profiled launch carries its own
+// start/stop events in the dispatch (hipExtLaunchKernel), so timing adds no
+// separate event records to the stream.  Off by default; `mask` selects the
+// families.  Each API call drains the events at its end (after its own
+// synchronisation) and accumulates duration, launch count and the
+// algorithmic bytes/flops the launch sites declare.
 #pragma once
 #include "common.h"
+#include <hip/hip_ext.h>
 #include <array>
 
 namespace nrg {
@@ -40,6 +42,7 @@ struct KStat {
 
 struct Profiler {
   bool on = false;
+  uint32_t mask = 0xffffffffu;   // families timed while on
   std::array<KStat, KF_COUNT> st{};
   struct Pending { hipEvent_t a, b; int fam; double bytes, flops; };
   std::vector<Pending> pend;
@@ -51,20 +54,7 @@ struct Profiler {
     NRG_HIP(hipEventCreate(&e));
     return e;
   }
-  // returns an event pair index, or -1 when off
-  int begin(hipStream_t s) {
-    if (!on) return -1;
-    Pending p{get(), get(), 0, 0, 0};
-    NRG_HIP(hipEventRecord(p.a, s));
-    pend.push_back(p);
-    return (int)pend.size() - 1;
-  }
-  void end(int id, hipStream_t s, int fam, double bytes, double flops) {
-    if (id < 0) return;
-    Pending& p = pend[id];
-    p.fam = fam; p.bytes = bytes; p.flops = flops;
-    NRG_HIP(hipEventRecord(p.b, s));
-  }
+  bool timed(int fam) const { return on && ((mask >> fam) & 1u); }
   // call after the stream has been synchronised
   void drain() {
     for (auto& p : pend) {
@@ -81,5 +71,19 @@ struct Profiler {
 };
 
 Profiler& profiler();
+
+// Launch `kernel`, timed when the profiler is on for family `fam`.
+template <typename F, typename... Args>
+inline void plaunch(int fam, double bytes, double flops, F kernel, const dim3& grid, const dim3& block,
+                    uint32_t shm, hipStream_t s, Args... args) {
+  Profiler& P = profiler();
+  if (P.timed(fam)) {
+    const hipEvent_t a = P.get(), b = P.get();
+    hipExtLaunchKernelGGL(kernel, grid, block, shm, s, a, b, 0, args...);
+    P.pend.push_back(Profiler::Pending{a, b, fam, bytes, flops});
+  } else {
+    hipLaunchKernelGGL(kernel, grid, block, shm, s, args...);
+  }
+}
 
 }  // namespace nrg

@@ -149,8 +149,18 @@ def rho_get(iteration: int, kept_per_block):
     return out
 
 
-def profile(enable: bool):
-    _check(load().nrg_gpu_profile(int(enable)))
+def profile(enable=True, families=None):
+    """Start (clearing) or stop the per-family kernel timing; `families`
+    restricts it to the named families (see KERNEL_FAMILIES)."""
+    if not enable:
+        code = 0
+    elif families is None:
+        code = -1
+    else:
+        code = 0
+        for f in families:
+            code |= 1 << KERNEL_FAMILIES.index(f)
+    _check(load().nrg_gpu_profile(code))
 
 
 def kernel_stats():
