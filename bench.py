@@ -44,6 +44,9 @@ def parse():
                          "timed steps (default: the dominant one, tridiag_symv; 'all' for every family)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run a plain device-to-device copy of BYTES bytes five times (the reference "
+                         "kernel that scales the FETCH_SIZE/WRITE_SIZE counters to bytes) and exit")
     ap.add_argument("--no-plain", dest="plain", action="store_false",
                     help="skip the one-iteration plain (one block per pass) comparison")
     return ap.parse_args()
@@ -89,8 +92,44 @@ def cpu_baseline(block_sizes, budget_s):
                 seconds_per_iteration=total)
 
 
+def profiled_traffic(family, workload, launches):
+    """HBM bytes per launch of `family` on `workload` from the committed
+    rocprofv3 FETCH_SIZE/WRITE_SIZE summary, used only when it covers the
+    same number of launches as this run (the same steps of the same path)."""
+    name = f"{family}_traffic_{workload.rsplit('_it', 1)[0]}.md"
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None, None
+    vals = {}
+    for line in open(path):
+        if "=" in line and line.split("=")[0].strip() in ("traffic_bytes_per_launch", "launches"):
+            k, v = line.split("=")
+            vals[k.strip()] = float(v)
+    if int(vals.get("launches", -1)) != int(launches):
+        return None, f"profiles/{name} covers {int(vals.get('launches', -1))} launches, this run {launches}"
+    return vals["traffic_bytes_per_launch"], f"profiles/{name}"
+
+
+def copy_calibration(nbytes):
+    """A plain HBM copy of known size: under `rocprofv3 --pmc FETCH_SIZE` (or
+    WRITE_SIZE) its counter reading per dispatch, against nbytes, gives the
+    factor that turns the counters of the NRG kernels into bytes."""
+    import torch
+    n = nbytes // 8
+    src = torch.arange(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    ok = bool(torch.equal(dst[-4:], src[-4:]))
+    print(json.dumps({"copy_bytes": 8 * n, "dispatches": 5, "ok": ok}))
+
+
 def main():
     a = parse()
+    if a.copy_calibration:
+        copy_calibration(a.copy_calibration)
+        return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if a.gpus != 1 or world != 1:
         print(f"bench.py: runs on one GPU only; --gpus {a.gpus} (WORLD_SIZE={world}) is not supported: "
@@ -192,6 +231,9 @@ def main():
         roof["kernel_family"] = fam
         roof["launches"] = k["launches"]
         roof["avg_launch_us"] = 1e3 * k["ms"] / max(k["launches"], 1)
+        roof["alg_bytes_per_launch"] = k["bytes"] / max(k["launches"], 1)
+        roof["alg_flops_per_launch"] = k["flops"] / max(k["launches"], 1)
+        roof["traffic"], roof["traffic_source"] = profiled_traffic(fam, a.workload, k["launches"])
         out["roofline"] = roof
         out["kernel_ms_per_step"] = {f: v["ms"] / a.steps for f, v in ks.items() if v["launches"]}
     else:

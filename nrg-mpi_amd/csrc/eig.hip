@@ -657,15 +657,45 @@ __global__ __launch_bounds__(256) void k_dc_rotgather(DcArgs a, const int32_t* t
   }
 }
 
-// Secular equation for tau = lambda - d_origin (one lane per root); poles and
-// weights are staged in LDS when they fit.
-__global__ __launch_bounds__(64) void k_dc_secular(DcArgs a, const int32_t* tasks, const int64_t* rootbase,
-                                                   int ntask) {
+// Secular equation for tau = lambda - d_origin: one wave per root, four
+// roots per workgroup.  The 64 lanes split every pole sum (lane l takes poles
+// l, l+64, ...) and combine it with an xor butterfly, which leaves the same
+// bits in every lane, so the iteration below runs uniformly across the wave.
+// Poles and weights are staged in LDS when they fit.
+constexpr int SEC_ROOTS = 4;
+
+__device__ inline void sec_sums(const double* dl, const double* zl, int k, int i, double dorg, double tau,
+                                double rho, double& psi, double& dpsi, double& phi, double& dphi) {
+  const int lane = threadIdx.x & 63;
+  double a0 = 0.0, a1 = 0.0, b0 = 0.0, b1 = 0.0;
+  for (int j = lane; j < k; j += 64) {
+    const double r = 1.0 / ((dl[j] - dorg) - tau);
+    const double w2 = rho * zl[j] * zl[j];
+    const double wr = w2 * r;
+    if (j <= i) { a0 += wr; a1 += wr * r; }
+    else { b0 += wr; b1 += wr * r; }
+  }
+  psi = wave_sum(a0);
+  dpsi = wave_sum(a1);
+  phi = wave_sum(b0);
+  dphi = wave_sum(b1);
+}
+
+__device__ inline double sec_value(const double* dl, const double* zl, int k, double dorg, double x,
+                                   double rho) {
+  const int lane = threadIdx.x & 63;
+  double f = 0.0;
+  for (int j = lane; j < k; j += 64) f += rho * zl[j] * zl[j] / ((dl[j] - dorg) - x);
+  return 1.0 + wave_sum(f);
+}
+
+__global__ __launch_bounds__(64 * SEC_ROOTS) void k_dc_secular(DcArgs a, const int32_t* tasks, int ntask) {
   extern __shared__ double sec_shm[];
   const int t = blockIdx.y;
-  const int i = blockIdx.x * 64 + threadIdx.x;
   const int k = a.kout[t];
-  if (blockIdx.x * 64 >= k) return;
+  if (blockIdx.x * SEC_ROOTS >= k) return;
+  const int i = blockIdx.x * SEC_ROOTS + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
   const int32_t* tk = tasks + 8 * t;
   const int m = tk[3] - tk[1];
   const int64_t wo = join64(tk[4], tk[5]);
@@ -673,7 +703,7 @@ __global__ __launch_bounds__(64) void k_dc_secular(DcArgs a, const int32_t* task
   double* W = a.ws + wo;
   const double *dl = W + 2 * m, *zl = W + 3 * m;
   if (k <= DC_LDS_M) {
-    for (int j = threadIdx.x; j < k; j += 64) { sec_shm[j] = dl[j]; sec_shm[k + j] = zl[j]; }
+    for (int j = threadIdx.x; j < k; j += 64 * SEC_ROOTS) { sec_shm[j] = dl[j]; sec_shm[k + j] = zl[j]; }
     __syncthreads();
     dl = sec_shm;
     zl = sec_shm + k;
@@ -687,17 +717,14 @@ __global__ __launch_bounds__(64) void k_dc_secular(DcArgs a, const int32_t* task
   if (i < k - 1) {
     const double gap = dl[i + 1] - dl[i];
     const double half = 0.5 * gap;
-    double f = 1.0;
-    for (int j = 0; j < k; ++j) f += rho * zl[j] * zl[j] / ((dl[j] - dl[i]) - half);
+    const double f = sec_value(dl, zl, k, dl[i], half, rho);
     if (f > 0.0) { o = i; lo = 0.0; hi = half; }
     else { o = i + 1; lo = -(gap - half); hi = 0.0; }
   } else {
     o = k - 1; lo = 0.0; hi = rho;
     // make sure g(hi) > 0 (sum z^2 = 1 up to rounding)
     for (int it = 0; it < 8; ++it) {
-      double f = 1.0;
-      for (int j = 0; j < k; ++j) f += rho * zl[j] * zl[j] / ((dl[j] - dl[o]) - hi);
-      if (f > 0.0) break;
+      if (sec_value(dl, zl, k, dl[o], hi, rho) > 0.0) break;
       hi *= 2.0;
     }
   }
@@ -712,19 +739,8 @@ __global__ __launch_bounds__(64) void k_dc_secular(DcArgs a, const int32_t* task
   const double eps = 2.220446049250313e-16;
   double tau = lo + 0.5 * (hi - lo);
   for (int it = 0; it < 200; ++it) {
-    double psi = 0, dpsi = 0, phi = 0, dphi = 0;
-    for (int j = 0; j <= i; ++j) {
-      const double r = 1.0 / ((dl[j] - dorg) - tau);
-      const double w2 = rho * zl[j] * zl[j];
-      psi += w2 * r;
-      dpsi += w2 * r * r;
-    }
-    for (int j = i + 1; j < k; ++j) {
-      const double r = 1.0 / ((dl[j] - dorg) - tau);
-      const double w2 = rho * zl[j] * zl[j];
-      phi += w2 * r;
-      dphi += w2 * r * r;
-    }
+    double psi, dpsi, phi, dphi;
+    sec_sums(dl, zl, k, i, dorg, tau, rho, psi, dpsi, phi, dphi);
     const double g = 1.0 + psi + phi;
     if (g == 0.0) break;
     if (g > 0.0) hi = tau; else lo = tau;
@@ -758,8 +774,10 @@ __global__ __launch_bounds__(64) void k_dc_secular(DcArgs a, const int32_t* task
     if (step <= 2.0 * eps * fabs(tau)) break;
   }
   if (tau == 0.0) tau = (o == i) ? hi : lo;     // never sit on the pole
-  tr[i] = tau;
-  org[i] = o;
+  if (lane == 0) {
+    tr[i] = tau;
+    org[i] = o;
+  }
 }
 
 // z-hat (Gu & Eisenstat): one lane per non-deflated pole.
@@ -928,8 +946,10 @@ __global__ void k_bt_buildv(BtArgs a, int p) {
   }
 }
 
-// T of the compact WY form from the Gram matrix G = V^T V.
-__global__ void k_bt_tmat(BtArgs a, int p) {
+// T of the compact WY form from the Gram matrix G = V^T V, built column by
+// column in LDS (NBB^2 doubles; gfx950 gives a workgroup up to 160 KB):
+// T(0:i, i) = T(0:i, 0:i) * (-tau_i G(0:i, i)),  T(i, i) = tau_i.
+__global__ __launch_bounds__(NBB) void k_bt_tmat(BtArgs a, int p) {
   const int b = a.order[blockIdx.x];
   const int n = a.n[b];
   const int c0 = p * NBB;
@@ -938,21 +958,29 @@ __global__ void k_bt_tmat(BtArgs a, int p) {
   const double* G = a.T + (int64_t)b * 2 * NBB * NBB;
   double* T = a.T + (int64_t)b * 2 * NBB * NBB + NBB * NBB;
   const double* tau = a.tau + a.voff[b] + c0;
+  __shared__ double sT[NBB * NBB];
   __shared__ double tmp[NBB];
   const int j = threadIdx.x;   // thread j owns row j of T (upper triangular)
-  for (int i = 0; i < NBB; ++i) T[j + i * NBB] = 0.0;
   for (int i = 0; i < nbp; ++i) {
-    // T(0:i, i) = T(0:i, 0:i) * (-tau_i G(0:i, i)),  T(i, i) = tau_i
     if (j < i) tmp[j] = -tau[i] * G[j + i * NBB];
     __syncthreads();
     if (j < i) {
-      double acc = 0.0;
-      for (int l = j; l < i; ++l) acc += T[j + l * NBB] * tmp[l];
-      T[j + i * NBB] = acc;
+      double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+      int l = j;
+      for (; l + 3 < i; l += 4) {
+        s0 += sT[j + l * NBB] * tmp[l];
+        s1 += sT[j + (l + 1) * NBB] * tmp[l + 1];
+        s2 += sT[j + (l + 2) * NBB] * tmp[l + 2];
+        s3 += sT[j + (l + 3) * NBB] * tmp[l + 3];
+      }
+      for (; l < i; ++l) s0 += sT[j + l * NBB] * tmp[l];
+      sT[j + i * NBB] = (s0 + s1) + (s2 + s3);
+    } else if (j == i) {
+      sT[i + i * NBB] = tau[i];
     }
-    if (j == i) T[i + i * NBB] = tau[i];
     __syncthreads();
   }
+  for (int i = 0; i < NBB; ++i) T[j + i * NBB] = (i < nbp && j <= i) ? sT[j + i * NBB] : 0.0;
 }
 
 // ===========================================================================
@@ -1209,9 +1237,8 @@ void BatchedEigensolver::divide_conquer(hipStream_t s) {
     for (int i = 0; i < nt; ++i) kmax = std::max(kmax, h_k_[i]);
     if (kmax > 0) {
       const int32_t* tk = d_tasks_.p;
-      plaunch(KF_DC_SECULAR, 0, 0, k_dc_secular, dim3(ceil_div(kmax, 64), nt), dim3(64),
-              (uint32_t)(2 * sizeof(double) * (size_t)std::min(kmax, DC_LDS_M)), s, da, tk,
-              (const int64_t*)nullptr, nt);
+      plaunch(KF_DC_SECULAR, 0, 0, k_dc_secular, dim3(ceil_div(kmax, SEC_ROOTS), nt), dim3(64 * SEC_ROOTS),
+              (uint32_t)(2 * sizeof(double) * (size_t)std::min(kmax, DC_LDS_M)), s, da, tk, nt);
       plaunch(KF_DC_VECTORS, 0, 0, k_dc_zhat, dim3(ceil_div(kmax, 64), nt), dim3(64), 0, s, da, tk, nt);
       plaunch(KF_DC_VECTORS, 0, 0, k_dc_vectors, dim3(kmax, nt), dim3(64), 0, s, da, tk, nt);
       batch_.clear();
