@@ -241,9 +241,14 @@ int nrg_gpu_diagonalize(int iteration, int nsub, const int32_t* subspace_info, i
     NRG_HIP(hipStreamSynchronize(C.stream));
     double t1 = wall_ms();
     C.timers.assemble = t1 - t0;
+    C.u_lazy = false;
     if (C.batch_limit <= 0 || C.batch_limit >= nsub) {
-      C.eig.solve(nsub, C.bn.data(), C.boff.data(), C.bvoff.data(), C.d_H.p, C.d_U.p, C.d_w.p, nullptr,
+      // without an eigenvector output the back-transform waits for the
+      // columns the invariant / stock calls read (ensure_columns)
+      const bool defer = (eigenvector == nullptr);
+      C.eig.solve(nsub, C.bn.data(), C.boff.data(), C.bvoff.data(), C.d_H.p, C.d_U.p, C.d_w.p, defer,
                   C.stream, &C.eig_stats);
+      C.u_lazy = defer;
     } else {
       // groups of at most batch_limit blocks, one solve each (batch_limit 1:
       // one block at a time, the reference's per-block dsyev schedule)
@@ -259,7 +264,7 @@ int nrg_gpu_diagonalize(int iteration, int nsub, const int32_t* subspace_info, i
         }
         EigStats st;
         C.eig.solve(nb, C.bn.data() + b0, ao.data(), vo.data(), C.d_H.p + C.boff[b0], C.d_U.p + C.boff[b0],
-                    C.d_w.p + C.bvoff[b0], nullptr, C.stream, &st);
+                    C.d_w.p + C.bvoff[b0], false, C.stream, &st);
         sum.ms_tridiag += st.ms_tridiag; sum.ms_dc += st.ms_dc; sum.ms_back += st.ms_back;
         sum.flops_alg += st.flops_alg;
         sum.max_n = std::max(sum.max_n, st.max_n);
@@ -308,6 +313,7 @@ int nrg_gpu_stock(int iteration, const int32_t* kept_per_block, int last) {
     need_init(C);
     NRG_REQUIRE(kept_per_block != nullptr, "stock: NULL kept_per_block");
     const double t0 = wall_ms();
+    ensure_columns(C, nullptr);   // kept and discarded columns enter the spectral products
     stock(C, iteration, kept_per_block, last);
     C.timers.cfs = wall_ms() - t0;
   });
@@ -402,6 +408,7 @@ int nrg_gpu_invariant(const int32_t* kept_per_block, double* out) {
     need_init(C);
     NRG_REQUIRE(C.nsub > 0, "invariant: no diagonalised iteration");
     const double t0 = wall_ms();
+    ensure_columns(C, kept_per_block);
     invariant(C, kept_per_block, out);
     C.timers.invariant = wall_ms() - t0;
   });
@@ -412,6 +419,7 @@ int nrg_gpu_invariant_spectrum(const int32_t* kept_per_block, double* out) {
     need_init(C);
     NRG_REQUIRE(C.nsub > 0, "invariant_spectrum: no diagonalised iteration");
     const double t0 = wall_ms();
+    ensure_columns(C, kept_per_block);
     invariant_spectrum(C, kept_per_block, out);
     C.timers.spectral = wall_ms() - t0;
   });
@@ -442,7 +450,8 @@ int nrg_gpu_syev_batched(int nblk, const int32_t* n, const double* H, double* w,
     dA.upload(H, a, C.stream);
     dZ.reserve(a);
     dw.reserve(v);
-    C.eig.solve(nblk, nn.data(), ao.data(), vo.data(), dA.p, dZ.p, dw.p, nullptr, C.stream, &C.eig_stats);
+    ensure_columns(C, nullptr);   // the solver's state is reused below
+    C.eig.solve(nblk, nn.data(), ao.data(), vo.data(), dA.p, dZ.p, dw.p, false, C.stream, &C.eig_stats);
     dw.download(w, v, C.stream);
     dZ.download(U, a, C.stream);
     NRG_HIP(hipStreamSynchronize(C.stream));

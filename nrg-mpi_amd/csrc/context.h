@@ -24,6 +24,7 @@ struct Context {
   Timers timers;
   int fault_iteration = -1000000;   // test hook: fail diagonalisation at this iteration
   int batch_limit = 0;              // max blocks per eigensolver launch sequence (0: all)
+  bool u_lazy = false;              // d_U columns are back-transformed on demand (ensure_columns)
 
   // ---- model constants (set once)
   int nv = 0, nops = 0, nchain = 0, ncm = 0, ncoef = 0, nmat = 0;
@@ -59,5 +60,21 @@ struct Context {
 };
 
 Context& ctx();
+
+// Make columns [0, hi[b]) of every block of d_U eigenvectors of the full
+// matrices (hi == nullptr: all columns).  Only the batched solve of
+// nrg_gpu_diagonalize without an eigenvector output defers its
+// back-transform; every reader of d_U calls this first.
+inline void ensure_columns(Context& C, const int32_t* hi) {
+  if (!C.u_lazy) return;
+  std::vector<int> h(C.nsub);
+  bool all = true;
+  for (int b = 0; b < C.nsub; ++b) {
+    h[b] = hi ? std::min(std::max((int)hi[b], 0), C.bn[b]) : C.bn[b];
+    all = all && h[b] == C.bn[b];
+  }
+  C.eig.back_columns(h.data(), C.stream, &C.timers.back);
+  if (all) C.u_lazy = false;
+}
 
 }  // namespace nrg

@@ -29,17 +29,25 @@ class BatchedEigensolver {
   // Solve all blocks.  On entry A holds the matrices (full storage); on exit
   // Z holds eigenvectors (ascending eigenvalue order, one column each) and w
   // the eigenvalues.  A is overwritten with Householder data.  A and Z are
-  // device buffers of sum n^2 doubles; w of sum n.  ncols_back[b] limits the
-  // back-transform to the first ncols_back[b] columns (n_b for all).
+  // device buffers of sum n^2 doubles; w of sum n.  With defer_back the
+  // columns of Z stay the eigenvectors of the tridiagonal matrices until
+  // back_columns() transforms them (A and the reflector scalars are kept
+  // until the next solve).
   void solve(int nblk, const int* n, const int64_t* aoff, const int64_t* voff,
-             double* A, double* Z, double* w, const int* ncols_back, hipStream_t s,
+             double* A, double* Z, double* w, bool defer_back, hipStream_t s,
              EigStats* st = nullptr);
+  // Back-transform (and chop) columns [done_b, hi[b]) of every block b
+  // (hi == nullptr: all columns); columns already transformed are skipped.
+  // *ms (if not null) accumulates the time.
+  void back_columns(const int* hi, hipStream_t s, double* ms = nullptr);
+  // Columns of block b that hold eigenvectors of the full matrix.
+  int columns_done(int b) const { return done_[b]; }
   void release();
 
  private:
   void tridiag(hipStream_t s);
   void divide_conquer(hipStream_t s);
-  void back_transform(const int* ncols_back, hipStream_t s);
+  void back_transform(const int* lo, const int* hi, hipStream_t s);
 
   int nblk_ = 0;
   std::vector<int> n_;
@@ -60,6 +68,9 @@ class BatchedEigensolver {
   std::vector<int32_t> h_k_;
   // back-transform
   DevBuf<double> d_V_, d_T_, d_X_, d_X2_;
+  DevBuf<int64_t> d_rng_;
+  std::vector<int64_t> h_rng_;
+  std::vector<int> done_;           // transformed columns per block
   GemmRunner gemm_;
   GemmBatch batch_;
 };

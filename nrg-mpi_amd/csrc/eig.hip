@@ -986,9 +986,12 @@ __global__ __launch_bounds__(NBB) void k_bt_tmat(BtArgs a, int p) {
 // ===========================================================================
 // chop |U| < 1e-14 (diagonalization.F90:273-280)
 // ===========================================================================
-__global__ void k_chop(double* Z, size_t n, double thr) {
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-    if (fabs(Z[i]) < thr) Z[i] = 0.0;
+// Grid: (chunks, ranges); range r is rng[2r] (start) and rng[2r+1] (length).
+__global__ void k_chop(double* Z, const int64_t* rng, double thr) {
+  double* z = Z + rng[2 * blockIdx.y];
+  const int64_t n = rng[2 * blockIdx.y + 1];
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    if (fabs(z[i]) < thr) z[i] = 0.0;
 }
 
 // ===========================================================================
@@ -1002,11 +1005,12 @@ void BatchedEigensolver::release() {
   d_n_.release(); d_order_.release(); d_ns_.release(); d_aoff_.release(); d_voff_.release();
   d_d_.release(); d_e_.release(); d_tau_.release(); d_y_.release(); d_P_.release(); d_part_.release();
   d_poff_.release(); d_ws_.release(); d_iws_.release(); d_tasks_.release(); d_k_.release();
-  d_V_.release(); d_T_.release(); d_X_.release(); d_X2_.release(); gemm_.release();
+  d_V_.release(); d_T_.release(); d_X_.release(); d_X2_.release(); d_rng_.release(); gemm_.release();
+  nblk_ = 0;
 }
 
 void BatchedEigensolver::solve(int nblk, const int* n, const int64_t* aoff, const int64_t* voff,
-                               double* A, double* Z, double* w, const int* ncols_back, hipStream_t s,
+                               double* A, double* Z, double* w, bool defer_back, hipStream_t s,
                                EigStats* st) {
   nblk_ = nblk;
   n_.assign(n, n + nblk);
@@ -1044,7 +1048,8 @@ void BatchedEigensolver::solve(int nblk, const int* n, const int64_t* aoff, cons
   if (st) { NRG_HIP(hipStreamSynchronize(s)); st->ms_tridiag = now_ms() - t0; t0 = now_ms(); }
   divide_conquer(s);
   if (st) { NRG_HIP(hipStreamSynchronize(s)); st->ms_dc = now_ms() - t0; t0 = now_ms(); }
-  back_transform(ncols_back, s);
+  done_.assign(nblk, 0);
+  if (!defer_back) back_columns(nullptr, s, nullptr);
   if (st) {
     NRG_HIP(hipStreamSynchronize(s));
     st->ms_back = now_ms() - t0;
@@ -1260,7 +1265,7 @@ void BatchedEigensolver::divide_conquer(hipStream_t s) {
   NRG_HIP(hipGetLastError());
 }
 
-void BatchedEigensolver::back_transform(const int* ncols_back, hipStream_t s) {
+void BatchedEigensolver::back_transform(const int* lo, const int* hi, hipStream_t s) {
   const int nblk = nblk_;
   const int nmax = n_[order_[0]];
   d_V_.reserve(tot_n_ * NBB);
@@ -1296,9 +1301,9 @@ void BatchedEigensolver::back_transform(const int* ncols_back, hipStream_t s) {
       const int n = n_[b];
       const int mr = n - c0 - 1;
       const int nbp = std::min(NBB, n - 1 - c0);
-      const int nc = ncols_back ? ncols_back[b] : n;
+      const int nc = hi[b] - lo[b];
       if (nc <= 0) continue;
-      batch_.add(gemm_task(nbp, nc, mr, 1, 0, voff_[b] * NBB, n, aoff_[b] + c0 + 1, n,
+      batch_.add(gemm_task(nbp, nc, mr, 1, 0, voff_[b] * NBB, n, aoff_[b] + (int64_t)lo[b] * n + c0 + 1, n,
                            voff_[b] * NBB, NBB, 1.0, 0.0));
     }
     gemm_.run(batch_, d_V_.p, Z_, d_X_.p, nullptr, nullptr, nullptr, s, KF_GEMM_BACK);
@@ -1308,7 +1313,7 @@ void BatchedEigensolver::back_transform(const int* ncols_back, hipStream_t s) {
       const int b = order_[q];
       const int n = n_[b];
       const int nbp = std::min(NBB, n - 1 - c0);
-      const int nc = ncols_back ? ncols_back[b] : n;
+      const int nc = hi[b] - lo[b];
       if (nc <= 0) continue;
       batch_.add(gemm_task(nbp, nc, nbp, 0, 0, (int64_t)b * 2 * NBB * NBB + NBB * NBB, NBB,
                            voff_[b] * NBB, NBB, voff_[b] * NBB, NBB, 1.0, 0.0));
@@ -1321,18 +1326,45 @@ void BatchedEigensolver::back_transform(const int* ncols_back, hipStream_t s) {
       const int n = n_[b];
       const int mr = n - c0 - 1;
       const int nbp = std::min(NBB, n - 1 - c0);
-      const int nc = ncols_back ? ncols_back[b] : n;
+      const int nc = hi[b] - lo[b];
       if (nc <= 0) continue;
       batch_.add(gemm_task(mr, nc, nbp, 0, 0, voff_[b] * NBB, n, voff_[b] * NBB, NBB,
-                           aoff_[b] + c0 + 1, n, -1.0, 1.0));
+                           aoff_[b] + (int64_t)lo[b] * n + c0 + 1, n, -1.0, 1.0));
     }
     gemm_.run(batch_, d_V_.p, d_X2_.p, Z_, nullptr, nullptr, nullptr, s, KF_GEMM_BACK);
   }
-  {
-    unsigned g = (unsigned)std::min<int64_t>((tot_n2_ + 255) / 256, 16384);
-    hipLaunchKernelGGL(k_chop, dim3(g), dim3(256), 0, s, Z_, (size_t)tot_n2_, 1e-14);
+  // chop the transformed columns (element ranges [aoff + lo n, aoff + hi n))
+  h_rng_.clear();
+  int64_t longest = 0;
+  for (int b = 0; b < nblk; ++b) {
+    if (hi[b] <= lo[b]) continue;
+    const int64_t a0 = aoff_[b] + (int64_t)lo[b] * n_[b], len = (int64_t)(hi[b] - lo[b]) * n_[b];
+    h_rng_.push_back(a0);
+    h_rng_.push_back(len);
+    longest = std::max(longest, len);
+  }
+  const int nr = (int)(h_rng_.size() / 2);
+  if (nr > 0) {
+    d_rng_.upload(h_rng_.data(), h_rng_.size(), s);
+    const unsigned g = (unsigned)std::min<int64_t>((longest + 1023) / 1024, 4096);
+    hipLaunchKernelGGL(k_chop, dim3(g, nr), dim3(256), 0, s, Z_, (const int64_t*)d_rng_.p, 1e-14);
   }
   NRG_HIP(hipGetLastError());
+}
+
+void BatchedEigensolver::back_columns(const int* hi, hipStream_t s, double* ms) {
+  NRG_REQUIRE(nblk_ > 0, "eigensolver: no solve to back-transform");
+  std::vector<int> lo(done_), h(nblk_);
+  bool any = false;
+  for (int b = 0; b < nblk_; ++b) {
+    h[b] = std::min(std::max(hi ? hi[b] : n_[b], done_[b]), n_[b]);
+    any = any || h[b] > lo[b];
+  }
+  if (!any) return;
+  const double t0 = now_ms();
+  back_transform(lo.data(), h.data(), s);
+  done_ = h;
+  if (ms) { NRG_HIP(hipStreamSynchronize(s)); *ms += now_ms() - t0; }
 }
 
 }  // namespace nrg

@@ -82,3 +82,28 @@ def test_one_block_per_pass_schedule(tmp_path):
 def test_two_orbital_canonical_sort(tmp_path):
     # canonical tie order: identical gauge-invariant results
     run_and_compare(tmp_path, "two_orbital_nk4000_it45", {"niter": 6}, last=2, sort_mode=1)
+
+
+def test_deferred_back_transform_matches_full(tmp_path):
+    # Without an eigenvector output the back-transform covers only the kept
+    # columns, when the invariant matrices need them; with the output every
+    # column is transformed at once.  The iterations must agree.
+    from nrgamd.host import Host
+    d = str(tmp_path / "two_orbital_nk300")
+    models.write_inputs(models.workload("two_orbital_nk4000_it45", nkeep=300, niter=10), d)
+    runs = []
+    for keep in (False, True):
+        h = Host(d, device=0, keep_vectors=keep)
+        rec = []
+        try:
+            for it in range(-1, 8):
+                info = h.iterate(it)
+                rec.append((info["nkept"], h.kept_per_block().copy(), h.eigenvalues().copy()))
+        finally:
+            h.close()
+        runs.append(rec)
+    for k, ((n0, kb0, e0), (n1, kb1, e1)) in enumerate(zip(*runs)):
+        assert n0 == n1, k
+        np.testing.assert_array_equal(kb0, kb1)
+        scale = max(1.0, float(np.max(np.abs(e1))))
+        assert np.max(np.abs(e0 - e1)) <= 1e-13 * scale, k
