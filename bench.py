@@ -47,6 +47,8 @@ def parse():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a plain device-to-device copy of BYTES bytes five times (the reference "
                          "kernel that scales the FETCH_SIZE/WRITE_SIZE counters to bytes) and exit")
+    ap.add_argument("--tridiag-align", choices=["end", "start"], default="end",
+                    help="column schedule of the batched tridiagonalisation (blocks end or start together)")
     ap.add_argument("--no-plain", dest="plain", action="store_false",
                     help="skip the one-iteration plain (one block per pass) comparison")
     return ap.parse_args()
@@ -147,6 +149,7 @@ def main():
     tmp = tempfile.mkdtemp(prefix="nrgbench_")
     models.write_inputs(spec, tmp)
     h = Host(tmp, device=0, sort_mode=0 if a.sort == "exact" else 1)
+    gpu.load().nrg_gpu_set_tridiag_schedule(1 if a.tridiag_align == "end" else 0)
     its = list(range(spec.first_iteration, spec.first_iteration + need))
     for it in its[:a.warmup]:
         h.iterate(it)
@@ -211,7 +214,8 @@ def main():
                             "Lambda=2, D=1, local dim 16, labels (Q,2Sz)",
                    "nkeep": spec.nkeep, "timed_iterations": [its[a.warmup], its[-1]],
                    "states_per_iteration": nfull, "blocks_per_iteration": nsub, "largest_block": nmax,
-                   "sort": a.sort, "form": "batched FP64 (plain precision)"},
+                   "sort": a.sort, "tridiag_schedule": a.tridiag_align,
+                   "form": "batched FP64 (plain precision)"},
         "stages_s_per_step": {k: v / a.steps for k, v in stages.items()},
         "eig_alg_tflops": flops_eig / max(stages.get("gpu_eig", 1e-9), 1e-9) / 1e12,
         "plain_fp64": plain,

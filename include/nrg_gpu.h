@@ -181,6 +181,12 @@ int nrg_gpu_kernel_stats(double* out, int nfam);
  * reference's dsyev loop, used as the plain-FP64 comparison). */
 int nrg_gpu_set_batch_limit(int max_blocks);
 
+/* Scheduling option with no counterpart in the reference: the column order
+ * of the batched tridiagonalisation.  align_end != 0 (the default) starts
+ * the blocks of an iteration at staggered steps so that they finish
+ * together; 0 starts them together. */
+int nrg_gpu_set_tridiag_schedule(int align_end);
+
 /* Test hook with no counterpart in the reference: make the next
  * nrg_gpu_diagonalize of `iteration` fail with status 5 (error-path tests). */
 int nrg_gpu_inject_fault(int iteration);

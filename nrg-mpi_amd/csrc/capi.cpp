@@ -486,6 +486,13 @@ int nrg_gpu_set_batch_limit(int max_blocks) {
   });
 }
 
+int nrg_gpu_set_tridiag_schedule(int align_end) {
+  return guarded([&](Context& C) {
+    need_init(C);
+    C.eig.set_align_end(align_end != 0);
+  });
+}
+
 int nrg_gpu_inject_fault(int iteration) {
   return guarded([&](Context& C) { C.fault_iteration = iteration; });
 }

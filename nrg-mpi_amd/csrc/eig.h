@@ -43,6 +43,9 @@ class BatchedEigensolver {
   // Columns of block b that hold eigenvectors of the full matrix.
   int columns_done(int b) const { return done_[b]; }
   void release();
+  // Column schedule of the tridiagonalisation: all blocks start together
+  // (false) or end together (true, the default).
+  void set_align_end(bool on) { align_end_ = on; }
 
  private:
   void tridiag(hipStream_t s);
@@ -57,6 +60,8 @@ class BatchedEigensolver {
   double *A_ = nullptr, *Z_ = nullptr, *w_ = nullptr;
 
   DevBuf<int32_t> d_n_, d_order_, d_ns_;   // sizes, size order, sizes in that order
+  DevBuf<int32_t> d_coff_;                  // tridiagonalisation start step per slot
+  bool align_end_ = true;                   // end-aligned column schedule (see tridiag)
   DevBuf<int64_t> d_aoff_, d_voff_;
   DevBuf<double> d_d_, d_e_, d_tau_, d_y_, d_P_, d_part_;
   DevBuf<int64_t> d_poff_;

@@ -57,6 +57,7 @@ struct TrArgs {
   const int32_t* n;
   const int32_t* order;
   const int32_t* ns;       // n[order[q]]: sizes in decreasing order
+  const int32_t* coff;     // step at which slot q starts: column c = step - coff[q]
   const int64_t* aoff;
   const int64_t* voff;
   const int64_t* poff;     // [0,nblk): P, [nblk,2nblk): records, [2nblk,3nblk): py
@@ -110,9 +111,15 @@ __host__ __device__ inline int64_t part_size(int n) { return colrec_size(n) + sy
 //   jj >= 0: update column c with the jj previous panel columns, write it
 //            back, record d[c], the partial norm of rows >= c+2 and the
 //            partial products sum V[r,q] x_r, sum W[r,q] x_r (q < jj).
-__global__ __launch_bounds__(1024) void k_tr_col(TrArgs a, int c, int jp, int jj, int vslot_prev, int pw, int act) {
+__global__ __launch_bounds__(1024) void k_tr_col(TrArgs a, int t, int jp, int jj, int vslot_prev, int pw, int act) {
   int qs, tile;
-  if (!flat_map((int)blockIdx.x, act, [&](int q) { return (a.ns[q] - c + RA - 1) / RA; }, qs, tile)) return;
+  auto tiles = [&](int q) {
+    const int cq = t - a.coff[q];
+    return (cq >= 0 && cq < a.ns[q]) ? (a.ns[q] - cq + RA - 1) / RA : 0;
+  };
+  if (!flat_map((int)blockIdx.x, act, tiles, qs, tile)) return;
+  const int c = t - a.coff[qs];
+  if (c == 0) jp = -1;   // a block starting at a panel boundary has no previous column
   const int b = a.order[qs];
   const int n = a.ns[qs];
   const int r0 = c + tile * RA;
@@ -252,11 +259,16 @@ __global__ __launch_bounds__(1024) void k_tr_col(TrArgs a, int c, int jp, int jj
 // Reflector + lower-triangle SYMV kernel, one workgroup (16 waves) per
 // PW = 16 CPW columns of the trailing block of one block.
 template <int CPW, int NW>
-__global__ __launch_bounds__(NW * 64) void k_tr_symv(TrArgs a, int c, int jj, int act) {
+__global__ __launch_bounds__(NW * 64) void k_tr_symv(TrArgs a, int t, int jj, int act) {
   constexpr int PW = CPW * NW;
   extern __shared__ double ax[];   // row parts, rows [j0, m)
   int qs, pan;
-  if (!flat_map((int)blockIdx.x, act, [&](int q) { return (a.ns[q] - c - 1 + PW - 1) / PW; }, qs, pan)) return;
+  auto tiles = [&](int q) {
+    const int cq = t - a.coff[q], mq = a.ns[q] - cq - 1;
+    return (cq >= 0 && mq > 0) ? (mq + PW - 1) / PW : 0;
+  };
+  if (!flat_map((int)blockIdx.x, act, tiles, qs, pan)) return;
+  const int c = t - a.coff[qs];
   const int b = a.order[qs];
   const int n = a.ns[qs];
   const int m = n - c - 1;
@@ -961,8 +973,13 @@ __global__ __launch_bounds__(NBB) void k_bt_tmat(BtArgs a, int p) {
   __shared__ double sT[NBB * NBB];
   __shared__ double tmp[NBB];
   const int j = threadIdx.x;   // thread j owns row j of T (upper triangular)
+  // G(j, i) is fetched one column ahead so that its latency overlaps the
+  // previous column's work
+  double gji = 0.0;
   for (int i = 0; i < nbp; ++i) {
-    if (j < i) tmp[j] = -tau[i] * G[j + i * NBB];
+    const double gcur = gji;
+    if (i + 1 < nbp && j < i + 1) gji = G[j + (i + 1) * NBB];
+    if (j < i) tmp[j] = -tau[i] * gcur;
     __syncthreads();
     if (j < i) {
       double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
@@ -1002,7 +1019,7 @@ static double now_ms() {
 }
 
 void BatchedEigensolver::release() {
-  d_n_.release(); d_order_.release(); d_ns_.release(); d_aoff_.release(); d_voff_.release();
+  d_n_.release(); d_order_.release(); d_ns_.release(); d_coff_.release(); d_aoff_.release(); d_voff_.release();
   d_d_.release(); d_e_.release(); d_tau_.release(); d_y_.release(); d_P_.release(); d_part_.release();
   d_poff_.release(); d_ws_.release(); d_iws_.release(); d_tasks_.release(); d_k_.release();
   d_V_.release(); d_T_.release(); d_X_.release(); d_X2_.release(); d_rng_.release(); gemm_.release();
@@ -1077,78 +1094,95 @@ void BatchedEigensolver::tridiag(hipStream_t s) {
   d_y_.reserve(std::max<int64_t>(yacc, 1));
   d_P_.reserve(tot_n_ * 4 * NB);
   d_part_.reserve(pacc);
-  TrArgs ta{d_n_.p, d_order_.p, d_ns_.p, d_aoff_.p, d_voff_.p, d_poff_.p, A_, d_d_.p, d_e_.p, d_tau_.p,
-            d_y_.p, d_P_.p, d_part_.p, nblk};
-  // number of blocks with n > x (order_ is by n descending)
-  auto active = [&](int x) {
+  // Column schedule.  Step t processes column c_q = t - coff[q] of every
+  // started slot q.  With end alignment the slots start at multiples of NB
+  // so that all blocks reach their last columns together: the light steps
+  // at the end of a block (small trailing matrices, latency-bound launches)
+  // are shared by all blocks instead of being left to the largest one, and
+  // every slot sees the same panel phase t mod NB.
+  std::vector<int32_t> coff(nblk);
+  for (int q = 0; q < nblk; ++q) coff[q] = align_end_ ? ((nmax - n_[order_[q]]) / NB) * NB : 0;
+  d_coff_.upload(coff.data(), nblk, s);
+  TrArgs ta{d_n_.p, d_order_.p, d_ns_.p, d_coff_.p, d_aoff_.p, d_voff_.p, d_poff_.p, A_, d_d_.p, d_e_.p,
+            d_tau_.p, d_y_.p, d_P_.p, d_part_.p, nblk};
+  // slots started by step t (coff is non-decreasing in slot order)
+  auto started = [&](int t) {
     int lo = 0, hi = nblk;
-    while (lo < hi) { int md = (lo + hi) / 2; if (n_[order_[md]] > x) lo = md + 1; else hi = md; }
+    while (lo < hi) { int md = (lo + hi) / 2; if (coff[md] <= t) lo = md + 1; else hi = md; }
     return lo;
+  };
+  auto rows_at = [&](int q, int t) {   // trailing rows n - c of slot q at step t (<= 0: idle)
+    const int c = t - coff[q];
+    return (c >= 0 && c < n_[order_[q]]) ? n_[order_[q]] - c : 0;
   };
   const size_t shm = (size_t)std::max(nmax, 1) * sizeof(double);
   int pw_prev = CG;   // panel width of the previous column's SYMV launch
-  auto colbytes = [&](int c, int act) {
+  auto colbytes = [&](int t, int act) {
     double b = 0;
     for (int q = 0; q < act; ++q) {
-      const double m = n_[order_[q]] - c;
+      const double m = rows_at(q, t);
       b += 8.0 * (4.0 * m * (1 + NB / 8) + m * m / (2.0 * pw_prev));   // column, panel rows, y partials
     }
     return b;
   };
   NRG_REQUIRE(shm <= 64 * 1024, "eigensolver: block too large for the SYMV stage (n > 8192)");
-  for (int c = 0; c < nmax; ++c) {
-    const int j0 = c - c % NB;
-    const int jj = c - j0;
-    const int actA = active(c);
+  int T = 0;
+  for (int q = 0; q < nblk; ++q) T = std::max(T, coff[q] + n_[order_[q]]);
+  for (int t = 0; t < T; ++t) {
+    const int jj = t % NB;   // panel position, the same for every slot
+    const int act = started(t);
     int gA = 0;   // flattened grid: the row tiles of the active blocks
-    for (int q = 0; q < actA; ++q) gA += ceil_div(n_[order_[q]] - c, RA);
-    if (jj == 0 && c > 0) {
+    for (int q = 0; q < act; ++q) gA += ceil_div(rows_at(q, t), RA);
+    if (gA == 0) continue;
+    if (jj == 0 && t > 0) {
       // close the previous panel: finalise its last w, trailing SYR2K
-      {
-        plaunch(KF_TR_COL, colbytes(c, actA), 0, k_tr_col, dim3(gA), dim3(1024), 0, s, ta, c, NB - 1, -1, NB - 1,
-                pw_prev, actA);
-      }
+      plaunch(KF_TR_COL, colbytes(t, act), 0, k_tr_col, dim3(gA), dim3(1024), 0, s, ta, t, NB - 1, -1, NB - 1,
+              pw_prev, act);
       batch_.clear();
-      for (int q = 0; q < actA; ++q) {
+      for (int q = 0; q < act; ++q) {
         const int b = order_[q];
         const int nb_ = n_[b];
+        const int c = t - coff[q];
         const int mt = nb_ - c;
-        if (mt <= 0) continue;
-        GemmTask t = gemm_task(mt, mt, 2 * NB, 0, 1,
-                               poff[b] + c, nb_,                       // [V W] rows c..
-                               poff[b] + c + (int64_t)2 * NB * nb_, nb_, // [W V] rows c..
-                               aoff_[b] + c + (int64_t)c * nb_, nb_, -1.0, 1.0);
-        t.lower = 1;   // only the lower triangle is read from here on
-        batch_.add(t);
+        if (c <= 0 || mt <= 0) continue;
+        GemmTask tk = gemm_task(mt, mt, 2 * NB, 0, 1,
+                                poff[b] + c, nb_,                       // [V W] rows c..
+                                poff[b] + c + (int64_t)2 * NB * nb_, nb_, // [W V] rows c..
+                                aoff_[b] + c + (int64_t)c * nb_, nb_, -1.0, 1.0);
+        tk.lower = 1;   // only the lower triangle is read from here on
+        batch_.add(tk);
       }
       gemm_.run(batch_, d_P_.p, d_P_.p, A_, nullptr, nullptr, nullptr, s, KF_GEMM_TRIDIAG);
-      plaunch(KF_TR_COL, colbytes(c, actA), 0, k_tr_col, dim3(gA), dim3(1024), 0, s, ta, c, -1, 0, NB - 1, pw_prev,
-              actA);
+      plaunch(KF_TR_COL, colbytes(t, act), 0, k_tr_col, dim3(gA), dim3(1024), 0, s, ta, t, -1, 0, NB - 1, pw_prev,
+              act);
     } else {
       const int jp = (jj >= 1) ? jj - 1 : -1;
-      const int vslot = (c >= 1) ? (c - 1) % NB : 0;
-      plaunch(KF_TR_COL, colbytes(c, actA), 0, k_tr_col, dim3(gA), dim3(1024), 0, s, ta, c, jp, jj, vslot, pw_prev,
-              actA);
+      const int vslot = (t >= 1) ? (t - 1) % NB : 0;
+      plaunch(KF_TR_COL, colbytes(t, act), 0, k_tr_col, dim3(gA), dim3(1024), 0, s, ta, t, jp, jj, vslot, pw_prev,
+              act);
     }
-    const int actB = active(c + 1);
-    if (actB > 0) {
-      // algorithmic traffic: the lower triangle of the symmetric trailing
-      // (n-c-1)^2 block once, plus the vectors
-      double bytes = 0, flops = 0;
-      for (int q = 0; q < actB; ++q) {
-        const double m = n_[order_[q]] - c - 1;
-        bytes += 8.0 * (0.5 * m * (m + 1.0) + 6.0 * m);
-        flops += 2.0 * m * m;
-      }
+    // algorithmic traffic: the lower triangle of the symmetric trailing
+    // (n-c-1)^2 block once, plus the vectors
+    double bytes = 0, flops = 0;
+    const int pw = CG;
+    int gB = 0, mB = 0;
+    for (int q = 0; q < act; ++q) {
+      const int m = rows_at(q, t) - 1;
+      if (m <= 0) continue;
+      bytes += 8.0 * (0.5 * (double)m * (m + 1.0) + 6.0 * m);
+      flops += 2.0 * (double)m * m;
+      gB += ceil_div(m, pw);
+      mB = std::max(mB, m);
+    }
+    if (gB > 0) {
+      // the LDS row vector covers rows [j0, m) of the longest trailing block
+      const uint32_t shB = (uint32_t)(mB * sizeof(double));
       // 16-column panels: 8 waves x 2 columns while the launch is
       // bandwidth-bound, 16 waves x 1 column once it is latency-bound
-      const int pw = CG;
-      int gB = 0;
-      for (int q = 0; q < actB; ++q) gB += ceil_div(n_[order_[q]] - c - 1, pw);
       if (bytes >= 64.0e6)
-        plaunch(KF_TR_SYMV, bytes, flops, k_tr_symv<2, 8>, dim3(gB), dim3(512), (uint32_t)shm, s, ta, c, jj, actB);
+        plaunch(KF_TR_SYMV, bytes, flops, k_tr_symv<2, 8>, dim3(gB), dim3(512), shB, s, ta, t, jj, act);
       else
-        plaunch(KF_TR_SYMV, bytes, flops, k_tr_symv<1, 16>, dim3(gB), dim3(1024), (uint32_t)shm, s, ta, c, jj, actB);
+        plaunch(KF_TR_SYMV, bytes, flops, k_tr_symv<1, 16>, dim3(gB), dim3(1024), shB, s, ta, t, jj, act);
       pw_prev = pw;
     }
   }

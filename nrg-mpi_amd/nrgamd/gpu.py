@@ -52,6 +52,7 @@ SIGNATURES = {
     "nrg_gpu_spectrum_bins": (ctypes.c_int, [_f64p, _f64p, ctypes.c_int64]),
     "nrg_gpu_history_clear": (ctypes.c_int, []),
     "nrg_gpu_set_batch_limit": (ctypes.c_int, [ctypes.c_int]),
+    "nrg_gpu_set_tridiag_schedule": (ctypes.c_int, [ctypes.c_int]),
 }
 
 KERNEL_FAMILIES = ["assemble", "tridiag_col", "tridiag_symv", "tridiag_gemm", "dc_leaf", "dc_prep",

@@ -107,3 +107,14 @@ def test_deferred_back_transform_matches_full(tmp_path):
         np.testing.assert_array_equal(kb0, kb1)
         scale = max(1.0, float(np.max(np.abs(e1))))
         assert np.max(np.abs(e0 - e1)) <= 1e-13 * scale, k
+
+
+def test_start_aligned_tridiagonal_schedule(tmp_path):
+    # the blocks may also start their column sequences together
+    from nrgamd import gpu
+    gpu.init(0)
+    gpu.load().nrg_gpu_set_tridiag_schedule(0)
+    try:
+        run_and_compare(tmp_path, "two_orbital_nk4000_it45", {"niter": 6}, last=2)
+    finally:
+        gpu.load().nrg_gpu_set_tridiag_schedule(1)
