@@ -39,6 +39,8 @@ def parse():
     ap.add_argument("--workload", default="two_orbital_nk4000_it45")
     ap.add_argument("--sort", choices=["exact", "canonical"], default="exact")
     ap.add_argument("--no-profile", action="store_true", help="skip the HIP-event kernel timing")
+    ap.add_argument("--profile-every", type=int, default=7,
+                    help="time every n-th launch of the profiled families (7: coprime to the 32-column panel)")
     ap.add_argument("--profile-families", default="tridiag_symv",
                     help="comma-separated kernel families whose launches carry HIP timing events in the "
                          "timed steps (default: the dominant one, tridiag_symv; 'all' for every family)")
@@ -49,6 +51,13 @@ def parse():
                          "kernel that scales the FETCH_SIZE/WRITE_SIZE counters to bytes) and exit")
     ap.add_argument("--tridiag-align", choices=["end", "start"], default="end",
                     help="column schedule of the batched tridiagonalisation (blocks end or start together)")
+    ap.add_argument("--split", type=int, default=0,
+                    help="slices of the split-precision contraction GEMMs in the timed steps (0: FP64 matrix "
+                         "cores, the faster form on MI355X; the split form is measured in its own iteration)")
+    ap.add_argument("--split-leg", type=int, default=8,
+                    help="slices of the one-iteration split-precision comparison (0: skip)")
+    ap.add_argument("--no-synthetic", dest="synthetic", action="store_false",
+                    help="skip the synthetic block-batch GEMM comparison (FP64 vs split precision)")
     ap.add_argument("--no-plain", dest="plain", action="store_false",
                     help="skip the one-iteration plain (one block per pass) comparison")
     return ap.parse_args()
@@ -112,6 +121,32 @@ def profiled_traffic(family, workload, launches):
     return vals["traffic_bytes_per_launch"], f"profiles/{name}"
 
 
+def synthetic_gemm(gpu, slices, nblk=32, median=600, seed=0):
+    """FP64 vs split-precision matrix-core products on a synthetic block batch:
+    block sizes log-normal (median `median`, sigma 1 in ln) clipped to
+    [16, 4096], one dense N(0,1) operator O and a dense N(0,1) U per block,
+    the product O U of every block in one batched launch, device-timed."""
+    import numpy as np
+    gpu.init(0)
+    rng = np.random.default_rng(seed)
+    sizes = np.clip(np.exp(rng.normal(np.log(median), 1.0, size=nblk)), 16, 4096).astype(int)
+    As = [rng.standard_normal((n, n)) for n in sizes]
+    Bs = [rng.standard_normal((n, n)) for n in sizes]
+    flops = float(sum(2.0 * n ** 3 for n in sizes))
+    C0, t0 = gpu.dgemm_batched(As, Bs, slices=0, repeat=3)
+    out = {"config": f"{nblk} blocks, n log-normal(median {median}, sigma 1) in [16, 4096], seed {seed}, "
+                     f"one product O U per block (n <= {int(sizes.max())}, sum 2n^3 = {flops:.3e})",
+           "fp64": {"ms": t0, "tflops": flops / (t0 * 1e-3) / 1e12}}
+    for s in slices:
+        Cs, ts = gpu.dgemm_batched(As, Bs, slices=s, repeat=3)
+        err = max(float(np.abs(c - r).max() / np.abs(r).max()) for c, r in zip(Cs, C0))
+        pairs = s * (s + 1) // 2
+        out[f"split{s}"] = {"ms": ts, "tflops_fp64_equiv": flops / (ts * 1e-3) / 1e12,
+                            "fp16_mfma_tflops": pairs * flops / (ts * 1e-3) / 1e12,
+                            "max_rel_err_vs_fp64": err, "digit_products": pairs}
+    return out
+
+
 def copy_calibration(nbytes):
     """A plain HBM copy of known size: under `rocprofv3 --pmc FETCH_SIZE` (or
     WRITE_SIZE) its counter reading per dispatch, against nbytes, gives the
@@ -150,6 +185,7 @@ def main():
     models.write_inputs(spec, tmp)
     h = Host(tmp, device=0, sort_mode=0 if a.sort == "exact" else 1)
     gpu.load().nrg_gpu_set_tridiag_schedule(1 if a.tridiag_align == "end" else 0)
+    gpu.load().nrg_gpu_set_gemm_split(a.split)
     its = list(range(spec.first_iteration, spec.first_iteration + need))
     for it in its[:a.warmup]:
         h.iterate(it)
@@ -157,7 +193,7 @@ def main():
               file=sys.stderr, flush=True)
     if not a.no_profile:
         fams = None if a.profile_families == "all" else [f for f in a.profile_families.split(",") if f]
-        gpu.profile(True, families=fams)
+        gpu.profile(True, families=fams, every=a.profile_every)
     stages = {}
     flops_eig = 0.0
     t0 = time.perf_counter()
@@ -178,19 +214,42 @@ def main():
     ks = gpu.kernel_stats() if not a.no_profile else {}
     if not a.no_profile:
         gpu.profile(False)
-    plain = None
-    if a.plain and its[-1] + 1 <= spec.last_iteration:
-        # the same path with one block per eigensolver pass (the per-block
-        # schedule of the reference's dsyev loop), one iteration
+    plain = batched64 = batched_split = None
+    nxt = its[-1] + 1
+    if a.plain and a.split > 0 and nxt <= spec.last_iteration:
+        # the batched path with FP64 contraction products, one iteration
+        gpu.load().nrg_gpu_set_gemm_split(0)
+        tb = time.perf_counter()
+        h.iterate(nxt)
+        tb = time.perf_counter() - tb
+        batched64 = {"value": 1.0 / tb, "unit": "iter/s", "iteration": nxt,
+                     "form": "batched FP64: all blocks per eigensolver pass, FP64 contraction products"}
+        print(f"batched FP64 iteration {nxt}: {tb:.3f} s", file=sys.stderr, flush=True)
+        nxt += 1
+    if a.plain and a.split == 0 and a.split_leg > 0 and nxt <= spec.last_iteration:
+        # the batched path with split-precision contraction products, one iteration
+        gpu.load().nrg_gpu_set_gemm_split(a.split_leg)
+        tb = time.perf_counter()
+        h.iterate(nxt)
+        tb = time.perf_counter() - tb
+        batched_split = {"value": 1.0 / tb, "unit": "iter/s", "iteration": nxt,
+                         "form": f"batched eigensolve + split-precision contraction products ({a.split_leg} "
+                                 f"slices, FP16 matrix cores)"}
+        print(f"batched split iteration {nxt}: {tb:.3f} s", file=sys.stderr, flush=True)
+        nxt += 1
+    gpu.load().nrg_gpu_set_gemm_split(0)
+    if a.plain and nxt <= spec.last_iteration:
+        # one block per eigensolver pass (the per-block schedule of the
+        # reference's dsyev loop) and FP64 products, one iteration
         gpu.load().nrg_gpu_set_batch_limit(1)
         tp = time.perf_counter()
-        h.iterate(its[-1] + 1)
+        h.iterate(nxt)
         tp = time.perf_counter() - tp
         gpu.load().nrg_gpu_set_batch_limit(0)
         st = gpu.stage_times()
-        plain = {"value": 1.0 / tp, "unit": "iter/s", "iteration": its[-1] + 1,
-                 "eig_s": st["eig"] * 1e-3, "form": "plain FP64: one block per eigensolver pass"}
-        print(f"plain iteration {its[-1] + 1}: {tp:.3f} s", file=sys.stderr, flush=True)
+        plain = {"value": 1.0 / tp, "unit": "iter/s", "iteration": nxt,
+                 "eig_s": st["eig"] * 1e-3, "form": "plain FP64: one block per eigensolver pass, FP64 products"}
+        print(f"plain iteration {nxt}: {tp:.3f} s", file=sys.stderr, flush=True)
     sizes = h.subspace()[:, h.i["nv"]].tolist()
     nfull, nsub, nmax = h.i["nfull"], h.i["nsub"], int(max(sizes))
     h.close()
@@ -215,12 +274,18 @@ def main():
                    "nkeep": spec.nkeep, "timed_iterations": [its[a.warmup], its[-1]],
                    "states_per_iteration": nfull, "blocks_per_iteration": nsub, "largest_block": nmax,
                    "sort": a.sort, "tridiag_schedule": a.tridiag_align,
-                   "form": "batched FP64 (plain precision)"},
+                   "form": ("batched eigensolve (FP64) + split-precision contraction GEMMs "
+                            f"({a.split} slices on FP16 matrix cores)") if a.split > 0 else
+                           "batched eigensolve (FP64) + FP64 contraction GEMMs"},
         "stages_s_per_step": {k: v / a.steps for k, v in stages.items()},
         "eig_alg_tflops": flops_eig / max(stages.get("gpu_eig", 1e-9), 1e-9) / 1e12,
         "plain_fp64": plain,
+        "batched_fp64": batched64,
+        "batched_split": batched_split,
         "batched_over_plain": (value / plain["value"]) if plain else None,
     }
+    if a.synthetic:
+        out["synthetic_split_gemm"] = synthetic_gemm(gpu, (8, 6))
     if ks:
         fam = max(ks, key=lambda f: ks[f]["ms"])
         k = ks[fam]
@@ -233,13 +298,17 @@ def main():
             roof = {"bound": "mfma", "achieved": ach, "peak": FP64_PEAK_TFS, "unit": "TFLOP/s",
                     "frac": ach / FP64_PEAK_TFS, "traffic": None}
         roof["kernel_family"] = fam
-        roof["launches"] = k["launches"]
+        roof["launches_timed"] = k["launches"]
+        roof["timing"] = (f"HIP events on every {a.profile_every}th launch of the family over the timed steps "
+                          f"(achieved = algorithmic bytes or flops / event time of those launches)")
         roof["avg_launch_us"] = 1e3 * k["ms"] / max(k["launches"], 1)
         roof["alg_bytes_per_launch"] = k["bytes"] / max(k["launches"], 1)
         roof["alg_flops_per_launch"] = k["flops"] / max(k["launches"], 1)
         roof["traffic"], roof["traffic_source"] = profiled_traffic(fam, a.workload, k["launches"])
         out["roofline"] = roof
-        out["kernel_ms_per_step"] = {f: v["ms"] / a.steps for f, v in ks.items() if v["launches"]}
+        # sampled launches scaled to all launches of the family
+        out["kernel_ms_per_step_est"] = {f: v["ms"] * a.profile_every / a.steps for f, v in ks.items()
+                                         if v["launches"]}
     else:
         out["roofline"] = None
     if not a.no_cpu_baseline:

@@ -158,6 +158,27 @@ int nrg_gpu_dgemm(int transa, int transb, int m, int n, int k, double alpha,
                   const double* A, int lda, const double* B, int ldb, double beta,
                   double* C, int ldc);
 
+/* ---- split-precision contraction GEMMs ------------------------------------
+ * Form with no counterpart in the reference: the contraction products of the
+ * invariant (invariantMatrixDGEMM.F90:300,316), spectral invariant
+ * (invariantMatrixForSpectrumDGEMM.F90:296,312), CFS weight
+ * (module.F90:1100,1113) and reduced-density (reducedDensityMatrixDGEMM.F90:
+ * 280,296) call sites run as FP64 products recovered from FP16 matrix-core
+ * products of `slices` signed 7-bit digits per element (Ozaki slicing, FP32
+ * accumulation flushed exactly to FP64).  0 (the default) keeps FP64 matrix
+ * cores; 3..8 slices.  The eigensolver always stays FP64. */
+int nrg_gpu_set_gemm_split(int slices);
+
+/* Batched GEMM on host arrays (kernel-level interface for the synthetic
+ * block batch): task i computes C_i = op(A_i) op(B_i), m x n x k = mnk[3i..],
+ * packed column-major at the given element offsets (ld = stored rows).
+ * slices 0: FP64 matrix cores, 3..8: split precision.  With repeat > 0 the
+ * product is run `repeat` more times on the device and *ms receives the mean
+ * milliseconds per batch (HIP events). */
+int nrg_gpu_dgemm_batched(int ntask, const int32_t* mnk, int transa, int transb, const double* A,
+                          const int64_t* offA, const double* B, const int64_t* offB, double* C,
+                          const int64_t* offC, int slices, int repeat, double* ms);
+
 /* Per-kernel-family HIP-event timing (no counterpart in the reference; the
  * reference's count.f90 timers measure host wall clock only).  enable = 0
  * stops, -1 times every family, any other value is a bitmask of the
@@ -174,6 +195,9 @@ int nrg_gpu_dgemm(int transa, int transb, int m, int n, int k, double alpha,
  * gather, 19 other. */
 int nrg_gpu_profile(int enable);
 int nrg_gpu_kernel_stats(double* out, int nfam);
+/* Time only every `every`-th launch of each timed family (default 1: all);
+ * the statistics then cover the sampled launches. */
+int nrg_gpu_profile_sample(int every);
 
 /* Scheduling option with no counterpart in the reference: solve the blocks
  * of an iteration in groups of at most max_blocks (0 = all in one batched

@@ -479,6 +479,61 @@ int nrg_gpu_dgemm(int transa, int transb, int m, int n, int k, double alpha, con
   });
 }
 
+int nrg_gpu_dgemm_batched(int ntask, const int32_t* mnk, int transa, int transb, const double* A,
+                          const int64_t* offA, const double* B, const int64_t* offB, double* C_,
+                          const int64_t* offC, int slices, int repeat, double* ms) {
+  return guarded([&](Context& C) {
+    need_init(C);
+    NRG_REQUIRE(ntask >= 0 && mnk && offA && offB && offC, "dgemm_batched: bad arguments");
+    NRG_REQUIRE(slices == 0 || (slices >= 3 && slices <= 8), "dgemm_batched: slices must be 0 or 3..8");
+    if (ntask == 0) return;
+    GemmBatch b;
+    int64_t na = 0, nb = 0, nc = 0;
+    for (int i = 0; i < ntask; ++i) {
+      const int m = mnk[3 * i], n = mnk[3 * i + 1], k = mnk[3 * i + 2];
+      NRG_REQUIRE(m > 0 && n > 0 && k > 0, "dgemm_batched: sizes must be positive");
+      const int lda = transa ? k : m, ldb = transb ? n : k;
+      na = std::max(na, offA[i] + (int64_t)m * k);
+      nb = std::max(nb, offB[i] + (int64_t)k * n);
+      nc = std::max(nc, offC[i] + (int64_t)m * n);
+      b.add(gemm_task(m, n, k, transa, transb, offA[i], lda, offB[i], ldb, offC[i], m, 1.0, 0.0));
+    }
+    DevBuf<double> dA, dB, dC;
+    dA.upload(A, na, C.stream);
+    dB.upload(B, nb, C.stream);
+    dC.reserve(nc);
+    auto once = [&]() {
+      if (slices > 0) C.gemm.run_split(b, dA.p, dB.p, dC.p, nullptr, nullptr, nullptr, slices, C.stream);
+      else {
+        const int keep = C.gemm.split;
+        C.gemm.split = 0;
+        C.gemm.run(b, dA.p, dB.p, dC.p, nullptr, nullptr, nullptr, C.stream);
+        C.gemm.split = keep;
+      }
+    };
+    once();
+    if (repeat > 0) {
+      NRG_HIP(hipEventRecord(C.ev0, C.stream));
+      for (int r = 0; r < repeat; ++r) once();
+      NRG_HIP(hipEventRecord(C.ev1, C.stream));
+      NRG_HIP(hipEventSynchronize(C.ev1));
+      float t = 0.f;
+      NRG_HIP(hipEventElapsedTime(&t, C.ev0, C.ev1));
+      if (ms) *ms = (double)t / repeat;
+    }
+    dC.download(C_, nc, C.stream);
+    NRG_HIP(hipStreamSynchronize(C.stream));
+    dA.release(); dB.release(); dC.release();
+  });
+}
+
+int nrg_gpu_set_gemm_split(int slices) {
+  return guarded([&](Context& C) {
+    NRG_REQUIRE(slices == 0 || (slices >= 3 && slices <= 8), "set_gemm_split: slices must be 0 or 3..8");
+    C.gemm.split = slices;
+  });
+}
+
 int nrg_gpu_set_batch_limit(int max_blocks) {
   return guarded([&](Context& C) {
     NRG_REQUIRE(max_blocks >= 0, "set_batch_limit: negative limit");
@@ -504,6 +559,13 @@ int nrg_gpu_profile(int enable) {
     profiler().reset();
     profiler().on = (enable != 0);
     profiler().mask = (enable == -1) ? 0xffffffffu : (uint32_t)enable;
+  });
+}
+
+int nrg_gpu_profile_sample(int every) {
+  return guarded([&](Context&) {
+    NRG_REQUIRE(every >= 1, "profile_sample: every must be >= 1");
+    profiler().every = (uint32_t)every;
   });
 }
 

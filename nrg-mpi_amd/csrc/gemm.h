@@ -42,9 +42,17 @@ struct GemmRunner {
   DevBuf<GemmTask> d_tasks;
   DevBuf<int32_t> d_tile_start;
   std::vector<int32_t> h_tile_start;
+  // split > 0: the contraction families (invariant, spectral invariant, CFS
+  // weights, rho) run as split-precision products with `split` slices
+  int split = 0;
   void run(const GemmBatch& b, const double* A, const double* B, double* C,
            const int32_t* gatherA, const int32_t* gatherB, const double* kscale,
            hipStream_t s, int family = KF_OTHER);
+  // FP64 product from FP16 matrix-core products of `slices` (3..8) signed
+  // 7-bit digits per element (gemm_split.hip)
+  void run_split(const GemmBatch& b, const double* A, const double* B, double* C,
+                 const int32_t* gatherA, const int32_t* gatherB, const double* kscale,
+                 int slices, hipStream_t s, int family = KF_OTHER);
   void release() { d_tasks.release(); d_tile_start.release(); }
 };
 

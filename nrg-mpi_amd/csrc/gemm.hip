@@ -274,6 +274,11 @@ void GemmRunner::run(const GemmBatch& b, const double* A, const double* B, doubl
                      hipStream_t s, int family) {
   const int nt = (int)b.tasks.size();
   if (nt == 0) return;
+  if (split > 0 && (family == KF_GEMM_INV || family == KF_GEMM_SPEC || family == KF_GEMM_CFSW ||
+                    family == KF_GEMM_CFSX || family == KF_GEMM_RHO)) {
+    run_split(b, A, B, C, gatherA, gatherB, kscale, split, s, family);
+    return;
+  }
   h_tile_start.resize(nt);
   long long total = 0;
   double flops = 0, bytes = 0;

@@ -43,6 +43,8 @@ struct KStat {
 struct Profiler {
   bool on = false;
   uint32_t mask = 0xffffffffu;   // families timed while on
+  uint32_t every = 1;            // time every `every`-th launch of a family
+  std::array<uint64_t, KF_COUNT> seen{};
   std::array<KStat, KF_COUNT> st{};
   struct Pending { hipEvent_t a, b; int fam; double bytes, flops; };
   std::vector<Pending> pend;
@@ -54,7 +56,10 @@ struct Profiler {
     NRG_HIP(hipEventCreate(&e));
     return e;
   }
-  bool timed(int fam) const { return on && ((mask >> fam) & 1u); }
+  bool timed(int fam) {
+    if (!on || !((mask >> fam) & 1u)) return false;
+    return (seen[fam]++ % every) == 0;
+  }
   // call after the stream has been synchronised
   void drain() {
     for (auto& p : pend) {
@@ -67,7 +72,7 @@ struct Profiler {
     }
     pend.clear();
   }
-  void reset() { st = {}; }
+  void reset() { st = {}; seen = {}; }
 };
 
 Profiler& profiler();

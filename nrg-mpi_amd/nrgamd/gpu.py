@@ -20,6 +20,7 @@ _lib = None
 
 _i32p = ctypes.POINTER(ctypes.c_int32)
 _f64p = ctypes.POINTER(ctypes.c_double)
+_i64p = ctypes.POINTER(ctypes.c_int64)
 
 SIGNATURES = {
     "nrg_gpu_init": (ctypes.c_int, [ctypes.c_int]),
@@ -41,6 +42,7 @@ SIGNATURES = {
     "nrg_gpu_inject_fault": (ctypes.c_int, [ctypes.c_int]),
     "nrg_gpu_profile": (ctypes.c_int, [ctypes.c_int]),
     "nrg_gpu_kernel_stats": (ctypes.c_int, [_f64p, ctypes.c_int]),
+    "nrg_gpu_profile_sample": (ctypes.c_int, [ctypes.c_int]),
     "nrg_gpu_set_spectrum_operators": (ctypes.c_int, [ctypes.c_int, _i32p, _i32p]),
     "nrg_gpu_stock": (ctypes.c_int, [ctypes.c_int, _i32p, ctypes.c_int]),
     "nrg_gpu_rho_init": (ctypes.c_int, [ctypes.c_int]),
@@ -53,6 +55,9 @@ SIGNATURES = {
     "nrg_gpu_history_clear": (ctypes.c_int, []),
     "nrg_gpu_set_batch_limit": (ctypes.c_int, [ctypes.c_int]),
     "nrg_gpu_set_tridiag_schedule": (ctypes.c_int, [ctypes.c_int]),
+    "nrg_gpu_set_gemm_split": (ctypes.c_int, [ctypes.c_int]),
+    "nrg_gpu_dgemm_batched": (ctypes.c_int, [ctypes.c_int, _i32p, ctypes.c_int, ctypes.c_int, _f64p, _i64p, _f64p,
+                                             _i64p, _f64p, _i64p, ctypes.c_int, ctypes.c_int, _f64p]),
 }
 
 KERNEL_FAMILIES = ["assemble", "tridiag_col", "tridiag_symv", "tridiag_gemm", "dc_leaf", "dc_prep",
@@ -115,6 +120,39 @@ def dgemm(A, B, C=None, transa=False, transb=False, alpha=1.0, beta=0.0):
     return C
 
 
+def dgemm_batched(As, Bs, transa=False, transb=False, slices=0, repeat=0):
+    """C_i = op(A_i) op(B_i) for every pair in one batched launch; slices 0:
+    FP64 matrix cores, 3..8: split precision.  Returns (list of C_i, mean ms
+    per batch over `repeat` device-timed runs or None)."""
+    mnk, oa, ob, oc = [], [], [], []
+    a_parts, b_parts = [], []
+    pa = pb = pc = 0
+    for A, B in zip(As, Bs):
+        A = np.asfortranarray(A, dtype=np.float64)
+        B = np.asfortranarray(B, dtype=np.float64)
+        m = A.shape[1] if transa else A.shape[0]
+        k = A.shape[0] if transa else A.shape[1]
+        n = B.shape[0] if transb else B.shape[1]
+        mnk += [m, n, k]
+        oa.append(pa); ob.append(pb); oc.append(pc)
+        a_parts.append(A.ravel(order="F")); b_parts.append(B.ravel(order="F"))
+        pa += A.size; pb += B.size; pc += m * n
+    Ah = np.concatenate(a_parts)
+    Bh = np.concatenate(b_parts)
+    Ch = np.zeros(pc)
+    mnk = np.array(mnk, dtype=np.int32)
+    oa, ob, oc = (np.array(x, dtype=np.int64) for x in (oa, ob, oc))
+    ms = np.zeros(1)
+    _check(load().nrg_gpu_dgemm_batched(len(As), _p(mnk, _i32p), int(transa), int(transb), _p(Ah, _f64p),
+                                        _p(oa, _i64p), _p(Bh, _f64p), _p(ob, _i64p), _p(Ch, _f64p),
+                                        _p(oc, _i64p), slices, repeat, _p(ms, _f64p)))
+    out = []
+    for i in range(len(As)):
+        m, n = int(mnk[3 * i]), int(mnk[3 * i + 1])
+        out.append(Ch[oc[i]:oc[i] + m * n].reshape(m, n, order="F"))
+    return out, (float(ms[0]) if repeat > 0 else None)
+
+
 def syev_batched(blocks):
     """Eigen-decompose a list of symmetric matrices in one batched call.
 
@@ -150,9 +188,11 @@ def rho_get(iteration: int, kept_per_block):
     return out
 
 
-def profile(enable=True, families=None):
+def profile(enable=True, families=None, every=1):
     """Start (clearing) or stop the per-family kernel timing; `families`
-    restricts it to the named families (see KERNEL_FAMILIES)."""
+    restricts it to the named families (see KERNEL_FAMILIES); `every` times
+    only every n-th launch of a family."""
+    _check(load().nrg_gpu_profile_sample(int(every)))
     if not enable:
         code = 0
     elif families is None:

@@ -24,8 +24,11 @@ CASES = {
 }
 
 
+@pytest.mark.parametrize("split", [0, 8], ids=["fp64", "split8"])
 @pytest.mark.parametrize("name", sorted(CASES))
-def test_cfs_spectrum(tmp_path, name):
+def test_cfs_spectrum(tmp_path, name, split):
+    # split 8: the invariant, spectral, CFS and rho products run as
+    # split-precision FP16 matrix-core products (8 slices); same tolerances
     from nrgamd import gpu
     from nrgamd.host import Host
     fx = np.load(os.path.join(GOLD, f"ref_{name}.npz"))
@@ -33,6 +36,7 @@ def test_cfs_spectrum(tmp_path, name):
     base, ov = CASES[name]
     models.write_inputs(models.workload(base, **ov), d)
     h = Host(d, device=0)
+    gpu.load().nrg_gpu_set_gemm_split(split)
     try:
         print(f"[{name}] forward", flush=True)
         h.run_forward()
@@ -73,4 +77,5 @@ def test_cfs_spectrum(tmp_path, name):
         print("rho: worst block trace/eigenvalue error", worst)
         assert worst <= RHO_TOL
     finally:
+        gpu.load().nrg_gpu_set_gemm_split(0)
         h.close()
