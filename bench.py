@@ -44,6 +44,9 @@ def parse():
     ap.add_argument("--profile-families", default="tridiag_symv",
                     help="comma-separated kernel families whose launches carry HIP timing events in the "
                          "timed steps (default: the dominant one, tridiag_symv; 'all' for every family)")
+    ap.add_argument("--launch-log", default="",
+                    help="write a CSV of every launch of the profiled families (ordinal per kernel function "
+                         "from the first iteration, algorithmic bytes and flops; phase warmup/timed)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -103,22 +106,23 @@ def cpu_baseline(block_sizes, budget_s):
                 seconds_per_iteration=total)
 
 
-def profiled_traffic(family, workload, launches):
+def profiled_traffic(family, workload):
     """HBM bytes per launch of `family` on `workload` from the committed
-    rocprofv3 FETCH_SIZE/WRITE_SIZE summary, used only when it covers the
-    same number of launches as this run (the same steps of the same path)."""
+    rocprofv3 FETCH_SIZE/WRITE_SIZE summary (a sample of the family's
+    dispatches in a timed step, with their algorithmic bytes)."""
     name = f"{family}_traffic_{workload.rsplit('_it', 1)[0]}.md"
     path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
-        return None, None
+        return None
     vals = {}
     for line in open(path):
-        if "=" in line and line.split("=")[0].strip() in ("traffic_bytes_per_launch", "launches"):
-            k, v = line.split("=")
+        k, _, v = line.partition("=")
+        if _ and k.strip() in ("traffic_bytes_per_launch", "alg_bytes_per_launch", "traffic_over_algorithmic"):
             vals[k.strip()] = float(v)
-    if int(vals.get("launches", -1)) != int(launches):
-        return None, f"profiles/{name} covers {int(vals.get('launches', -1))} launches, this run {launches}"
-    return vals["traffic_bytes_per_launch"], f"profiles/{name}"
+    if "traffic_bytes_per_launch" not in vals:
+        return None
+    vals["source"] = f"profiles/{name}"
+    return vals
 
 
 def synthetic_gemm(gpu, slices, nblk=32, median=600, seed=0):
@@ -186,6 +190,9 @@ def main():
     h = Host(tmp, device=0, sort_mode=0 if a.sort == "exact" else 1)
     gpu.load().nrg_gpu_set_tridiag_schedule(1 if a.tridiag_align == "end" else 0)
     gpu.load().nrg_gpu_set_gemm_split(a.split)
+    log_fams = [f for f in a.profile_families.split(",") if f and f != "all"] or list(gpu.KERNEL_FAMILIES)
+    if a.launch_log:
+        gpu.launch_log(log_fams)
     its = list(range(spec.first_iteration, spec.first_iteration + need))
     for it in its[:a.warmup]:
         h.iterate(it)
@@ -196,6 +203,7 @@ def main():
         gpu.profile(True, families=fams, every=a.profile_every)
     stages = {}
     flops_eig = 0.0
+    n_warm_log = len(gpu.launch_log_get()) if a.launch_log else 0
     t0 = time.perf_counter()
     for it in its[a.warmup:]:
         h.iterate(it)
@@ -211,6 +219,14 @@ def main():
               f"{st['back']:.0f} ms], post {t['postprocess']:.3f}) nfull={h.i['nfull']} nsub={h.i['nsub']}",
               file=sys.stderr, flush=True)
     elapsed = time.perf_counter() - t0
+    if a.launch_log:
+        L = gpu.launch_log_get()
+        gpu.launch_log(None)
+        with open(a.launch_log, "w") as f:
+            f.write("family,threads,ordinal,bytes,flops,phase\n")
+            for i, r in enumerate(L):
+                f.write(f"{gpu.KERNEL_FAMILIES[int(r[0])]},{int(r[1])},{int(r[2])},{r[3]:.0f},{r[4]:.0f},"
+                        f"{'warmup' if i < n_warm_log else 'timed'}\n")
     ks = gpu.kernel_stats() if not a.no_profile else {}
     if not a.no_profile:
         gpu.profile(False)
@@ -304,7 +320,12 @@ def main():
         roof["avg_launch_us"] = 1e3 * k["ms"] / max(k["launches"], 1)
         roof["alg_bytes_per_launch"] = k["bytes"] / max(k["launches"], 1)
         roof["alg_flops_per_launch"] = k["flops"] / max(k["launches"], 1)
-        roof["traffic"], roof["traffic_source"] = profiled_traffic(fam, a.workload, k["launches"])
+        tr = profiled_traffic(fam, a.workload)
+        if tr:
+            roof["traffic"] = tr["traffic_bytes_per_launch"]
+            roof["traffic_sample"] = {"alg_bytes_per_launch": tr.get("alg_bytes_per_launch"),
+                                      "traffic_over_algorithmic": tr.get("traffic_over_algorithmic"),
+                                      "source": tr["source"]}
         out["roofline"] = roof
         # sampled launches scaled to all launches of the family
         out["kernel_ms_per_step_est"] = {f: v["ms"] * a.profile_every / a.steps for f, v in ks.items()

@@ -198,6 +198,14 @@ int nrg_gpu_kernel_stats(double* out, int nfam);
 /* Time only every `every`-th launch of each timed family (default 1: all);
  * the statistics then cover the sampled launches. */
 int nrg_gpu_profile_sample(int every);
+/* Launch log for matching profiler counters to algorithmic work: with
+ * fam_mask != 0 every launch of a family in the mask is recorded (the call
+ * clears the log; 0 stops).  nrg_gpu_launch_log_get writes up to cap records
+ * out[5i..5i+4] = {family, threads per workgroup, ordinal of the launch among
+ * all launches of the same kernel function since this call,
+ * algorithmic bytes, algorithmic flops} and the record count to *count. */
+int nrg_gpu_launch_log(int fam_mask);
+int nrg_gpu_launch_log_get(double* out, int64_t cap, int64_t* count);
 
 /* Scheduling option with no counterpart in the reference: solve the blocks
  * of an iteration in groups of at most max_blocks (0 = all in one batched

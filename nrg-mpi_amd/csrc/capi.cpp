@@ -569,6 +569,29 @@ int nrg_gpu_profile_sample(int every) {
   });
 }
 
+int nrg_gpu_launch_log(int fam_mask) {
+  return guarded([&](Context&) {
+    profiler().log.clear();
+    profiler().nth.clear();
+    profiler().log_mask = (uint32_t)fam_mask;
+  });
+}
+
+int nrg_gpu_launch_log_get(double* out, int64_t cap, int64_t* count) {
+  return guarded([&](Context&) {
+    const auto& L = profiler().log;
+    if (count) *count = (int64_t)L.size();
+    const int64_t n = std::min<int64_t>(cap, (int64_t)L.size());
+    for (int64_t i = 0; i < n && out; ++i) {
+      out[5 * i + 0] = L[i].fam;
+      out[5 * i + 1] = L[i].threads;
+      out[5 * i + 2] = (double)L[i].nth;
+      out[5 * i + 3] = L[i].bytes;
+      out[5 * i + 4] = L[i].flops;
+    }
+  });
+}
+
 int nrg_gpu_kernel_stats(double* out, int nfam) {
   return guarded([&](Context&) {
     for (int f = 0; f < nfam && f < KF_COUNT; ++f) {

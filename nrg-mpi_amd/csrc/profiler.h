@@ -45,6 +45,19 @@ struct Profiler {
   uint32_t mask = 0xffffffffu;   // families timed while on
   uint32_t every = 1;            // time every `every`-th launch of a family
   std::array<uint64_t, KF_COUNT> seen{};
+  // launch log: one record per launch of the logged families, with the
+  // launch's ordinal among launches of the same kernel function (the
+  // numbering a profiler's per-kernel dispatch counter uses)
+  struct LogRec { int fam, threads; uint64_t nth; double bytes, flops; };
+  uint32_t log_mask = 0;
+  std::vector<LogRec> log;
+  std::vector<std::pair<const void*, uint64_t>> nth;
+  uint64_t next_nth(const void* k) {
+    for (auto& e : nth)
+      if (e.first == k) return ++e.second;
+    nth.emplace_back(k, 1);
+    return 1;
+  }
   std::array<KStat, KF_COUNT> st{};
   struct Pending { hipEvent_t a, b; int fam; double bytes, flops; };
   std::vector<Pending> pend;
@@ -82,6 +95,11 @@ template <typename F, typename... Args>
 inline void plaunch(int fam, double bytes, double flops, F kernel, const dim3& grid, const dim3& block,
                     uint32_t shm, hipStream_t s, Args... args) {
   Profiler& P = profiler();
+  if (P.log_mask) {
+    const uint64_t k = P.next_nth((const void*)kernel);   // counted for every kernel
+    if ((P.log_mask >> fam) & 1u)
+      P.log.push_back(Profiler::LogRec{fam, (int)(block.x * block.y * block.z), k, bytes, flops});
+  }
   if (P.timed(fam)) {
     const hipEvent_t a = P.get(), b = P.get();
     hipExtLaunchKernelGGL(kernel, grid, block, shm, s, a, b, 0, args...);

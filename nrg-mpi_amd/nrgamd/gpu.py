@@ -43,6 +43,8 @@ SIGNATURES = {
     "nrg_gpu_profile": (ctypes.c_int, [ctypes.c_int]),
     "nrg_gpu_kernel_stats": (ctypes.c_int, [_f64p, ctypes.c_int]),
     "nrg_gpu_profile_sample": (ctypes.c_int, [ctypes.c_int]),
+    "nrg_gpu_launch_log": (ctypes.c_int, [ctypes.c_int]),
+    "nrg_gpu_launch_log_get": (ctypes.c_int, [_f64p, ctypes.c_int64, _i64p]),
     "nrg_gpu_set_spectrum_operators": (ctypes.c_int, [ctypes.c_int, _i32p, _i32p]),
     "nrg_gpu_stock": (ctypes.c_int, [ctypes.c_int, _i32p, ctypes.c_int]),
     "nrg_gpu_rho_init": (ctypes.c_int, [ctypes.c_int]),
@@ -202,6 +204,23 @@ def profile(enable=True, families=None, every=1):
         for f in families:
             code |= 1 << KERNEL_FAMILIES.index(f)
     _check(load().nrg_gpu_profile(code))
+
+
+def launch_log(families=None):
+    """Start (families: names) or stop (None) the launch log."""
+    code = 0
+    for f in families or []:
+        code |= 1 << KERNEL_FAMILIES.index(f)
+    _check(load().nrg_gpu_launch_log(code))
+
+
+def launch_log_get():
+    """Records (family, threads, ordinal, bytes, flops) as an (n, 5) array."""
+    n = np.zeros(1, dtype=np.int64)
+    _check(load().nrg_gpu_launch_log_get(None, 0, _p(n, _i64p)))
+    a = np.zeros(5 * max(int(n[0]), 1))
+    _check(load().nrg_gpu_launch_log_get(_p(a, _f64p), int(n[0]), _p(n, _i64p)))
+    return a[:5 * int(n[0])].reshape(-1, 5)
 
 
 def kernel_stats():

@@ -3,15 +3,19 @@
     python -m nrgamd.profsum stats  DIR [--title T] [--command C]
         kernel table from the ``*kernel_stats.csv`` of a
         ``rocprofv3 --kernel-trace --stats -f csv`` run;
-    python -m nrgamd.profsum traffic --fetch DIR --write DIR --copy-fetch DIR
-        --copy-write DIR --copy-bytes B --kernel NAME --last N [--alg-bytes A]
-        HBM bytes per dispatch of NAME over its last N dispatches, from
-        ``rocprofv3 --pmc FETCH_SIZE`` and ``--pmc WRITE_SIZE`` runs (KiB per
-        dispatch), each scaled by what the same counter reads for a plain
-        device copy of B bytes (B read and B written per dispatch).
+        with ``--log LOG --family F``: also the mean duration of the family's
+        dispatches in the timed steps, located through the launch log that
+        bench.py --launch-log wrote in the same run;
+    python -m nrgamd.profsum traffic --log LOG --copy-fetch DIR --copy-write DIR
+        --copy-bytes B --variant REGEX THREADS FIRST FETCH_DIR WRITE_DIR [--variant ...]
+        HBM bytes per dispatch of each kernel variant over the dispatches that
+        ``rocprofv3 --pmc FETCH_SIZE`` / ``--pmc WRITE_SIZE`` runs counted
+        (KiB per dispatch), each scaled by what the same counter reads for a
+        plain device copy of B bytes (B read and B written per dispatch), next
+        to the algorithmic bytes of the same dispatches from the launch log.
 
-The traffic summary ends with a ``traffic_bytes_per_launch = X`` line and a
-``launches = N`` line that bench.py reads back.
+The traffic summary ends with ``traffic_bytes_per_launch``,
+``alg_bytes_per_launch`` and ``launches`` lines that bench.py reads back.
 """
 from __future__ import annotations
 
@@ -42,7 +46,50 @@ def _col(row, *names):
     raise KeyError(f"none of {names} in {list(row)}")
 
 
-def stats(d, title, command, top):
+def read_log(path):
+    """Launch log of bench.py --launch-log: list of dicts."""
+    with open(path, newline="") as fh:
+        return list(csv.DictReader(fh))
+
+
+def _ordinals(rows, pat):
+    """Kernel-trace or counter rows of kernels matching pat, numbered per
+    kernel name in dispatch order from 1 (the profiler's per-kernel dispatch
+    count), as {name: [(ordinal, row), ...]}."""
+    out = {}
+    rows = sorted(rows, key=lambda r: int(_col(r, "Dispatch_Id", "Dispatch_ID")))
+    for r in rows:
+        name = _col(r, "Kernel_Name")
+        if not re.search(pat, name):
+            continue
+        lst = out.setdefault(name, [])
+        lst.append((len(lst) + 1, r))
+    return out
+
+
+def family_window(d, log, family):
+    """Mean duration (us) of the family's dispatches in the timed steps."""
+    rows = _rows(d, "kernel_trace.csv")
+    L = [r for r in read_log(log) if r["family"] == family and r["phase"] == "timed"]
+    want = {}
+    for r in L:
+        want.setdefault(int(r["threads"]), set()).add(int(r["ordinal"]))
+    tot, cnt = 0.0, 0
+    for name, lst in _ordinals(rows, r"k_tr_symv" if family == "tridiag_symv" else family).items():
+        th = None
+        for o, r in lst:
+            wg = int(_col(r, "Workgroup_Size_X", "Workgroup_Size"))
+            th = wg
+            break
+        sel = want.get(th, set())
+        for o, r in lst:
+            if o in sel:
+                tot += int(_col(r, "End_Timestamp")) - int(_col(r, "Start_Timestamp"))
+                cnt += 1
+    return tot / max(cnt, 1) / 1e3, cnt
+
+
+def stats(d, title, command, top, log=None, family=None):
     rows = _rows(d, "kernel_stats.csv")
     recs = []
     for r in rows:
@@ -55,50 +102,64 @@ def stats(d, title, command, top):
     for name, calls, tot, avg, pct in recs[:top]:
         short = re.sub(r"\(.*$", "", name)
         lines.append(f"| `{short}` | {calls} | {tot / 1e3:.0f} | {avg / 1e3:.2f} | {pct:.1f} |")
+    if log and family:
+        us, n = family_window(d, log, family)
+        lines += ["", f"Family `{family}` in the timed steps (dispatches located through the launch log of the "
+                      f"same run): {n} dispatches, mean duration {us:.2f} us.", "",
+                  f"timed_family_avg_us = {us:.3f}", f"timed_family_launches = {n}"]
     return "\n".join(lines) + "\n"
 
 
-def _counter(d, counter, kernel_re):
-    vals = []
-    pat = re.compile(kernel_re)
-    rows = _rows(d, "counter_collection.csv")
-    rows.sort(key=lambda r: int(_col(r, "Dispatch_Id", "Dispatch_ID")))
-    for r in rows:
-        if _col(r, "Counter_Name") != counter:
-            continue
-        if not pat.search(_col(r, "Kernel_Name")):
-            continue
-        vals.append(float(_col(r, "Counter_Value")))
-    if not vals:
-        raise SystemExit(f"no {counter} rows for /{kernel_re}/ under {d}")
-    return vals
+def _counter(d, counter, kernel_re, first=1):
+    """{ordinal: value} of `counter` for dispatches of the kernel matching
+    kernel_re (one kernel name).  The counted dispatches are the per-kernel
+    ordinals first, first+1, ... (--kernel-iteration-range [first-...])."""
+    rows = [r for r in _rows(d, "counter_collection.csv") if _col(r, "Counter_Name") == counter]
+    found = _ordinals(rows, kernel_re)
+    if len(found) != 1:
+        raise SystemExit(f"{counter}: /{kernel_re}/ matches {sorted(found)} under {d}")
+    (name, lst), = found.items()
+    return {first - 1 + o: float(_col(r, "Counter_Value")) for o, r in lst}
 
 
 def traffic(a):
     cf = _counter(a.copy_fetch, "FETCH_SIZE", a.copy_kernel)
     cw = _counter(a.copy_write, "WRITE_SIZE", a.copy_kernel)
-    f_fetch = a.copy_bytes / (1024.0 * max(cf))
-    f_write = a.copy_bytes / (1024.0 * max(cw))
-    kf = _counter(a.fetch, "FETCH_SIZE", a.kernel)[-a.last:]
-    kw = _counter(a.write, "WRITE_SIZE", a.kernel)[-a.last:]
-    if len(kf) != a.last or len(kw) != a.last:
-        raise SystemExit(f"fewer than {a.last} dispatches of {a.kernel} ({len(kf)}, {len(kw)})")
-    fetch_b = sum(kf) / a.last * 1024.0 * f_fetch
-    write_b = sum(kw) / a.last * 1024.0 * f_write
-    t = fetch_b + write_b
+    f_fetch = a.copy_bytes / (1024.0 * max(cf.values()))
+    f_write = a.copy_bytes / (1024.0 * max(cw.values()))
+    log = read_log(a.log)
     lines = [f"# {a.title}", "", "Commands (repo root as cwd):", ""]
     lines += ["    " + c for c in a.command]
-    lines += ["", f"Copy calibration: {a.copy_bytes} bytes read and written per copy dispatch; the counters read "
-                  f"{max(cf):.0f} KiB (FETCH_SIZE) and {max(cw):.0f} KiB (WRITE_SIZE), factors "
-                  f"{f_fetch:.4f} and {f_write:.4f} bytes per counted byte.", "",
-              f"`{a.kernel}`, last {a.last} dispatches (the timed steps):", "",
-              "| quantity | bytes per dispatch |", "|---|---:|",
-              f"| FETCH_SIZE (scaled) | {fetch_b:.0f} |", f"| WRITE_SIZE (scaled) | {write_b:.0f} |",
-              f"| HBM traffic | {t:.0f} |"]
-    if a.alg_bytes:
-        lines.append(f"| algorithmic bytes | {a.alg_bytes:.0f} |")
-        lines += ["", f"traffic / algorithmic = {t / a.alg_bytes:.3f}"]
-    lines += ["", f"traffic_bytes_per_launch = {t:.0f}", f"launches = {a.last}", ""]
+    lines += ["", f"Copy calibration: {a.copy_bytes:.0f} bytes read and written per copy dispatch; the counters "
+                  f"read {max(cf.values()):.0f} KiB (FETCH_SIZE) and {max(cw.values()):.0f} KiB (WRITE_SIZE): "
+                  f"factors {f_fetch:.4f} and {f_write:.4f} bytes per counted byte.", "",
+              "| kernel | dispatches counted | fetch B/dispatch | write B/dispatch | HBM B/dispatch | "
+              "algorithmic B/dispatch | traffic / algorithmic | timed launches |",
+              "|---|---:|---:|---:|---:|---:|---:|---:|"]
+    tot_t = tot_a = tot_n = 0.0
+    for regex, threads, first, fdir, wdir in a.variant:
+        threads, first = int(threads), int(first)
+        kf = _counter(fdir, "FETCH_SIZE", regex, first)
+        kw = _counter(wdir, "WRITE_SIZE", regex, first)
+        common = sorted(set(kf) & set(kw))
+        alg = {int(r["ordinal"]): float(r["bytes"]) for r in log if int(r["threads"]) == threads}
+        timed = sum(1 for r in log if int(r["threads"]) == threads and r["phase"] == "timed")
+        common = [o for o in common if o in alg]
+        if not common:
+            raise SystemExit(f"no counted dispatches of /{regex}/ found in the launch log")
+        fb = sum(kf[o] for o in common) / len(common) * 1024.0 * f_fetch
+        wb = sum(kw[o] for o in common) / len(common) * 1024.0 * f_write
+        ab = sum(alg[o] for o in common) / len(common)
+        lines.append(f"| `{regex}` | {len(common)} (ordinals {common[0]}-{common[-1]}) | {fb:.0f} | {wb:.0f} | "
+                     f"{fb + wb:.0f} | {ab:.0f} | {(fb + wb) / ab:.3f} | {timed} |")
+        # family figure: each variant weighted by its launches in the timed steps
+        tot_t += timed * (fb + wb)
+        tot_a += timed * ab
+        tot_n += timed
+    lines += ["", "Family figures weight each variant by its launches in the timed steps of the logged run.",
+              "", f"traffic_bytes_per_launch = {tot_t / tot_n:.0f}",
+              f"alg_bytes_per_launch = {tot_a / tot_n:.0f}", f"traffic_over_algorithmic = {tot_t / tot_a:.4f}",
+              f"launches = {tot_n:.0f}", ""]
     return "\n".join(lines)
 
 
@@ -110,19 +171,21 @@ def main(argv=None):
     s.add_argument("--title", default="rocprofv3 kernel trace")
     s.add_argument("--command", default="")
     s.add_argument("--top", type=int, default=30)
+    s.add_argument("--log", default=None)
+    s.add_argument("--family", default=None)
     t = sub.add_parser("traffic")
-    for k in ("fetch", "write", "copy_fetch", "copy_write"):
-        t.add_argument("--" + k.replace("_", "-"), dest=k, required=True)
+    t.add_argument("--log", required=True)
+    t.add_argument("--copy-fetch", required=True)
+    t.add_argument("--copy-write", required=True)
     t.add_argument("--copy-bytes", type=float, required=True)
     t.add_argument("--copy-kernel", default="copyBuffer|copy")
-    t.add_argument("--kernel", required=True)
-    t.add_argument("--last", type=int, required=True)
-    t.add_argument("--alg-bytes", type=float, default=0.0, help="algorithmic bytes per dispatch")
+    t.add_argument("--variant", nargs=5, action="append", required=True,
+                   metavar=("REGEX", "THREADS", "FIRST", "FETCH_DIR", "WRITE_DIR"))
     t.add_argument("--title", default="HBM traffic per dispatch")
     t.add_argument("--command", action="append", default=[])
     a = ap.parse_args(argv)
     if a.cmd == "stats":
-        sys.stdout.write(stats(a.dir, a.title, a.command, a.top))
+        sys.stdout.write(stats(a.dir, a.title, a.command, a.top, a.log, a.family))
     else:
         sys.stdout.write(traffic(a))
 
