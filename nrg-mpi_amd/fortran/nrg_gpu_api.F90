@@ -102,6 +102,19 @@ module nrg_gpu_api
        real(c_double), intent(out) :: t(*)
        integer(c_int), value :: n
      end function
+     ! options with no counterpart in the reference
+     integer(c_int) function nrg_gpu_set_gemm_split(slices) bind(C, name="nrg_gpu_set_gemm_split")
+       import :: c_int
+       integer(c_int), value :: slices
+     end function
+     integer(c_int) function nrg_gpu_set_batch_limit(max_blocks) bind(C, name="nrg_gpu_set_batch_limit")
+       import :: c_int
+       integer(c_int), value :: max_blocks
+     end function
+     integer(c_int) function nrg_gpu_set_tridiag_schedule(align_end) bind(C, name="nrg_gpu_set_tridiag_schedule")
+       import :: c_int
+       integer(c_int), value :: align_end
+     end function
   end interface
 
 contains
