@@ -138,6 +138,7 @@ __global__ __launch_bounds__(1024) void k_tr_col(TrArgs a, int t, int jp, int jj
   __shared__ double s_y[16][64];
   __shared__ double s_red[16];
   __shared__ double s_w[64], s_x[64];
+  __shared__ double s_pp[4][64];   // panel sums split over waves 1..4
 
   // ---- every global read in one round: panel rows, row c, the SYMV
   // partials of column c-1 (y, y.v, u1, u2), the column and the reflector
@@ -168,21 +169,19 @@ __global__ __launch_bounds__(1024) void k_tr_col(TrArgs a, int t, int jp, int jj
   if (jp >= 0) {
     const int npan = (n - c + pw - 1) / pw;   // SYMV workgroups of column c-1 (m = n - c)
     // y of row r: the 16 waves split the partial vectors py[q][r-c], q <= (r-c)/pw
-    double t0 = 0.0, t1 = 0.0, t2 = 0.0, t3 = 0.0;
+    double t[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};   // 8 loads in flight per lane
     if (r < n) {
       const int i = r - c;
       const int nq = i / pw + 1;
       const double* col = py + i;
       int q = wv;
-      for (; q + 48 < nq; q += 64) {
-        t0 += col[(int64_t)q * n];
-        t1 += col[(int64_t)(q + 16) * n];
-        t2 += col[(int64_t)(q + 32) * n];
-        t3 += col[(int64_t)(q + 48) * n];
+      for (; q + 112 < nq; q += 128) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) t[u] += col[(int64_t)(q + 16 * u) * n];
       }
-      for (; q < nq; q += 16) t0 += col[(int64_t)q * n];
+      for (; q < nq; q += 16) t[0] += col[(int64_t)q * n];
     }
-    s_y[wv][lane] = (t0 + t1) + (t2 + t3);
+    s_y[wv][lane] = ((t[0] + t[1]) + (t[2] + t[3])) + ((t[4] + t[5]) + (t[6] + t[7]));
     // y.v: one partial per SYMV workgroup
     double s = 0.0;
     for (int q = tid; q < npan; q += 1024) s += symrec[q];
@@ -194,6 +193,20 @@ __global__ __launch_bounds__(1024) void k_tr_col(TrArgs a, int t, int jp, int jj
   }
   __syncthreads();
 
+  // ---- waves 1..4: the panel sums of w (i < jp) and of the column update
+  // (i < jj, i != jp), even and odd i apart; wave 0 finishes them below
+  if (wv >= 1 && wv <= 4) {
+    const int par = (wv - 1) & 1;
+    double acc = 0.0;
+    if (wv <= 2) {
+      for (int i = par; i < jp; i += 2) acc += sW[i][lane] * su1[i] + sV[i][lane] * su2[i];
+    } else {
+      for (int i = par; i < jj; i += 2)
+        if (i != jp) acc += sV[i][lane] * sWc[i] + sW[i][lane] * sVc[i];
+    }
+    s_pp[wv - 1][lane] = acc;
+  }
+  __syncthreads();
   // ---- wave 0: w of column c-1, the Householder vector, column c
   if (wv == 0) {
     double w_r = 0.0, wc = 0.0;
@@ -212,8 +225,7 @@ __global__ __launch_bounds__(1024) void k_tr_col(TrArgs a, int t, int jp, int jj
       wc = tp * (y_c - cc) + al * sVc[jp];
       double acc = 0.0;
       for (int w = 0; w < 16; ++w) acc += s_y[w][lane];
-#pragma unroll 2
-      for (int i = 0; i < jp; ++i) acc -= sW[i][lane] * su1[i] + sV[i][lane] * su2[i];
+      acc -= s_pp[0][lane] + s_pp[1][lane];
       w_r = tp * acc + al * sV[jp][lane];
       if (r < n) {
         P[r + (NB + jp) * ld] = w_r;
@@ -222,14 +234,8 @@ __global__ __launch_bounds__(1024) void k_tr_col(TrArgs a, int t, int jp, int jj
     }
     if (copy) A[r + (c - 1) * ld] = vsl;
     if (jj >= 0) {
-      double col = a_rc;
-#pragma unroll 2
-      for (int i = 0; i < jj; ++i) {
-        const bool cur = (i == jp);
-        const double Wri = cur ? w_r : sW[i][lane];
-        const double Wci = cur ? wc : sWc[i];
-        col -= sV[i][lane] * Wci + Wri * sVc[i];
-      }
+      double col = a_rc - (s_pp[2][lane] + s_pp[3][lane]);
+      if (jp >= 0 && jp < jj) col -= sV[jp][lane] * wc + w_r * sVc[jp];
       if (r < n) {
         A[r + c * ld] = col;
         if (r == c) a.d[a.voff[b] + c] = col;
