@@ -267,7 +267,7 @@ __global__ __launch_bounds__(1024) void k_tr_col(TrArgs a, int t, int jp, int jj
 template <int CPW, int NW>
 __global__ __launch_bounds__(NW * 64) void k_tr_symv(TrArgs a, int t, int jj, int act) {
   constexpr int PW = CPW * NW;
-  extern __shared__ double ax[];   // row parts, rows [j0, m)
+  extern __shared__ double ax[];   // row parts, rows [j0, m), then v of those rows
   int qs, pan;
   auto tiles = [&](int q) {
     const int cq = t - a.coff[q], mq = a.ns[q] - cq - 1;
@@ -326,7 +326,11 @@ __global__ __launch_bounds__(NW * 64) void k_tr_symv(TrArgs a, int t, int jj, in
     }
     s_u[wv][lane] = s;
   }
-  for (int i = j0 + tid; i < m; i += NW * 64) ax[i - j0] = 0.0;
+  double* sv = ax + (m - j0);   // v_i of rows [j0, m), staged once per workgroup
+  for (int i = j0 + tid; i < m; i += NW * 64) {
+    ax[i - j0] = 0.0;
+    sv[i - j0] = vv(i);
+  }
   __syncthreads();
   if (pan == 0 && tid < 2 * jj) {
     const int col = (tid < jj) ? tid : NB + (tid - jj);
@@ -340,17 +344,15 @@ __global__ __launch_bounds__(NW * 64) void k_tr_symv(TrArgs a, int t, int jj, in
     const int j = j0 + wv * CPW + q;
     if (j >= m) break;
     const double* col = At + (int64_t)j * ld;
-    const double vj = vv(j);
-    // the hot loop uses vcol[i]*scal for v_i; only row 0 (column 0, lane 0)
-    // has v_0 = 1 instead, corrected after the loop
+    const double vj = sv[j - j0];
     double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
     int i = j + lane;
     for (; i + 192 < m; i += 256) {
       const double x0 = col[i], x1 = col[i + 64], x2 = col[i + 128], x3 = col[i + 192];
-      s0 += x0 * (vcol[i] * scal);
-      s1 += x1 * (vcol[i + 64] * scal);
-      s2 += x2 * (vcol[i + 128] * scal);
-      s3 += x3 * (vcol[i + 192] * scal);
+      s0 += x0 * sv[i - j0];
+      s1 += x1 * sv[i + 64 - j0];
+      s2 += x2 * sv[i + 128 - j0];
+      s3 += x3 * sv[i + 192 - j0];
       if (i > j) atomicAdd(&ax[i - j0], x0 * vj);
       atomicAdd(&ax[i + 64 - j0], x1 * vj);
       atomicAdd(&ax[i + 128 - j0], x2 * vj);
@@ -358,10 +360,9 @@ __global__ __launch_bounds__(NW * 64) void k_tr_symv(TrArgs a, int t, int jj, in
     }
     for (; i < m; i += 64) {
       const double x = col[i];
-      s0 += x * (vcol[i] * scal);
+      s0 += x * sv[i - j0];
       if (i > j) atomicAdd(&ax[i - j0], x * vj);
     }
-    if (j == 0 && lane == 0) s0 += col[0] * (1.0 - vcol[0] * scal);
     const double t = wave_sum((s0 + s1) + (s2 + s3));
     if (lane == 0) atomicAdd(&ax[j - j0], t);
   }
@@ -371,12 +372,12 @@ __global__ __launch_bounds__(NW * 64) void k_tr_symv(TrArgs a, int t, int jj, in
   for (int i = j0 + tid; i < m; i += NW * 64) {
     const double t = ax[i - j0];
     py[i] = t;
-    yv += t * vv(i);
+    yv += t * sv[i - j0];
   }
   if (tid < PW && j0 + tid < m) {
     const int i = j0 + tid;
     const int r = c + 1 + i;
-    const double vi = vv(i);
+    const double vi = sv[i - j0];
     P[r + jj * ld] = vi;
     P[r + (3 * NB + jj) * ld] = vi;
   }
@@ -1131,7 +1132,7 @@ void BatchedEigensolver::tridiag(hipStream_t s) {
     }
     return b;
   };
-  NRG_REQUIRE(shm <= 64 * 1024, "eigensolver: block too large for the SYMV stage (n > 8192)");
+  NRG_REQUIRE(2 * shm <= 160 * 1024, "eigensolver: block too large for the SYMV stage (n > 10240)");
   int T = 0;
   for (int q = 0; q < nblk; ++q) T = std::max(T, coff[q] + n_[order_[q]]);
   for (int t = 0; t < T; ++t) {
@@ -1170,23 +1171,28 @@ void BatchedEigensolver::tridiag(hipStream_t s) {
     // algorithmic traffic: the lower triangle of the symmetric trailing
     // (n-c-1)^2 block once, plus the vectors
     double bytes = 0, flops = 0;
-    const int pw = CG;
-    int gB = 0, mB = 0;
+    int mB = 0;
     for (int q = 0; q < act; ++q) {
       const int m = rows_at(q, t) - 1;
       if (m <= 0) continue;
       bytes += 8.0 * (0.5 * (double)m * (m + 1.0) + 6.0 * m);
       flops += 2.0 * (double)m * m;
-      gB += ceil_div(m, pw);
       mB = std::max(mB, m);
     }
+    // 32-column panels (16 waves x 2 columns) while the launch is
+    // bandwidth-bound, 16-column panels (16 waves x 1 column) once it is
+    // latency-bound
+    const int pw = (bytes >= 64.0e6) ? 2 * CG : CG;
+    int gB = 0;
+    for (int q = 0; q < act; ++q) {
+      const int m = rows_at(q, t) - 1;
+      if (m > 0) gB += ceil_div(m, pw);
+    }
     if (gB > 0) {
-      // the LDS row vector covers rows [j0, m) of the longest trailing block
-      const uint32_t shB = (uint32_t)(mB * sizeof(double));
-      // 16-column panels: 8 waves x 2 columns while the launch is
-      // bandwidth-bound, 16 waves x 1 column once it is latency-bound
-      if (bytes >= 64.0e6)
-        plaunch(KF_TR_SYMV, bytes, flops, k_tr_symv<2, 8>, dim3(gB), dim3(512), shB, s, ta, t, jj, act);
+      // LDS: row parts and v of rows [j0, m) of the longest trailing block
+      const uint32_t shB = (uint32_t)(2 * mB * sizeof(double));
+      if (pw == 2 * CG)
+        plaunch(KF_TR_SYMV, bytes, flops, k_tr_symv<2, 16>, dim3(gB), dim3(1024), shB, s, ta, t, jj, act);
       else
         plaunch(KF_TR_SYMV, bytes, flops, k_tr_symv<1, 16>, dim3(gB), dim3(1024), shB, s, ta, t, jj, act);
       pw_prev = pw;
