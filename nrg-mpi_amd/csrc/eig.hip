@@ -45,9 +45,10 @@ __device__ inline double wave_sum(double v) {
 // workgroup 0 completes u1 = V^T v and u2 = W^T v, and all workgroups
 // compute y = A v of the trailing block from its lower triangle only:
 // y_i = sum_{j<=i} A(i,j) v_j + sum_{j>i} A(j,i) v_j.  Workgroup p takes the
-// PW columns [PW p, PW p + PW) (PW = 16 or 64: 16 waves, 1 or 4 columns each),
-// streams each column's lower part
-// contiguously, keeps the column dot in registers and adds the row parts
+// PW columns [PW p, PW p + PW) (PW = 32 while the launch is bandwidth-bound,
+// 16 once it is latency-bound: 16 waves, 2 or 1 columns each), stages v of
+// its rows in LDS, streams each column's lower part contiguously, keeps the
+// column dot in registers and adds the row parts
 // A(i,j) v_j into an LDS vector with LDS atomics; that vector is written as
 // py[p][i] (rows i >= PW p) and y_i = sum_{p <= i/PW} py[p][i] is completed
 // in the next column kernel (16 waves share the sum, fixed order), as is
