@@ -969,7 +969,10 @@ __global__ void k_bt_buildv(BtArgs a, int p) {
 // T of the compact WY form from the Gram matrix G = V^T V, built column by
 // column in LDS (NBB^2 doubles; gfx950 gives a workgroup up to 160 KB):
 // T(0:i, i) = T(0:i, 0:i) * (-tau_i G(0:i, i)),  T(i, i) = tau_i.
-__global__ __launch_bounds__(NBB) void k_bt_tmat(BtArgs a, int p) {
+// Row j of the product is shared by the TQ = 4 adjacent lanes 4j..4j+3
+// (terms l = j + sub, j + sub + 4, ...), combined with two xor shuffles.
+constexpr int TQ = 4;
+__global__ __launch_bounds__(NBB * TQ) void k_bt_tmat(BtArgs a, int p) {
   const int b = a.order[blockIdx.x];
   const int n = a.n[b];
   const int c0 = p * NBB;
@@ -980,32 +983,30 @@ __global__ __launch_bounds__(NBB) void k_bt_tmat(BtArgs a, int p) {
   const double* tau = a.tau + a.voff[b] + c0;
   __shared__ double sT[NBB * NBB];
   __shared__ double tmp[NBB];
-  const int j = threadIdx.x;   // thread j owns row j of T (upper triangular)
+  const int j = threadIdx.x / TQ, sub = threadIdx.x % TQ;   // row j, term phase sub
   // G(j, i) is fetched one column ahead so that its latency overlaps the
   // previous column's work
   double gji = 0.0;
   for (int i = 0; i < nbp; ++i) {
     const double gcur = gji;
-    if (i + 1 < nbp && j < i + 1) gji = G[j + (i + 1) * NBB];
-    if (j < i) tmp[j] = -tau[i] * gcur;
+    if (sub == 0 && i + 1 < nbp && j < i + 1) gji = G[j + (i + 1) * NBB];
+    if (sub == 0 && j < i) tmp[j] = -tau[i] * gcur;
     __syncthreads();
-    if (j < i) {
-      double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-      int l = j;
-      for (; l + 3 < i; l += 4) {
-        s0 += sT[j + l * NBB] * tmp[l];
-        s1 += sT[j + (l + 1) * NBB] * tmp[l + 1];
-        s2 += sT[j + (l + 2) * NBB] * tmp[l + 2];
-        s3 += sT[j + (l + 3) * NBB] * tmp[l + 3];
-      }
-      for (; l < i; ++l) s0 += sT[j + l * NBB] * tmp[l];
-      sT[j + i * NBB] = (s0 + s1) + (s2 + s3);
-    } else if (j == i) {
-      sT[i + i * NBB] = tau[i];
+    double acc = 0.0;
+    if (j < i)
+      for (int l = j + sub; l < i; l += TQ) acc += sT[j + l * NBB] * tmp[l];
+    acc += __shfl_xor(acc, 1, 64);
+    acc += __shfl_xor(acc, 2, 64);
+    if (sub == 0) {
+      if (j < i) sT[j + i * NBB] = acc;
+      else if (j == i) sT[i + i * NBB] = tau[i];
     }
     __syncthreads();
   }
-  for (int i = 0; i < NBB; ++i) T[j + i * NBB] = (i < nbp && j <= i) ? sT[j + i * NBB] : 0.0;
+  for (int e = threadIdx.x; e < NBB * NBB; e += NBB * TQ) {
+    const int r = e % NBB, i = e / NBB;
+    T[e] = (i < nbp && r <= i) ? sT[r + i * NBB] : 0.0;
+  }
 }
 
 // ===========================================================================
@@ -1340,7 +1341,7 @@ void BatchedEigensolver::back_transform(const int* lo, const int* hi, hipStream_
                            (int64_t)b * 2 * NBB * NBB, NBB, 1.0, 0.0));
     }
     gemm_.run(batch_, d_V_.p, d_V_.p, d_T_.p, nullptr, nullptr, nullptr, s, KF_GEMM_BACK);
-    hipLaunchKernelGGL(k_bt_tmat, dim3(act), dim3(NBB), 0, s, ba, p);
+    plaunch(KF_BT_SMALL, 0, 0, k_bt_tmat, dim3(act), dim3(NBB * TQ), 0, s, ba, p);
     // X = V^T Z(c0+1:, :nc)
     batch_.clear();
     for (int q = 0; q < act; ++q) {
