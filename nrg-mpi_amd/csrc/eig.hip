@@ -599,10 +599,11 @@ __global__ __launch_bounds__(256) void k_dc_merge_prep(DcArgs a, const int32_t* 
       if (rp2 * fabs(szs[j]) <= tol) { df[nd_def++] = j; continue; }
       if (pj < 0) { pj = j; continue; }
       double sv = szs[pj], cv = szs[j];
-      const double tt = hypot(cv, sv);
+      const double tt = sqrt(cv * cv + sv * sv);   // |z| <= 1: no overflow
       const double dt = sds[j] - sds[pj];
-      cv /= tt;
-      sv = -sv / tt;
+      const double rt = 1.0 / tt;
+      cv *= rt;
+      sv = -sv * rt;
       if (fabs(dt * cv * sv) <= tol) {
         szs[j] = tt;
         szs[pj] = 0.0;
@@ -638,9 +639,62 @@ __global__ __launch_bounds__(256) void k_dc_merge_prep(DcArgs a, const int32_t* 
   (void)Qcd;
 }
 
-// Apply the deflation rotations to the merged columns and gather them as
-// [non-deflated | deflated] into Qcd (m x m, ld m).  Grid: (row chunks of 64,
-// merges); each workgroup owns 64 rows of one merge.
+// Apply the deflation rotations to the merged columns, one lane per row.
+// The scan makes chains: a rotation (pj, j) finishes column pj and hands j on
+// as the next pj, so the running column stays in a register and is stored
+// when its chain ends.  The second columns of the next 8 rotations are
+// fetched together: each is a fresh scan position that no earlier rotation
+// has written.  Grid: (row chunks of 256, merges).
+__global__ __launch_bounds__(256) void k_dc_rotate(DcArgs a, const int32_t* tasks, int ntask) {
+  const int t = blockIdx.y;
+  const int32_t* tk = tasks + 8 * t;
+  const int b = tk[0], s = tk[1], e = tk[3];
+  const int m = e - s;
+  const int row = blockIdx.x * 256 + threadIdx.x;
+  if (row >= m) return;
+  const int32_t* I = a.iws + join64(tk[6], tk[7]);
+  const int nrot = I[7 * m];
+  if (nrot == 0) return;
+  const int n = a.n[b];
+  const double* W = a.ws + join64(tk[4], tk[5]);
+  const int32_t *col = I, *rp = I + 4 * m;
+  const double *rc = W + 7 * m, *rsn = W + 8 * m;
+  double* Zr = a.Z + a.aoff[b] + (s + row);
+  int cc = -1;        // carried column and its current value
+  double cvl = 0.0;
+  for (int q0 = 0; q0 < nrot; q0 += 8) {
+    const int nq = min(8, nrot - q0);
+    double y[8];
+    int c1[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (u < nq) {
+        c1[u] = col[rp[2 * (q0 + u) + 1]];
+        y[u] = Zr[(int64_t)c1[u] * n];
+      }
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (u < nq) {
+        const int c0 = col[rp[2 * (q0 + u)]];
+        double x;
+        if (c0 == cc) {
+          x = cvl;
+        } else {
+          if (cc >= 0) Zr[(int64_t)cc * n] = cvl;
+          x = Zr[(int64_t)c0 * n];
+        }
+        const double r = rc[q0 + u], sn = rsn[q0 + u];
+        Zr[(int64_t)c0 * n] = r * x + sn * y[u];
+        cvl = r * y[u] - sn * x;
+        cc = c1[u];
+      }
+  }
+  if (cc >= 0) Zr[(int64_t)cc * n] = cvl;
+}
+
+// Gather the rotated merge columns as [non-deflated | deflated] into Qcd
+// (m x m, ld m).  Grid: (row chunks of 64, merges); each workgroup owns 64
+// rows of one merge, its 4 waves take every 4th column.
 __global__ __launch_bounds__(256) void k_dc_rotgather(DcArgs a, const int32_t* tasks, int ntask) {
   const int t = blockIdx.y;
   const int32_t* tk = tasks + 8 * t;
@@ -653,24 +707,13 @@ __global__ __launch_bounds__(256) void k_dc_rotgather(DcArgs a, const int32_t* t
   double* Z = a.Z + a.aoff[b];
   double* W = a.ws + join64(tk[4], tk[5]);
   const int32_t* I = a.iws + join64(tk[6], tk[7]);
-  const int32_t *col = I, *nd = I + m, *df = I + 2 * m, *rp = I + 4 * m;
-  const double *rc = W + 7 * m, *rsn = W + 8 * m;
+  const int32_t *col = I, *nd = I + m, *df = I + 2 * m;
   double* Qcd = W + dws_vec(m);
-  const int nrot = I[7 * m];
   const int tid = threadIdx.x;
   const int rl = tid & 63, g = tid >> 6;
   const int row = row0 + rl;
-  if (nrot > 0 && g == 0 && row < m) {
-    double* Zr = Z + (s + row);
-    for (int q = 0; q < nrot; ++q) {
-      const int64_t cp = (int64_t)col[rp[2 * q]] * n, cj = (int64_t)col[rp[2 * q + 1]] * n;
-      const double x = Zr[cp], yv = Zr[cj];
-      Zr[cp] = rc[q] * x + rsn[q] * yv;
-      Zr[cj] = rc[q] * yv - rsn[q] * x;
-    }
-  }
-  __syncthreads();
   if (row >= m) return;
+#pragma unroll 8
   for (int c = g; c < m; c += 4) {
     const int srcc = (c < k) ? col[nd[c]] : col[df[c - k]];
     Qcd[row + (int64_t)c * m] = Z[(int64_t)(s + row) + (int64_t)srcc * n];
@@ -923,6 +966,7 @@ __global__ __launch_bounds__(256) void k_dc_scatter(DcArgs a, const int32_t* tas
   const int32_t* I = a.iws + join64(tk[6], tk[7]);
   const double* Qcd = W + dws_vec(m);
   const double* Qt = W + dws_vec(m) + 2 * (int64_t)m * m;
+#pragma unroll 8
   for (int c = g; c < m; c += 4) {
     const int src = I[5 * m + c];
     const double v = (src < 0) ? Qt[row + (int64_t)(-1 - src) * m] : Qcd[row + (int64_t)(k + src) * m];
@@ -1281,6 +1325,7 @@ void BatchedEigensolver::divide_conquer(hipStream_t s) {
       const size_t shm = 2 * sizeof(double) * (size_t)std::min(mmax, DC_LDS_M);
       const int32_t* tk = d_tasks_.p;
       plaunch(KF_DC_PREP, 0, 0, k_dc_merge_prep, dim3(nt), dim3(256), (uint32_t)shm, s, da, tk, nt);
+      plaunch(KF_DC_PREP, 0, 0, k_dc_rotate, dim3(ceil_div(mmax, 256), nt), dim3(256), 0, s, da, tk, nt);
       plaunch(KF_DC_PREP, 0, 0, k_dc_rotgather, dim3(ceil_div(mmax, 64), nt), dim3(256), 0, s, da, tk, nt);
     }
     h_k_.resize(nt);
