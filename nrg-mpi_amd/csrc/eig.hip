@@ -21,7 +21,8 @@ constexpr int RA = 64;     // rows per workgroup, column-update kernel
 constexpr int CG = 16;     // narrowest SYMV panel (16 waves x 1 column); sizes the partial buffers
 constexpr int USTRIDE = 2 * NB + 1;  // column-kernel record: V^T x, W^T x, norm
 constexpr int LEAF = 32;   // divide-and-conquer leaf size
-constexpr int DC_LDS_M = 4000;  // merges up to this size run the deflation scan in LDS
+constexpr int DC_LDS_M = 8192;  // merges up to this size run the deflation scan in LDS (128 KB)
+constexpr int SEC_LDS_M = 4000; // secular solves stage poles and weights in LDS up to this size
 constexpr int NBB = 128;   // back-transform panel width (Z is streamed once per panel)
 constexpr int MAXN = 16384;
 
@@ -765,7 +766,7 @@ __global__ __launch_bounds__(64 * SEC_ROOTS) void k_dc_secular(DcArgs a, const i
   const int64_t io = join64(tk[6], tk[7]);
   double* W = a.ws + wo;
   const double *dl = W + 2 * m, *zl = W + 3 * m;
-  if (k <= DC_LDS_M) {
+  if (k <= SEC_LDS_M) {
     for (int j = threadIdx.x; j < k; j += 64 * SEC_ROOTS) { sec_shm[j] = dl[j]; sec_shm[k + j] = zl[j]; }
     __syncthreads();
     dl = sec_shm;
@@ -909,12 +910,20 @@ __global__ __launch_bounds__(256) void k_dc_finish(DcArgs a, const int32_t* task
   double* Z = a.Z + a.aoff[b];
   const int md = m - k;
   const int tid = threadIdx.x;
-  // rank sort of the deflated values (ties by index)
+  // rank sort of the deflated values (ties by index), values staged in LDS
+  extern __shared__ double fin_shm[];
+  const double* dvs = dv;
+  if (md <= DC_LDS_M) {
+    for (int i = tid; i < md; i += 256) fin_shm[i] = dv[i];
+    __syncthreads();
+    dvs = fin_shm;
+  }
   for (int i = tid; i < md; i += 256) {
-    const double v = dv[i];
+    const double v = dvs[i];
     int r = 0;
+#pragma unroll 4
     for (int j = 0; j < md; ++j) {
-      const double u = dv[j];
+      const double u = dvs[j];
       r += (u < v) || (u == v && j < i);
     }
     dord[r] = i;
@@ -1336,7 +1345,7 @@ void BatchedEigensolver::divide_conquer(hipStream_t s) {
     if (kmax > 0) {
       const int32_t* tk = d_tasks_.p;
       plaunch(KF_DC_SECULAR, 0, 0, k_dc_secular, dim3(ceil_div(kmax, SEC_ROOTS), nt), dim3(64 * SEC_ROOTS),
-              (uint32_t)(2 * sizeof(double) * (size_t)std::min(kmax, DC_LDS_M)), s, da, tk, nt);
+              (uint32_t)(2 * sizeof(double) * (size_t)std::min(kmax, SEC_LDS_M)), s, da, tk, nt);
       plaunch(KF_DC_VECTORS, 0, 0, k_dc_zhat, dim3(ceil_div(kmax, 64), nt), dim3(64), 0, s, da, tk, nt);
       plaunch(KF_DC_VECTORS, 0, 0, k_dc_vectors, dim3(kmax, nt), dim3(64), 0, s, da, tk, nt);
       batch_.clear();
@@ -1352,7 +1361,8 @@ void BatchedEigensolver::divide_conquer(hipStream_t s) {
       gemm_.run(batch_, d_ws_.p, d_ws_.p, d_ws_.p, nullptr, nullptr, nullptr, s, KF_GEMM_DC);
     }
     const int32_t* tk = d_tasks_.p;
-    plaunch(KF_DC_FINISH, 0, 0, k_dc_finish, dim3(nt), dim3(256), 0, s, da, tk, nt);
+    plaunch(KF_DC_FINISH, 0, 0, k_dc_finish, dim3(nt), dim3(256),
+            (uint32_t)(sizeof(double) * (size_t)std::min(mmax, DC_LDS_M)), s, da, tk, nt);
     plaunch(KF_DC_FINISH, 0, 0, k_dc_scatter, dim3(ceil_div(mmax, 64), nt), dim3(256), 0, s, da, tk, nt);
   }
   NRG_HIP(hipGetLastError());
