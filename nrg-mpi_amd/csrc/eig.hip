@@ -554,6 +554,13 @@ __global__ __launch_bounds__(256) void k_dc_merge_prep(DcArgs a, const int32_t* 
   const double sg = (beta >= 0.0) ? 1.0 : -1.0;
   const double* D1 = a.w + vo + s;
   const double* D2 = a.w + vo + mid;
+  if (m <= DC_LDS_M) {
+    // the two ascending lists are adjacent in w: search them in LDS
+    for (int i = tid; i < m; i += 256) dc_shm[i] = D1[i];
+    __syncthreads();
+    D1 = dc_shm;
+    D2 = dc_shm + m1;
+  }
   // merge ranks of the two ascending lists (ties: left first)
   for (int i = tid; i < m; i += 256) {
     double v, z;

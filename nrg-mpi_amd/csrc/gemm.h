@@ -11,6 +11,7 @@
 // signed partial permutations or per-operator dense sub-blocks.
 #pragma once
 #include "common.h"
+#include <cstring>
 #include "profiler.h"
 
 namespace nrg {
@@ -39,9 +40,20 @@ struct GemmBatch {
 // pointers are shared by all tasks; tasks address them through offsets.
 // Buffers for the task table live in the caller-owned workspace.
 struct GemmRunner {
-  DevBuf<GemmTask> d_tasks;
-  DevBuf<int32_t> d_tile_start;
+  // task table and tile prefix sums, uploaded together in one copy
+  DevBuf<char> d_pack;
+  std::vector<char> h_pack;
   std::vector<int32_t> h_tile_start;
+  void upload_tables(const GemmBatch& b, hipStream_t s, const GemmTask*& tk, const int32_t*& ts) {
+    const size_t nt = b.tasks.size();
+    const size_t tb = (nt * sizeof(GemmTask) + 255) / 256 * 256;
+    h_pack.resize(tb + nt * sizeof(int32_t));
+    std::memcpy(h_pack.data(), b.tasks.data(), nt * sizeof(GemmTask));
+    std::memcpy(h_pack.data() + tb, h_tile_start.data(), nt * sizeof(int32_t));
+    d_pack.upload(h_pack.data(), h_pack.size(), s);
+    tk = reinterpret_cast<const GemmTask*>(d_pack.p);
+    ts = reinterpret_cast<const int32_t*>(d_pack.p + tb);
+  }
   // split > 0: the contraction families (invariant, spectral invariant, CFS
   // weights, rho) run as split-precision products with `split` slices
   int split = 0;
@@ -53,7 +65,7 @@ struct GemmRunner {
   void run_split(const GemmBatch& b, const double* A, const double* B, double* C,
                  const int32_t* gatherA, const int32_t* gatherB, const double* kscale,
                  int slices, hipStream_t s, int family = KF_OTHER);
-  void release() { d_tasks.release(); d_tile_start.release(); }
+  void release() { d_pack.release(); }
 };
 
 inline GemmTask gemm_task(int m, int n, int k, int transA, int transB,

@@ -294,15 +294,14 @@ void GemmRunner::run(const GemmBatch& b, const double* A, const double* B, doubl
     total += t.lower ? tm * (tm + 1) / 2 : tm * ceil_div(t.n, TN);
   }
   NRG_REQUIRE(total < (1LL << 31), "gemm: too many tiles");
-  d_tasks.upload(b.tasks, s);
-  d_tile_start.upload(h_tile_start, s);
+  const GemmTask* tk;
+  const int32_t* ts;
+  upload_tables(b, s, tk, ts);
   bool uniform = true;
   for (int i = 1; i < nt && uniform; ++i)
     uniform = (b.tasks[i].transA == b.tasks[0].transA && b.tasks[i].transB == b.tasks[0].transB);
   const int tv = b.tasks[0].transA * 2 + b.tasks[0].transB;
   const dim3 g((unsigned)total), blk(256);
-  const GemmTask* tk = d_tasks.p;
-  const int32_t* ts = d_tile_start.p;
   if (!gatherA && !gatherB && !kscale && uniform) {
     if (tv == 0) plaunch(family, bytes, flops, k_gemm_f64_fast<0, 0>, g, blk, 0, s, tk, ts, nt, A, B, C);
     else if (tv == 1) plaunch(family, bytes, flops, k_gemm_f64_fast<0, 1>, g, blk, 0, s, tk, ts, nt, A, B, C);

@@ -254,11 +254,10 @@ void GemmRunner::run_split(const GemmBatch& b, const double* A, const double* B,
     total += t.lower ? tm * (tm + 1) / 2 : tm * ceil_div(t.n, SP_T);
   }
   NRG_REQUIRE(total < (1LL << 31), "gemm: too many tiles");
-  d_tasks.upload(b.tasks, s);
-  d_tile_start.upload(h_tile_start, s);
+  const GemmTask* tk;
+  const int32_t* ts;
+  upload_tables(b, s, tk, ts);
   const dim3 g((unsigned)total), blk(256);
-  const GemmTask* tk = d_tasks.p;
-  const int32_t* ts = d_tile_start.p;
   switch (slices) {
     case 3: plaunch(family, bytes, flops, k_gemm_split<3>, g, blk, 0, s, tk, ts, nt, A, B, C, gatherA, gatherB, kscale); break;
     case 4: plaunch(family, bytes, flops, k_gemm_split<4>, g, blk, 0, s, tk, ts, nt, A, B, C, gatherA, gatherB, kscale); break;
