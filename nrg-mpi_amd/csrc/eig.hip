@@ -601,36 +601,48 @@ __global__ __launch_bounds__(256) void k_dc_merge_prep(DcArgs a, const int32_t* 
     const double rp2 = 2.0 * rho;
     const double tol = 8.0 * 2.220446049250313e-16 * fmax(fmax(s_red[0], s_red[1]), fmax(s_red[2], s_red[3]));
     s_rho = rp2;
-    // serial deflation scan in ascending order
+    // serial deflation scan in ascending order.  The pole and weight of
+    // the running column pj are carried in registers (written back when pj
+    // is settled) and those of the next position are fetched one step
+    // ahead, so the loop-carried chain is arithmetic only.
     int k = 0, nd_def = 0, nrot = 0, pj = -1;
+    double zp = 0.0, dp = 0.0;
+    double zn = szs[0], dn = sds[0];
     for (int j = 0; j < m; ++j) {
-      if (rp2 * fabs(szs[j]) <= tol) { df[nd_def++] = j; continue; }
-      if (pj < 0) { pj = j; continue; }
-      double sv = szs[pj], cv = szs[j];
-      const double tt = sqrt(cv * cv + sv * sv);   // |z| <= 1: no overflow
-      const double dt = sds[j] - sds[pj];
+      const double zj = zn, dj = dn;
+      if (j + 1 < m) { zn = szs[j + 1]; dn = sds[j + 1]; }
+      if (rp2 * fabs(zj) <= tol) { df[nd_def++] = j; continue; }
+      if (pj < 0) { pj = j; zp = zj; dp = dj; continue; }
+      const double tt = sqrt(zj * zj + zp * zp);   // |z| <= 1: no overflow
       const double rt = 1.0 / tt;
-      cv *= rt;
-      sv = -sv * rt;
+      const double cv = zj * rt, sv = -zp * rt;
+      const double dt = dj - dp;
       if (fabs(dt * cv * sv) <= tol) {
-        szs[j] = tt;
         szs[pj] = 0.0;
         rp[2 * nrot] = pj;
         rp[2 * nrot + 1] = j;
         rc[nrot] = cv;
         rsn[nrot] = sv;
         ++nrot;
-        const double t2 = sds[pj] * cv * cv + sds[j] * sv * sv;
-        sds[j] = sds[pj] * sv * sv + sds[j] * cv * cv;
-        sds[pj] = t2;
+        sds[pj] = dp * cv * cv + dj * sv * sv;
         df[nd_def++] = pj;
         pj = j;
+        zp = tt;
+        dp = dp * sv * sv + dj * cv * cv;
       } else {
+        szs[pj] = zp;
+        sds[pj] = dp;
         nd[k++] = pj;
         pj = j;
+        zp = zj;
+        dp = dj;
       }
     }
-    if (pj >= 0) nd[k++] = pj;
+    if (pj >= 0) {
+      szs[pj] = zp;
+      sds[pj] = dp;
+      nd[k++] = pj;
+    }
     s_k = k;
     s_nrot = nrot;
     a.kout[t] = k;
@@ -654,15 +666,18 @@ __global__ __launch_bounds__(256) void k_dc_merge_prep(DcArgs a, const int32_t* 
 // fetched together: each is a fresh scan position that no earlier rotation
 // has written.  Grid: (row chunks of 256, merges).
 __global__ __launch_bounds__(256) void k_dc_rotate(DcArgs a, const int32_t* tasks, int ntask) {
+  constexpr int CH = 1024;   // rotations staged in LDS per round
+  __shared__ int32_t s_c[2 * CH];
+  __shared__ double s_r[2 * CH];
   const int t = blockIdx.y;
   const int32_t* tk = tasks + 8 * t;
   const int b = tk[0], s = tk[1], e = tk[3];
   const int m = e - s;
-  const int row = blockIdx.x * 256 + threadIdx.x;
-  if (row >= m) return;
+  if (blockIdx.x * 256 >= m) return;
   const int32_t* I = a.iws + join64(tk[6], tk[7]);
   const int nrot = I[7 * m];
   if (nrot == 0) return;
+  const int row = blockIdx.x * 256 + threadIdx.x;
   const int n = a.n[b];
   const double* W = a.ws + join64(tk[4], tk[5]);
   const int32_t *col = I, *rp = I + 4 * m;
@@ -670,34 +685,43 @@ __global__ __launch_bounds__(256) void k_dc_rotate(DcArgs a, const int32_t* task
   double* Zr = a.Z + a.aoff[b] + (s + row);
   int cc = -1;        // carried column and its current value
   double cvl = 0.0;
-  for (int q0 = 0; q0 < nrot; q0 += 8) {
-    const int nq = min(8, nrot - q0);
-    double y[8];
-    int c1[8];
+  for (int c0q = 0; c0q < nrot; c0q += CH) {
+    const int nch = min(CH, nrot - c0q);
+    __syncthreads();   // the previous round's tables are consumed
+    for (int q = threadIdx.x; q < nch; q += 256) {
+      s_c[2 * q] = col[rp[2 * (c0q + q)]];
+      s_c[2 * q + 1] = col[rp[2 * (c0q + q) + 1]];
+      s_r[2 * q] = rc[c0q + q];
+      s_r[2 * q + 1] = rsn[c0q + q];
+    }
+    __syncthreads();
+    if (row >= m) continue;
+    for (int q0 = 0; q0 < nch; q0 += 8) {
+      const int nq = min(8, nch - q0);
+      double y[8];
 #pragma unroll
-    for (int u = 0; u < 8; ++u)
-      if (u < nq) {
-        c1[u] = col[rp[2 * (q0 + u) + 1]];
-        y[u] = Zr[(int64_t)c1[u] * n];
-      }
+      for (int u = 0; u < 8; ++u)
+        if (u < nq) y[u] = Zr[(int64_t)s_c[2 * (q0 + u) + 1] * n];
 #pragma unroll
-    for (int u = 0; u < 8; ++u)
-      if (u < nq) {
-        const int c0 = col[rp[2 * (q0 + u)]];
-        double x;
-        if (c0 == cc) {
-          x = cvl;
-        } else {
-          if (cc >= 0) Zr[(int64_t)cc * n] = cvl;
-          x = Zr[(int64_t)c0 * n];
+      for (int u = 0; u < 8; ++u)
+        if (u < nq) {
+          const int q = q0 + u;
+          const int c0 = s_c[2 * q];
+          double x;
+          if (c0 == cc) {
+            x = cvl;
+          } else {
+            if (cc >= 0) Zr[(int64_t)cc * n] = cvl;
+            x = Zr[(int64_t)c0 * n];
+          }
+          const double r = s_r[2 * q], sn = s_r[2 * q + 1];
+          Zr[(int64_t)c0 * n] = r * x + sn * y[u];
+          cvl = r * y[u] - sn * x;
+          cc = s_c[2 * q + 1];
         }
-        const double r = rc[q0 + u], sn = rsn[q0 + u];
-        Zr[(int64_t)c0 * n] = r * x + sn * y[u];
-        cvl = r * y[u] - sn * x;
-        cc = c1[u];
-      }
+    }
   }
-  if (cc >= 0) Zr[(int64_t)cc * n] = cvl;
+  if (row < m && cc >= 0) Zr[(int64_t)cc * n] = cvl;
 }
 
 // Gather the rotated merge columns as [non-deflated | deflated] into Qcd
