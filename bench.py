@@ -221,12 +221,13 @@ def main():
     elapsed = time.perf_counter() - t0
     if a.launch_log:
         L = gpu.launch_log_get()
+        names = {int(k): gpu.launch_log_kernel(int(k)) for k in set(L[:, 2].tolist())}
         gpu.launch_log(None)
         with open(a.launch_log, "w") as f:
-            f.write("family,threads,ordinal,bytes,flops,phase\n")
+            f.write("family,kernel,threads,ordinal,bytes,flops,phase\n")
             for i, r in enumerate(L):
-                f.write(f"{gpu.KERNEL_FAMILIES[int(r[0])]},{int(r[1])},{int(r[2])},{r[3]:.0f},{r[4]:.0f},"
-                        f"{'warmup' if i < n_warm_log else 'timed'}\n")
+                f.write(f"{gpu.KERNEL_FAMILIES[int(r[0])]},{names[int(r[2])]},{int(r[1])},{int(r[3])},"
+                        f"{r[4]:.0f},{r[5]:.0f},{'warmup' if i < n_warm_log else 'timed'}\n")
     ks = gpu.kernel_stats() if not a.no_profile else {}
     if not a.no_profile:
         gpu.profile(False)

@@ -583,12 +583,25 @@ int nrg_gpu_launch_log_get(double* out, int64_t cap, int64_t* count) {
     if (count) *count = (int64_t)L.size();
     const int64_t n = std::min<int64_t>(cap, (int64_t)L.size());
     for (int64_t i = 0; i < n && out; ++i) {
-      out[5 * i + 0] = L[i].fam;
-      out[5 * i + 1] = L[i].threads;
-      out[5 * i + 2] = (double)L[i].nth;
-      out[5 * i + 3] = L[i].bytes;
-      out[5 * i + 4] = L[i].flops;
+      out[6 * i + 0] = L[i].fam;
+      out[6 * i + 1] = L[i].threads;
+      out[6 * i + 2] = L[i].kid;
+      out[6 * i + 3] = (double)L[i].nth;
+      out[6 * i + 4] = L[i].bytes;
+      out[6 * i + 5] = L[i].flops;
     }
+  });
+}
+
+int nrg_gpu_launch_log_kernel(int kid, char* name, int cap) {
+  return guarded([&](Context&) {
+    const auto& K = profiler().nth;
+    NRG_REQUIRE(kid >= 0 && kid < (int)K.size(), "launch_log_kernel: no such kernel id");
+    NRG_REQUIRE(name && cap > 0, "launch_log_kernel: no buffer");
+    const std::string& nm = K[kid].name;
+    const size_t len = std::min(nm.size(), (size_t)cap - 1);
+    std::memcpy(name, nm.data(), len);
+    name[len] = 0;
   });
 }
 

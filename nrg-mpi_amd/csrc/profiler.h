@@ -5,6 +5,7 @@
 // synchronisation) and accumulates duration, launch count and the
 // algorithmic bytes/flops the launch sites declare.
 #pragma once
+#include <string>
 #include "common.h"
 #include <hip/hip_ext.h>
 #include <array>
@@ -48,14 +49,17 @@ struct Profiler {
   // launch log: one record per launch of the logged families, with the
   // launch's ordinal among launches of the same kernel function (the
   // numbering a profiler's per-kernel dispatch counter uses)
-  struct LogRec { int fam, threads; uint64_t nth; double bytes, flops; };
+  struct LogRec { int fam, threads, kid; uint64_t nth; double bytes, flops; };
+  struct KernelSeen { const void* fn; uint64_t count; std::string name; };
   uint32_t log_mask = 0;
   std::vector<LogRec> log;
-  std::vector<std::pair<const void*, uint64_t>> nth;
-  uint64_t next_nth(const void* k) {
-    for (auto& e : nth)
-      if (e.first == k) return ++e.second;
-    nth.emplace_back(k, 1);
+  std::vector<KernelSeen> nth;   // kernels in order of first launch (kid = index)
+  uint64_t next_nth(const void* k, hipStream_t s, int& kid) {
+    for (size_t i = 0; i < nth.size(); ++i)
+      if (nth[i].fn == k) { kid = (int)i; return ++nth[i].count; }
+    const char* nm = hipKernelNameRefByPtr(k, s);
+    nth.push_back(KernelSeen{k, 1, nm ? std::string(nm) : std::string()});
+    kid = (int)nth.size() - 1;
     return 1;
   }
   std::array<KStat, KF_COUNT> st{};
@@ -96,9 +100,10 @@ inline void plaunch(int fam, double bytes, double flops, F kernel, const dim3& g
                     uint32_t shm, hipStream_t s, Args... args) {
   Profiler& P = profiler();
   if (P.log_mask) {
-    const uint64_t k = P.next_nth((const void*)kernel);   // counted for every kernel
+    int kid = 0;
+    const uint64_t k = P.next_nth((const void*)kernel, s, kid);   // counted for every kernel
     if ((P.log_mask >> fam) & 1u)
-      P.log.push_back(Profiler::LogRec{fam, (int)(block.x * block.y * block.z), k, bytes, flops});
+      P.log.push_back(Profiler::LogRec{fam, (int)(block.x * block.y * block.z), kid, k, bytes, flops});
   }
   if (P.timed(fam)) {
     const hipEvent_t a = P.get(), b = P.get();

@@ -45,6 +45,7 @@ SIGNATURES = {
     "nrg_gpu_profile_sample": (ctypes.c_int, [ctypes.c_int]),
     "nrg_gpu_launch_log": (ctypes.c_int, [ctypes.c_int]),
     "nrg_gpu_launch_log_get": (ctypes.c_int, [_f64p, ctypes.c_int64, _i64p]),
+    "nrg_gpu_launch_log_kernel": (ctypes.c_int, [ctypes.c_int, ctypes.c_char_p, ctypes.c_int]),
     "nrg_gpu_set_spectrum_operators": (ctypes.c_int, [ctypes.c_int, _i32p, _i32p]),
     "nrg_gpu_stock": (ctypes.c_int, [ctypes.c_int, _i32p, ctypes.c_int]),
     "nrg_gpu_rho_init": (ctypes.c_int, [ctypes.c_int]),
@@ -215,12 +216,19 @@ def launch_log(families=None):
 
 
 def launch_log_get():
-    """Records (family, threads, ordinal, bytes, flops) as an (n, 5) array."""
+    """Records (family, threads, kernel id, ordinal, bytes, flops) as an
+    (n, 6) array."""
     n = np.zeros(1, dtype=np.int64)
     _check(load().nrg_gpu_launch_log_get(None, 0, _p(n, _i64p)))
-    a = np.zeros(5 * max(int(n[0]), 1))
+    a = np.zeros(6 * max(int(n[0]), 1))
     _check(load().nrg_gpu_launch_log_get(_p(a, _f64p), int(n[0]), _p(n, _i64p)))
-    return a[:5 * int(n[0])].reshape(-1, 5)
+    return a[:6 * int(n[0])].reshape(-1, 6)
+
+
+def launch_log_kernel(kid):
+    buf = ctypes.create_string_buffer(512)
+    _check(load().nrg_gpu_launch_log_kernel(int(kid), buf, 512))
+    return buf.value.decode()
 
 
 def kernel_stats():

@@ -7,7 +7,7 @@
         dispatches in the timed steps, located through the launch log that
         bench.py --launch-log wrote in the same run;
     python -m nrgamd.profsum traffic --log LOG --copy-fetch DIR --copy-write DIR
-        --copy-bytes B --variant REGEX THREADS FIRST FETCH_DIR WRITE_DIR [--variant ...]
+        --copy-bytes B --variant REGEX FIRST FETCH_DIR WRITE_DIR [--variant ...]
         HBM bytes per dispatch of each kernel variant over the dispatches that
         ``rocprofv3 --pmc FETCH_SIZE`` / ``--pmc WRITE_SIZE`` runs counted
         (KiB per dispatch), each scaled by what the same counter reads for a
@@ -52,6 +52,20 @@ def read_log(path):
         return list(csv.DictReader(fh))
 
 
+def kernel_key(name):
+    """(base name, template integers) of a demangled or mangled kernel name,
+    e.g. 'void nrg::k_tr_symv<2, 16>(...)' and '_ZN3nrg9k_tr_symvILi2ELi16E...'
+    both give ('k_tr_symv', (2, 16))."""
+    m = re.search(r"(k_[a-z0-9_]+)", name)
+    base = m.group(1) if m else name
+    if "<" in name:
+        t = re.search(r"<([^>]*)>", name)
+        ints = tuple(int(x) for x in re.findall(r"-?\d+", t.group(1))) if t else ()
+    else:
+        ints = tuple(int(x) for x in re.findall(r"Li(-?\d+)E", name))
+    return base, ints
+
+
 def _ordinals(rows, pat):
     """Kernel-trace or counter rows of kernels matching pat, numbered per
     kernel name in dispatch order from 1 (the profiler's per-kernel dispatch
@@ -70,18 +84,13 @@ def _ordinals(rows, pat):
 def family_window(d, log, family):
     """Mean duration (us) of the family's dispatches in the timed steps."""
     rows = _rows(d, "kernel_trace.csv")
-    L = [r for r in read_log(log) if r["family"] == family and r["phase"] == "timed"]
     want = {}
-    for r in L:
-        want.setdefault(int(r["threads"]), set()).add(int(r["ordinal"]))
+    for r in read_log(log):
+        if r["family"] == family and r["phase"] == "timed":
+            want.setdefault(kernel_key(r["kernel"]), set()).add(int(r["ordinal"]))
     tot, cnt = 0.0, 0
-    for name, lst in _ordinals(rows, r"k_tr_symv" if family == "tridiag_symv" else family).items():
-        th = None
-        for o, r in lst:
-            wg = int(_col(r, "Workgroup_Size_X", "Workgroup_Size"))
-            th = wg
-            break
-        sel = want.get(th, set())
+    for name, lst in _ordinals(rows, "|".join(sorted({k[0] for k in want}))).items():
+        sel = want.get(kernel_key(name), set())
         for o, r in lst:
             if o in sel:
                 tot += int(_col(r, "End_Timestamp")) - int(_col(r, "Start_Timestamp"))
@@ -119,12 +128,12 @@ def _counter(d, counter, kernel_re, first=1):
     if len(found) != 1:
         raise SystemExit(f"{counter}: /{kernel_re}/ matches {sorted(found)} under {d}")
     (name, lst), = found.items()
-    return {first - 1 + o: float(_col(r, "Counter_Value")) for o, r in lst}
+    return {first - 1 + o: float(_col(r, "Counter_Value")) for o, r in lst}, name
 
 
 def traffic(a):
-    cf = _counter(a.copy_fetch, "FETCH_SIZE", a.copy_kernel)
-    cw = _counter(a.copy_write, "WRITE_SIZE", a.copy_kernel)
+    cf, _ = _counter(a.copy_fetch, "FETCH_SIZE", a.copy_kernel)
+    cw, _ = _counter(a.copy_write, "WRITE_SIZE", a.copy_kernel)
     f_fetch = a.copy_bytes / (1024.0 * max(cf.values()))
     f_write = a.copy_bytes / (1024.0 * max(cw.values()))
     log = read_log(a.log)
@@ -137,13 +146,15 @@ def traffic(a):
               "algorithmic B/dispatch | traffic / algorithmic | timed launches |",
               "|---|---:|---:|---:|---:|---:|---:|---:|"]
     tot_t = tot_a = tot_n = 0.0
-    for regex, threads, first, fdir, wdir in a.variant:
-        threads, first = int(threads), int(first)
-        kf = _counter(fdir, "FETCH_SIZE", regex, first)
-        kw = _counter(wdir, "WRITE_SIZE", regex, first)
+    for regex, first, fdir, wdir in a.variant:
+        first = int(first)
+        kf, name = _counter(fdir, "FETCH_SIZE", regex, first)
+        kw, _ = _counter(wdir, "WRITE_SIZE", regex, first)
+        key = kernel_key(name)
         common = sorted(set(kf) & set(kw))
-        alg = {int(r["ordinal"]): float(r["bytes"]) for r in log if int(r["threads"]) == threads}
-        timed = sum(1 for r in log if int(r["threads"]) == threads and r["phase"] == "timed")
+        mine = [r for r in log if kernel_key(r["kernel"]) == key]
+        alg = {int(r["ordinal"]): float(r["bytes"]) for r in mine}
+        timed = sum(1 for r in mine if r["phase"] == "timed")
         common = [o for o in common if o in alg]
         if not common:
             raise SystemExit(f"no counted dispatches of /{regex}/ found in the launch log")
@@ -179,8 +190,8 @@ def main(argv=None):
     t.add_argument("--copy-write", required=True)
     t.add_argument("--copy-bytes", type=float, required=True)
     t.add_argument("--copy-kernel", default="copyBuffer|copy")
-    t.add_argument("--variant", nargs=5, action="append", required=True,
-                   metavar=("REGEX", "THREADS", "FIRST", "FETCH_DIR", "WRITE_DIR"))
+    t.add_argument("--variant", nargs=4, action="append", required=True,
+                   metavar=("REGEX", "FIRST", "FETCH_DIR", "WRITE_DIR"))
     t.add_argument("--title", default="HBM traffic per dispatch")
     t.add_argument("--command", action="append", default=[])
     a = ap.parse_args(argv)
