@@ -267,7 +267,7 @@ __global__ __launch_bounds__(1024) void k_tr_col(TrArgs a, int t, int jp, int jj
 // Reflector + lower-triangle SYMV kernel, one workgroup (16 waves) per
 // PW = 16 CPW columns of the trailing block of one block.
 template <int CPW, int NW>
-__global__ __launch_bounds__(NW * 64) void k_tr_symv(TrArgs a, int t, int jj, int act) {
+__global__ __launch_bounds__(NW * 64) void k_tr_symv(TrArgs a, int t, int jj, int act, int panel_major) {
   constexpr int PW = CPW * NW;
   extern __shared__ double ax[];   // row parts, rows [j0, m), then v of those rows
   int qs, pan;
@@ -275,7 +275,16 @@ __global__ __launch_bounds__(NW * 64) void k_tr_symv(TrArgs a, int t, int jj, in
     const int cq = t - a.coff[q], mq = a.ns[q] - cq - 1;
     return (cq >= 0 && mq > 0) ? (mq + PW - 1) / PW : 0;
   };
-  if (!flat_map((int)blockIdx.x, act, tiles, qs, pan)) return;
+  if (panel_major) {
+    // workgroup L takes panel L / act of slot L % act: the longest panels of
+    // all blocks are dispatched first (the trailing sizes of the blocks are
+    // nearly equal under the end-aligned schedule)
+    qs = (int)blockIdx.x % act;
+    pan = (int)blockIdx.x / act;
+    if (pan >= tiles(qs)) return;
+  } else if (!flat_map((int)blockIdx.x, act, tiles, qs, pan)) {
+    return;
+  }
   const int c = t - a.coff[qs];
   const int b = a.order[qs];
   const int n = a.ns[qs];
@@ -1269,18 +1278,22 @@ void BatchedEigensolver::tridiag(hipStream_t s) {
     // bandwidth-bound, 16-column panels (16 waves x 1 column) once it is
     // latency-bound
     const int pw = (bytes >= 64.0e6) ? 2 * CG : CG;
-    int gB = 0;
+    int gB = 0, tmax = 0;
     for (int q = 0; q < act; ++q) {
       const int m = rows_at(q, t) - 1;
       if (m > 0) gB += ceil_div(m, pw);
+      if (m > 0) tmax = std::max(tmax, ceil_div(m, pw));
     }
     if (gB > 0) {
       // LDS: row parts and v of rows [j0, m) of the longest trailing block
       const uint32_t shB = (uint32_t)(2 * mB * sizeof(double));
+      // panel-major order when the panel counts are nearly uniform
+      const int pm = (align_end_ && (int64_t)act * tmax <= gB + gB / 8) ? 1 : 0;
+      const int g = pm ? act * tmax : gB;
       if (pw == 2 * CG)
-        plaunch(KF_TR_SYMV, bytes, flops, k_tr_symv<2, 16>, dim3(gB), dim3(1024), shB, s, ta, t, jj, act);
+        plaunch(KF_TR_SYMV, bytes, flops, k_tr_symv<2, 16>, dim3(g), dim3(1024), shB, s, ta, t, jj, act, pm);
       else
-        plaunch(KF_TR_SYMV, bytes, flops, k_tr_symv<1, 16>, dim3(gB), dim3(1024), shB, s, ta, t, jj, act);
+        plaunch(KF_TR_SYMV, bytes, flops, k_tr_symv<1, 16>, dim3(g), dim3(1024), shB, s, ta, t, jj, act, pm);
       pw_prev = pw;
     }
   }
