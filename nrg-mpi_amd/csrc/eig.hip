@@ -113,13 +113,22 @@ __host__ __device__ inline int64_t part_size(int n) { return colrec_size(n) + sy
 //   jj >= 0: update column c with the jj previous panel columns, write it
 //            back, record d[c], the partial norm of rows >= c+2 and the
 //            partial products sum V[r,q] x_r, sum W[r,q] x_r (q < jj).
-__global__ __launch_bounds__(1024) void k_tr_col(TrArgs a, int t, int jp, int jj, int vslot_prev, int pw, int act) {
+__global__ __launch_bounds__(1024) void k_tr_col(TrArgs a, int t, int jp, int jj, int vslot_prev, int pw, int act,
+                                                 int tmax) {
   int qs, tile;
   auto tiles = [&](int q) {
     const int cq = t - a.coff[q];
     return (cq >= 0 && cq < a.ns[q]) ? (a.ns[q] - cq + RA - 1) / RA : 0;
   };
-  if (!flat_map((int)blockIdx.x, act, tiles, qs, tile)) return;
+  if (tmax > 0) {
+    // tile-major order, deepest tiles first: a tile's share of the SYMV
+    // partial sums grows with its row index
+    qs = (int)blockIdx.x % act;
+    tile = tiles(qs) - 1 - (int)blockIdx.x / act;
+    if (tile < 0) return;
+  } else if (!flat_map((int)blockIdx.x, act, tiles, qs, tile)) {
+    return;
+  }
   const int c = t - a.coff[qs];
   if (c == 0) jp = -1;   // a block starting at a panel boundary has no previous column
   const int b = a.order[qs];
@@ -1233,13 +1242,19 @@ void BatchedEigensolver::tridiag(hipStream_t s) {
   for (int t = 0; t < T; ++t) {
     const int jj = t % NB;   // panel position, the same for every slot
     const int act = started(t);
-    int gA = 0;   // flattened grid: the row tiles of the active blocks
-    for (int q = 0; q < act; ++q) gA += ceil_div(rows_at(q, t), RA);
+    int gA = 0, tA = 0;   // flattened grid: the row tiles of the active blocks
+    for (int q = 0; q < act; ++q) {
+      gA += ceil_div(rows_at(q, t), RA);
+      tA = std::max(tA, ceil_div(rows_at(q, t), RA));
+    }
     if (gA == 0) continue;
+    // tile-major order when the tile counts are nearly uniform
+    const int tmA = (align_end_ && (int64_t)act * tA <= gA + gA / 8) ? tA : 0;
+    const int gAl = tmA ? act * tA : gA;
     if (jj == 0 && t > 0) {
       // close the previous panel: finalise its last w, trailing SYR2K
-      plaunch(KF_TR_COL, colbytes(t, act), 0, k_tr_col, dim3(gA), dim3(1024), 0, s, ta, t, NB - 1, -1, NB - 1,
-              pw_prev, act);
+      plaunch(KF_TR_COL, colbytes(t, act), 0, k_tr_col, dim3(gAl), dim3(1024), 0, s, ta, t, NB - 1, -1, NB - 1,
+              pw_prev, act, tmA);
       batch_.clear();
       for (int q = 0; q < act; ++q) {
         const int b = order_[q];
@@ -1255,13 +1270,13 @@ void BatchedEigensolver::tridiag(hipStream_t s) {
         batch_.add(tk);
       }
       gemm_.run(batch_, d_P_.p, d_P_.p, A_, nullptr, nullptr, nullptr, s, KF_GEMM_TRIDIAG);
-      plaunch(KF_TR_COL, colbytes(t, act), 0, k_tr_col, dim3(gA), dim3(1024), 0, s, ta, t, -1, 0, NB - 1, pw_prev,
-              act);
+      plaunch(KF_TR_COL, colbytes(t, act), 0, k_tr_col, dim3(gAl), dim3(1024), 0, s, ta, t, -1, 0, NB - 1, pw_prev,
+              act, tmA);
     } else {
       const int jp = (jj >= 1) ? jj - 1 : -1;
       const int vslot = (t >= 1) ? (t - 1) % NB : 0;
-      plaunch(KF_TR_COL, colbytes(t, act), 0, k_tr_col, dim3(gA), dim3(1024), 0, s, ta, t, jp, jj, vslot, pw_prev,
-              act);
+      plaunch(KF_TR_COL, colbytes(t, act), 0, k_tr_col, dim3(gAl), dim3(1024), 0, s, ta, t, jp, jj, vslot, pw_prev,
+              act, tmA);
     }
     // algorithmic traffic: the lower triangle of the symmetric trailing
     // (n-c-1)^2 block once, plus the vectors
