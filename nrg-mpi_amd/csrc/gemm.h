@@ -36,24 +36,40 @@ struct GemmBatch {
   void clear() { tasks.clear(); }
 };
 
+// One split-K reduction: C(offC, m x n, ldc) = alpha sum of `parts` partial
+// products at wsoff (m x n each, ld m) + beta C.
+struct SplitRed {
+  int64_t offC, wsoff;
+  int m, n, ldc, parts;
+  double alpha, beta;
+};
+
 // Runs all tasks of the batch in one launch.  A/B/C/gather/scale base
 // pointers are shared by all tasks; tasks address them through offsets.
 // Buffers for the task table live in the caller-owned workspace.
 struct GemmRunner {
-  // task table and tile prefix sums, uploaded together in one copy
-  DevBuf<char> d_pack;
+  // task table and tile prefix sums, uploaded together in one copy (two
+  // slots: the unsplit and the split-K launch of one call)
+  DevBuf<char> d_pack[2];
   std::vector<char> h_pack;
   std::vector<int32_t> h_tile_start;
-  void upload_tables(const GemmBatch& b, hipStream_t s, const GemmTask*& tk, const int32_t*& ts) {
+  GemmBatch plain_, part_;
+  std::vector<SplitRed> h_red_;
+  DevBuf<SplitRed> d_red_;
+  DevBuf<double> d_splitw_;
+  void upload_tables(const GemmBatch& b, hipStream_t s, const GemmTask*& tk, const int32_t*& ts, int slot = 0) {
     const size_t nt = b.tasks.size();
     const size_t tb = (nt * sizeof(GemmTask) + 255) / 256 * 256;
     h_pack.resize(tb + nt * sizeof(int32_t));
     std::memcpy(h_pack.data(), b.tasks.data(), nt * sizeof(GemmTask));
     std::memcpy(h_pack.data() + tb, h_tile_start.data(), nt * sizeof(int32_t));
-    d_pack.upload(h_pack.data(), h_pack.size(), s);
-    tk = reinterpret_cast<const GemmTask*>(d_pack.p);
-    ts = reinterpret_cast<const int32_t*>(d_pack.p + tb);
+    d_pack[slot].upload(h_pack.data(), h_pack.size(), s);
+    tk = reinterpret_cast<const GemmTask*>(d_pack[slot].p);
+    ts = reinterpret_cast<const int32_t*>(d_pack[slot].p + tb);
   }
+  void launch(const GemmBatch& b, const double* A, const double* B, double* C,
+              const int32_t* gatherA, const int32_t* gatherB, const double* kscale,
+              hipStream_t s, int family, int slot);
   // split > 0: the contraction families (invariant, spectral invariant, CFS
   // weights, rho) run as split-precision products with `split` slices
   int split = 0;
@@ -65,7 +81,9 @@ struct GemmRunner {
   void run_split(const GemmBatch& b, const double* A, const double* B, double* C,
                  const int32_t* gatherA, const int32_t* gatherB, const double* kscale,
                  int slices, hipStream_t s, int family = KF_OTHER);
-  void release() { d_pack.release(); }
+  void release() {
+    d_pack[0].release(); d_pack[1].release(); d_red_.release(); d_splitw_.release();
+  }
 };
 
 inline GemmTask gemm_task(int m, int n, int k, int transA, int transB,

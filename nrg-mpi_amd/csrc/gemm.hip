@@ -269,6 +269,22 @@ __global__ __launch_bounds__(256) void k_gemm_f64_fast(
       }
 }
 
+// Deterministic split-K reduction: C = alpha sum_c W_c + beta C, partials
+// summed in chunk order.  Grid: (element chunks, records).
+__global__ void k_splitk_reduce(const SplitRed* __restrict__ recs, const double* __restrict__ W, double* C) {
+  const SplitRed r = recs[blockIdx.y];
+  const int64_t mn = (int64_t)r.m * r.n;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < mn; e += (int64_t)gridDim.x * blockDim.x) {
+    double acc = 0.0;
+    for (int c = 0; c < r.parts; ++c) acc += W[r.wsoff + c * mn + e];
+    const int i = (int)(e % r.m), j = (int)(e / r.m);
+    double* cp = C + r.offC + i + (int64_t)j * r.ldc;
+    double v = r.alpha * acc;
+    if (r.beta != 0.0) v += r.beta * (*cp);
+    *cp = v;
+  }
+}
+
 void GemmRunner::run(const GemmBatch& b, const double* A, const double* B, double* C,
                      const int32_t* gatherA, const int32_t* gatherB, const double* kscale,
                      hipStream_t s, int family) {
@@ -279,6 +295,61 @@ void GemmRunner::run(const GemmBatch& b, const double* A, const double* B, doubl
     run_split(b, A, B, C, gatherA, gatherB, kscale, split, s, family);
     return;
   }
+  // Products with a long K and few output tiles (the back-transform's
+  // V^T V and V^T Z) are split along K into partial products written to a
+  // workspace and summed in a fixed order, so that they fill the device.
+  plain_.clear();
+  part_.clear();
+  h_red_.clear();
+  int64_t wsz = 0;
+  for (const GemmTask& t : b.tasks) {
+    const int64_t tiles = (int64_t)ceil_div(t.m, TM) * ceil_div(t.n, TN);
+    int parts = 1;
+    if (!t.lower && t.k >= 1024 && tiles <= 16) parts = std::min(16, t.k / 512);
+    if (parts < 2) { plain_.add(t); continue; }
+    const int chunk = (int)(ceil_div(t.k, parts * TK2) * TK2);
+    parts = ceil_div(t.k, chunk);
+    SplitRed r;
+    r.offC = t.offC; r.wsoff = wsz; r.m = t.m; r.n = t.n; r.ldc = t.ldc; r.parts = parts;
+    r.alpha = t.alpha; r.beta = t.beta;
+    h_red_.push_back(r);
+    for (int c = 0; c < parts; ++c) {
+      const int k0 = c * chunk, kc = std::min(chunk, t.k - k0);
+      GemmTask u = t;
+      u.k = kc;
+      if (!t.transA) u.offA += (int64_t)k0 * t.lda;
+      else if (t.offGA >= 0) u.offGA += k0;
+      else u.offA += k0;
+      if (t.transB) u.offB += (int64_t)k0 * t.ldb;
+      else if (t.offGB >= 0) u.offGB += k0;
+      else u.offB += k0;
+      if (t.offS >= 0) u.offS += k0;
+      u.offC = wsz + (int64_t)c * t.m * t.n;
+      u.ldc = t.m;
+      u.alpha = 1.0;
+      u.beta = 0.0;
+      part_.add(u);
+    }
+    wsz += (int64_t)parts * t.m * t.n;
+  }
+  if (!plain_.tasks.empty()) launch(plain_, A, B, C, gatherA, gatherB, kscale, s, family, 0);
+  if (!part_.tasks.empty()) {
+    d_splitw_.reserve(wsz);
+    launch(part_, A, B, d_splitw_.p, gatherA, gatherB, kscale, s, family, 1);
+    d_red_.upload(h_red_, s);
+    int64_t mx = 0;
+    for (const SplitRed& r : h_red_) mx = std::max<int64_t>(mx, (int64_t)r.m * r.n);
+    const unsigned gx = (unsigned)std::min<int64_t>(ceil_div(mx, 256), 256);
+    plaunch(family, 0, 0, k_splitk_reduce, dim3(gx, (unsigned)h_red_.size()), dim3(256), 0, s,
+            (const SplitRed*)d_red_.p, (const double*)d_splitw_.p, C);
+  }
+  NRG_HIP(hipGetLastError());
+}
+
+void GemmRunner::launch(const GemmBatch& b, const double* A, const double* B, double* C,
+                        const int32_t* gatherA, const int32_t* gatherB, const double* kscale,
+                        hipStream_t s, int family, int slot) {
+  const int nt = (int)b.tasks.size();
   h_tile_start.resize(nt);
   long long total = 0;
   double flops = 0, bytes = 0;
@@ -296,7 +367,7 @@ void GemmRunner::run(const GemmBatch& b, const double* A, const double* B, doubl
   NRG_REQUIRE(total < (1LL << 31), "gemm: too many tiles");
   const GemmTask* tk;
   const int32_t* ts;
-  upload_tables(b, s, tk, ts);
+  upload_tables(b, s, tk, ts, slot);
   bool uniform = true;
   for (int i = 1; i < nt && uniform; ++i)
     uniform = (b.tasks[i].transA == b.tasks[0].transA && b.tasks[i].transB == b.tasks[0].transB);
@@ -310,7 +381,6 @@ void GemmRunner::run(const GemmBatch& b, const double* A, const double* B, doubl
   } else {
     plaunch(family, bytes, flops, k_gemm_f64, g, blk, 0, s, tk, ts, nt, A, B, C, gatherA, gatherB, kscale);
   }
-  NRG_HIP(hipGetLastError());
 }
 
 // ---------------------------------------------------------------------------

@@ -6,7 +6,8 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.mark.parametrize("ta,tb", [(0, 0), (1, 0), (0, 1), (1, 1)])
-@pytest.mark.parametrize("m,n,k", [(1, 1, 1), (17, 5, 3), (64, 64, 64), (130, 77, 200), (5, 300, 0)])
+@pytest.mark.parametrize("m,n,k", [(1, 1, 1), (17, 5, 3), (64, 64, 64), (130, 77, 200), (5, 300, 0),
+                                   (128, 100, 4475), (33, 7, 1030)])   # the last two split along K
 def test_dgemm(gpu, ta, tb, m, n, k):
     rng = np.random.default_rng(m * 1000 + n * 10 + k)
     A = rng.standard_normal((k, m) if ta else (m, k))
