@@ -275,7 +275,7 @@ __global__ __launch_bounds__(1024) void k_tr_col(TrArgs a, int t, int jp, int jj
 
 // Reflector + lower-triangle SYMV kernel, one workgroup (16 waves) per
 // PW = 16 CPW columns of the trailing block of one block.
-template <int CPW, int NW>
+template <int CPW, int NW, int U>
 __global__ __launch_bounds__(NW * 64) void k_tr_symv(TrArgs a, int t, int jj, int act, int panel_major) {
   constexpr int PW = CPW * NW;
   extern __shared__ double ax[];   // row parts, rows [j0, m), then v of those rows
@@ -365,25 +365,30 @@ __global__ __launch_bounds__(NW * 64) void k_tr_symv(TrArgs a, int t, int jj, in
     if (j >= m) break;
     const double* col = At + (int64_t)j * ld;
     const double vj = sv[j - j0];
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    // U loads in flight per lane
+    double sa[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) sa[u] = 0.0;
     int i = j + lane;
-    for (; i + 192 < m; i += 256) {
-      const double x0 = col[i], x1 = col[i + 64], x2 = col[i + 128], x3 = col[i + 192];
-      s0 += x0 * sv[i - j0];
-      s1 += x1 * sv[i + 64 - j0];
-      s2 += x2 * sv[i + 128 - j0];
-      s3 += x3 * sv[i + 192 - j0];
-      if (i > j) atomicAdd(&ax[i - j0], x0 * vj);
-      atomicAdd(&ax[i + 64 - j0], x1 * vj);
-      atomicAdd(&ax[i + 128 - j0], x2 * vj);
-      atomicAdd(&ax[i + 192 - j0], x3 * vj);
+    for (; i + 64 * (U - 1) < m; i += 64 * U) {
+      double x[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) x[u] = col[i + 64 * u];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        sa[u] += x[u] * sv[i + 64 * u - j0];
+        if (u > 0 || i > j) atomicAdd(&ax[i + 64 * u - j0], x[u] * vj);
+      }
     }
     for (; i < m; i += 64) {
       const double x = col[i];
-      s0 += x * sv[i - j0];
+      sa[0] += x * sv[i - j0];
       if (i > j) atomicAdd(&ax[i - j0], x * vj);
     }
-    const double t = wave_sum((s0 + s1) + (s2 + s3));
+    double st = 0.0;
+#pragma unroll
+    for (int u = 0; u < U; u += 2) st += (u + 1 < U) ? (sa[u] + sa[u + 1]) : sa[u];
+    const double t = wave_sum(st);
     if (lane == 0) atomicAdd(&ax[j - j0], t);
   }
   __syncthreads();
@@ -1306,9 +1311,9 @@ void BatchedEigensolver::tridiag(hipStream_t s) {
       const int pm = (align_end_ && (int64_t)act * tmax <= gB + gB / 8) ? 1 : 0;
       const int g = pm ? act * tmax : gB;
       if (pw == 2 * CG)
-        plaunch(KF_TR_SYMV, bytes, flops, k_tr_symv<2, 16>, dim3(g), dim3(1024), shB, s, ta, t, jj, act, pm);
+        plaunch(KF_TR_SYMV, bytes, flops, k_tr_symv<2, 16, 4>, dim3(g), dim3(1024), shB, s, ta, t, jj, act, pm);
       else
-        plaunch(KF_TR_SYMV, bytes, flops, k_tr_symv<1, 16>, dim3(g), dim3(1024), shB, s, ta, t, jj, act, pm);
+        plaunch(KF_TR_SYMV, bytes, flops, k_tr_symv<1, 16, 4>, dim3(g), dim3(1024), shB, s, ta, t, jj, act, pm);
       pw_prev = pw;
     }
   }
