@@ -59,6 +59,8 @@ def parse():
                          "cores, the faster form on MI355X; the split form is measured in its own iteration)")
     ap.add_argument("--split-leg", type=int, default=8,
                     help="slices of the one-iteration split-precision comparison (0: skip)")
+    ap.add_argument("--no-spectrum-leg", dest="spectrum_leg", action="store_false",
+                    help="skip the full-pipeline comparison with spectra (forward + rho sweep + CFS)")
     ap.add_argument("--no-synthetic", dest="synthetic", action="store_false",
                     help="skip the synthetic block-batch GEMM comparison (FP64 vs split precision)")
     ap.add_argument("--no-plain", dest="plain", action="store_false",
@@ -148,6 +150,44 @@ def synthetic_gemm(gpu, slices, nblk=32, median=600, seed=0):
         out[f"split{s}"] = {"ms": ts, "tflops_fp64_equiv": flops / (ts * 1e-3) / 1e12,
                             "fp16_mfma_tflops": pairs * flops / (ts * 1e-3) / 1e12,
                             "max_rel_err_vs_fp64": err, "digit_products": pairs}
+    return out
+
+
+def spectrum_pipeline(gpu, models, Host, nkeep=1000, niter=25):
+    """Full pipeline with spectra (forward with stocks, rho sweep, CFS
+    spikes and binning) on the two-orbital Kanamori model at a reduced
+    Nkeep, once with FP64 contraction products and once split (8 slices)."""
+    import numpy as np
+    out = {"config": f"two-orbital Kanamori, Nkeep={nkeep}, {niter} iterations, 4 spectral types"}
+    bins = {}
+    for split in (0, 8):
+        tmp = tempfile.mkdtemp(prefix="nrgspec_")
+        models.write_inputs(models.workload("two_orbital_nk4000_it45", nkeep=nkeep, niter=niter, spectrum=True), tmp)
+        h = Host(tmp, device=0)
+        gpu.load().nrg_gpu_set_gemm_split(split)
+        try:
+            t0 = time.perf_counter()
+            h.run_forward()
+            t1 = time.perf_counter()
+            h.spectrum()
+            t2 = time.perf_counter()
+            st = h.spectrum_times()
+            neg, pos = h.bins()
+            i = h.info()
+            nit = i["n_iter"] - i["i_first"] + 1
+            out["split8" if split else "fp64"] = {
+                "forward_s": t1 - t0, "forward_iter_per_s": nit / (t1 - t0),
+                "stock_total_s": st["stock"], "rho_sweep_s": st["rho"], "cfs_s": st["cfs"],
+                "spectrum_phase_s": t2 - t1, "pipeline_s": t2 - t0,
+                "sum_rule_max_dev": float(np.max(np.abs(neg.sum(1) + pos.sum(1) - 1.0)))}
+            bins[split] = (neg, pos)
+            if split:
+                peak = max(np.abs(bins[0][0]).max(), np.abs(bins[0][1]).max())
+                diff = max(np.abs(neg - bins[0][0]).max(), np.abs(pos - bins[0][1]).max())
+                out["split8"]["max_bin_diff_vs_fp64_rel_peak"] = float(diff / peak)
+        finally:
+            gpu.load().nrg_gpu_set_gemm_split(0)
+            h.close()
     return out
 
 
@@ -301,6 +341,8 @@ def main():
         "batched_split": batched_split,
         "batched_over_plain": (value / plain["value"]) if plain else None,
     }
+    if a.spectrum_leg:
+        out["spectrum_pipeline"] = spectrum_pipeline(gpu, models, Host)
     if a.synthetic:
         out["synthetic_split_gemm"] = synthetic_gemm(gpu, (8, 6))
     if ks:

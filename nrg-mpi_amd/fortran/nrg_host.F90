@@ -757,7 +757,7 @@ contains
        ierr = nrg_gpu_stock(int(it, c_int), kept_blk, merge(1_c_int, 0_c_int, it == n_iter))
        if (ierr /= 0) return
        call cpu_time_wall(t1)
-       t_stock = t1 - t0
+       t_stock = t_stock + (t1 - t0)   ! summed over the truncated iterations
     end if
     if (it < n_iter) then
        call cpu_time_wall(t0)
@@ -881,6 +881,7 @@ contains
     integer, intent(in) :: device
     integer, intent(out) :: ierr
     double precision :: emax, log10emax
+    t_stock = 0; t_rho = 0; t_cfs = 0
     call read_inputs(dir, ierr)
     if (ierr /= 0) then
        ierr = 100 + ierr
