@@ -155,62 +155,84 @@ __global__ __launch_bounds__(256) void k_gemm_split(
     }
   };
 
-  int slice = 0;
-  for (int k0 = 0; k0 < t.k; k0 += SP_K, ++slice) {
-    // element e = tid + 256 u of the 64 x 32 slices, index order chosen so a
-    // wave reads contiguous stored memory
-#pragma unroll 2
+  // element e = tid + 256 u of the 64 x 32 slices, index order chosen so a
+  // wave reads contiguous stored memory; the next slice's values are
+  // fetched into registers while the matrix cores work on the current one
+  double ra[8], rb[8];
+  auto fetch = [&](int k0) {
+#pragma unroll
     for (int u = 0; u < 8; ++u) {
       const int e = tid + 256 * u;
-      {
-        int i, kk;
-        if (!t.transA) { i = e & 63; kk = e >> 6; } else { kk = e & 31; i = e >> 5; }
-        const int gi = m0 + i, gk = k0 + kk;
-        double r = (gi < t.m && gk < t.k) ? opA(gi, gk) * s_sa[i] : 0.0;
+      int i, kk;
+      if (!t.transA) { i = e & 63; kk = e >> 6; } else { kk = e & 31; i = e >> 5; }
+      const int gi = m0 + i, gk = k0 + kk;
+      ra[u] = (gi < t.m && gk < t.k) ? opA(gi, gk) : 0.0;
+      int j, kb;
+      if (!t.transB) { kb = e & 31; j = e >> 5; } else { j = e & 63; kb = e >> 6; }
+      const int gj = n0 + j, gk2 = k0 + kb;
+      rb[u] = (gj < t.n && gk2 < t.k) ? opB(gk2, gj) : 0.0;
+    }
+  };
+  auto digits = [&]() {
 #pragma unroll
-        for (int s = 0; s < S; ++s) {
-          const double dg = rint(ldexp(r, SP_BETA));
-          r = ldexp(r, SP_BETA) - dg;
-          As[s][i][kk] = (_Float16)dg;
-        }
+    for (int u = 0; u < 8; ++u) {
+      const int e = tid + 256 * u;
+      int i, kk;
+      if (!t.transA) { i = e & 63; kk = e >> 6; } else { kk = e & 31; i = e >> 5; }
+      double r = ra[u] * s_sa[i];
+#pragma unroll
+      for (int s = 0; s < S; ++s) {
+        const double dg = rint(ldexp(r, SP_BETA));
+        r = ldexp(r, SP_BETA) - dg;
+        As[s][i][kk] = (_Float16)dg;
       }
-      {
-        int j, kb;
-        if (!t.transB) { kb = e & 31; j = e >> 5; } else { j = e & 63; kb = e >> 6; }
-        const int gj = n0 + j, gk = k0 + kb;
-        double r = (gj < t.n && gk < t.k) ? opB(gk, gj) * s_sb[j] : 0.0;
+      int j, kb;
+      if (!t.transB) { kb = e & 31; j = e >> 5; } else { j = e & 63; kb = e >> 6; }
+      double q = rb[u] * s_sb[j];
 #pragma unroll
-        for (int s = 0; s < S; ++s) {
-          const double dg = rint(ldexp(r, SP_BETA));
-          r = ldexp(r, SP_BETA) - dg;
-          Bs[s][j][kb] = (_Float16)dg;
-        }
+      for (int s = 0; s < S; ++s) {
+        const double dg = rint(ldexp(q, SP_BETA));
+        q = ldexp(q, SP_BETA) - dg;
+        Bs[s][j][kb] = (_Float16)dg;
       }
     }
+  };
+
+  // lane layout of v_mfma_f32_16x16x32_f16: A row (lane & 15), k 8*(lane>>4)..+7;
+  // B column (lane & 15), same k; D row 4*(lane>>4)+r, column lane & 15
+  const int li = lane & 15, kg = (lane >> 4) * 8;
+  if (t.k > 0) {
+    fetch(0);
+    digits();
     __syncthreads();
-    // lane layout of v_mfma_f32_16x16x32_f16: A row (lane & 15), k 8*(lane>>4)..+7;
-    // B column (lane & 15), same k; D row 4*(lane>>4)+r, column lane & 15
-    const int li = lane & 15, kg = (lane >> 4) * 8;
-    h8 av[S][2];
+  }
+  int slice = 0;
+  for (int k0 = 0; k0 < t.k; k0 += SP_K, ++slice) {
+    const bool more = (k0 + SP_K < t.k);
+    if (more) fetch(k0 + SP_K);
 #pragma unroll
-    for (int s = 0; s < S; ++s)
+    for (int sa = 0; sa < S; ++sa) {
+      h8 av[2];
 #pragma unroll
-      for (int mi = 0; mi < 2; ++mi) av[s][mi] = *(const h8*)&As[s][wm * 32 + mi * 16 + li][kg];
+      for (int mi = 0; mi < 2; ++mi) av[mi] = *(const h8*)&As[sa][wm * 32 + mi * 16 + li][kg];
 #pragma unroll
-    for (int tt = 0; tt < S; ++tt) {
-      h8 bv[2];
+      for (int tt = 0; sa + tt < S; ++tt) {
+        h8 bv[2];
 #pragma unroll
-      for (int ni = 0; ni < 2; ++ni) bv[ni] = *(const h8*)&Bs[tt][wn * 32 + ni * 16 + li][kg];
-#pragma unroll
-      for (int s = 0; s + tt < S; ++s)
+        for (int ni = 0; ni < 2; ++ni) bv[ni] = *(const h8*)&Bs[tt][wn * 32 + ni * 16 + li][kg];
 #pragma unroll
         for (int mi = 0; mi < 2; ++mi)
 #pragma unroll
           for (int ni = 0; ni < 2; ++ni)
-            acc[s + tt][mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_f16(av[s][mi], bv[ni], acc[s + tt][mi][ni], 0, 0, 0);
+            acc[sa + tt][mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_f16(av[mi], bv[ni], acc[sa + tt][mi][ni], 0, 0, 0);
+      }
     }
-    __syncthreads();
     if ((slice + 1) % SP_FLUSH == 0) flush();
+    if (more) {
+      __syncthreads();
+      digits();
+      __syncthreads();
+    }
   }
   flush();
 
