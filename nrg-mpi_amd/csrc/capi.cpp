@@ -93,6 +93,10 @@ int nrg_gpu_init(int device) {
     NRG_HIP(hipStreamCreateWithFlags(&C.stream, hipStreamNonBlocking));
     NRG_HIP(hipEventCreate(&C.ev0));
     NRG_HIP(hipEventCreate(&C.ev1));
+    for (int i = 0; i < 2; ++i) {
+      NRG_HIP(hipEventCreate(&C.ev_back[i]));
+      NRG_HIP(hipEventCreate(&C.ev_inv[i]));
+    }
     C.initialised = true;
   });
 }
@@ -110,6 +114,10 @@ int nrg_gpu_finalize(void) {
     C.d_tmp.release(); C.d_gidx.release(); C.d_gscale.release();
     (void)hipEventDestroy(C.ev0);
     (void)hipEventDestroy(C.ev1);
+    for (int i = 0; i < 2; ++i) {
+      (void)hipEventDestroy(C.ev_back[i]);
+      (void)hipEventDestroy(C.ev_inv[i]);
+    }
     (void)hipStreamDestroy(C.stream);
     C.stream = nullptr;
     C.initialised = false;
@@ -277,6 +285,7 @@ int nrg_gpu_diagonalize(int iteration, int nsub, const int32_t* subspace_info, i
     C.timers.tridiag = C.eig_stats.ms_tridiag;
     C.timers.dc = C.eig_stats.ms_dc;
     C.timers.back = C.eig_stats.ms_back;
+    C.back_timed = C.inv_timed = false;
     C.d_w.download(eigenvalue, nfull, C.stream);
     if (eigenvector) C.d_U.download(eigenvector, n2, C.stream);
     NRG_HIP(hipStreamSynchronize(C.stream));
@@ -407,10 +416,13 @@ int nrg_gpu_invariant(const int32_t* kept_per_block, double* out) {
   return guarded([&](Context& C) {
     need_init(C);
     NRG_REQUIRE(C.nsub > 0, "invariant: no diagonalised iteration");
-    const double t0 = wall_ms();
+    // returns once the work is queued (unless `out` is asked for): the
+    // caller's host work for the next iteration overlaps the device
     ensure_columns(C, kept_per_block);
+    NRG_HIP(hipEventRecord(C.ev_inv[0], C.stream));
     invariant(C, kept_per_block, out);
-    C.timers.invariant = wall_ms() - t0;
+    NRG_HIP(hipEventRecord(C.ev_inv[1], C.stream));
+    C.inv_timed = true;
   });
 }
 
@@ -427,8 +439,17 @@ int nrg_gpu_invariant_spectrum(const int32_t* kept_per_block, double* out) {
 
 int nrg_gpu_stage_times(double* t, int n) {
   return guarded([&](Context& C) {
-    const double v[10] = {C.timers.assemble, C.timers.eig, C.timers.tridiag, C.timers.dc, C.timers.back,
-                          C.timers.invariant, C.timers.spectral, C.timers.rho, C.timers.cfs,
+    auto span = [&](hipEvent_t* e) {
+      NRG_HIP(hipEventSynchronize(e[1]));
+      float ms = 0.f;
+      NRG_HIP(hipEventElapsedTime(&ms, e[0], e[1]));
+      return (double)ms;
+    };
+    double back = C.timers.back;
+    if (C.back_timed) back += span(C.ev_back);
+    const double inv = C.inv_timed ? span(C.ev_inv) : 0.0;
+    const double v[10] = {C.timers.assemble, C.timers.eig, C.timers.tridiag, C.timers.dc, back,
+                          inv, C.timers.spectral, C.timers.rho, C.timers.cfs,
                           C.eig_stats.flops_alg};
     for (int i = 0; i < n && i < 10; ++i) t[i] = v[i];
   });

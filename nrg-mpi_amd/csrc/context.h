@@ -25,6 +25,10 @@ struct Context {
   int fault_iteration = -1000000;   // test hook: fail diagonalisation at this iteration
   int batch_limit = 0;              // max blocks per eigensolver launch sequence (0: all)
   bool u_lazy = false;              // d_U columns are back-transformed on demand (ensure_columns)
+  // stages that return without waiting for the device are timed with event
+  // pairs, read when the stage times are asked for
+  hipEvent_t ev_back[2] = {nullptr, nullptr}, ev_inv[2] = {nullptr, nullptr};
+  bool back_timed = false, inv_timed = false;
 
   // ---- model constants (set once)
   int nv = 0, nops = 0, nchain = 0, ncm = 0, ncoef = 0, nmat = 0;
@@ -73,7 +77,10 @@ inline void ensure_columns(Context& C, const int32_t* hi) {
     h[b] = hi ? std::min(std::max((int)hi[b], 0), C.bn[b]) : C.bn[b];
     all = all && h[b] == C.bn[b];
   }
-  C.eig.back_columns(h.data(), C.stream, &C.timers.back);
+  if (!C.back_timed) NRG_HIP(hipEventRecord(C.ev_back[0], C.stream));   // first transform of the iteration
+  C.eig.back_columns(h.data(), C.stream, nullptr);
+  NRG_HIP(hipEventRecord(C.ev_back[1], C.stream));
+  C.back_timed = true;
   if (all) C.u_lazy = false;
 }
 

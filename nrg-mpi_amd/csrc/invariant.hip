@@ -103,8 +103,10 @@ void invariant(Context& C, const int32_t* kept, double* out_host) {
   }
   std::swap(C.d_inv, C.d_inv_next);
   C.kold = (int)k;
-  if (out_host) C.d_inv.download(out_host, ksz, C.stream);
-  NRG_HIP(hipStreamSynchronize(C.stream));
+  if (out_host) {
+    C.d_inv.download(out_host, ksz, C.stream);
+    NRG_HIP(hipStreamSynchronize(C.stream));
+  }
 }
 
 void invariant_spectrum(Context& C, const int32_t* kept, double* out_host) {
