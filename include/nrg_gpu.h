@@ -85,7 +85,9 @@ int nrg_gpu_diagonalize(int iteration, int nsub, const int32_t* subspace_info, i
  * invariant matrices of the kept basis from the eigenvectors of the last
  * nrg_gpu_diagonalize call.  kept_per_block(nsub) is keeped_basis_number
  * (:84-100).  The result stays on the device as the input of the next
- * diagonalization; invariant_matrix_out (k,k,ncm) may be NULL. */
+ * diagonalization; invariant_matrix_out (k,k,ncm) may be NULL, and then the
+ * call returns as soon as the work is queued on the library's stream (every
+ * later call of the library is ordered after it). */
 int nrg_gpu_invariant(const int32_t* kept_per_block, double* invariant_matrix_out);
 
 /* invariantMatrixForSpectrumDGEMM (invariantMatrixForSpectrumDGEMM.F90:1-392). */
