@@ -107,6 +107,9 @@ int nrg_gpu_finalize(void) {
     (void)hipStreamSynchronize(C.stream);
     cfs().release();
     C.eig.release();
+    C.u_lazy = false;   // no solve left whose columns could be transformed
+    C.back_timed = false;
+    C.inv_timed = false;
     C.gemm.release();
     C.d_op.release(); C.d_hop.release(); C.d_hop_coef.release();
     C.d_basis_in.release(); C.d_H.release(); C.d_U.release(); C.d_w.release(); C.d_Eold.release();

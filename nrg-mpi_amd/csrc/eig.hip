@@ -31,6 +31,42 @@ __device__ inline double wave_sum(double v) {
   return v;
 }
 
+// DPP lane move of a double (two 32-bit moves), all lanes of the row enabled
+template <int CTRL>
+__device__ inline double dpp_move(double v) {
+  const long long b = __builtin_bit_cast(long long, v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+  return __builtin_bit_cast(double, (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
+// Sum over each 16-lane row of the wave, in every lane of the row: quad
+// exchanges (lane ^ 1, lane ^ 2), then rotations of the row by 4 and 8.
+// Register-to-register moves only (no LDS-pipe permutes); the caller keeps
+// all 64 lanes active.
+__device__ inline double row_sum16(double v) {
+  v += dpp_move<0xB1>(v);    // quad_perm [1,0,3,2]
+  v += dpp_move<0x4E>(v);    // quad_perm [2,3,0,1]
+  v += dpp_move<0x124>(v);   // row_ror:4
+  v += dpp_move<0x128>(v);   // row_ror:8
+  return v;
+}
+
+__device__ inline double lane_value(double v, int l) {
+  const long long b = __builtin_bit_cast(long long, v);
+  const int lo = __builtin_amdgcn_readlane((int)b, l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __builtin_bit_cast(double, (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
+// Sum over the 64 lanes of a fully active wave: row sums, then the first
+// lane of each row read into scalars.  The same value in every lane, and
+// the same order of additions in every wave.
+__device__ inline double wave_sum_full(double v) {
+  v = row_sum16(v);
+  return (lane_value(v, 0) + lane_value(v, 16)) + (lane_value(v, 32) + lane_value(v, 48));
+}
+
 // ===========================================================================
 // 1. tridiagonalisation
 // ===========================================================================
@@ -149,21 +185,21 @@ __global__ __launch_bounds__(1024) void k_tr_col(TrArgs a, int t, int jp, int jj
   __shared__ double s_y[16][64];
   __shared__ double s_red[16];
   __shared__ double s_w[64], s_x[64];
-  __shared__ double s_pp[4][64];   // panel sums split over waves 1..4
+  __shared__ double s_pp[16][64];  // panel sums: waves 0-7 for w, 8-15 for the column
 
-  // ---- every global read in one round: panel rows, row c, the SYMV
+  // ---- every global read in one round: panel rows (into registers first,
+  // stored to LDS after the other reads are issued), row c, the SYMV
   // partials of column c-1 (y, y.v, u1, u2), the column and the reflector
   const int nvc = max(jp + 1, jj);          // V columns used (w reads V[:, jp])
   const int nwc = (jp >= 0) ? jp : max(jj, 0);  // W columns read (W[:, jp] is formed here)
-#pragma unroll 1
-  for (int e = tid; e < nvc * RA; e += 1024) {
-    const int col = e / RA, row = e - col * RA, rr = r0 + row;
-    sV[col][row] = (rr < n) ? P[rr + (int64_t)col * ld] : 0.0;
-  }
-#pragma unroll 1
-  for (int e = tid; e < nwc * RA; e += 1024) {
-    const int col = e / RA, row = e - col * RA, rr = r0 + row;
-    sW[col][row] = (rr < n) ? P[rr + (int64_t)(NB + col) * ld] : 0.0;
+  static_assert(NB * RA == 2 * 1024, "two panel elements per thread");
+  double pv[2], pwc[2];
+#pragma unroll
+  for (int u = 0; u < 2; ++u) {
+    const int e = tid + 1024 * u;
+    const int col = e / RA, rr = r0 + (e - col * RA);
+    pv[u] = (col < nvc && rr < n) ? P[rr + (int64_t)col * ld] : 0.0;
+    pwc[u] = (col < nwc && rr < n) ? P[rr + (int64_t)(NB + col) * ld] : 0.0;
   }
   if (tid < nvc) sVc[tid] = P[c + (int64_t)tid * ld];
   else if (tid >= 64 && tid - 64 < nwc) sWc[tid - 64] = P[c + (int64_t)(NB + tid - 64) * ld];
@@ -196,26 +232,38 @@ __global__ __launch_bounds__(1024) void k_tr_col(TrArgs a, int t, int jp, int jj
     // y.v: one partial per SYMV workgroup
     double s = 0.0;
     for (int q = tid; q < npan; q += 1024) s += symrec[q];
-    s = wave_sum(s);
+    s = wave_sum_full(s);
     if (lane == 0) s_red[wv] = s;
     // u1 = V^T v, u2 = W^T v of column c-1 (completed by the SYMV kernel)
     if (tid >= 128 && tid - 128 < jp) su1[tid - 128] = ubuf[tid - 128];
     else if (tid >= 192 && tid - 192 < jp) su2[tid - 192] = ubuf[NB + tid - 192];
   }
+#pragma unroll
+  for (int u = 0; u < 2; ++u) {
+    const int e = tid + 1024 * u;
+    const int col = e / RA, row = e - col * RA;
+    sV[col][row] = pv[u];
+    sW[col][row] = pwc[u];
+  }
   __syncthreads();
 
-  // ---- waves 1..4: the panel sums of w (i < jp) and of the column update
-  // (i < jj, i != jp), even and odd i apart; wave 0 finishes them below
-  if (wv >= 1 && wv <= 4) {
-    const int par = (wv - 1) & 1;
+  // ---- panel sums of w (waves 0-7, i < jp) and of the column update
+  // (waves 8-15, i < jj, i != jp): wave w takes i = w mod 8, at most 4 terms,
+  // all LDS reads issued together; wave 0 adds the 8 partials below
+  static_assert(NB == 32, "4 panel terms per wave");
+  {
+    const int par = wv & 7;
     double acc = 0.0;
-    if (wv <= 2) {
-      for (int i = par; i < jp; i += 2) acc += sW[i][lane] * su1[i] + sV[i][lane] * su2[i];
-    } else {
-      for (int i = par; i < jj; i += 2)
-        if (i != jp) acc += sV[i][lane] * sWc[i] + sW[i][lane] * sVc[i];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int i = par + 8 * u;
+      if (wv < 8) {
+        if (i < jp) acc += sW[i][lane] * su1[i] + sV[i][lane] * su2[i];
+      } else if (i < jj && i != jp) {
+        acc += sV[i][lane] * sWc[i] + sW[i][lane] * sVc[i];
+      }
     }
-    s_pp[wv - 1][lane] = acc;
+    s_pp[wv][lane] = acc;
   }
   __syncthreads();
   // ---- wave 0: w of column c-1, the Householder vector, column c
@@ -228,15 +276,19 @@ __global__ __launch_bounds__(1024) void k_tr_col(TrArgs a, int t, int jp, int jj
         uu = su1[lane] * su2[lane];
         cc = sWc[lane] * su1[lane] + sVc[lane] * su2[lane];
       }
-      uu = wave_sum(uu);
-      cc = wave_sum(cc);
+      uu = wave_sum_full(uu);
+      cc = wave_sum_full(cc);
       double yv = 0.0;
+#pragma unroll
       for (int w = 0; w < 16; ++w) yv += s_red[w];
       const double al = -0.5 * tp * tp * (yv - 2.0 * uu);
       wc = tp * (y_c - cc) + al * sVc[jp];
-      double acc = 0.0;
+      double acc = 0.0, pp = 0.0;
+#pragma unroll
       for (int w = 0; w < 16; ++w) acc += s_y[w][lane];
-      acc -= s_pp[0][lane] + s_pp[1][lane];
+#pragma unroll
+      for (int w = 0; w < 8; ++w) pp += s_pp[w][lane];
+      acc -= pp;
       w_r = tp * acc + al * sV[jp][lane];
       if (r < n) {
         P[r + (NB + jp) * ld] = w_r;
@@ -245,7 +297,10 @@ __global__ __launch_bounds__(1024) void k_tr_col(TrArgs a, int t, int jp, int jj
     }
     if (copy) A[r + (c - 1) * ld] = vsl;
     if (jj >= 0) {
-      double col = a_rc - (s_pp[2][lane] + s_pp[3][lane]);
+      double cp = 0.0;
+#pragma unroll
+      for (int w = 8; w < 16; ++w) cp += s_pp[w][lane];
+      double col = a_rc - cp;
       if (jp >= 0 && jp < jj) col -= sV[jp][lane] * wc + w_r * sVc[jp];
       if (r < n) {
         A[r + c * ld] = col;
@@ -254,22 +309,24 @@ __global__ __launch_bounds__(1024) void k_tr_col(TrArgs a, int t, int jp, int jj
       const double x = (r < n && r >= c + 2) ? col : 0.0;
       s_x[lane] = x;
       s_w[lane] = w_r;
-      const double sq = wave_sum(x * x);
+      const double sq = wave_sum_full(x * x);
       if (lane == 0) colrec[(int64_t)tile * USTRIDE + 2 * NB] = sq;
     }
   }
   if (jj < 0) return;
   __syncthreads();
-  // ---- partial V^T x (slots q < jj) and W^T x (slots NB + q): the waves
-  // take every 16th product, lanes over the 64 rows
-  for (int k = wv; k < 2 * jj; k += 16) {
-    double vw;
-    if (k < jj) vw = sV[k][lane];
-    else if (k - jj == jp) vw = s_w[lane];
-    else vw = sW[k - jj][lane];
-    double t = vw * s_x[lane];
-    t = wave_sum(t);
-    if (lane == 0) colrec[(int64_t)tile * USTRIDE + (k < jj ? k : NB + (k - jj))] = t;
+  // ---- partial V^T x (slots q < jj) and W^T x (slots NB + q): product
+  // k = tid / 16 over the rows g, g + 16, g + 32, g + 48 (g = tid mod 16),
+  // summed over the 16 lanes of its DPP row; a wave holds products 4 wv..+3
+  // and skips as a whole when none of them exists
+  if (4 * wv < 2 * jj) {
+    const int k = tid >> 4, g = tid & 15;
+    const double* src = (k < jj) ? sV[k] : (k - jj == jp) ? s_w : sW[min(k - jj, NB - 1)];
+    double t = 0.0;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) t += src[g + 16 * u] * s_x[g + 16 * u];
+    t = row_sum16(t);
+    if (g == 0 && k < 2 * jj) colrec[(int64_t)tile * USTRIDE + (k < jj ? k : NB + (k - jj))] = t;
   }
 }
 
@@ -318,7 +375,7 @@ __global__ __launch_bounds__(NW * 64) void k_tr_symv(TrArgs a, int t, int jj, in
   {
     double sl = 0.0;
     for (int q = lane; q < ncol; q += 64) sl += colrec[(int64_t)q * USTRIDE + 2 * NB];
-    xn2 = wave_sum(sl);   // every wave forms the same sum
+    xn2 = wave_sum_full(sl);   // every wave forms the same sum
   }
   const double alpha = A[(c + 1) + c * ld];
   double beta, tau, scal;
@@ -388,7 +445,7 @@ __global__ __launch_bounds__(NW * 64) void k_tr_symv(TrArgs a, int t, int jj, in
     double st = 0.0;
 #pragma unroll
     for (int u = 0; u < U; u += 2) st += (u + 1 < U) ? (sa[u] + sa[u + 1]) : sa[u];
-    const double t = wave_sum(st);
+    const double t = wave_sum_full(st);
     if (lane == 0) atomicAdd(&ax[j - j0], t);
   }
   __syncthreads();
@@ -406,7 +463,7 @@ __global__ __launch_bounds__(NW * 64) void k_tr_symv(TrArgs a, int t, int jj, in
     P[r + jj * ld] = vi;
     P[r + (3 * NB + jj) * ld] = vi;
   }
-  yv = wave_sum(yv);
+  yv = wave_sum_full(yv);
   if (lane == 0) s_red[wv] = yv;
   __syncthreads();
   if (tid == 0) {

@@ -118,3 +118,28 @@ def test_start_aligned_tridiagonal_schedule(tmp_path):
         run_and_compare(tmp_path, "two_orbital_nk4000_it45", {"niter": 6}, last=2)
     finally:
         gpu.load().nrg_gpu_set_tridiag_schedule(1)
+
+
+def test_solver_usable_after_host_close(tmp_path):
+    # closing a host whose last iteration deferred its back-transform leaves
+    # no pending columns behind: a following batched solve runs normally
+    from nrgamd import gpu
+    from nrgamd.host import Host
+    d = str(tmp_path / "two_orbital_nk200")
+    models.write_inputs(models.workload("two_orbital_nk4000_it45", nkeep=200, niter=4), d)
+    h = Host(d, device=0)
+    try:
+        for it in range(-1, 2):
+            h.iterate(it)
+    finally:
+        h.close()
+    gpu.init(0)
+    rng = np.random.default_rng(3)
+    blocks = []
+    for n in (7, 40):
+        a = rng.standard_normal((n, n))
+        blocks.append(a + a.T)
+    ws, Us = gpu.syev_batched(blocks)
+    for H, w, U in zip(blocks, ws, Us):
+        np.testing.assert_allclose(np.sort(w), np.linalg.eigvalsh(H), rtol=0, atol=1e-12 * np.abs(H).max() * len(H))
+        assert np.max(np.abs(H @ U - U * w)) <= 1e-11 * np.abs(H).max() * len(H)
