@@ -59,12 +59,11 @@ class BatchedEigensolver {
   int64_t tot_n_ = 0, tot_n2_ = 0;
   double *A_ = nullptr, *Z_ = nullptr, *w_ = nullptr;
 
-  DevBuf<int32_t> d_n_, d_order_, d_ns_;   // sizes, size order, sizes in that order
-  DevBuf<int32_t> d_coff_;                  // tridiagonalisation start step per slot
+  DevBuf<int32_t> d_n_, d_order_;          // sizes, size order
+  DevBuf<char> d_slot_;                     // tridiagonalisation slot records (size order)
   bool align_end_ = true;                   // end-aligned column schedule (see tridiag)
   DevBuf<int64_t> d_aoff_, d_voff_;
   DevBuf<double> d_d_, d_e_, d_tau_, d_y_, d_P_, d_part_;
-  DevBuf<int64_t> d_poff_;
   // divide and conquer
   DevBuf<double> d_ws_;
   DevBuf<int32_t> d_iws_;

@@ -91,14 +91,25 @@ __device__ inline double wave_sum_full(double v) {
 // in the next column kernel (16 waves share the sum, fixed order), as is
 // y.v from one partial per SYMV workgroup.  Every reduction has a fixed
 // order: results are reproducible bit for bit.
+// Slot q of the size order (blocks by decreasing n): everything a
+// workgroup needs to address its block, in one 64-byte record so that the
+// workgroup-to-block map and the block's pointers cost one load.
+struct TrSlot {
+  int32_t n;       // block size
+  int32_t coff;    // step at which the slot starts: column c = step - coff
+  int32_t b;       // block index
+  int32_t pad;
+  int64_t aoff;    // matrix offset
+  int64_t voff;    // vector offset (d, e, tau)
+  int64_t pP;      // panel buffer offset
+  int64_t pRec;    // column / SYMV record offset
+  int64_t pPy;     // SYMV partial-vector offset
+  int64_t pad2;
+};
+static_assert(sizeof(TrSlot) == 64, "slot record is 64 bytes");
+
 struct TrArgs {
-  const int32_t* n;
-  const int32_t* order;
-  const int32_t* ns;       // n[order[q]]: sizes in decreasing order
-  const int32_t* coff;     // step at which slot q starts: column c = step - coff[q]
-  const int64_t* aoff;
-  const int64_t* voff;
-  const int64_t* poff;     // [0,nblk): P, [nblk,2nblk): records, [2nblk,3nblk): py
+  const TrSlot* slot;
   double* A;
   double* d;
   double* e;
@@ -106,8 +117,13 @@ struct TrArgs {
   double* py;              // per block: ceil(n/CG) partial vectors of length n
   double* P;
   double* part;            // per block: column records, SYMV records, u1/u2
-  int nblk;
 };
+
+// Pins values that are uniform over the workgroup in scalar registers at
+// this point (an empty asm statement that reads them): the slot record and
+// the kernel's base pointers are then loaded in one round before the
+// workgroup's early exit instead of partly after it.
+#define TR_PIN(...) asm volatile("" ::__VA_ARGS__)
 
 // Flattened launch over the active blocks (slots q < act of the size order):
 // slot q owns tiles(q) consecutive workgroups.  Maps workgroup L to its slot
@@ -153,30 +169,35 @@ __global__ __launch_bounds__(1024) void k_tr_col(TrArgs a, int t, int jp, int jj
                                                  int tmax) {
   int qs, tile;
   auto tiles = [&](int q) {
-    const int cq = t - a.coff[q];
-    return (cq >= 0 && cq < a.ns[q]) ? (a.ns[q] - cq + RA - 1) / RA : 0;
+    const int cq = t - a.slot[q].coff, nq = a.slot[q].n;
+    return (cq >= 0 && cq < nq) ? (nq - cq + RA - 1) / RA : 0;
   };
+  TrSlot sl;
   if (tmax > 0) {
     // tile-major order, deepest tiles first: a tile's share of the SYMV
     // partial sums grows with its row index
     qs = (int)blockIdx.x % act;
-    tile = tiles(qs) - 1 - (int)blockIdx.x / act;
+    sl = a.slot[qs];
+    TR_PIN("s"(sl.n), "s"(sl.coff), "s"(sl.aoff), "s"(sl.voff), "s"(sl.pP), "s"(sl.pRec), "s"(sl.pPy),
+           "s"(a.A), "s"(a.P), "s"(a.part), "s"(a.py), "s"(a.tau), "s"(a.d));
+    const int cq = t - sl.coff;
+    tile = ((cq >= 0 && cq < sl.n) ? (sl.n - cq + RA - 1) / RA : 0) - 1 - (int)blockIdx.x / act;
     if (tile < 0) return;
-  } else if (!flat_map((int)blockIdx.x, act, tiles, qs, tile)) {
-    return;
+  } else {
+    if (!flat_map((int)blockIdx.x, act, tiles, qs, tile)) return;
+    sl = a.slot[qs];
   }
-  const int c = t - a.coff[qs];
+  const int c = t - sl.coff;
   if (c == 0) jp = -1;   // a block starting at a panel boundary has no previous column
-  const int b = a.order[qs];
-  const int n = a.ns[qs];
+  const int n = sl.n;
   const int r0 = c + tile * RA;
   const int64_t ld = n;
-  double* A = a.A + a.aoff[b];
-  double* P = a.P + a.poff[b];
-  double* colrec = a.part + a.poff[a.nblk + b];
+  double* A = a.A + sl.aoff;
+  double* P = a.P + sl.pP;
+  double* colrec = a.part + sl.pRec;
   const double* symrec = colrec + colrec_size(n);
   const double* ubuf = symrec + symrec_size(n);
-  const double* py = a.py + a.poff[2 * a.nblk + b];
+  const double* py = a.py + sl.pPy;
   const int tid = threadIdx.x;
   const int lane = tid & 63, wv = tid >> 6;
   const int r = r0 + lane;
@@ -210,7 +231,7 @@ __global__ __launch_bounds__(1024) void k_tr_col(TrArgs a, int t, int jp, int jj
     if (copy) vsl = P[r + vslot_prev * ld];
     if (jp >= 0) {
       y_c = py[0];
-      tp = a.tau[a.voff[b] + c - 1];
+      tp = a.tau[sl.voff + c - 1];
     }
   }
   if (jp >= 0) {
@@ -304,7 +325,7 @@ __global__ __launch_bounds__(1024) void k_tr_col(TrArgs a, int t, int jp, int jj
       if (jp >= 0 && jp < jj) col -= sV[jp][lane] * wc + w_r * sVc[jp];
       if (r < n) {
         A[r + c * ld] = col;
-        if (r == c) a.d[a.voff[b] + c] = col;
+        if (r == c) a.d[sl.voff + c] = col;
       }
       const double x = (r < n && r >= c + 2) ? col : 0.0;
       s_x[lane] = x;
@@ -338,31 +359,36 @@ __global__ __launch_bounds__(NW * 64) void k_tr_symv(TrArgs a, int t, int jj, in
   extern __shared__ double ax[];   // row parts, rows [j0, m), then v of those rows
   int qs, pan;
   auto tiles = [&](int q) {
-    const int cq = t - a.coff[q], mq = a.ns[q] - cq - 1;
+    const int cq = t - a.slot[q].coff, mq = a.slot[q].n - cq - 1;
     return (cq >= 0 && mq > 0) ? (mq + PW - 1) / PW : 0;
   };
+  TrSlot sl;
   if (panel_major) {
     // workgroup L takes panel L / act of slot L % act: the longest panels of
     // all blocks are dispatched first (the trailing sizes of the blocks are
     // nearly equal under the end-aligned schedule)
     qs = (int)blockIdx.x % act;
     pan = (int)blockIdx.x / act;
-    if (pan >= tiles(qs)) return;
-  } else if (!flat_map((int)blockIdx.x, act, tiles, qs, pan)) {
-    return;
+    sl = a.slot[qs];
+    TR_PIN("s"(sl.n), "s"(sl.coff), "s"(sl.aoff), "s"(sl.voff), "s"(sl.pP), "s"(sl.pRec), "s"(sl.pPy),
+           "s"(a.A), "s"(a.P), "s"(a.part), "s"(a.py), "s"(a.tau), "s"(a.e));
+    const int cq = t - sl.coff, mq = sl.n - cq - 1;
+    if (pan >= ((cq >= 0 && mq > 0) ? (mq + PW - 1) / PW : 0)) return;
+  } else {
+    if (!flat_map((int)blockIdx.x, act, tiles, qs, pan)) return;
+    sl = a.slot[qs];
   }
-  const int c = t - a.coff[qs];
-  const int b = a.order[qs];
-  const int n = a.ns[qs];
+  const int c = t - sl.coff;
+  const int n = sl.n;
   const int m = n - c - 1;
   const int j0 = pan * PW;
   const int64_t ld = n;
-  const double* A = a.A + a.aoff[b];
-  double* P = a.P + a.poff[b];
-  const double* colrec = a.part + a.poff[a.nblk + b];
-  double* symrec = a.part + a.poff[a.nblk + b] + colrec_size(n);
+  const double* A = a.A + sl.aoff;
+  double* P = a.P + sl.pP;
+  const double* colrec = a.part + sl.pRec;
+  double* symrec = a.part + sl.pRec + colrec_size(n);
   double* ubuf = symrec + symrec_size(n);
-  double* py = a.py + a.poff[2 * a.nblk + b] + (int64_t)pan * n;
+  double* py = a.py + sl.pPy + (int64_t)pan * n;
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wv = tid >> 6;
@@ -391,8 +417,8 @@ __global__ __launch_bounds__(NW * 64) void k_tr_symv(TrArgs a, int t, int jj, in
   auto vv = [&](int i) { return (i == 0) ? 1.0 : vcol[i] * scal; };
   if (pan == 0) {
     if (tid == 0) {
-      a.e[a.voff[b] + c] = beta;
-      a.tau[a.voff[b] + c] = tau;
+      a.e[sl.voff + c] = beta;
+      a.tau[sl.voff + c] = tau;
     }
     // u1_q = V[c+1,q] + scal sum_{r>=c+2} V[r,q] x_r (same for W): the
     // waves split the records, lane = slot, fixed order
@@ -1192,9 +1218,9 @@ static double now_ms() {
 }
 
 void BatchedEigensolver::release() {
-  d_n_.release(); d_order_.release(); d_ns_.release(); d_coff_.release(); d_aoff_.release(); d_voff_.release();
+  d_n_.release(); d_order_.release(); d_slot_.release(); d_aoff_.release(); d_voff_.release();
   d_d_.release(); d_e_.release(); d_tau_.release(); d_y_.release(); d_P_.release(); d_part_.release();
-  d_poff_.release(); d_ws_.release(); d_iws_.release(); d_tasks_.release(); d_k_.release();
+  d_ws_.release(); d_iws_.release(); d_tasks_.release(); d_k_.release();
   d_V_.release(); d_T_.release(); d_X_.release(); d_X2_.release(); d_rng_.release(); gemm_.release();
   nblk_ = 0;
 }
@@ -1221,11 +1247,6 @@ void BatchedEigensolver::solve(int nblk, const int* n, const int64_t* aoff, cons
   std::stable_sort(order_.begin(), order_.end(), [&](int x, int y) { return n_[x] > n_[y]; });
   d_n_.upload(n_.data(), nblk, s);
   d_order_.upload(order_.data(), nblk, s);
-  {
-    std::vector<int32_t> ns(nblk);
-    for (int q = 0; q < nblk; ++q) ns[q] = n_[order_[q]];
-    d_ns_.upload(ns.data(), nblk, s);
-  }
   d_aoff_.upload(aoff_.data(), nblk, s);
   d_voff_.upload(voff_.data(), nblk, s);
   d_d_.reserve(tot_n_);
@@ -1263,7 +1284,6 @@ void BatchedEigensolver::tridiag(hipStream_t s) {
     poff[2 * nblk + b] = yacc;
     yacc += (int64_t)((n_[b] + CG - 1) / CG) * n_[b];
   }
-  d_poff_.upload(poff.data(), poff.size(), s);
   d_y_.reserve(std::max<int64_t>(yacc, 1));
   d_P_.reserve(tot_n_ * 4 * NB);
   d_part_.reserve(pacc);
@@ -1275,9 +1295,16 @@ void BatchedEigensolver::tridiag(hipStream_t s) {
   // every slot sees the same panel phase t mod NB.
   std::vector<int32_t> coff(nblk);
   for (int q = 0; q < nblk; ++q) coff[q] = align_end_ ? ((nmax - n_[order_[q]]) / NB) * NB : 0;
-  d_coff_.upload(coff.data(), nblk, s);
-  TrArgs ta{d_n_.p, d_order_.p, d_ns_.p, d_coff_.p, d_aoff_.p, d_voff_.p, d_poff_.p, A_, d_d_.p, d_e_.p,
-            d_tau_.p, d_y_.p, d_P_.p, d_part_.p, nblk};
+  std::vector<TrSlot> slots(nblk);
+  for (int q = 0; q < nblk; ++q) {
+    const int b = order_[q];
+    TrSlot& r = slots[q];
+    r.n = n_[b]; r.coff = coff[q]; r.b = b; r.pad = 0;
+    r.aoff = aoff_[b]; r.voff = voff_[b];
+    r.pP = poff[b]; r.pRec = poff[nblk + b]; r.pPy = poff[2 * nblk + b]; r.pad2 = 0;
+  }
+  d_slot_.upload(reinterpret_cast<const char*>(slots.data()), slots.size() * sizeof(TrSlot), s);
+  TrArgs ta{reinterpret_cast<const TrSlot*>(d_slot_.p), A_, d_d_.p, d_e_.p, d_tau_.p, d_y_.p, d_P_.p, d_part_.p};
   // slots started by step t (coff is non-decreasing in slot order)
   auto started = [&](int t) {
     int lo = 0, hi = nblk;
