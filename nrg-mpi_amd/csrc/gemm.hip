@@ -161,8 +161,12 @@ template <int TA, int TB>
 __global__ __launch_bounds__(256) void k_gemm_f64_fast(
     const GemmTask* __restrict__ tasks, const int32_t* __restrict__ tile_start, int ntasks,
     const double* __restrict__ A, const double* __restrict__ B, double* __restrict__ C) {
-  __shared__ double As[TK2 * LDS_STRIDE];
-  __shared__ double Bs[TK2 * LDS_STRIDE];
+  // A and B slices; after the K loop the same space holds the C tile
+  // (64 columns of LDS_STRIDE rows) for the column-wise epilogue
+  static_assert(2 * TK2 == TN, "the C tile fits in the A/B slices");
+  __shared__ double ABs[2 * TK2 * LDS_STRIDE];
+  double* As = ABs;
+  double* Bs = ABs + TK2 * LDS_STRIDE;
   __shared__ int32_t s_ts[MAXT_LDS];
   const int bid = blockIdx.x;
   const int ti = find_task(tile_start, ntasks, bid, s_ts);
@@ -223,6 +227,21 @@ __global__ __launch_bounds__(256) void k_gemm_f64_fast(
     }
   };
 
+  // C columns of this wave (16 wave .. 16 wave + 15), row = lane: fetched
+  // into the slice registers while the last slice is multiplied
+  const bool readc = (t.beta != 0.0);
+  double* Cb = C + t.offC;
+  auto fetch_c = [&]() {
+    const int gi = m0 + lane;
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      const int gj = n0 + 16 * wave + u;
+      double v = 0.0;
+      if (gi < t.m && gj < t.n) v = Cb[(int64_t)gi + (int64_t)gj * t.ldc];
+      (u < 8 ? ra[u] : rb[u - 8]) = v;
+    }
+  };
+
   d4 acc[2][2];
   for (int a = 0; a < 2; ++a)
     for (int b = 0; b < 2; ++b) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
@@ -231,10 +250,13 @@ __global__ __launch_bounds__(256) void k_gemm_f64_fast(
     fetch(0);
     store();
     __syncthreads();
+  } else if (readc) {
+    fetch_c();
   }
   for (int k0 = 0; k0 < t.k; k0 += TK2) {
     const bool more = (k0 + TK2 < t.k);
     if (more) fetch(k0 + TK2);   // in flight while this slice is multiplied
+    else if (readc) fetch_c();
 #pragma unroll
     for (int s = 0; s < TK2 / 4; ++s) {
       const int kk = s * 4 + (lane >> 4);
@@ -254,20 +276,35 @@ __global__ __launch_bounds__(256) void k_gemm_f64_fast(
       __syncthreads();
     }
   }
-  double* Cb = C + t.offC;
+  // epilogue: the accumulators go to LDS in the matrix-core layout and
+  // come back as whole 64-row columns, so that every C store (and the C
+  // prefetch above) moves one contiguous column per wave instruction
+  __syncthreads();   // every wave is done with the A/B slices
+#pragma unroll
   for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
     for (int ni = 0; ni < 2; ++ni)
+#pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int gi = m0 + wm * 32 + mi * 16 + (lane >> 4) + 4 * r;
-        const int gj = n0 + wn * 32 + ni * 16 + (lane & 15);
-        if (gi < t.m && gj < t.n) {
-          double* c = Cb + (int64_t)gi + (int64_t)gj * t.ldc;
-          double v = t.alpha * acc[mi][ni][r];
-          if (t.beta != 0.0) v += t.beta * (*c);
-          *c = v;
-        }
+        const int il = wm * 32 + mi * 16 + (lane >> 4) + 4 * r;
+        const int jl = wn * 32 + ni * 16 + (lane & 15);
+        ABs[jl * LDS_STRIDE + il] = acc[mi][ni][r];
       }
+  __syncthreads();
+  const int gi = m0 + lane;
+#pragma unroll
+  for (int u = 0; u < 16; ++u) {
+    const int jl = 16 * wave + u;
+    const int gj = n0 + jl;
+    if (gi < t.m && gj < t.n) {
+      double v = t.alpha * ABs[jl * LDS_STRIDE + lane];
+      if (readc) v += t.beta * (u < 8 ? ra[u] : rb[u - 8]);
+      Cb[(int64_t)gi + (int64_t)gj * t.ldc] = v;
+    }
+  }
 }
+
+
 
 // Deterministic split-K reduction: C = alpha sum_c W_c + beta C, partials
 // summed in chunk order.  Grid: (element chunks, records).

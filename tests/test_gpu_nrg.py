@@ -102,11 +102,15 @@ def test_deferred_back_transform_matches_full(tmp_path):
         finally:
             h.close()
         runs.append(rec)
+    # The two runs send different column ranges through the back-transform
+    # products (other tile counts, other split-K choices), so their vectors
+    # agree to rounding, not bit for bit, and the difference carries into
+    # later iterations: the eigenvalues are held to the parity tolerance.
     for k, ((n0, kb0, e0), (n1, kb1, e1)) in enumerate(zip(*runs)):
         assert n0 == n1, k
         np.testing.assert_array_equal(kb0, kb1)
         scale = max(1.0, float(np.max(np.abs(e1))))
-        assert np.max(np.abs(e0 - e1)) <= 1e-13 * scale, k
+        assert np.max(np.abs(e0 - e1)) <= EIG_TOL * scale, k
 
 
 def test_start_aligned_tridiagonal_schedule(tmp_path):
