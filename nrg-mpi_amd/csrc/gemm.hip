@@ -158,7 +158,7 @@ __device__ inline int find_task(const int32_t* __restrict__ tile_start, int ntas
 }
 
 template <int TA, int TB>
-__global__ __launch_bounds__(256) void k_gemm_f64_fast(
+__global__ __launch_bounds__(256, 3) void k_gemm_f64_fast(
     const GemmTask* __restrict__ tasks, const int32_t* __restrict__ tile_start, int ntasks,
     const double* __restrict__ A, const double* __restrict__ B, double* __restrict__ C) {
   // A and B slices; after the K loop the same space holds the C tile
@@ -186,31 +186,41 @@ __global__ __launch_bounds__(256) void k_gemm_f64_fast(
   }
   const int tid = threadIdx.x;
   const int lane = tid & 63;
-  const int wave = tid >> 6;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);   // uniform over the wave
   const int wm = wave & 1, wn = wave >> 1;
   const double* Ab = A + t.offA;
   const double* Bb = B + t.offB;
 
   // element e = tid + 256 u of the 64 x 32 slice of opA / 32 x 64 of opB
-  // (index order chosen so a wave reads contiguous stored memory)
+  // (index order chosen so a wave reads contiguous stored memory).  Each
+  // load is a uniform row base (scalar registers) plus one per-lane 32-bit
+  // byte offset shared by all 16 loads of the slice.
   double ra[8], rb[8];
+  // lane part of the element indices: (contiguous index, strided index)
+  const int a_c = !TA ? (tid & 63) : (tid & 31), a_s = !TA ? (tid >> 6) : (tid >> 5);
+  const int b_c = TB ? (tid & 63) : (tid & 31), b_s = TB ? (tid >> 6) : (tid >> 5);
+  const uint32_t a_off = (uint32_t)(a_s * t.lda + a_c) * 8u;
+  const uint32_t b_off = (uint32_t)(b_s * t.ldb + b_c) * 8u;
+  // strides of u along the strided index: 4 (64-wide contiguous) or 8
+  constexpr int a_su = !TA ? 4 : 8, b_su = TB ? 4 : 8;
   auto fetch = [&](int k0) {
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
-      const int e = tid + 256 * u;
-      int i, kk;
-      if (!TA) { i = e & 63; kk = e >> 6; } else { kk = e & 31; i = e >> 5; }
-      const int gi = m0 + i, gk = k0 + kk;
+      // opA: TA = 0 -> row i = m0 + a_c, k = k0 + a_s + 4u; TA = 1 -> k = k0 + a_c, row = m0 + a_s + 8u
+      const int sa = a_s + a_su * u;
+      const int gi = !TA ? m0 + a_c : m0 + sa, gk = !TA ? k0 + sa : k0 + a_c;
+      const char* pa = (const char*)(Ab + (!TA ? (int64_t)(k0 + a_su * u) * t.lda + m0
+                                               : (int64_t)(m0 + a_su * u) * t.lda + k0));
       double v = 0.0;
-      if (gi < t.m && gk < t.k)
-        v = !TA ? Ab[(int64_t)gi + (int64_t)gk * t.lda] : Ab[(int64_t)gk + (int64_t)gi * t.lda];
+      if (gi < t.m && gk < t.k) v = *(const double*)(pa + a_off);
       ra[u] = v;
-      int j, kb;
-      if (!TB) { kb = e & 31; j = e >> 5; } else { j = e & 63; kb = e >> 6; }
-      const int gj = n0 + j, gk2 = k0 + kb;
+      // opB: TB = 1 -> col j = n0 + b_c, k = k0 + b_s + 4u; TB = 0 -> k = k0 + b_c, col = n0 + b_s + 8u
+      const int sb = b_s + b_su * u;
+      const int gj = TB ? n0 + b_c : n0 + sb, gk2 = TB ? k0 + sb : k0 + b_c;
+      const char* pb = (const char*)(Bb + (TB ? (int64_t)(k0 + b_su * u) * t.ldb + n0
+                                              : (int64_t)(n0 + b_su * u) * t.ldb + k0));
       double w = 0.0;
-      if (gj < t.n && gk2 < t.k)
-        w = !TB ? Bb[(int64_t)gk2 + (int64_t)gj * t.ldb] : Bb[(int64_t)gj + (int64_t)gk2 * t.ldb];
+      if (gj < t.n && gk2 < t.k) w = *(const double*)(pb + b_off);
       rb[u] = w;
     }
   };
@@ -231,13 +241,15 @@ __global__ __launch_bounds__(256) void k_gemm_f64_fast(
   // into the slice registers while the last slice is multiplied
   const bool readc = (t.beta != 0.0);
   double* Cb = C + t.offC;
+  const uint32_t c_off = (uint32_t)lane * 8u;   // row m0 + lane of a column
+  auto c_col = [&](int u) { return (char*)(Cb + (int64_t)(n0 + 16 * wave + u) * t.ldc + m0); };
   auto fetch_c = [&]() {
     const int gi = m0 + lane;
 #pragma unroll
     for (int u = 0; u < 16; ++u) {
       const int gj = n0 + 16 * wave + u;
       double v = 0.0;
-      if (gi < t.m && gj < t.n) v = Cb[(int64_t)gi + (int64_t)gj * t.ldc];
+      if (gi < t.m && gj < t.n) v = *(const double*)(c_col(u) + c_off);
       (u < 8 ? ra[u] : rb[u - 8]) = v;
     }
   };
@@ -299,7 +311,7 @@ __global__ __launch_bounds__(256) void k_gemm_f64_fast(
     if (gi < t.m && gj < t.n) {
       double v = t.alpha * ABs[jl * LDS_STRIDE + lane];
       if (readc) v += t.beta * (u < 8 ? ra[u] : rb[u - 8]);
-      Cb[(int64_t)gi + (int64_t)gj * t.ldc] = v;
+      *(double*)(c_col(u) + c_off) = v;
     }
   }
 }
