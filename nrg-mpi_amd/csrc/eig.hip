@@ -8,6 +8,7 @@
 //    & Sorensen (1987) and the eigenvector formula of Gu & Eisenstat (1995);
 //  * compact WY accumulation of reflectors (Schreiber & Van Loan 1989).
 #include "eig.h"
+#include <climits>
 #include "profiler.h"
 #include <algorithm>
 #include <chrono>
@@ -658,38 +659,40 @@ __global__ __launch_bounds__(256) void k_dc_merge_prep(DcArgs a, const int32_t* 
   const double beta = a.e[vo + mid - 1];
   const double rho = fabs(beta);
   const double sg = (beta >= 0.0) ? 1.0 : -1.0;
-  const double* D1 = a.w + vo + s;
-  const double* D2 = a.w + vo + mid;
-  if (m <= DC_LDS_M) {
-    // the two ascending lists are adjacent in w: search them in LDS
-    for (int i = tid; i < m; i += 256) dc_shm[i] = D1[i];
-    __syncthreads();
-    D1 = dc_shm;
-    D2 = dc_shm + m1;
-  }
-  // merge ranks of the two ascending lists (ties: left first)
-  for (int i = tid; i < m; i += 256) {
-    double v, z;
-    int rank, cidx;
-    if (i < m1) {
-      v = D1[i];
-      z = Z[(int64_t)(mid - 1) + (int64_t)(s + i) * n];
-      int lo = 0, hi = m2;
-      while (lo < hi) { int md = (lo + hi) >> 1; if (D2[md] < v) lo = md + 1; else hi = md; }
-      rank = i + lo;
-      cidx = s + i;
-    } else {
-      const int j = i - m1;
-      v = D2[j];
-      z = sg * Z[(int64_t)mid + (int64_t)(mid + j) * n];
-      int lo = 0, hi = m1;
-      while (lo < hi) { int md = (lo + hi) >> 1; if (D1[md] <= v) lo = md + 1; else hi = md; }
-      rank = j + lo;
-      cidx = mid + j;
+  // merge ranks of the two ascending lists (ties: left first), searched in
+  // LDS when they fit (instantiated per memory kind, as the scan below)
+  auto merge_ranks = [&](const auto* D1, const auto* D2) {
+    for (int i = tid; i < m; i += 256) {
+      double v, z;
+      int rank, cidx;
+      if (i < m1) {
+        v = D1[i];
+        z = Z[(int64_t)(mid - 1) + (int64_t)(s + i) * n];
+        int lo = 0, hi = m2;
+        while (lo < hi) { int md = (lo + hi) >> 1; if (D2[md] < v) lo = md + 1; else hi = md; }
+        rank = i + lo;
+        cidx = s + i;
+      } else {
+        const int j = i - m1;
+        v = D2[j];
+        z = sg * Z[(int64_t)mid + (int64_t)(mid + j) * n];
+        int lo = 0, hi = m1;
+        while (lo < hi) { int md = (lo + hi) >> 1; if (D1[md] <= v) lo = md + 1; else hi = md; }
+        rank = j + lo;
+        cidx = mid + j;
+      }
+      ds[rank] = v;
+      zs[rank] = z * 0.7071067811865476;  // z / sqrt(2): |z| = 1
+      col[rank] = cidx;
     }
-    ds[rank] = v;
-    zs[rank] = z * 0.7071067811865476;  // z / sqrt(2): |z| = 1
-    col[rank] = cidx;
+  };
+  if (m <= DC_LDS_M) {
+    // the two ascending lists are adjacent in w
+    for (int i = tid; i < m; i += 256) dc_shm[i] = a.w[vo + s + i];
+    __syncthreads();
+    merge_ranks(dc_shm, dc_shm + m1);
+  } else {
+    merge_ranks(a.w + vo + s, a.w + vo + mid);
   }
   __syncthreads();
   double mx = 0.0;
@@ -698,12 +701,14 @@ __global__ __launch_bounds__(256) void k_dc_merge_prep(DcArgs a, const int32_t* 
   if ((tid & 63) == 0) s_red[tid >> 6] = mx;
   // stage the sorted poles and weights in LDS for the serial scan
   const bool use_lds = (m <= DC_LDS_M);
-  double* sds = use_lds ? dc_shm : ds;
-  double* szs = use_lds ? dc_shm + m : zs;
   if (use_lds)
-    for (int i = tid; i < m; i += 256) { sds[i] = ds[i]; szs[i] = zs[i]; }
+    for (int i = tid; i < m; i += 256) { dc_shm[i] = ds[i]; dc_shm[m + i] = zs[i]; }
   __syncthreads();
-  if (tid == 0) {
+  // The scan and the copies after it are written once and instantiated for
+  // LDS and for global poles/weights: with one pointer that may be either,
+  // the compiler emits flat accesses, and a flat load waits for every global
+  // store before it (the scan's index stores), one memory latency per step.
+  auto scan = [&](auto* sds, auto* szs) {
     const double rp2 = 2.0 * rho;
     const double tol = 8.0 * 2.220446049250313e-16 * fmax(fmax(s_red[0], s_red[1]), fmax(s_red[2], s_red[3]));
     s_rho = rp2;
@@ -753,14 +758,24 @@ __global__ __launch_bounds__(256) void k_dc_merge_prep(DcArgs a, const int32_t* 
     s_nrot = nrot;
     a.kout[t] = k;
     I[7 * m] = nrot;
+  };
+  auto gather = [&](const auto* sds, const auto* szs) {
+    const int k = s_k;
+    for (int i = tid; i < k; i += 256) {
+      dl[i] = sds[nd[i]];
+      zl[i] = szs[nd[i]];
+    }
+    for (int i = tid; i < m - k; i += 256) dv[i] = sds[df[i]];
+  };
+  if (use_lds) {
+    if (tid == 0) scan(dc_shm, dc_shm + m);
+    __syncthreads();
+    gather(dc_shm, dc_shm + m);
+  } else {
+    if (tid == 0) scan(ds, zs);
+    __syncthreads();
+    gather(ds, zs);
   }
-  __syncthreads();
-  const int k = s_k;
-  for (int i = tid; i < k; i += 256) {
-    dl[i] = sds[nd[i]];
-    zl[i] = szs[nd[i]];
-  }
-  for (int i = tid; i < m - k; i += 256) dv[i] = sds[df[i]];
   if (tid == 0) W[5 * m] = s_rho;  // zh[0] slot keeps rho until the z-hat pass
   (void)Qcd;
 }
@@ -1047,24 +1062,53 @@ __global__ __launch_bounds__(256) void k_dc_finish(DcArgs a, const int32_t* task
   double* Z = a.Z + a.aoff[b];
   const int md = m - k;
   const int tid = threadIdx.x;
-  // rank sort of the deflated values (ties by index), values staged in LDS
+  // sort of the deflated values, ties by index: (value, index) pairs in
+  // LDS, bitonic network over the next power of two (padding +inf / INT_MAX
+  // sorts last).  Merges too large for LDS rank each value against all.
   extern __shared__ double fin_shm[];
-  const double* dvs = dv;
   if (md <= DC_LDS_M) {
-    for (int i = tid; i < md; i += 256) fin_shm[i] = dv[i];
-    __syncthreads();
-    dvs = fin_shm;
-  }
-  for (int i = tid; i < md; i += 256) {
-    const double v = dvs[i];
-    int r = 0;
-#pragma unroll 4
-    for (int j = 0; j < md; ++j) {
-      const double u = dvs[j];
-      r += (u < v) || (u == v && j < i);
+    int P = 1;
+    while (P < md) P <<= 1;
+    double* kv = fin_shm;
+    int32_t* ki = reinterpret_cast<int32_t*>(fin_shm + P);
+    for (int i = tid; i < P; i += 256) {
+      kv[i] = (i < md) ? dv[i] : __builtin_inf();
+      ki[i] = (i < md) ? i : INT_MAX;
     }
-    dord[r] = i;
-    sdv[r] = v;
+    __syncthreads();
+    for (int len = 2; len <= P; len <<= 1) {
+      for (int st = len >> 1; st > 0; st >>= 1) {
+        for (int x = tid; x < P / 2; x += 256) {
+          // pair (i, i + st) with i having bit st clear; direction by bit len
+          const int i = ((x & ~(st - 1)) << 1) | (x & (st - 1));
+          const int j = i + st;
+          const bool up = ((i & len) == 0);
+          const double vi = kv[i], vj = kv[j];
+          const int ii = ki[i], ij = ki[j];
+          const bool gt = (vi > vj) || (vi == vj && ii > ij);   // (i) after (j)
+          if (gt == up) {
+            kv[i] = vj; kv[j] = vi;
+            ki[i] = ij; ki[j] = ii;
+          }
+        }
+        __syncthreads();
+      }
+    }
+    for (int r = tid; r < md; r += 256) {
+      dord[r] = ki[r];
+      sdv[r] = kv[r];
+    }
+  } else {
+    for (int i = tid; i < md; i += 256) {
+      const double v = dv[i];
+      int r = 0;
+      for (int j = 0; j < md; ++j) {
+        const double u = dv[j];
+        r += (u < v) || (u == v && j < i);
+      }
+      dord[r] = i;
+      sdv[r] = v;
+    }
   }
   __syncthreads();
   // eigenvalues: lambda_i then deflated, merged by rank
@@ -1509,8 +1553,11 @@ void BatchedEigensolver::divide_conquer(hipStream_t s) {
       gemm_.run(batch_, d_ws_.p, d_ws_.p, d_ws_.p, nullptr, nullptr, nullptr, s, KF_GEMM_DC);
     }
     const int32_t* tk = d_tasks_.p;
+    // LDS: (value, index) pairs of the deflated values, padded to a power of two
+    int P = 1;
+    while (P < std::min(mmax, DC_LDS_M)) P <<= 1;
     plaunch(KF_DC_FINISH, 0, 0, k_dc_finish, dim3(nt), dim3(256),
-            (uint32_t)(sizeof(double) * (size_t)std::min(mmax, DC_LDS_M)), s, da, tk, nt);
+            (uint32_t)((sizeof(double) + sizeof(int32_t)) * (size_t)P), s, da, tk, nt);
     plaunch(KF_DC_FINISH, 0, 0, k_dc_scatter, dim3(ceil_div(mmax, 64), nt), dim3(256), 0, s, da, tk, nt);
   }
   NRG_HIP(hipGetLastError());
