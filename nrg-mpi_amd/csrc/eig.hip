@@ -26,6 +26,7 @@ constexpr int DC_LDS_M = 8192;  // merges up to this size run the deflation scan
 constexpr int SEC_LDS_M = 4000; // secular solves stage poles and weights in LDS up to this size
 constexpr int NBB = 128;   // back-transform panel width (Z is streamed once per panel)
 constexpr int MAXN = 16384;
+constexpr int SYMV_WIDE_MIN_WG = 512;   // 32-column SYMV panels only for launches of at least this many
 
 __device__ inline double wave_sum(double v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -1423,9 +1424,17 @@ void BatchedEigensolver::tridiag(hipStream_t s) {
       mB = std::max(mB, m);
     }
     // 32-column panels (16 waves x 2 columns) while the launch is
-    // bandwidth-bound, 16-column panels (16 waves x 1 column) once it is
-    // latency-bound
-    const int pw = (bytes >= 64.0e6) ? 2 * CG : CG;
+    // bandwidth-bound and has enough of them to occupy the device,
+    // 16-column panels (16 waves x 1 column) otherwise: a workgroup's
+    // bandwidth is bounded by the loads it keeps in flight, so a launch of
+    // few 32-column panels (the first columns, where one or two blocks are
+    // active) runs faster as twice as many 16-column panels
+    int g32 = 0;
+    for (int q = 0; q < act; ++q) {
+      const int m = rows_at(q, t) - 1;
+      if (m > 0) g32 += ceil_div(m, 2 * CG);
+    }
+    const int pw = (bytes >= 64.0e6 && g32 >= SYMV_WIDE_MIN_WG) ? 2 * CG : CG;
     int gB = 0, tmax = 0;
     for (int q = 0; q < act; ++q) {
       const int m = rows_at(q, t) - 1;
