@@ -46,6 +46,8 @@ class BatchedEigensolver {
   // Column schedule of the tridiagonalisation: all blocks start together
   // (false) or end together (true, the default).
   void set_align_end(bool on) { align_end_ = on; }
+  // Blocks split over this many streams in the tridiagonalisation (1 or 2).
+  void set_groups(int g) { groups_ = (g >= 2) ? 2 : 1; }
 
  private:
   void tridiag(hipStream_t s);
@@ -77,6 +79,14 @@ class BatchedEigensolver {
   std::vector<int> done_;           // transformed columns per block
   GemmRunner gemm_;
   GemmBatch batch_;
+  // second group of the tridiagonalisation: its own stream, slot table and
+  // trailing-update runner
+  int groups_ = 2;
+  hipStream_t s2_ = nullptr;
+  hipEvent_t ev_fork_ = nullptr, ev_join_ = nullptr;
+  DevBuf<char> d_slot2_;
+  GemmRunner gemm2_;
+  GemmBatch batch2_;
 };
 
 }  // namespace nrg

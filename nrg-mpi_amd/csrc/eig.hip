@@ -27,6 +27,7 @@ constexpr int SEC_LDS_M = 4000; // secular solves stage poles and weights in LDS
 constexpr int NBB = 128;   // back-transform panel width (Z is streamed once per panel)
 constexpr int MAXN = 16384;
 constexpr int SYMV_WIDE_MIN_WG = 512;   // 32-column SYMV panels only for launches of at least this many
+constexpr double SYMV_WIDE_MIN_BYTES = 64.0e6;   // ... and of at least this many bytes (both per group count)
 
 __device__ inline double wave_sum(double v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -1263,7 +1264,14 @@ static double now_ms() {
 }
 
 void BatchedEigensolver::release() {
-  d_n_.release(); d_order_.release(); d_slot_.release(); d_aoff_.release(); d_voff_.release();
+  d_n_.release(); d_order_.release(); d_slot_.release(); d_slot2_.release(); gemm2_.release();
+  if (s2_) {
+    (void)hipStreamSynchronize(s2_);
+    (void)hipStreamDestroy(s2_);
+    (void)hipEventDestroy(ev_fork_);
+    (void)hipEventDestroy(ev_join_);
+    s2_ = nullptr;
+  } d_aoff_.release(); d_voff_.release();
   d_d_.release(); d_e_.release(); d_tau_.release(); d_y_.release(); d_P_.release(); d_part_.release();
   d_ws_.release(); d_iws_.release(); d_tasks_.release(); d_k_.release();
   d_V_.release(); d_T_.release(); d_X_.release(); d_X2_.release(); d_rng_.release(); gemm_.release();
@@ -1340,60 +1348,101 @@ void BatchedEigensolver::tridiag(hipStream_t s) {
   // every slot sees the same panel phase t mod NB.
   std::vector<int32_t> coff(nblk);
   for (int q = 0; q < nblk; ++q) coff[q] = align_end_ ? ((nmax - n_[order_[q]]) / NB) * NB : 0;
-  std::vector<TrSlot> slots(nblk);
-  for (int q = 0; q < nblk; ++q) {
-    const int b = order_[q];
-    TrSlot& r = slots[q];
-    r.n = n_[b]; r.coff = coff[q]; r.b = b; r.pad = 0;
-    r.aoff = aoff_[b]; r.voff = voff_[b];
-    r.pP = poff[b]; r.pRec = poff[nblk + b]; r.pPy = poff[2 * nblk + b]; r.pad2 = 0;
-  }
-  d_slot_.upload(reinterpret_cast<const char*>(slots.data()), slots.size() * sizeof(TrSlot), s);
-  TrArgs ta{reinterpret_cast<const TrSlot*>(d_slot_.p), A_, d_d_.p, d_e_.p, d_tau_.p, d_y_.p, d_P_.p, d_part_.p};
-  // slots started by step t (coff is non-decreasing in slot order)
-  auto started = [&](int t) {
-    int lo = 0, hi = nblk;
-    while (lo < hi) { int md = (lo + hi) / 2; if (coff[md] <= t) lo = md + 1; else hi = md; }
-    return lo;
-  };
-  auto rows_at = [&](int q, int t) {   // trailing rows n - c of slot q at step t (<= 0: idle)
-    const int c = t - coff[q];
-    return (c >= 0 && c < n_[order_[q]]) ? n_[order_[q]] - c : 0;
-  };
   const size_t shm = (size_t)std::max(nmax, 1) * sizeof(double);
-  int pw_prev = CG;   // panel width of the previous column's SYMV launch
-  auto colbytes = [&](int t, int act) {
-    double b = 0;
-    for (int q = 0; q < act; ++q) {
-      const double m = rows_at(q, t);
-      b += 8.0 * (4.0 * m * (1 + NB / 8) + m * m / (2.0 * pw_prev));   // column, panel rows, y partials
-    }
-    return b;
-  };
   NRG_REQUIRE(2 * shm <= 160 * 1024, "eigensolver: block too large for the SYMV stage (n > 10240)");
   int T = 0;
   for (int q = 0; q < nblk; ++q) T = std::max(T, coff[q] + n_[order_[q]]);
-  for (int t = 0; t < T; ++t) {
-    const int jj = t % NB;   // panel position, the same for every slot
-    const int act = started(t);
-    int gA = 0, tA = 0;   // flattened grid: the row tiles of the active blocks
-    for (int q = 0; q < act; ++q) {
-      gA += ceil_div(rows_at(q, t), RA);
-      tA = std::max(tA, ceil_div(rows_at(q, t), RA));
+
+  // Two groups of blocks (alternate slots of the size order) follow the same
+  // schedule on two streams.  Each step of a block is a latency-bound column
+  // kernel followed by a bandwidth-bound SYMV; with the groups on separate
+  // streams the column kernel of one group runs while the SYMV of the other
+  // streams the trailing matrices.
+  const int ng = (nblk >= 2 && groups_ >= 2) ? 2 : 1;
+  struct Group {
+    std::vector<int> q;        // global slots of this group, size order
+    std::vector<int32_t> coff;
+    TrArgs ta;
+    GemmRunner* gemm;
+    GemmBatch* batch;
+    DevBuf<char>* d_slot;
+    hipStream_t s;
+    int pw_prev = CG;          // panel width of the previous column's SYMV launch
+  };
+  Group grp[2];
+  if (ng == 2 && !s2_) {
+    NRG_HIP(hipStreamCreateWithFlags(&s2_, hipStreamNonBlocking));
+    NRG_HIP(hipEventCreateWithFlags(&ev_fork_, hipEventDisableTiming));
+    NRG_HIP(hipEventCreateWithFlags(&ev_join_, hipEventDisableTiming));
+  }
+  for (int g = 0; g < ng; ++g) {
+    Group& G = grp[g];
+    for (int q = g; q < nblk; q += ng) G.q.push_back(q);
+    std::vector<TrSlot> slots(G.q.size());
+    for (size_t i = 0; i < G.q.size(); ++i) {
+      const int q = G.q[i], b = order_[q];
+      G.coff.push_back(coff[q]);
+      TrSlot& r = slots[i];
+      r.n = n_[b]; r.coff = coff[q]; r.b = b; r.pad = 0;
+      r.aoff = aoff_[b]; r.voff = voff_[b];
+      r.pP = poff[b]; r.pRec = poff[nblk + b]; r.pPy = poff[2 * nblk + b]; r.pad2 = 0;
     }
-    if (gA == 0) continue;
+    G.gemm = g ? &gemm2_ : &gemm_;
+    G.batch = g ? &batch2_ : &batch_;
+    G.d_slot = g ? &d_slot2_ : &d_slot_;
+    G.s = g ? s2_ : s;
+    G.d_slot->upload(reinterpret_cast<const char*>(slots.data()), slots.size() * sizeof(TrSlot), s);
+    G.ta = TrArgs{reinterpret_cast<const TrSlot*>(G.d_slot->p), A_, d_d_.p, d_e_.p, d_tau_.p, d_y_.p, d_P_.p,
+                  d_part_.p};
+  }
+  if (ng == 2) {
+    // the second stream starts after everything queued so far (inputs, tables)
+    NRG_HIP(hipEventRecord(ev_fork_, s));
+    NRG_HIP(hipStreamWaitEvent(s2_, ev_fork_, 0));
+  }
+
+  auto step = [&](Group& G, int t) {
+    const int ns = (int)G.q.size();
+    // slots of the group started by step t (coff is non-decreasing)
+    int act;
+    {
+      int lo = 0, hi = ns;
+      while (lo < hi) { int md = (lo + hi) / 2; if (G.coff[md] <= t) lo = md + 1; else hi = md; }
+      act = lo;
+    }
+    auto rows_at = [&](int i, int tt) {   // trailing rows n - c of group slot i at step tt (<= 0: idle)
+      const int nq = n_[order_[G.q[i]]];
+      const int c = tt - G.coff[i];
+      return (c >= 0 && c < nq) ? nq - c : 0;
+    };
+    auto colbytes = [&](int tt) {
+      double b = 0;
+      for (int i = 0; i < act; ++i) {
+        const double m = rows_at(i, tt);
+        b += 8.0 * (4.0 * m * (1 + NB / 8) + m * m / (2.0 * G.pw_prev));   // column, panel rows, y partials
+      }
+      return b;
+    };
+    const int jj = t % NB;   // panel position, the same for every slot
+    int gA = 0, tA = 0;      // flattened grid: the row tiles of the active blocks
+    for (int i = 0; i < act; ++i) {
+      gA += ceil_div(rows_at(i, t), RA);
+      tA = std::max(tA, ceil_div(rows_at(i, t), RA));
+    }
+    if (gA == 0) return;
     // tile-major order when the tile counts are nearly uniform
     const int tmA = (align_end_ && (int64_t)act * tA <= gA + gA / 8) ? tA : 0;
     const int gAl = tmA ? act * tA : gA;
+    const TrArgs& ta = G.ta;
     if (jj == 0 && t > 0) {
       // close the previous panel: finalise its last w, trailing SYR2K
-      plaunch(KF_TR_COL, colbytes(t, act), 0, k_tr_col, dim3(gAl), dim3(1024), 0, s, ta, t, NB - 1, -1, NB - 1,
-              pw_prev, act, tmA);
-      batch_.clear();
-      for (int q = 0; q < act; ++q) {
-        const int b = order_[q];
+      plaunch(KF_TR_COL, colbytes(t), 0, k_tr_col, dim3(gAl), dim3(1024), 0, G.s, ta, t, NB - 1, -1, NB - 1,
+              G.pw_prev, act, tmA);
+      G.batch->clear();
+      for (int i = 0; i < act; ++i) {
+        const int b = order_[G.q[i]];
         const int nb_ = n_[b];
-        const int c = t - coff[q];
+        const int c = t - G.coff[i];
         const int mt = nb_ - c;
         if (c <= 0 || mt <= 0) continue;
         GemmTask tk = gemm_task(mt, mt, 2 * NB, 0, 1,
@@ -1401,23 +1450,23 @@ void BatchedEigensolver::tridiag(hipStream_t s) {
                                 poff[b] + c + (int64_t)2 * NB * nb_, nb_, // [W V] rows c..
                                 aoff_[b] + c + (int64_t)c * nb_, nb_, -1.0, 1.0);
         tk.lower = 1;   // only the lower triangle is read from here on
-        batch_.add(tk);
+        G.batch->add(tk);
       }
-      gemm_.run(batch_, d_P_.p, d_P_.p, A_, nullptr, nullptr, nullptr, s, KF_GEMM_TRIDIAG);
-      plaunch(KF_TR_COL, colbytes(t, act), 0, k_tr_col, dim3(gAl), dim3(1024), 0, s, ta, t, -1, 0, NB - 1, pw_prev,
-              act, tmA);
+      G.gemm->run(*G.batch, d_P_.p, d_P_.p, A_, nullptr, nullptr, nullptr, G.s, KF_GEMM_TRIDIAG);
+      plaunch(KF_TR_COL, colbytes(t), 0, k_tr_col, dim3(gAl), dim3(1024), 0, G.s, ta, t, -1, 0, NB - 1,
+              G.pw_prev, act, tmA);
     } else {
       const int jp = (jj >= 1) ? jj - 1 : -1;
       const int vslot = (t >= 1) ? (t - 1) % NB : 0;
-      plaunch(KF_TR_COL, colbytes(t, act), 0, k_tr_col, dim3(gAl), dim3(1024), 0, s, ta, t, jp, jj, vslot, pw_prev,
-              act, tmA);
+      plaunch(KF_TR_COL, colbytes(t), 0, k_tr_col, dim3(gAl), dim3(1024), 0, G.s, ta, t, jp, jj, vslot,
+              G.pw_prev, act, tmA);
     }
     // algorithmic traffic: the lower triangle of the symmetric trailing
     // (n-c-1)^2 block once, plus the vectors
     double bytes = 0, flops = 0;
     int mB = 0;
-    for (int q = 0; q < act; ++q) {
-      const int m = rows_at(q, t) - 1;
+    for (int i = 0; i < act; ++i) {
+      const int m = rows_at(i, t) - 1;
       if (m <= 0) continue;
       bytes += 8.0 * (0.5 * (double)m * (m + 1.0) + 6.0 * m);
       flops += 2.0 * (double)m * m;
@@ -1430,14 +1479,14 @@ void BatchedEigensolver::tridiag(hipStream_t s) {
     // few 32-column panels (the first columns, where one or two blocks are
     // active) runs faster as twice as many 16-column panels
     int g32 = 0;
-    for (int q = 0; q < act; ++q) {
-      const int m = rows_at(q, t) - 1;
+    for (int i = 0; i < act; ++i) {
+      const int m = rows_at(i, t) - 1;
       if (m > 0) g32 += ceil_div(m, 2 * CG);
     }
-    const int pw = (bytes >= 64.0e6 && g32 >= SYMV_WIDE_MIN_WG) ? 2 * CG : CG;
+    const int pw = (bytes >= SYMV_WIDE_MIN_BYTES / ng && g32 >= SYMV_WIDE_MIN_WG / ng) ? 2 * CG : CG;
     int gB = 0, tmax = 0;
-    for (int q = 0; q < act; ++q) {
-      const int m = rows_at(q, t) - 1;
+    for (int i = 0; i < act; ++i) {
+      const int m = rows_at(i, t) - 1;
       if (m > 0) gB += ceil_div(m, pw);
       if (m > 0) tmax = std::max(tmax, ceil_div(m, pw));
     }
@@ -1448,11 +1497,17 @@ void BatchedEigensolver::tridiag(hipStream_t s) {
       const int pm = (align_end_ && (int64_t)act * tmax <= gB + gB / 8) ? 1 : 0;
       const int g = pm ? act * tmax : gB;
       if (pw == 2 * CG)
-        plaunch(KF_TR_SYMV, bytes, flops, k_tr_symv<2, 16, 4>, dim3(g), dim3(1024), shB, s, ta, t, jj, act, pm);
+        plaunch(KF_TR_SYMV, bytes, flops, k_tr_symv<2, 16, 4>, dim3(g), dim3(1024), shB, G.s, ta, t, jj, act, pm);
       else
-        plaunch(KF_TR_SYMV, bytes, flops, k_tr_symv<1, 16, 4>, dim3(g), dim3(1024), shB, s, ta, t, jj, act, pm);
-      pw_prev = pw;
+        plaunch(KF_TR_SYMV, bytes, flops, k_tr_symv<1, 16, 4>, dim3(g), dim3(1024), shB, G.s, ta, t, jj, act, pm);
+      G.pw_prev = pw;
     }
+  };
+  for (int t = 0; t < T; ++t)
+    for (int g = 0; g < ng; ++g) step(grp[g], t);
+  if (ng == 2) {
+    NRG_HIP(hipEventRecord(ev_join_, s2_));
+    NRG_HIP(hipStreamWaitEvent(s, ev_join_, 0));
   }
   NRG_HIP(hipGetLastError());
 }
