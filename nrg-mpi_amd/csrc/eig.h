@@ -46,8 +46,8 @@ class BatchedEigensolver {
   // Column schedule of the tridiagonalisation: all blocks start together
   // (false) or end together (true, the default).
   void set_align_end(bool on) { align_end_ = on; }
-  // Blocks split over this many streams in the tridiagonalisation (1 or 2).
-  void set_groups(int g) { groups_ = (g >= 2) ? 2 : 1; }
+  // Blocks split over this many streams in the tridiagonalisation (1..4).
+  void set_groups(int g) { groups_ = std::max(1, std::min(g, MAX_TR_GROUPS)); }
 
  private:
   void tridiag(hipStream_t s);
@@ -62,7 +62,6 @@ class BatchedEigensolver {
   double *A_ = nullptr, *Z_ = nullptr, *w_ = nullptr;
 
   DevBuf<int32_t> d_n_, d_order_;          // sizes, size order
-  DevBuf<char> d_slot_;                     // tridiagonalisation slot records (size order)
   bool align_end_ = true;                   // end-aligned column schedule (see tridiag)
   DevBuf<int64_t> d_aoff_, d_voff_;
   DevBuf<double> d_d_, d_e_, d_tau_, d_y_, d_P_, d_part_;
@@ -79,14 +78,16 @@ class BatchedEigensolver {
   std::vector<int> done_;           // transformed columns per block
   GemmRunner gemm_;
   GemmBatch batch_;
-  // second group of the tridiagonalisation: its own stream, slot table and
-  // trailing-update runner
+  // block groups of the tridiagonalisation: per group a stream (group 0:
+  // the caller's), a slot table and a trailing-update runner
+  static constexpr int MAX_TR_GROUPS = 4;
   int groups_ = 2;
-  hipStream_t s2_ = nullptr;
-  hipEvent_t ev_fork_ = nullptr, ev_join_ = nullptr;
-  DevBuf<char> d_slot2_;
-  GemmRunner gemm2_;
-  GemmBatch batch2_;
+  hipStream_t sg_[MAX_TR_GROUPS] = {};
+  hipEvent_t evg_[MAX_TR_GROUPS] = {};
+  hipEvent_t ev_fork_ = nullptr;
+  DevBuf<char> d_slotg_[MAX_TR_GROUPS];
+  GemmRunner gemmg_[MAX_TR_GROUPS];
+  GemmBatch batchg_[MAX_TR_GROUPS];
 };
 
 }  // namespace nrg

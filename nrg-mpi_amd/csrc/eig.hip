@@ -1264,14 +1264,19 @@ static double now_ms() {
 }
 
 void BatchedEigensolver::release() {
-  d_n_.release(); d_order_.release(); d_slot_.release(); d_slot2_.release(); gemm2_.release();
-  if (s2_) {
-    (void)hipStreamSynchronize(s2_);
-    (void)hipStreamDestroy(s2_);
-    (void)hipEventDestroy(ev_fork_);
-    (void)hipEventDestroy(ev_join_);
-    s2_ = nullptr;
-  } d_aoff_.release(); d_voff_.release();
+  d_n_.release(); d_order_.release();
+  for (int g = 0; g < MAX_TR_GROUPS; ++g) {
+    d_slotg_[g].release();
+    gemmg_[g].release();
+    if (sg_[g]) {
+      (void)hipStreamSynchronize(sg_[g]);
+      (void)hipStreamDestroy(sg_[g]);
+      (void)hipEventDestroy(evg_[g]);
+      sg_[g] = nullptr;
+      evg_[g] = nullptr;
+    }
+  }
+  if (ev_fork_) { (void)hipEventDestroy(ev_fork_); ev_fork_ = nullptr; } d_aoff_.release(); d_voff_.release();
   d_d_.release(); d_e_.release(); d_tau_.release(); d_y_.release(); d_P_.release(); d_part_.release();
   d_ws_.release(); d_iws_.release(); d_tasks_.release(); d_k_.release();
   d_V_.release(); d_T_.release(); d_X_.release(); d_X2_.release(); d_rng_.release(); gemm_.release();
@@ -1353,12 +1358,12 @@ void BatchedEigensolver::tridiag(hipStream_t s) {
   int T = 0;
   for (int q = 0; q < nblk; ++q) T = std::max(T, coff[q] + n_[order_[q]]);
 
-  // Two groups of blocks (alternate slots of the size order) follow the same
-  // schedule on two streams.  Each step of a block is a latency-bound column
-  // kernel followed by a bandwidth-bound SYMV; with the groups on separate
-  // streams the column kernel of one group runs while the SYMV of the other
-  // streams the trailing matrices.
-  const int ng = (nblk >= 2 && groups_ >= 2) ? 2 : 1;
+  // Groups of blocks (slot q in group q mod ng of the size order) follow the
+  // same schedule on their own streams.  Each step of a block is a
+  // latency-bound column kernel followed by a bandwidth-bound SYMV; with the
+  // groups on separate streams the column kernel of one group runs while the
+  // SYMV of another streams its trailing matrices.
+  const int ng = std::max(1, std::min(groups_, std::min(nblk, MAX_TR_GROUPS)));
   struct Group {
     std::vector<int> q;        // global slots of this group, size order
     std::vector<int32_t> coff;
@@ -1369,12 +1374,13 @@ void BatchedEigensolver::tridiag(hipStream_t s) {
     hipStream_t s;
     int pw_prev = CG;          // panel width of the previous column's SYMV launch
   };
-  Group grp[2];
-  if (ng == 2 && !s2_) {
-    NRG_HIP(hipStreamCreateWithFlags(&s2_, hipStreamNonBlocking));
-    NRG_HIP(hipEventCreateWithFlags(&ev_fork_, hipEventDisableTiming));
-    NRG_HIP(hipEventCreateWithFlags(&ev_join_, hipEventDisableTiming));
-  }
+  Group grp[MAX_TR_GROUPS];
+  for (int g = 1; g < ng; ++g)
+    if (!sg_[g]) {
+      NRG_HIP(hipStreamCreateWithFlags(&sg_[g], hipStreamNonBlocking));
+      NRG_HIP(hipEventCreateWithFlags(&evg_[g], hipEventDisableTiming));
+    }
+  if (ng > 1 && !ev_fork_) NRG_HIP(hipEventCreateWithFlags(&ev_fork_, hipEventDisableTiming));
   for (int g = 0; g < ng; ++g) {
     Group& G = grp[g];
     for (int q = g; q < nblk; q += ng) G.q.push_back(q);
@@ -1387,18 +1393,18 @@ void BatchedEigensolver::tridiag(hipStream_t s) {
       r.aoff = aoff_[b]; r.voff = voff_[b];
       r.pP = poff[b]; r.pRec = poff[nblk + b]; r.pPy = poff[2 * nblk + b]; r.pad2 = 0;
     }
-    G.gemm = g ? &gemm2_ : &gemm_;
-    G.batch = g ? &batch2_ : &batch_;
-    G.d_slot = g ? &d_slot2_ : &d_slot_;
-    G.s = g ? s2_ : s;
+    G.gemm = &gemmg_[g];
+    G.batch = &batchg_[g];
+    G.d_slot = &d_slotg_[g];
+    G.s = g ? sg_[g] : s;
     G.d_slot->upload(reinterpret_cast<const char*>(slots.data()), slots.size() * sizeof(TrSlot), s);
     G.ta = TrArgs{reinterpret_cast<const TrSlot*>(G.d_slot->p), A_, d_d_.p, d_e_.p, d_tau_.p, d_y_.p, d_P_.p,
                   d_part_.p};
   }
-  if (ng == 2) {
-    // the second stream starts after everything queued so far (inputs, tables)
+  if (ng > 1) {
+    // the other streams start after everything queued so far (inputs, tables)
     NRG_HIP(hipEventRecord(ev_fork_, s));
-    NRG_HIP(hipStreamWaitEvent(s2_, ev_fork_, 0));
+    for (int g = 1; g < ng; ++g) NRG_HIP(hipStreamWaitEvent(sg_[g], ev_fork_, 0));
   }
 
   auto step = [&](Group& G, int t) {
@@ -1505,9 +1511,9 @@ void BatchedEigensolver::tridiag(hipStream_t s) {
   };
   for (int t = 0; t < T; ++t)
     for (int g = 0; g < ng; ++g) step(grp[g], t);
-  if (ng == 2) {
-    NRG_HIP(hipEventRecord(ev_join_, s2_));
-    NRG_HIP(hipStreamWaitEvent(s, ev_join_, 0));
+  for (int g = 1; g < ng; ++g) {
+    NRG_HIP(hipEventRecord(evg_[g], sg_[g]));
+    NRG_HIP(hipStreamWaitEvent(s, evg_[g], 0));
   }
   NRG_HIP(hipGetLastError());
 }
