@@ -39,8 +39,10 @@ def parse():
     ap.add_argument("--workload", default="two_orbital_nk4000_it45")
     ap.add_argument("--sort", choices=["exact", "canonical"], default="exact")
     ap.add_argument("--no-profile", action="store_true", help="skip the HIP-event kernel timing")
-    ap.add_argument("--profile-every", type=int, default=7,
-                    help="time every n-th launch of the profiled families (7: coprime to the 32-column panel)")
+    ap.add_argument("--profile-every", type=int, default=61,
+                    help="time every n-th launch of the profiled families (61: prime, so coprime to the "
+                         "32-column panel; a timed tridiagonal SYMV launch runs with the device to itself, "
+                         "so sampling more often also slows the multi-stream schedule)")
     ap.add_argument("--profile-families", default="tridiag_symv",
                     help="comma-separated kernel families whose launches carry HIP timing events in the "
                          "timed steps (default: the dominant one, tridiag_symv; 'all' for every family)")
@@ -264,10 +266,10 @@ def main():
         names = {int(k): gpu.launch_log_kernel(int(k)) for k in set(L[:, 2].tolist())}
         gpu.launch_log(None)
         with open(a.launch_log, "w") as f:
-            f.write("family,kernel,threads,ordinal,bytes,flops,phase\n")
+            f.write("family,kernel,threads,ordinal,bytes,flops,phase,events\n")
             for i, r in enumerate(L):
                 f.write(f"{gpu.KERNEL_FAMILIES[int(r[0])]},{names[int(r[2])]},{int(r[1])},{int(r[3])},"
-                        f"{r[4]:.0f},{r[5]:.0f},{'warmup' if i < n_warm_log else 'timed'}\n")
+                        f"{r[4]:.0f},{r[5]:.0f},{'warmup' if i < n_warm_log else 'timed'},{int(r[6])}\n")
     ks = gpu.kernel_stats() if not a.no_profile else {}
     if not a.no_profile:
         gpu.profile(False)

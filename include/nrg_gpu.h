@@ -203,9 +203,10 @@ int nrg_gpu_profile_sample(int every);
 /* Launch log for matching profiler counters to algorithmic work: with
  * fam_mask != 0 every launch of a family in the mask is recorded (the call
  * clears the log; 0 stops).  nrg_gpu_launch_log_get writes up to cap records
- * out[6i..6i+5] = {family, threads per workgroup, kernel id, ordinal of the
+ * out[7i..7i+6] = {family, threads per workgroup, kernel id, ordinal of the
  * launch among all launches of the same kernel function since this call,
- * algorithmic bytes, algorithmic flops} and the record count to *count;
+ * algorithmic bytes, algorithmic flops, 1 if the launch carried timing
+ * events (nrg_gpu_profile) else 0} and the record count to *count;
  * nrg_gpu_launch_log_kernel gives the (mangled) name of a kernel id. */
 int nrg_gpu_launch_log(int fam_mask);
 int nrg_gpu_launch_log_get(double* out, int64_t cap, int64_t* count);

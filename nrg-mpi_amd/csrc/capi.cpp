@@ -607,12 +607,13 @@ int nrg_gpu_launch_log_get(double* out, int64_t cap, int64_t* count) {
     if (count) *count = (int64_t)L.size();
     const int64_t n = std::min<int64_t>(cap, (int64_t)L.size());
     for (int64_t i = 0; i < n && out; ++i) {
-      out[6 * i + 0] = L[i].fam;
-      out[6 * i + 1] = L[i].threads;
-      out[6 * i + 2] = L[i].kid;
-      out[6 * i + 3] = (double)L[i].nth;
-      out[6 * i + 4] = L[i].bytes;
-      out[6 * i + 5] = L[i].flops;
+      out[7 * i + 0] = L[i].fam;
+      out[7 * i + 1] = L[i].threads;
+      out[7 * i + 2] = L[i].kid;
+      out[7 * i + 3] = (double)L[i].nth;
+      out[7 * i + 4] = L[i].bytes;
+      out[7 * i + 5] = L[i].flops;
+      out[7 * i + 6] = L[i].timed;
     }
   });
 }

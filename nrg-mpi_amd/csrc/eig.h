@@ -85,6 +85,7 @@ class BatchedEigensolver {
   hipStream_t sg_[MAX_TR_GROUPS] = {};
   hipEvent_t evg_[MAX_TR_GROUPS] = {};
   hipEvent_t ev_fork_ = nullptr;
+  hipEvent_t evf_[MAX_TR_GROUPS] = {};   // fences around timed SYMV launches
   DevBuf<char> d_slotg_[MAX_TR_GROUPS];
   GemmRunner gemmg_[MAX_TR_GROUPS];
   GemmBatch batchg_[MAX_TR_GROUPS];

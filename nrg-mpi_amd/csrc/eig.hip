@@ -1276,7 +1276,9 @@ void BatchedEigensolver::release() {
       evg_[g] = nullptr;
     }
   }
-  if (ev_fork_) { (void)hipEventDestroy(ev_fork_); ev_fork_ = nullptr; } d_aoff_.release(); d_voff_.release();
+  if (ev_fork_) { (void)hipEventDestroy(ev_fork_); ev_fork_ = nullptr; }
+  for (int g = 0; g < MAX_TR_GROUPS; ++g)
+    if (evf_[g]) { (void)hipEventDestroy(evf_[g]); evf_[g] = nullptr; } d_aoff_.release(); d_voff_.release();
   d_d_.release(); d_e_.release(); d_tau_.release(); d_y_.release(); d_P_.release(); d_part_.release();
   d_ws_.release(); d_iws_.release(); d_tasks_.release(); d_k_.release();
   d_V_.release(); d_T_.release(); d_X_.release(); d_X2_.release(); d_rng_.release(); gemm_.release();
@@ -1381,6 +1383,8 @@ void BatchedEigensolver::tridiag(hipStream_t s) {
       NRG_HIP(hipEventCreateWithFlags(&evg_[g], hipEventDisableTiming));
     }
   if (ng > 1 && !ev_fork_) NRG_HIP(hipEventCreateWithFlags(&ev_fork_, hipEventDisableTiming));
+  for (int g = 0; g < ng && ng > 1; ++g)
+    if (!evf_[g]) NRG_HIP(hipEventCreateWithFlags(&evf_[g], hipEventDisableTiming));
   for (int g = 0; g < ng; ++g) {
     Group& G = grp[g];
     for (int q = g; q < nblk; q += ng) G.q.push_back(q);
@@ -1502,10 +1506,27 @@ void BatchedEigensolver::tridiag(hipStream_t s) {
       // panel-major order when the panel counts are nearly uniform
       const int pm = (align_end_ && (int64_t)act * tmax <= gB + gB / 8) ? 1 : 0;
       const int g = pm ? act * tmax : gB;
+      // A SYMV launch that carries timing events runs with the device to
+      // itself: the other groups' streams are fenced around it, so that its
+      // event interval is its execution and not also time spent waiting for
+      // workgroup slots held by another stream's kernel.
+      const int gi = (int)(&G - grp);
+      const bool excl = (ng > 1) && profiler().peek(KF_TR_SYMV);
+      if (excl)
+        for (int h = 0; h < ng; ++h)
+          if (h != gi) {
+            NRG_HIP(hipEventRecord(evf_[h], grp[h].s));
+            NRG_HIP(hipStreamWaitEvent(G.s, evf_[h], 0));
+          }
       if (pw == 2 * CG)
         plaunch(KF_TR_SYMV, bytes, flops, k_tr_symv<2, 16, 4>, dim3(g), dim3(1024), shB, G.s, ta, t, jj, act, pm);
       else
         plaunch(KF_TR_SYMV, bytes, flops, k_tr_symv<1, 16, 4>, dim3(g), dim3(1024), shB, G.s, ta, t, jj, act, pm);
+      if (excl) {
+        NRG_HIP(hipEventRecord(evf_[gi], G.s));
+        for (int h = 0; h < ng; ++h)
+          if (h != gi) NRG_HIP(hipStreamWaitEvent(grp[h].s, evf_[gi], 0));
+      }
       G.pw_prev = pw;
     }
   };

@@ -49,7 +49,7 @@ struct Profiler {
   // launch log: one record per launch of the logged families, with the
   // launch's ordinal among launches of the same kernel function (the
   // numbering a profiler's per-kernel dispatch counter uses)
-  struct LogRec { int fam, threads, kid; uint64_t nth; double bytes, flops; };
+  struct LogRec { int fam, threads, kid, timed; uint64_t nth; double bytes, flops; };
   struct KernelSeen { const void* fn; uint64_t count; std::string name; };
   uint32_t log_mask = 0;
   std::vector<LogRec> log;
@@ -77,6 +77,11 @@ struct Profiler {
     if (!on || !((mask >> fam) & 1u)) return false;
     return (seen[fam]++ % every) == 0;
   }
+  // whether the next launch of `fam` will be timed (does not count it)
+  bool peek(int fam) const {
+    if (!on || !((mask >> fam) & 1u)) return false;
+    return (seen[fam] % every) == 0;
+  }
   // call after the stream has been synchronised
   void drain() {
     for (auto& p : pend) {
@@ -99,13 +104,14 @@ template <typename F, typename... Args>
 inline void plaunch(int fam, double bytes, double flops, F kernel, const dim3& grid, const dim3& block,
                     uint32_t shm, hipStream_t s, Args... args) {
   Profiler& P = profiler();
+  const bool tm = P.timed(fam);
   if (P.log_mask) {
     int kid = 0;
     const uint64_t k = P.next_nth((const void*)kernel, s, kid);   // counted for every kernel
     if ((P.log_mask >> fam) & 1u)
-      P.log.push_back(Profiler::LogRec{fam, (int)(block.x * block.y * block.z), kid, k, bytes, flops});
+      P.log.push_back(Profiler::LogRec{fam, (int)(block.x * block.y * block.z), kid, tm ? 1 : 0, k, bytes, flops});
   }
-  if (P.timed(fam)) {
+  if (tm) {
     const hipEvent_t a = P.get(), b = P.get();
     hipExtLaunchKernelGGL(kernel, grid, block, shm, s, a, b, 0, args...);
     P.pend.push_back(Profiler::Pending{a, b, fam, bytes, flops});

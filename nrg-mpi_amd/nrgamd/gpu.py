@@ -216,13 +216,13 @@ def launch_log(families=None):
 
 
 def launch_log_get():
-    """Records (family, threads, kernel id, ordinal, bytes, flops) as an
-    (n, 6) array."""
+    """Records (family, threads, kernel id, ordinal, bytes, flops, timed) as
+    an (n, 7) array; timed is 1 for launches that carried timing events."""
     n = np.zeros(1, dtype=np.int64)
     _check(load().nrg_gpu_launch_log_get(None, 0, _p(n, _i64p)))
-    a = np.zeros(6 * max(int(n[0]), 1))
+    a = np.zeros(7 * max(int(n[0]), 1))
     _check(load().nrg_gpu_launch_log_get(_p(a, _f64p), int(n[0]), _p(n, _i64p)))
-    return a[:6 * int(n[0])].reshape(-1, 6)
+    return a[:7 * int(n[0])].reshape(-1, 7)
 
 
 def launch_log_kernel(kid):

@@ -81,12 +81,15 @@ def _ordinals(rows, pat):
     return out
 
 
-def family_window(d, log, family):
-    """Mean duration (us) of the family's dispatches in the timed steps."""
+def family_window(d, log, family, events_only=False):
+    """Mean duration (us) of the family's dispatches in the timed steps
+    (events_only: only those that carried the bench's timing events)."""
     rows = _rows(d, "kernel_trace.csv")
     want = {}
     for r in read_log(log):
         if r["family"] == family and r["phase"] == "timed":
+            if events_only and r.get("events", "1") != "1":
+                continue
             want.setdefault(kernel_key(r["kernel"]), set()).add(int(r["ordinal"]))
     tot, cnt = 0.0, 0
     for name, lst in _ordinals(rows, "|".join(sorted({k[0] for k in want}))).items():
@@ -109,13 +112,17 @@ def stats(d, title, command, top, log=None, family=None):
              "Kernel summary (`*kernel_stats.csv` of the run; durations in microseconds):", "",
              "| kernel | calls | total us | avg us | % |", "|---|---:|---:|---:|---:|"]
     for name, calls, tot, avg, pct in recs[:top]:
-        short = re.sub(r"\(.*$", "", name)
+        short = re.sub(r"\(.*$", "", name.replace("(anonymous namespace)::", ""))
         lines.append(f"| `{short}` | {calls} | {tot / 1e3:.0f} | {avg / 1e3:.2f} | {pct:.1f} |")
     if log and family:
-        us, n = family_window(d, log, family)
+        us_all, n_all = family_window(d, log, family)
+        us, n = family_window(d, log, family, events_only=True)
         lines += ["", f"Family `{family}` in the timed steps (dispatches located through the launch log of the "
-                      f"same run): {n} dispatches, mean duration {us:.2f} us.", "",
-                  f"timed_family_avg_us = {us:.3f}", f"timed_family_launches = {n}"]
+                      f"same run): {n_all} dispatches, mean duration {us_all:.2f} us; the {n} of them that "
+                      f"carried the bench's timing events (run with the device to themselves): mean duration "
+                      f"{us:.2f} us.", "",
+                  f"timed_family_avg_us = {us:.3f}", f"timed_family_launches = {n}",
+                  f"timed_family_all_avg_us = {us_all:.3f}", f"timed_family_all_launches = {n_all}"]
     return "\n".join(lines) + "\n"
 
 
