@@ -360,8 +360,9 @@ def main():
                     "frac": ach / FP64_PEAK_TFS, "traffic": None}
         roof["kernel_family"] = fam
         roof["launches_timed"] = k["launches"]
-        roof["timing"] = (f"HIP events on every {a.profile_every}th launch of the family over the timed steps "
-                          f"(achieved = algorithmic bytes or flops / event time of those launches)")
+        roof["timing"] = (f"HIP events on every {a.profile_every}-th launch of the family over the timed steps, "
+                          f"each such tridiagonal SYMV launch with the device to itself (achieved = algorithmic "
+                          f"bytes or flops / event time of those launches)")
         roof["avg_launch_us"] = 1e3 * k["ms"] / max(k["launches"], 1)
         roof["alg_bytes_per_launch"] = k["bytes"] / max(k["launches"], 1)
         roof["alg_flops_per_launch"] = k["flops"] / max(k["launches"], 1)
@@ -372,7 +373,9 @@ def main():
                                       "traffic_over_algorithmic": tr.get("traffic_over_algorithmic"),
                                       "source": tr["source"]}
         out["roofline"] = roof
-        # sampled launches scaled to all launches of the family
+        # sampled launches scaled to all launches of the family: kernel time
+        # summed over launches (launches of different streams overlap in the
+        # wall time of a step)
         out["kernel_ms_per_step_est"] = {f: v["ms"] * a.profile_every / a.steps for f, v in ks.items()
                                          if v["launches"]}
     else:
