@@ -53,6 +53,7 @@ class BatchedEigensolver {
   void tridiag(hipStream_t s);
   void divide_conquer(hipStream_t s);
   void back_transform(const int* lo, const int* hi, hipStream_t s);
+  void panel_factors(hipStream_t s);
 
   int nblk_ = 0;
   std::vector<int> n_;
@@ -74,6 +75,15 @@ class BatchedEigensolver {
   // back-transform
   DevBuf<double> d_V_, d_T_, d_X_, d_X2_;
   DevBuf<int64_t> d_rng_;
+  // reflector panels: V at vpo[b] + p n NBB, [G | T] at index tpo[b] + p
+  std::vector<int64_t> h_vpo_;
+  std::vector<int32_t> h_tpo_, h_plist_;
+  DevBuf<int64_t> d_vpo_;
+  DevBuf<int32_t> d_tpo_, d_plist_;
+  hipEvent_t ev_bt_ = nullptr;   // panel factors built (second stream)
+  GemmRunner gemm_bt_;           // their V^T V products
+  GemmBatch batch_bt_;
+  bool bt_wait_ = false;
   std::vector<int64_t> h_rng_;
   std::vector<int> done_;           // transformed columns per block
   GemmRunner gemm_;

@@ -1169,27 +1169,32 @@ __global__ __launch_bounds__(256) void k_dc_scatter(DcArgs a, const int32_t* tas
 // ===========================================================================
 // 3. back-transform
 // ===========================================================================
+// Reflector panels of all blocks: panel p (reflectors c0 = p NBB ..
+// c0 + NBB - 1) of block b has its explicit V (rows c0+1..n-1, ld n) at
+// vpo[b] + p n NBB and its [G | T] pair (2 NBB^2) at index tpo[b] + p.
 struct BtArgs {
   const int32_t* n;
-  const int32_t* order;
   const int64_t* aoff;
   const int64_t* voff;
   const double* A;
   const double* tau;
-  double* V;   // per block n x NBB (ld n)
-  double* T;   // per block 2 * NBB^2: [G | T]
+  double* V;
+  double* T;
+  const int64_t* vpo;
+  const int32_t* tpo;
 };
 
-// Explicit reflector panel V (rows c0+1..n-1) for panel p.
-__global__ void k_bt_buildv(BtArgs a, int p) {
-  const int b = a.order[blockIdx.y];
+// Explicit reflector panel V (rows c0+1..n-1).  Grid: (chunks, panels);
+// panel i of the list is block pl[2i], panel pl[2i+1].
+__global__ void k_bt_buildv(BtArgs a, const int32_t* pl) {
+  const int b = pl[2 * blockIdx.y], p = pl[2 * blockIdx.y + 1];
   const int n = a.n[b];
   const int c0 = p * NBB;
   if (c0 >= n - 1) return;
   const int nbp = min(NBB, n - 1 - c0);
   const int mr = n - c0 - 1;
   const double* A = a.A + a.aoff[b];
-  double* V = a.V + a.voff[b] * NBB;
+  double* V = a.V + a.vpo[b] + (int64_t)p * n * NBB;
   for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < mr * NBB; q += gridDim.x * blockDim.x) {
     const int r = q % mr, i = q / mr;
     const int row = c0 + 1 + r, c = c0 + i;
@@ -1208,14 +1213,14 @@ __global__ void k_bt_buildv(BtArgs a, int p) {
 // Row j of the product is shared by the TQ = 4 adjacent lanes 4j..4j+3
 // (terms l = j + sub, j + sub + 4, ...), combined with two xor shuffles.
 constexpr int TQ = 4;
-__global__ __launch_bounds__(NBB * TQ) void k_bt_tmat(BtArgs a, int p) {
-  const int b = a.order[blockIdx.x];
+__global__ __launch_bounds__(NBB * TQ) void k_bt_tmat(BtArgs a, const int32_t* pl) {
+  const int b = pl[2 * blockIdx.x], p = pl[2 * blockIdx.x + 1];
   const int n = a.n[b];
   const int c0 = p * NBB;
   if (c0 >= n - 1) return;
   const int nbp = min(NBB, n - 1 - c0);
-  const double* G = a.T + (int64_t)b * 2 * NBB * NBB;
-  double* T = a.T + (int64_t)b * 2 * NBB * NBB + NBB * NBB;
+  const double* G = a.T + (int64_t)(a.tpo[b] + p) * 2 * NBB * NBB;
+  double* T = a.T + (int64_t)(a.tpo[b] + p) * 2 * NBB * NBB + NBB * NBB;
   const double* tau = a.tau + a.voff[b] + c0;
   __shared__ double sT[NBB * NBB];
   __shared__ double tmp[NBB];
@@ -1277,6 +1282,9 @@ void BatchedEigensolver::release() {
     }
   }
   if (ev_fork_) { (void)hipEventDestroy(ev_fork_); ev_fork_ = nullptr; }
+  if (ev_bt_) { (void)hipEventDestroy(ev_bt_); ev_bt_ = nullptr; }
+  bt_wait_ = false;
+  d_vpo_.release(); d_tpo_.release(); d_plist_.release(); gemm_bt_.release();
   for (int g = 0; g < MAX_TR_GROUPS; ++g)
     if (evf_[g]) { (void)hipEventDestroy(evf_[g]); evf_[g] = nullptr; } d_aoff_.release(); d_voff_.release();
   d_d_.release(); d_e_.release(); d_tau_.release(); d_y_.release(); d_P_.release(); d_part_.release();
@@ -1317,6 +1325,18 @@ void BatchedEigensolver::solve(int nblk, const int* n, const int64_t* aoff, cons
   double t0 = now_ms();
   tridiag(s);
   if (st) { NRG_HIP(hipStreamSynchronize(s)); st->ms_tridiag = now_ms() - t0; t0 = now_ms(); }
+  // the back-transform's panel factors overlap the divide and conquer
+  if (sg_[1]) {
+    NRG_HIP(hipEventRecord(ev_fork_, s));
+    NRG_HIP(hipStreamWaitEvent(sg_[1], ev_fork_, 0));
+    panel_factors(sg_[1]);
+    if (!ev_bt_) NRG_HIP(hipEventCreateWithFlags(&ev_bt_, hipEventDisableTiming));
+    NRG_HIP(hipEventRecord(ev_bt_, sg_[1]));
+    bt_wait_ = true;
+  } else {
+    panel_factors(s);
+    bt_wait_ = false;
+  }
   divide_conquer(s);
   if (st) { NRG_HIP(hipStreamSynchronize(s)); st->ms_dc = now_ms() - t0; t0 = now_ms(); }
   done_.assign(nblk, 0);
@@ -1654,14 +1674,67 @@ void BatchedEigensolver::divide_conquer(hipStream_t s) {
   NRG_HIP(hipGetLastError());
 }
 
+// V, G = V^T V and T of every reflector panel of every block.  They depend
+// only on the tridiagonalisation (A below the subdiagonal, tau), not on the
+// columns transformed, so they are built once per solve, in three launches
+// over all panels, on stream s (issued while the divide and conquer runs on
+// the caller's stream when s is the second stream).
+void BatchedEigensolver::panel_factors(hipStream_t s) {
+  const int nblk = nblk_;
+  h_vpo_.assign(nblk, 0);
+  h_tpo_.assign(nblk, 0);
+  h_plist_.clear();
+  int64_t vacc = 0;
+  int32_t tacc = 0;
+  int64_t rows_max = 0;
+  for (int b = 0; b < nblk; ++b) {
+    const int n = n_[b];
+    const int np = (n - 1 + NBB - 1) / NBB;   // panels of reflectors 0..n-2
+    h_vpo_[b] = vacc;
+    h_tpo_[b] = tacc;
+    vacc += (int64_t)np * n * NBB;
+    tacc += np;
+    for (int p = 0; p < np; ++p) {
+      h_plist_.push_back(b);
+      h_plist_.push_back(p);
+    }
+    rows_max = std::max<int64_t>(rows_max, n);
+  }
+  const int npl = (int)(h_plist_.size() / 2);
+  if (npl == 0) return;
+  d_V_.reserve(std::max<int64_t>(vacc, 1));
+  d_T_.reserve((int64_t)tacc * 2 * NBB * NBB);
+  d_vpo_.upload(h_vpo_.data(), nblk, s);
+  d_tpo_.upload(h_tpo_.data(), nblk, s);
+  d_plist_.upload(h_plist_.data(), h_plist_.size(), s);
+  BtArgs ba{d_n_.p, d_aoff_.p, d_voff_.p, A_, d_tau_.p, d_V_.p, d_T_.p, d_vpo_.p, d_tpo_.p};
+  hipLaunchKernelGGL(k_bt_buildv, dim3((unsigned)std::max<int64_t>(1, ceil_div(rows_max * NBB, 256 * 4)), npl),
+                     dim3(256), 0, s, ba, (const int32_t*)d_plist_.p);
+  // G = V^T V for every panel
+  GemmBatch& bt = batch_bt_;
+  bt.clear();
+  for (int i = 0; i < npl; ++i) {
+    const int b = h_plist_[2 * i], p = h_plist_[2 * i + 1];
+    const int n = n_[b], c0 = p * NBB;
+    const int mr = n - c0 - 1;
+    const int nbp = std::min(NBB, n - 1 - c0);
+    const int64_t vo = h_vpo_[b] + (int64_t)p * n * NBB;
+    bt.add(gemm_task(nbp, nbp, mr, 1, 0, vo, n, vo, n, (int64_t)(h_tpo_[b] + p) * 2 * NBB * NBB, NBB, 1.0, 0.0));
+  }
+  gemm_bt_.run(bt, d_V_.p, d_V_.p, d_T_.p, nullptr, nullptr, nullptr, s, KF_GEMM_BACK);
+  plaunch(KF_BT_SMALL, 0, 0, k_bt_tmat, dim3(npl), dim3(NBB * TQ), 0, s, ba, (const int32_t*)d_plist_.p);
+  NRG_HIP(hipGetLastError());
+}
+
 void BatchedEigensolver::back_transform(const int* lo, const int* hi, hipStream_t s) {
   const int nblk = nblk_;
   const int nmax = n_[order_[0]];
-  d_V_.reserve(tot_n_ * NBB);
-  d_T_.reserve((int64_t)nblk * 2 * NBB * NBB);
+  if (bt_wait_) {   // the panel factors were built on the second stream
+    NRG_HIP(hipStreamWaitEvent(s, ev_bt_, 0));
+    bt_wait_ = false;
+  }
   d_X_.reserve(tot_n_ * NBB);
   d_X2_.reserve(tot_n_ * NBB);
-  BtArgs ba{d_n_.p, d_order_.p, d_aoff_.p, d_voff_.p, A_, d_tau_.p, d_V_.p, d_T_.p};
   const int npan = (nmax - 1 + NBB - 1) / NBB;
   for (int p = npan - 1; p >= 0; --p) {
     const int c0 = p * NBB;
@@ -1669,20 +1742,8 @@ void BatchedEigensolver::back_transform(const int* lo, const int* hi, hipStream_
     int act = 0;
     while (act < nblk && n_[order_[act]] > c0 + 1) ++act;
     if (act == 0) continue;
-    hipLaunchKernelGGL(k_bt_buildv, dim3(std::max(1, ceil_div((int64_t)(nmax - c0) * NBB, 256 * 4)), act),
-                       dim3(256), 0, s, ba, p);
-    // G = V^T V
-    batch_.clear();
-    for (int q = 0; q < act; ++q) {
-      const int b = order_[q];
-      const int n = n_[b];
-      const int mr = n - c0 - 1;
-      const int nbp = std::min(NBB, n - 1 - c0);
-      batch_.add(gemm_task(nbp, nbp, mr, 1, 0, voff_[b] * NBB, n, voff_[b] * NBB, n,
-                           (int64_t)b * 2 * NBB * NBB, NBB, 1.0, 0.0));
-    }
-    gemm_.run(batch_, d_V_.p, d_V_.p, d_T_.p, nullptr, nullptr, nullptr, s, KF_GEMM_BACK);
-    plaunch(KF_BT_SMALL, 0, 0, k_bt_tmat, dim3(act), dim3(NBB * TQ), 0, s, ba, p);
+    auto vo = [&](int b) { return h_vpo_[b] + (int64_t)p * n_[b] * NBB; };
+    auto to = [&](int b) { return (int64_t)(h_tpo_[b] + p) * 2 * NBB * NBB; };
     // X = V^T Z(c0+1:, :nc)
     batch_.clear();
     for (int q = 0; q < act; ++q) {
@@ -1692,9 +1753,10 @@ void BatchedEigensolver::back_transform(const int* lo, const int* hi, hipStream_
       const int nbp = std::min(NBB, n - 1 - c0);
       const int nc = hi[b] - lo[b];
       if (nc <= 0) continue;
-      batch_.add(gemm_task(nbp, nc, mr, 1, 0, voff_[b] * NBB, n, aoff_[b] + (int64_t)lo[b] * n + c0 + 1, n,
+      batch_.add(gemm_task(nbp, nc, mr, 1, 0, vo(b), n, aoff_[b] + (int64_t)lo[b] * n + c0 + 1, n,
                            voff_[b] * NBB, NBB, 1.0, 0.0));
     }
+    if (batch_.tasks.empty()) continue;
     gemm_.run(batch_, d_V_.p, Z_, d_X_.p, nullptr, nullptr, nullptr, s, KF_GEMM_BACK);
     // X2 = T X
     batch_.clear();
@@ -1704,8 +1766,8 @@ void BatchedEigensolver::back_transform(const int* lo, const int* hi, hipStream_
       const int nbp = std::min(NBB, n - 1 - c0);
       const int nc = hi[b] - lo[b];
       if (nc <= 0) continue;
-      batch_.add(gemm_task(nbp, nc, nbp, 0, 0, (int64_t)b * 2 * NBB * NBB + NBB * NBB, NBB,
-                           voff_[b] * NBB, NBB, voff_[b] * NBB, NBB, 1.0, 0.0));
+      batch_.add(gemm_task(nbp, nc, nbp, 0, 0, to(b) + NBB * NBB, NBB, voff_[b] * NBB, NBB, voff_[b] * NBB, NBB,
+                           1.0, 0.0));
     }
     gemm_.run(batch_, d_T_.p, d_X_.p, d_X2_.p, nullptr, nullptr, nullptr, s, KF_GEMM_BACK);
     // Z(c0+1:, :nc) -= V X2
@@ -1717,7 +1779,7 @@ void BatchedEigensolver::back_transform(const int* lo, const int* hi, hipStream_
       const int nbp = std::min(NBB, n - 1 - c0);
       const int nc = hi[b] - lo[b];
       if (nc <= 0) continue;
-      batch_.add(gemm_task(mr, nc, nbp, 0, 0, voff_[b] * NBB, n, voff_[b] * NBB, NBB,
+      batch_.add(gemm_task(mr, nc, nbp, 0, 0, vo(b), n, voff_[b] * NBB, NBB,
                            aoff_[b] + (int64_t)lo[b] * n + c0 + 1, n, -1.0, 1.0));
     }
     gemm_.run(batch_, d_V_.p, d_X2_.p, Z_, nullptr, nullptr, nullptr, s, KF_GEMM_BACK);
