@@ -186,7 +186,8 @@ int nrg_gpu_dgemm_batched(int ntask, const int32_t* mnk, int transa, int transb,
  * stops, -1 times every family, any other value is a bitmask of the
  * families to time (bit f = family f); each call clears the accumulators.
  * A timed launch carries its start/stop events in the dispatch itself
- * (hipExtLaunchKernel).  nrg_gpu_kernel_stats writes, for family
+ * (hipExtLaunchKernel); timing does not make any call synchronous.
+ * nrg_gpu_kernel_stats waits for the device and writes, for family
  * f < nfam, out[4f..4f+3] = {milliseconds, launches, algorithmic bytes,
  * algorithmic flops}.  Families: 0 H assembly, 1 tridiagonal column update,
  * 2 tridiagonal SYMV, 3 tridiagonal trailing GEMM, 4 D&C leaves, 5 D&C merge

@@ -29,10 +29,7 @@ static int guarded(F&& f) {
   Context& C = ctx();
   try {
     f(C);
-    if (profiler().on && C.stream) {
-      NRG_HIP(hipStreamSynchronize(C.stream));
-      profiler().drain();
-    }
+    if (profiler().on && C.stream) profiler().drain_ready();
     C.last_error.clear();
     return NRG_OK;
   } catch (const Error& e) {
@@ -578,7 +575,7 @@ int nrg_gpu_inject_fault(int iteration) {
 
 int nrg_gpu_profile(int enable) {
   return guarded([&](Context& C) {
-    if (C.stream) NRG_HIP(hipStreamSynchronize(C.stream));
+    if (C.stream) NRG_HIP(hipDeviceSynchronize());   // every stream of the library
     profiler().drain();
     profiler().reset();
     profiler().on = (enable != 0);
@@ -631,7 +628,9 @@ int nrg_gpu_launch_log_kernel(int kid, char* name, int cap) {
 }
 
 int nrg_gpu_kernel_stats(double* out, int nfam) {
-  return guarded([&](Context&) {
+  return guarded([&](Context& C) {
+    if (C.stream) NRG_HIP(hipDeviceSynchronize());
+    profiler().drain();
     for (int f = 0; f < nfam && f < KF_COUNT; ++f) {
       const KStat& k = profiler().st[f];
       out[4 * f + 0] = k.ms;

@@ -1,9 +1,11 @@
 // Optional per-kernel-family timing.  A profiled launch carries its own
 // start/stop events in the dispatch (hipExtLaunchKernel), so timing adds no
 // separate event records to the stream.  Off by default; `mask` selects the
-// families.  Each API call drains the events at its end (after its own
-// synchronisation) and accumulates duration, launch count and the
-// algorithmic bytes/flops the launch sites declare.
+// families.  Each API call collects the event pairs that have completed by
+// its end (without waiting, so that calls keep their asynchrony), and
+// reading the statistics waits for the rest; a collected pair adds its
+// duration, a launch and the algorithmic bytes/flops the launch site
+// declared.
 #pragma once
 #include <string>
 #include "common.h"
@@ -82,7 +84,22 @@ struct Profiler {
     if (!on || !((mask >> fam) & 1u)) return false;
     return (seen[fam] % every) == 0;
   }
-  // call after the stream has been synchronised
+  // collect the pairs whose stop event has completed, keep the others
+  void drain_ready() {
+    size_t k = 0;
+    for (size_t i = 0; i < pend.size(); ++i) {
+      const Pending p = pend[i];
+      if (hipEventQuery(p.b) != hipSuccess) { pend[k++] = p; continue; }
+      float ms = 0;
+      NRG_HIP(hipEventElapsedTime(&ms, p.a, p.b));
+      KStat& s = st[p.fam];
+      s.ms += ms; s.bytes += p.bytes; s.flops += p.flops; s.launches += 1;
+      pool.push_back(p.a);
+      pool.push_back(p.b);
+    }
+    pend.resize(k);
+  }
+  // call after the device has been synchronised
   void drain() {
     for (auto& p : pend) {
       float ms = 0;
