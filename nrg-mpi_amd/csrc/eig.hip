@@ -726,11 +726,15 @@ __global__ __launch_bounds__(256) void k_dc_merge_prep(DcArgs a, const int32_t* 
       if (j + 1 < m) { zn = szs[j + 1]; dn = sds[j + 1]; }
       if (rp2 * fabs(zj) <= tol) { df[nd_def++] = j; continue; }
       if (pj < 0) { pj = j; zp = zj; dp = dj; continue; }
-      const double tt = sqrt(zj * zj + zp * zp);   // |z| <= 1: no overflow
-      const double rt = 1.0 / tt;
-      const double cv = zj * rt, sv = -zp * rt;
       const double dt = dj - dp;
-      if (fabs(dt * cv * sv) <= tol) {
+      const double r2 = zj * zj + zp * zp;   // |z| <= 1: no overflow
+      // |dt c s| <= tol with c = zj / r, s = -zp / r, r^2 = zj^2 + zp^2,
+      // tested as |dt zj zp| <= tol r^2: the square root and the division
+      // stay off the loop-carried chain unless the pair deflates
+      if (fabs(dt * zj * zp) <= tol * r2) {
+        const double tt = sqrt(r2);
+        const double rt = 1.0 / tt;
+        const double cv = zj * rt, sv = -zp * rt;
         szs[pj] = 0.0;
         rp[2 * nrot] = pj;
         rp[2 * nrot + 1] = j;
