@@ -852,15 +852,20 @@ __global__ __launch_bounds__(256) void k_dc_rotate(DcArgs a, const int32_t* task
 }
 
 // Gather the rotated merge columns as [non-deflated | deflated] into Qcd
-// (m x m, ld m).  Grid: (row chunks of 64, merges); each workgroup owns 64
-// rows of one merge, its 4 waves take every 4th column.
+// (m x m, ld m).  Grid: (row chunks of 64, column chunks of DC_CCH,
+// merges); each workgroup owns 64 rows and DC_CCH columns of one merge, its
+// 4 waves take every 4th column (a top-level merge is thousands of
+// workgroups, enough to stream the m^2 copy at bandwidth).
+constexpr int DC_CCH = 256;
 __global__ __launch_bounds__(256) void k_dc_rotgather(DcArgs a, const int32_t* tasks, int ntask) {
-  const int t = blockIdx.y;
+  const int t = blockIdx.z;
   const int32_t* tk = tasks + 8 * t;
   const int b = tk[0], s = tk[1], e = tk[3];
   const int m = e - s;
   const int row0 = blockIdx.x * 64;
-  if (row0 >= m) return;
+  const int c0 = blockIdx.y * DC_CCH;
+  if (row0 >= m || c0 >= m) return;
+  const int c1 = min(m, c0 + DC_CCH);
   const int n = a.n[b];
   const int k = a.kout[t];
   double* Z = a.Z + a.aoff[b];
@@ -873,7 +878,7 @@ __global__ __launch_bounds__(256) void k_dc_rotgather(DcArgs a, const int32_t* t
   const int row = row0 + rl;
   if (row >= m) return;
 #pragma unroll 8
-  for (int c = g; c < m; c += 4) {
+  for (int c = c0 + g; c < c1; c += 4) {
     const int srcc = (c < k) ? col[nd[c]] : col[df[c - k]];
     Qcd[row + (int64_t)c * m] = Z[(int64_t)(s + row) + (int64_t)srcc * n];
   }
@@ -1146,14 +1151,17 @@ __global__ __launch_bounds__(256) void k_dc_finish(DcArgs a, const int32_t* task
 }
 
 // Write the merged eigenvectors in final (ascending) order into the merge's
-// square region of Z.  Grid: (row chunks of 64, merges).
+// square region of Z.  Grid: (row chunks of 64, column chunks of DC_CCH,
+// merges).
 __global__ __launch_bounds__(256) void k_dc_scatter(DcArgs a, const int32_t* tasks, int ntask) {
-  const int t = blockIdx.y;
+  const int t = blockIdx.z;
   const int32_t* tk = tasks + 8 * t;
   const int b = tk[0], s = tk[1], e = tk[3];
   const int m = e - s;
   const int row = blockIdx.x * 64 + (threadIdx.x & 63);
-  if (row >= m) return;
+  const int c0 = blockIdx.y * DC_CCH;
+  if (row >= m || c0 >= m) return;
+  const int c1 = min(m, c0 + DC_CCH);
   const int g = threadIdx.x >> 6;
   const int k = a.kout[t];
   const int n = a.n[b];
@@ -1163,7 +1171,7 @@ __global__ __launch_bounds__(256) void k_dc_scatter(DcArgs a, const int32_t* tas
   const double* Qcd = W + dws_vec(m);
   const double* Qt = W + dws_vec(m) + 2 * (int64_t)m * m;
 #pragma unroll 8
-  for (int c = g; c < m; c += 4) {
+  for (int c = c0 + g; c < c1; c += 4) {
     const int src = I[5 * m + c];
     const double v = (src < 0) ? Qt[row + (int64_t)(-1 - src) * m] : Qcd[row + (int64_t)(k + src) * m];
     Z[(int64_t)(s + row) + (int64_t)(s + c) * n] = v;
@@ -1642,7 +1650,8 @@ void BatchedEigensolver::divide_conquer(hipStream_t s) {
       const int32_t* tk = d_tasks_.p;
       plaunch(KF_DC_PREP, 0, 0, k_dc_merge_prep, dim3(nt), dim3(256), (uint32_t)shm, s, da, tk, nt);
       plaunch(KF_DC_PREP, 0, 0, k_dc_rotate, dim3(ceil_div(mmax, 256), nt), dim3(256), 0, s, da, tk, nt);
-      plaunch(KF_DC_PREP, 0, 0, k_dc_rotgather, dim3(ceil_div(mmax, 64), nt), dim3(256), 0, s, da, tk, nt);
+      plaunch(KF_DC_PREP, 0, 0, k_dc_rotgather, dim3(ceil_div(mmax, 64), ceil_div(mmax, DC_CCH), nt), dim3(256), 0,
+              s, da, tk, nt);
     }
     h_k_.resize(nt);
     d_k_.download(h_k_.data(), nt, s);
@@ -1673,7 +1682,8 @@ void BatchedEigensolver::divide_conquer(hipStream_t s) {
     while (P < std::min(mmax, DC_LDS_M)) P <<= 1;
     plaunch(KF_DC_FINISH, 0, 0, k_dc_finish, dim3(nt), dim3(256),
             (uint32_t)((sizeof(double) + sizeof(int32_t)) * (size_t)P), s, da, tk, nt);
-    plaunch(KF_DC_FINISH, 0, 0, k_dc_scatter, dim3(ceil_div(mmax, 64), nt), dim3(256), 0, s, da, tk, nt);
+    plaunch(KF_DC_FINISH, 0, 0, k_dc_scatter, dim3(ceil_div(mmax, 64), ceil_div(mmax, DC_CCH), nt), dim3(256), 0,
+            s, da, tk, nt);
   }
   NRG_HIP(hipGetLastError());
 }
