@@ -54,6 +54,7 @@ class BatchedEigensolver {
   void divide_conquer(hipStream_t s);
   void back_transform(const int* lo, const int* hi, hipStream_t s);
   void panel_factors(hipStream_t s);
+  void materialize(const int* hi, hipStream_t s);
 
   int nblk_ = 0;
   std::vector<int> n_;
@@ -72,6 +73,12 @@ class BatchedEigensolver {
   DevBuf<int32_t> d_tasks_;
   DevBuf<int32_t> d_k_;
   std::vector<int32_t> h_k_;
+  // deferred top merge of each block (lev 0: the block is a leaf)
+  struct TopMerge { int lev = 0, m = 0, k = 0; int64_t wo = 0, io = 0; };
+  std::vector<TopMerge> top_;
+  std::vector<int> mat_;                    // columns of Z holding tridiagonal eigenvectors
+  DevBuf<double> d_usel_;
+  DevBuf<int32_t> d_mtask_;
   // back-transform
   DevBuf<double> d_V_, d_T_, d_X_, d_X2_;
   DevBuf<int64_t> d_rng_;

@@ -1178,6 +1178,51 @@ __global__ __launch_bounds__(256) void k_dc_scatter(DcArgs a, const int32_t* tas
   }
 }
 
+// Deferred top merges.  A block's last merge (the whole block) keeps its
+// workspace until the columns are asked for; final column c of the block is
+// Qcd[:, :k] U[:, i] when it is root i of the secular equation
+// (I[5m + c] = -1 - i) or the deflated column Qcd[:, k + src] (src >= 0).
+// Task record (12 ints): b, lo, nc, k, m, pad, wo (2), io (2), uoff (2).
+struct DcTop {
+  const int32_t* n;
+  const int64_t* aoff;
+  const double* ws;
+  const int32_t* iws;
+  double* Z;
+  double* usel;
+};
+
+// Usel (k x nc, ld k): column j = U[:, i] for a root, zero for a deflated
+// column.  Grid: (chunks of 256 rows of U, nc, tasks).
+__global__ __launch_bounds__(256) void k_dc_top_select(DcTop a, const int32_t* tasks) {
+  const int32_t* tk = tasks + 12 * blockIdx.z;
+  const int lo = tk[1], nc = tk[2], k = tk[3], m = tk[4];
+  const int j = blockIdx.y;
+  const int r = blockIdx.x * 256 + threadIdx.x;
+  if (j >= nc || r >= k) return;
+  const double* W = a.ws + join64(tk[6], tk[7]);
+  const int32_t* I = a.iws + join64(tk[8], tk[9]);
+  const int src = I[5 * m + lo + j];
+  double* us = a.usel + join64(tk[10], tk[11]);
+  us[r + (int64_t)j * k] = (src < 0) ? W[dws_vec(m) + (int64_t)m * m + r + (int64_t)(-1 - src) * k] : 0.0;
+}
+
+// Deflated columns into Z (after the product wrote the root columns).
+// Grid: (chunks of 256 rows, nc, tasks).
+__global__ __launch_bounds__(256) void k_dc_top_deflated(DcTop a, const int32_t* tasks) {
+  const int32_t* tk = tasks + 12 * blockIdx.z;
+  const int b = tk[0], lo = tk[1], nc = tk[2], k = tk[3], m = tk[4];
+  const int j = blockIdx.y;
+  const int r = blockIdx.x * 256 + threadIdx.x;
+  if (j >= nc || r >= m) return;
+  const int32_t* I = a.iws + join64(tk[8], tk[9]);
+  const int src = I[5 * m + lo + j];
+  if (src < 0) return;
+  const double* W = a.ws + join64(tk[6], tk[7]);
+  const int n = a.n[b];
+  a.Z[a.aoff[b] + r + (int64_t)(lo + j) * n] = W[dws_vec(m) + r + (int64_t)(k + src) * m];
+}
+
 // ===========================================================================
 // 3. back-transform
 // ===========================================================================
@@ -1300,7 +1345,7 @@ void BatchedEigensolver::release() {
   for (int g = 0; g < MAX_TR_GROUPS; ++g)
     if (evf_[g]) { (void)hipEventDestroy(evf_[g]); evf_[g] = nullptr; } d_aoff_.release(); d_voff_.release();
   d_d_.release(); d_e_.release(); d_tau_.release(); d_y_.release(); d_P_.release(); d_part_.release();
-  d_ws_.release(); d_iws_.release(); d_tasks_.release(); d_k_.release();
+  d_ws_.release(); d_iws_.release(); d_tasks_.release(); d_k_.release(); d_usel_.release(); d_mtask_.release();
   d_V_.release(); d_T_.release(); d_X_.release(); d_X2_.release(); d_rng_.release(); gemm_.release();
   nblk_ = 0;
 }
@@ -1593,6 +1638,11 @@ void BatchedEigensolver::divide_conquer(hipStream_t s) {
     }
   }
   levels.resize(maxL);
+  // A block's last merge (level L[b], the whole block) is deferred: its
+  // eigenvector product and final scatter run when columns are asked for
+  // (materialize), so its workspace lives past the per-level range and its
+  // records follow the level's other merges.
+  std::vector<std::vector<int32_t>> tops(maxL);
   int64_t wsmax = 0, iwsmax = 0;
   for (int lev = 1; lev <= maxL; ++lev) {
     auto& rec = levels[lev - 1];
@@ -1607,6 +1657,10 @@ void BatchedEigensolver::divide_conquer(hipStream_t s) {
         const int s0 = bd(j << lev), mid = bd((j << lev) + (1 << (lev - 1))), e0 = bd((j + 1) << lev);
         if (!(s0 < mid && mid < e0)) continue;
         const int m = e0 - s0;
+        if (lev == l) {
+          for (int32_t v : {b, s0, mid, e0, 0, 0, 0, 0}) tops[lev - 1].push_back(v);
+          continue;
+        }
         rec.push_back(b); rec.push_back(s0); rec.push_back(mid); rec.push_back(e0);
         rec.push_back((int32_t)(wacc & 0xffffffff)); rec.push_back((int32_t)(wacc >> 32));
         rec.push_back((int32_t)(iacc & 0xffffffff)); rec.push_back((int32_t)(iacc >> 32));
@@ -1617,8 +1671,25 @@ void BatchedEigensolver::divide_conquer(hipStream_t s) {
     wsmax = std::max(wsmax, wacc);
     iwsmax = std::max(iwsmax, iacc);
   }
-  d_ws_.reserve(std::max<int64_t>(wsmax, 1));
-  d_iws_.reserve(std::max<int64_t>(iwsmax, 1));
+  top_.assign(nblk, TopMerge{});
+  std::vector<int> nreg(maxL);
+  int64_t wtop = wsmax, itop = iwsmax;
+  for (int lev = 1; lev <= maxL; ++lev) {
+    auto& rec = levels[lev - 1];
+    nreg[lev - 1] = (int)rec.size() / 8;
+    const auto& tr = tops[lev - 1];
+    for (size_t i = 0; i < tr.size(); i += 8) {
+      const int b = tr[i], m = tr[i + 3] - tr[i + 1];
+      for (int q = 0; q < 4; ++q) rec.push_back(tr[i + q]);
+      rec.push_back((int32_t)(wtop & 0xffffffff)); rec.push_back((int32_t)(wtop >> 32));
+      rec.push_back((int32_t)(itop & 0xffffffff)); rec.push_back((int32_t)(itop >> 32));
+      top_[b].lev = lev; top_[b].m = m; top_[b].wo = wtop; top_[b].io = itop;
+      wtop += 9 * (int64_t)m + 3 * (int64_t)m * m + 8;
+      itop += 8 * (int64_t)m + 8;
+    }
+  }
+  d_ws_.reserve(std::max<int64_t>(wtop, 1));
+  d_iws_.reserve(std::max<int64_t>(itop, 1));
   // zero Z (off-diagonal blocks of the merge tree must be zero)
   fill(Z_, 0.0, tot_n2_, s);
   DcArgs da{d_n_.p, d_aoff_.p, d_voff_.p, d_d_.p, d_e_.p, w_, Z_, d_ws_.p, d_iws_.p, nullptr};
@@ -1665,7 +1736,7 @@ void BatchedEigensolver::divide_conquer(hipStream_t s) {
       plaunch(KF_DC_VECTORS, 0, 0, k_dc_zhat, dim3(ceil_div(kmax, 64), nt), dim3(64), 0, s, da, tk, nt);
       plaunch(KF_DC_VECTORS, 0, 0, k_dc_vectors, dim3(kmax, nt), dim3(64), 0, s, da, tk, nt);
       batch_.clear();
-      for (int i = 0; i < nt; ++i) {
+      for (int i = 0; i < nreg[lev - 1]; ++i) {
         const int k = h_k_[i];
         if (k == 0) continue;
         const int m = rec[8 * i + 3] - rec[8 * i + 1];
@@ -1676,15 +1747,19 @@ void BatchedEigensolver::divide_conquer(hipStream_t s) {
       }
       gemm_.run(batch_, d_ws_.p, d_ws_.p, d_ws_.p, nullptr, nullptr, nullptr, s, KF_GEMM_DC);
     }
+    for (int i = nreg[lev - 1]; i < nt; ++i) top_[rec[8 * i]].k = h_k_[i];
     const int32_t* tk = d_tasks_.p;
     // LDS: (value, index) pairs of the deflated values, padded to a power of two
     int P = 1;
     while (P < std::min(mmax, DC_LDS_M)) P <<= 1;
     plaunch(KF_DC_FINISH, 0, 0, k_dc_finish, dim3(nt), dim3(256),
             (uint32_t)((sizeof(double) + sizeof(int32_t)) * (size_t)P), s, da, tk, nt);
-    plaunch(KF_DC_FINISH, 0, 0, k_dc_scatter, dim3(ceil_div(mmax, 64), ceil_div(mmax, DC_CCH), nt), dim3(256), 0,
-            s, da, tk, nt);
+    if (nreg[lev - 1] > 0)
+      plaunch(KF_DC_FINISH, 0, 0, k_dc_scatter, dim3(ceil_div(mmax, 64), ceil_div(mmax, DC_CCH), nreg[lev - 1]),
+              dim3(256), 0, s, da, tk, nreg[lev - 1]);
   }
+  mat_.resize(nblk);
+  for (int b = 0; b < nblk; ++b) mat_[b] = (top_[b].lev > 0) ? 0 : n_[b];   // leaves write Z directly
   NRG_HIP(hipGetLastError());
 }
 
@@ -1817,6 +1892,46 @@ void BatchedEigensolver::back_transform(const int* lo, const int* hi, hipStream_
   NRG_HIP(hipGetLastError());
 }
 
+// Columns [mat_b, hi_b) of the deferred top merges into Z: Usel gathers the
+// roots' secular vectors (zero for deflated columns), Z = Qcd[:, :k] Usel on
+// the matrix cores, then the deflated columns are copied.
+void BatchedEigensolver::materialize(const int* hi, hipStream_t s) {
+  std::vector<int32_t> tk;
+  int64_t uacc = 0;
+  int ncmax = 0, kmax = 0, mmax = 0;
+  batch_.clear();
+  for (int b = 0; b < nblk_; ++b) {
+    const TopMerge& T = top_[b];
+    if (T.lev <= 0 || hi[b] <= mat_[b]) continue;
+    const int lo = mat_[b], nc = hi[b] - lo, k = T.k, m = T.m;
+    for (int32_t v : {b, lo, nc, k, m, 0, (int32_t)(T.wo & 0xffffffff), (int32_t)(T.wo >> 32),
+                      (int32_t)(T.io & 0xffffffff), (int32_t)(T.io >> 32), (int32_t)(uacc & 0xffffffff),
+                      (int32_t)(uacc >> 32)})
+      tk.push_back(v);
+    if (k > 0)
+      batch_.add(gemm_task(m, nc, k, 0, 0, T.wo + 9 * (int64_t)m, m, uacc, k, aoff_[b] + (int64_t)lo * n_[b], n_[b],
+                           1.0, 0.0));
+    uacc += (int64_t)k * nc;
+    ncmax = std::max(ncmax, nc);
+    kmax = std::max(kmax, k);
+    mmax = std::max(mmax, m);
+    mat_[b] = hi[b];
+  }
+  if (tk.empty()) return;
+  const int nt = (int)tk.size() / 12;
+  d_usel_.reserve(std::max<int64_t>(uacc, 1));
+  d_mtask_.upload(tk.data(), tk.size(), s);
+  DcTop a{d_n_.p, d_aoff_.p, d_ws_.p, d_iws_.p, Z_, d_usel_.p};
+  if (kmax > 0)
+    plaunch(KF_DC_FINISH, 0, 0, k_dc_top_select, dim3(ceil_div(kmax, 256), ncmax, nt), dim3(256), 0, s, a,
+            (const int32_t*)d_mtask_.p);
+  if (!batch_.tasks.empty())
+    gemm_.run(batch_, d_ws_.p, d_usel_.p, Z_, nullptr, nullptr, nullptr, s, KF_GEMM_DC);
+  plaunch(KF_DC_FINISH, 0, 0, k_dc_top_deflated, dim3(ceil_div(mmax, 256), ncmax, nt), dim3(256), 0, s, a,
+          (const int32_t*)d_mtask_.p);
+  NRG_HIP(hipGetLastError());
+}
+
 void BatchedEigensolver::back_columns(const int* hi, hipStream_t s, double* ms) {
   NRG_REQUIRE(nblk_ > 0, "eigensolver: no solve to back-transform");
   std::vector<int> lo(done_), h(nblk_);
@@ -1827,6 +1942,7 @@ void BatchedEigensolver::back_columns(const int* hi, hipStream_t s, double* ms) 
   }
   if (!any) return;
   const double t0 = now_ms();
+  materialize(h.data(), s);
   back_transform(lo.data(), h.data(), s);
   done_ = h;
   if (ms) { NRG_HIP(hipStreamSynchronize(s)); *ms += now_ms() - t0; }
