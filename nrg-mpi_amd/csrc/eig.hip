@@ -1825,6 +1825,8 @@ void BatchedEigensolver::back_transform(const int* lo, const int* hi, hipStream_
   d_X_.reserve(tot_n_ * NBB);
   d_X2_.reserve(tot_n_ * NBB);
   const int npan = (nmax - 1 + NBB - 1) / NBB;
+  // the panels' products are recorded and issued after one table upload
+  gemm_.begin();
   for (int p = npan - 1; p >= 0; --p) {
     const int c0 = p * NBB;
     // active: n - 1 > c0  <=>  n > c0 + 1
@@ -1873,6 +1875,7 @@ void BatchedEigensolver::back_transform(const int* lo, const int* hi, hipStream_
     }
     gemm_.run(batch_, d_V_.p, d_X2_.p, Z_, nullptr, nullptr, nullptr, s, KF_GEMM_BACK);
   }
+  gemm_.flush(s);
   // chop the transformed columns (element ranges [aoff + lo n, aoff + hi n))
   h_rng_.clear();
   int64_t longest = 0;

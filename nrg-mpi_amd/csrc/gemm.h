@@ -81,8 +81,38 @@ struct GemmRunner {
   void run_split(const GemmBatch& b, const double* A, const double* B, double* C,
                  const int32_t* gatherA, const int32_t* gatherB, const double* kscale,
                  int slices, hipStream_t s, int family = KF_OTHER);
+  // Recording: between begin() and flush(), run() appends its task tables
+  // (and split-K reduction records) to one host buffer and its launches to a
+  // list; flush() uploads the buffer in one copy and issues the launches in
+  // the recorded order on its stream.  Saves a copy per product where a
+  // caller issues many small batched products in a row.
+  struct RLaunch {
+    int kind;                  // 0: GEMM, 1: split-K reduction
+    int fast, tv, family;
+    unsigned grid, gx;
+    int nt;
+    size_t tab, tst;           // byte offsets of the task table / tile starts (or reduction records)
+    double bytes, flops;
+    const double *A, *B;
+    double* C;                 // nullptr: the split-K workspace
+    const int32_t *gA, *gB;
+    const double* ks;
+  };
+  bool rec_ = false;
+  std::vector<char> r_host_;
+  DevBuf<char> r_dev_;
+  std::vector<RLaunch> r_l_;
+  int64_t r_ws_ = 0;
+  void begin() { rec_ = true; r_host_.clear(); r_l_.clear(); r_ws_ = 0; }
+  void flush(hipStream_t s);
+  size_t r_append(const void* p, size_t n) {
+    const size_t o = (r_host_.size() + 255) / 256 * 256;
+    r_host_.resize(o + n);
+    std::memcpy(r_host_.data() + o, p, n);
+    return o;
+  }
   void release() {
-    d_pack[0].release(); d_pack[1].release(); d_red_.release(); d_splitw_.release();
+    d_pack[0].release(); d_pack[1].release(); d_red_.release(); d_splitw_.release(); r_dev_.release();
   }
 };
 

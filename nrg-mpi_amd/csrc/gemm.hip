@@ -383,12 +383,23 @@ void GemmRunner::run(const GemmBatch& b, const double* A, const double* B, doubl
   }
   if (!plain_.tasks.empty()) launch(plain_, A, B, C, gatherA, gatherB, kscale, s, family, 0);
   if (!part_.tasks.empty()) {
-    d_splitw_.reserve(wsz);
-    launch(part_, A, B, d_splitw_.p, gatherA, gatherB, kscale, s, family, 1);
-    d_red_.upload(h_red_, s);
     int64_t mx = 0;
     for (const SplitRed& r : h_red_) mx = std::max<int64_t>(mx, (int64_t)r.m * r.n);
     const unsigned gx = (unsigned)std::min<int64_t>(ceil_div(mx, 256), 256);
+    if (rec_) {
+      // the partial products go to the shared workspace (C = nullptr)
+      r_ws_ = std::max(r_ws_, wsz);
+      launch(part_, A, B, nullptr, gatherA, gatherB, kscale, s, family, 1);
+      RLaunch r{};
+      r.kind = 1; r.family = family; r.gx = gx; r.nt = (int)h_red_.size();
+      r.tab = r_append(h_red_.data(), h_red_.size() * sizeof(SplitRed));
+      r.C = C;
+      r_l_.push_back(r);
+      return;
+    }
+    d_splitw_.reserve(wsz);
+    launch(part_, A, B, d_splitw_.p, gatherA, gatherB, kscale, s, family, 1);
+    d_red_.upload(h_red_, s);
     plaunch(family, 0, 0, k_splitk_reduce, dim3(gx, (unsigned)h_red_.size()), dim3(256), 0, s,
             (const SplitRed*)d_red_.p, (const double*)d_splitw_.p, C);
   }
@@ -414,15 +425,28 @@ void GemmRunner::launch(const GemmBatch& b, const double* A, const double* B, do
     total += t.lower ? tm * (tm + 1) / 2 : tm * ceil_div(t.n, TN);
   }
   NRG_REQUIRE(total < (1LL << 31), "gemm: too many tiles");
-  const GemmTask* tk;
-  const int32_t* ts;
-  upload_tables(b, s, tk, ts, slot);
   bool uniform = true;
   for (int i = 1; i < nt && uniform; ++i)
     uniform = (b.tasks[i].transA == b.tasks[0].transA && b.tasks[i].transB == b.tasks[0].transB);
   const int tv = b.tasks[0].transA * 2 + b.tasks[0].transB;
+  const bool fast = (!gatherA && !gatherB && !kscale && uniform);
+  if (rec_) {
+    RLaunch r{};
+    r.kind = 0; r.fast = fast ? 1 : 0; r.tv = tv; r.family = family;
+    r.grid = (unsigned)total; r.nt = nt;
+    r.tab = r_append(b.tasks.data(), (size_t)nt * sizeof(GemmTask));
+    r.tst = r_append(h_tile_start.data(), (size_t)nt * sizeof(int32_t));
+    r.bytes = bytes; r.flops = flops;
+    r.A = A; r.B = B; r.C = (slot == 1) ? nullptr : C;
+    r.gA = gatherA; r.gB = gatherB; r.ks = kscale;
+    r_l_.push_back(r);
+    return;
+  }
+  const GemmTask* tk;
+  const int32_t* ts;
+  upload_tables(b, s, tk, ts, slot);
   const dim3 g((unsigned)total), blk(256);
-  if (!gatherA && !gatherB && !kscale && uniform) {
+  if (fast) {
     if (tv == 0) plaunch(family, bytes, flops, k_gemm_f64_fast<0, 0>, g, blk, 0, s, tk, ts, nt, A, B, C);
     else if (tv == 1) plaunch(family, bytes, flops, k_gemm_f64_fast<0, 1>, g, blk, 0, s, tk, ts, nt, A, B, C);
     else if (tv == 2) plaunch(family, bytes, flops, k_gemm_f64_fast<1, 0>, g, blk, 0, s, tk, ts, nt, A, B, C);
@@ -430,6 +454,36 @@ void GemmRunner::launch(const GemmBatch& b, const double* A, const double* B, do
   } else {
     plaunch(family, bytes, flops, k_gemm_f64, g, blk, 0, s, tk, ts, nt, A, B, C, gatherA, gatherB, kscale);
   }
+}
+
+void GemmRunner::flush(hipStream_t s) {
+  rec_ = false;
+  if (r_l_.empty()) return;
+  if (r_ws_ > 0) d_splitw_.reserve(r_ws_);
+  r_dev_.upload(r_host_.data(), r_host_.size(), s);
+  const char* base = r_dev_.p;
+  const dim3 blk(256);
+  for (const RLaunch& r : r_l_) {
+    if (r.kind == 1) {
+      plaunch(r.family, 0, 0, k_splitk_reduce, dim3(r.gx, (unsigned)r.nt), blk, 0, s,
+              reinterpret_cast<const SplitRed*>(base + r.tab), (const double*)d_splitw_.p, r.C);
+      continue;
+    }
+    const GemmTask* tk = reinterpret_cast<const GemmTask*>(base + r.tab);
+    const int32_t* ts = reinterpret_cast<const int32_t*>(base + r.tst);
+    double* C = r.C ? r.C : d_splitw_.p;
+    const dim3 g(r.grid);
+    if (r.fast) {
+      if (r.tv == 0) plaunch(r.family, r.bytes, r.flops, k_gemm_f64_fast<0, 0>, g, blk, 0, s, tk, ts, r.nt, r.A, r.B, C);
+      else if (r.tv == 1) plaunch(r.family, r.bytes, r.flops, k_gemm_f64_fast<0, 1>, g, blk, 0, s, tk, ts, r.nt, r.A, r.B, C);
+      else if (r.tv == 2) plaunch(r.family, r.bytes, r.flops, k_gemm_f64_fast<1, 0>, g, blk, 0, s, tk, ts, r.nt, r.A, r.B, C);
+      else plaunch(r.family, r.bytes, r.flops, k_gemm_f64_fast<1, 1>, g, blk, 0, s, tk, ts, r.nt, r.A, r.B, C);
+    } else {
+      plaunch(r.family, r.bytes, r.flops, k_gemm_f64, g, blk, 0, s, tk, ts, r.nt, r.A, r.B, C, r.gA, r.gB, r.ks);
+    }
+  }
+  r_l_.clear();
+  NRG_HIP(hipGetLastError());
 }
 
 // ---------------------------------------------------------------------------

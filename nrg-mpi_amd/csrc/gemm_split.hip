@@ -261,6 +261,7 @@ void GemmRunner::run_split(const GemmBatch& b, const double* A, const double* B,
   const int nt = (int)b.tasks.size();
   if (nt == 0) return;
   NRG_REQUIRE(slices >= 3 && slices <= SP_MAXS, "gemm: split slices must be 3..8");
+  NRG_REQUIRE(!rec_, "gemm: split-precision products are not recorded");
   h_tile_start.resize(nt);
   long long total = 0;
   double flops = 0, bytes = 0;
