@@ -1452,6 +1452,7 @@ void BatchedEigensolver::tridiag(hipStream_t s) {
     DevBuf<char>* d_slot;
     hipStream_t s;
     int pw_prev = CG;          // panel width of the previous column's SYMV launch
+    std::vector<size_t> syr2k; // recorded trailing-update launches of panel t / NB: [syr2k[i], syr2k[i+1])
   };
   Group grp[MAX_TR_GROUPS];
   for (int g = 1; g < ng; ++g)
@@ -1486,6 +1487,36 @@ void BatchedEigensolver::tridiag(hipStream_t s) {
     // the other streams start after everything queued so far (inputs, tables)
     NRG_HIP(hipEventRecord(ev_fork_, s));
     for (int g = 1; g < ng; ++g) NRG_HIP(hipStreamWaitEvent(sg_[g], ev_fork_, 0));
+  }
+  // trailing updates [V W][W V]^T of panel boundary t (t = NB, 2 NB, ...):
+  // every group's products are recorded up front and their task tables
+  // uploaded once; the step loop issues each panel's launches in place
+  auto syr2k_batch = [&](Group& G, int t) {
+    G.batch->clear();
+    for (size_t i = 0; i < G.q.size(); ++i) {
+      const int b = order_[G.q[i]];
+      const int nb_ = n_[b];
+      const int c = t - G.coff[i];
+      const int mt = nb_ - c;
+      if (c <= 0 || mt <= 0) continue;
+      GemmTask tk = gemm_task(mt, mt, 2 * NB, 0, 1,
+                              poff[b] + c, nb_,                       // [V W] rows c..
+                              poff[b] + c + (int64_t)2 * NB * nb_, nb_, // [W V] rows c..
+                              aoff_[b] + c + (int64_t)c * nb_, nb_, -1.0, 1.0);
+      tk.lower = 1;   // only the lower triangle is read from here on
+      G.batch->add(tk);
+    }
+  };
+  for (int g = 0; g < ng; ++g) {
+    Group& G = grp[g];
+    G.gemm->begin();
+    G.syr2k.assign(1, 0);
+    for (int t = NB; t < T; t += NB) {
+      syr2k_batch(G, t);
+      G.gemm->run(*G.batch, d_P_.p, d_P_.p, A_, nullptr, nullptr, nullptr, G.s, KF_GEMM_TRIDIAG);
+      G.syr2k.push_back(G.gemm->recorded());
+    }
+    G.gemm->upload_recorded(G.s);
   }
 
   auto step = [&](Group& G, int t) {
@@ -1525,21 +1556,8 @@ void BatchedEigensolver::tridiag(hipStream_t s) {
       // close the previous panel: finalise its last w, trailing SYR2K
       plaunch(KF_TR_COL, colbytes(t), 0, k_tr_col, dim3(gAl), dim3(1024), 0, G.s, ta, t, NB - 1, -1, NB - 1,
               G.pw_prev, act, tmA);
-      G.batch->clear();
-      for (int i = 0; i < act; ++i) {
-        const int b = order_[G.q[i]];
-        const int nb_ = n_[b];
-        const int c = t - G.coff[i];
-        const int mt = nb_ - c;
-        if (c <= 0 || mt <= 0) continue;
-        GemmTask tk = gemm_task(mt, mt, 2 * NB, 0, 1,
-                                poff[b] + c, nb_,                       // [V W] rows c..
-                                poff[b] + c + (int64_t)2 * NB * nb_, nb_, // [W V] rows c..
-                                aoff_[b] + c + (int64_t)c * nb_, nb_, -1.0, 1.0);
-        tk.lower = 1;   // only the lower triangle is read from here on
-        G.batch->add(tk);
-      }
-      G.gemm->run(*G.batch, d_P_.p, d_P_.p, A_, nullptr, nullptr, nullptr, G.s, KF_GEMM_TRIDIAG);
+      const int pi = t / NB - 1;
+      G.gemm->issue(G.syr2k[pi], G.syr2k[pi + 1], G.s);
       plaunch(KF_TR_COL, colbytes(t), 0, k_tr_col, dim3(gAl), dim3(1024), 0, G.s, ta, t, -1, 0, NB - 1,
               G.pw_prev, act, tmA);
     } else {
@@ -1609,6 +1627,7 @@ void BatchedEigensolver::tridiag(hipStream_t s) {
   };
   for (int t = 0; t < T; ++t)
     for (int g = 0; g < ng; ++g) step(grp[g], t);
+  for (int g = 0; g < ng; ++g) grp[g].gemm->end_replay();
   for (int g = 1; g < ng; ++g) {
     NRG_HIP(hipEventRecord(evg_[g], sg_[g]));
     NRG_HIP(hipStreamWaitEvent(s, evg_[g], 0));

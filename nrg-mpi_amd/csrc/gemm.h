@@ -105,6 +105,14 @@ struct GemmRunner {
   int64_t r_ws_ = 0;
   void begin() { rec_ = true; r_host_.clear(); r_l_.clear(); r_ws_ = 0; }
   void flush(hipStream_t s);
+  // flush() in two parts, for callers that interleave the recorded launches
+  // with other work: upload the tables once, then issue recorded launches
+  // [i0, i1) where they belong (recorded() before and after a run() gives
+  // its range); end_replay() drops the record
+  size_t recorded() const { return r_l_.size(); }
+  void upload_recorded(hipStream_t s);
+  void issue(size_t i0, size_t i1, hipStream_t s);
+  void end_replay() { r_l_.clear(); }
   size_t r_append(const void* p, size_t n) {
     const size_t o = (r_host_.size() + 255) / 256 * 256;
     r_host_.resize(o + n);

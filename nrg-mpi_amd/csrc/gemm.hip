@@ -457,13 +457,23 @@ void GemmRunner::launch(const GemmBatch& b, const double* A, const double* B, do
 }
 
 void GemmRunner::flush(hipStream_t s) {
+  upload_recorded(s);
+  issue(0, r_l_.size(), s);
+  r_l_.clear();
+}
+
+void GemmRunner::upload_recorded(hipStream_t s) {
   rec_ = false;
   if (r_l_.empty()) return;
   if (r_ws_ > 0) d_splitw_.reserve(r_ws_);
   r_dev_.upload(r_host_.data(), r_host_.size(), s);
+}
+
+void GemmRunner::issue(size_t i0, size_t i1, hipStream_t s) {
   const char* base = r_dev_.p;
   const dim3 blk(256);
-  for (const RLaunch& r : r_l_) {
+  for (size_t i = i0; i < i1 && i < r_l_.size(); ++i) {
+    const RLaunch& r = r_l_[i];
     if (r.kind == 1) {
       plaunch(r.family, 0, 0, k_splitk_reduce, dim3(r.gx, (unsigned)r.nt), blk, 0, s,
               reinterpret_cast<const SplitRed*>(base + r.tab), (const double*)d_splitw_.p, r.C);
@@ -482,7 +492,6 @@ void GemmRunner::flush(hipStream_t s) {
       plaunch(r.family, r.bytes, r.flops, k_gemm_f64, g, blk, 0, s, tk, ts, r.nt, r.A, r.B, C, r.gA, r.gB, r.ks);
     }
   }
-  r_l_.clear();
   NRG_HIP(hipGetLastError());
 }
 
