@@ -28,6 +28,9 @@ constexpr int NBB = 128;   // back-transform panel width (Z is streamed once per
 constexpr int MAXN = 16384;
 constexpr int SYMV_WIDE_MIN_WG = 512;   // 32-column SYMV panels only for launches of at least this many
 constexpr double SYMV_WIDE_MIN_BYTES = 64.0e6;   // ... and of at least this many bytes (both per group count)
+#ifndef SYMV_U
+#define SYMV_U 4   // column loads in flight per lane in the SYMV
+#endif
 
 __device__ inline double wave_sum(double v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -451,10 +454,9 @@ __global__ __launch_bounds__(NW * 64) void k_tr_symv(TrArgs a, int t, int jj, in
     if (j >= m) break;
     const double* col = At + (int64_t)j * ld;
     const double vj = sv[j - j0];
-    // U loads in flight per lane
-    double sa[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) sa[u] = 0.0;
+    // U loads in flight per lane, all issued before the first use; two
+    // accumulators keep the kernel within 64 VGPRs (two workgroups per CU)
+    double s0 = 0.0, s1 = 0.0;
     int i = j + lane;
     for (; i + 64 * (U - 1) < m; i += 64 * U) {
       double x[U];
@@ -462,19 +464,29 @@ __global__ __launch_bounds__(NW * 64) void k_tr_symv(TrArgs a, int t, int jj, in
       for (int u = 0; u < U; ++u) x[u] = col[i + 64 * u];
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        sa[u] += x[u] * sv[i + 64 * u - j0];
+        if (u & 1) s1 += x[u] * sv[i + 64 * u - j0];
+        else s0 += x[u] * sv[i + 64 * u - j0];
         if (u > 0 || i > j) atomicAdd(&ax[i + 64 * u - j0], x[u] * vj);
       }
     }
-    for (; i < m; i += 64) {
-      const double x = col[i];
-      sa[0] += x * sv[i - j0];
-      if (i > j) atomicAdd(&ax[i - j0], x * vj);
-    }
-    double st = 0.0;
+    if (i < m) {
+      // the last rows of the column as one guarded batch (one round trip)
+      double x[U];
 #pragma unroll
-    for (int u = 0; u < U; u += 2) st += (u + 1 < U) ? (sa[u] + sa[u + 1]) : sa[u];
-    const double t = wave_sum_full(st);
+      for (int u = 0; u < U; ++u) {
+        x[u] = 0.0;
+        if (i + 64 * u < m) x[u] = col[i + 64 * u];
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        if (i + 64 * u < m) {
+          if (u & 1) s1 += x[u] * sv[i + 64 * u - j0];
+          else s0 += x[u] * sv[i + 64 * u - j0];
+          if (u > 0 || i > j) atomicAdd(&ax[i + 64 * u - j0], x[u] * vj);
+        }
+      }
+    }
+    const double t = wave_sum_full(s0 + s1);
     if (lane == 0) atomicAdd(&ax[j - j0], t);
   }
   __syncthreads();
@@ -1614,9 +1626,9 @@ void BatchedEigensolver::tridiag(hipStream_t s) {
             NRG_HIP(hipStreamWaitEvent(G.s, evf_[h], 0));
           }
       if (pw == 2 * CG)
-        plaunch(KF_TR_SYMV, bytes, flops, k_tr_symv<2, 16, 4>, dim3(g), dim3(1024), shB, G.s, ta, t, jj, act, pm);
+        plaunch(KF_TR_SYMV, bytes, flops, k_tr_symv<2, 16, SYMV_U>, dim3(g), dim3(1024), shB, G.s, ta, t, jj, act, pm);
       else
-        plaunch(KF_TR_SYMV, bytes, flops, k_tr_symv<1, 16, 4>, dim3(g), dim3(1024), shB, G.s, ta, t, jj, act, pm);
+        plaunch(KF_TR_SYMV, bytes, flops, k_tr_symv<1, 16, SYMV_U>, dim3(g), dim3(1024), shB, G.s, ta, t, jj, act, pm);
       if (excl) {
         NRG_HIP(hipEventRecord(evf_[gi], G.s));
         for (int h = 0; h < ng; ++h)
