@@ -48,6 +48,10 @@ void invariant(Context& C, const int32_t* kept, double* out_host) {
   std::vector<int32_t> gidx;
   std::vector<double> gsc;
   GemmBatch batch;
+  // table rows of each type, in table order
+  std::vector<std::vector<int>> qt(C.ncm);
+  for (size_t q = 0; q < C.inv_t.size(); ++q)
+    if (C.inv_t[q] >= 0 && C.inv_t[q] < C.ncm) qt[C.inv_t[q]].push_back((int)q);
   for (int L = 0; L < nsub; ++L) {
     if (kept[L] == 0) continue;
     const int32_t* labL = &C.sub[L * (nv + 3)];
@@ -63,27 +67,31 @@ void invariant(Context& C, const int32_t* kept, double* out_host) {
         if (dq != 1 || std::abs(dsz) != 1) continue;
       }
       for (int t = 0; t < C.ncm; ++t) {
-        const int64_t g0 = (int64_t)gidx.size();
+        // matched row pairs (a_i, b_i) of all table rows of type t
+        int K = 0;
+        for (int q : qt[t])
+          K += (int)std::min(rows[L].by_op[C.inv_a[q]].size(), rows[R].by_op[C.inv_b[q]].size());
+        if (K == 0) continue;
         // A gathers rows b of U_R, B gathers rows a of U_L: two index lists
-        std::vector<int32_t> la, lb;
-        std::vector<double> lc;
-        for (size_t q = 0; q < C.inv_t.size(); ++q) {
-          if (C.inv_t[q] != t) continue;
+        // [b_1..b_K | a_1..a_K], and the coefficient of each pair
+        const int64_t g0 = (int64_t)gidx.size();
+        const int64_t s0 = (int64_t)gsc.size();
+        gidx.resize(g0 + 2 * (int64_t)K);
+        gsc.resize(s0 + K);
+        int32_t* pb = gidx.data() + g0;
+        int32_t* pa = pb + K;
+        double* pc = gsc.data() + s0;
+        for (int q : qt[t]) {
           const auto& ra = rows[L].by_op[C.inv_a[q]];
           const auto& rb = rows[R].by_op[C.inv_b[q]];
           const size_t np = std::min(ra.size(), rb.size());
-          for (size_t v = 0; v < np; ++v) {
-            la.push_back(ra[v]);
-            lb.push_back(rb[v]);
-            lc.push_back(C.inv_c[q]);
-          }
+          std::copy(ra.begin(), ra.begin() + np, pa);
+          std::copy(rb.begin(), rb.begin() + np, pb);
+          std::fill(pc, pc + np, C.inv_c[q]);
+          pa += np;
+          pb += np;
+          pc += np;
         }
-        const int K = (int)la.size();
-        if (K == 0) continue;
-        gidx.insert(gidx.end(), lb.begin(), lb.end());
-        gidx.insert(gidx.end(), la.begin(), la.end());
-        const int64_t s0 = (int64_t)gsc.size();
-        gsc.insert(gsc.end(), lc.begin(), lc.end());
         GemmTask tk = gemm_task(kept[R], kept[L], K, 1, 0, C.boff[R], C.bn[R], C.boff[L], C.bn[L],
                                 st[R] + st[L] * k + (int64_t)t * k * k, (int)k, 1.0, 0.0);
         tk.offGA = g0;

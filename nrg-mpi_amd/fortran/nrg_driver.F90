@@ -102,6 +102,27 @@ contains
     nrg_host_truncate = nkept
   end function nrg_host_truncate
 
+  ! The eigenvalue sort of the truncation on caller values (CPU tests of the
+  ! tie order): perm(n) receives the 1-based source position of each sorted
+  ! element, val is sorted in place.
+  integer(c_int) function nrg_host_sort_values(n, val, perm) bind(C, name="nrg_host_sort_values")
+    integer(c_int), value :: n
+    real(c_double), intent(inout) :: val(*)
+    integer(c_int), intent(out) :: perm(*)
+    double precision, allocatable :: v(:)
+    integer, allocatable :: p(:)
+    integer :: i
+    allocate(v(n), p(n))
+    v = val(1:n)
+    do i = 1, n
+       p(i) = i
+    end do
+    call run_merge_sort_r8(int(n), v, p)
+    val(1:n) = v
+    perm(1:n) = p
+    nrg_host_sort_values = 0
+  end function nrg_host_sort_values
+
   ! The basis sort on caller keys (CPU tests of the exact tie order):
   ! mode 0 the O(N log N) replay, 1 the swap-by-swap exchange sort; perm(n)
   ! receives the 1-based source position of each sorted element.

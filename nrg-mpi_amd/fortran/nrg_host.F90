@@ -448,48 +448,92 @@ contains
     end do
   end subroutine insertion_i8
 
-  recursive subroutine merge_sort_r8(n, val, perm, wv, wp)
+  ! Stable ascending sort of (val, perm) by val that merges the ascending
+  ! runs the input already holds (the eigenvalues arrive as one ascending
+  ! run per block): ceil(log2 r) passes for r runs.  Ties keep their input
+  ! order, so the permutation is that of the reference's insertion sort
+  ! with strict comparisons (module.F90:16-44).
+  subroutine run_merge_sort_r8(n, val, perm)
     integer, intent(in) :: n
     double precision, intent(inout) :: val(n)
     integer, intent(inout) :: perm(n)
-    double precision, intent(inout) :: wv(n)
-    integer, intent(inout) :: wp(n)
-    integer :: h, i, j, k, tp
-    double precision :: tv
-    if (n <= 16) then
-       do i = 2, n
-          tv = val(i); tp = perm(i)
-          j = i - 1
-          do while (j >= 1)
-             if (.not. (tv < val(j))) exit
-             val(j + 1) = val(j); perm(j + 1) = perm(j)
-             j = j - 1
-          end do
-          val(j + 1) = tv; perm(j + 1) = tp
-       end do
-       return
-    end if
-    h = n / 2
-    call merge_sort_r8(h, val(1:h), perm(1:h), wv(1:h), wp(1:h))
-    call merge_sort_r8(n - h, val(h + 1:n), perm(h + 1:n), wv(h + 1:n), wp(h + 1:n))
-    i = 1; j = h + 1; k = 1
-    do while (i <= h .and. j <= n)
-       if (val(j) < val(i)) then
-          wv(k) = val(j); wp(k) = perm(j); j = j + 1
-       else
-          wv(k) = val(i); wp(k) = perm(i); i = i + 1
+    double precision, allocatable :: vb(:)
+    integer, allocatable :: pb(:), rs(:)
+    integer :: nr, i
+    logical :: inbuf
+    if (n < 2) return
+    allocate(rs(n + 1))
+    nr = 1
+    rs(1) = 1
+    do i = 2, n
+       if (val(i) < val(i - 1)) then
+          nr = nr + 1
+          rs(nr) = i
        end if
-       k = k + 1
     end do
-    do while (i <= h)
-       wv(k) = val(i); wp(k) = perm(i); i = i + 1; k = k + 1
+    rs(nr + 1) = n + 1
+    if (nr == 1) return
+    allocate(vb(n), pb(n))
+    inbuf = .false.
+    do while (nr > 1)
+       if (inbuf) then
+          call merge_run_pairs(n, vb, pb, val, perm, nr, rs)
+       else
+          call merge_run_pairs(n, val, perm, vb, pb, nr, rs)
+       end if
+       inbuf = .not. inbuf
     end do
-    do while (j <= n)
-       wv(k) = val(j); wp(k) = perm(j); j = j + 1; k = k + 1
+    if (inbuf) then
+       val = vb
+       perm = pb
+    end if
+  end subroutine run_merge_sort_r8
+
+  ! One pass: runs 2i-1 and 2i of (sv, sp) merged into (dv, dp), the right
+  ! element taken only when strictly smaller; an odd last run is copied.
+  ! rs(1:nr+1) holds the run starts and n + 1; on return the merged ones.
+  subroutine merge_run_pairs(n, sv, sp, dv, dp, nr, rs)
+    integer, intent(in) :: n
+    double precision, intent(in) :: sv(n)
+    integer, intent(in) :: sp(n)
+    double precision, intent(out) :: dv(n)
+    integer, intent(out) :: dp(n)
+    integer, intent(inout) :: nr, rs(:)
+    integer :: r, i, j, k, m, e, nr2
+    nr2 = 0
+    r = 1
+    do while (r <= nr)
+       nr2 = nr2 + 1
+       i = rs(r)
+       rs(nr2) = i
+       if (r == nr) then
+          dv(i:n) = sv(i:n)
+          dp(i:n) = sp(i:n)
+          exit
+       end if
+       m = rs(r + 1)
+       e = rs(r + 2)
+       j = m
+       k = i
+       do while (i < m .and. j < e)
+          if (sv(j) < sv(i)) then
+             dv(k) = sv(j); dp(k) = sp(j); j = j + 1
+          else
+             dv(k) = sv(i); dp(k) = sp(i); i = i + 1
+          end if
+          k = k + 1
+       end do
+       do while (i < m)
+          dv(k) = sv(i); dp(k) = sp(i); i = i + 1; k = k + 1
+       end do
+       do while (j < e)
+          dv(k) = sv(j); dp(k) = sp(j); j = j + 1; k = k + 1
+       end do
+       r = r + 2
     end do
-    val = wv
-    perm = wp
-  end subroutine merge_sort_r8
+    rs(nr2 + 1) = n + 1
+    nr = nr2
+  end subroutine merge_run_pairs
 
   ! =====================================================================
   ! prepareBasis (prepareBasis.F90:41-114)
@@ -625,10 +669,13 @@ contains
     integer :: i, j
     double precision :: be, ex, zn, en, sz2, sz, cn, rs
     integer :: szt
+    double precision, allocatable :: exv(:)
+    allocate(exv(nfull))
     zn = 0; en = 0; sz2 = 0; sz = 0; cn = 0
     do i = 1, nfull
        be = betabar * eigval(i)
        ex = exp(-be)
+       exv(i) = ex
        zn = zn + ex
        en = en + be * ex
        szt = 0
@@ -641,8 +688,7 @@ contains
     end do
     do i = 1, nfull
        be = betabar * eigval(i)
-       ex = exp(-be)
-       cn = cn + (be - en / zn) * (be - en / zn) * ex
+       cn = cn + (be - en / zn) * (be - en / zn) * exv(i)
     end do
     thermo_c = cn / zn
     thermo_s = en / zn + log(zn)
@@ -655,16 +701,16 @@ contains
   subroutine truncate(it)
     integer, intent(in) :: it
     integer, allocatable :: idx(:), wp(:), kidx(:), order(:), tmp(:, :)
-    double precision, allocatable :: es(:), wv(:), e2(:)
+    double precision, allocatable :: es(:), e2(:)
     integer(8), allocatable :: kk(:), wk(:)
     integer :: i, limit, cutoff, nb
     double precision :: ecut
-    allocate(idx(nfull), wp(nfull), es(nfull), wv(nfull))
+    allocate(idx(nfull), wp(nfull), es(nfull))
     do i = 1, nfull
        idx(i) = i
     end do
     es = eigval
-    call merge_sort_r8(nfull, es, idx, wv, wp)
+    call run_merge_sort_r8(nfull, es, idx)
     nb = nfull
     trunc_margin = -1.0d0
     if (it /= i_first .and. it /= n_iter .and. nfull > nkeep) then

@@ -40,6 +40,7 @@ SIGNATURES = {
     "nrg_host_get_rho_traces": (ctypes.c_int, [_f64p, ctypes.c_int]),
     "nrg_host_get_spectrum_times": (ctypes.c_int, [_f64p]),
     "nrg_host_sort_keys": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_int64), _i32p, ctypes.c_int]),
+    "nrg_host_sort_values": (ctypes.c_int, [ctypes.c_int, _f64p, _i32p]),
 }
 
 
@@ -51,6 +52,15 @@ def sort_keys(keys, mode=0):
     load().nrg_host_sort_keys(len(k), k.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), p.ctypes.data_as(_i32p),
                               mode)
     return k, p.astype(np.int64) - 1
+
+
+def sort_values(values):
+    """The truncation's eigenvalue sort on float64 values: (sorted values,
+    0-based source positions), ties in input order."""
+    v = np.ascontiguousarray(values, dtype=np.float64).copy()
+    p = np.zeros(len(v), dtype=np.int32)
+    load().nrg_host_sort_values(len(v), v.ctypes.data_as(_f64p), p.ctypes.data_as(_i32p))
+    return v, p.astype(np.int64) - 1
 
 
 class HostError(RuntimeError):

@@ -124,3 +124,25 @@ def test_lazy_sort_replays_exchange_sort(seed):
     np.testing.assert_array_equal(k0, np.sort(keys))
     np.testing.assert_array_equal(p0, p1)
     np.testing.assert_array_equal(p0, O.exchange_sort_permutation(keys))
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_eigenvalue_sort_is_stable(seed):
+    """The truncation's eigenvalue sort merges the ascending runs of its input
+    (one per block); its permutation must be the stable one (ties in input
+    order, as the reference's insertion sort with strict comparisons), which
+    numpy's stable argsort fixes.  Inputs: ascending runs of random length
+    with many exact ties, plus single-element, descending and all-equal cases."""
+    from nrgamd import host
+    rng = np.random.default_rng(seed)
+    runs = []
+    for _ in range(int(rng.integers(1, 40))):
+        r = np.sort(rng.integers(0, 50, int(rng.integers(1, 60))).astype(np.float64) * 0.25)
+        runs.append(r)
+    cases = [np.concatenate(runs), np.array([3.0]), np.arange(30.0)[::-1].copy(), np.full(17, 2.5),
+             rng.integers(0, 5, 200).astype(np.float64)]
+    for v in cases:
+        vs, p = host.sort_values(v)
+        ref = np.argsort(v, kind="stable")
+        np.testing.assert_array_equal(p, ref)
+        np.testing.assert_array_equal(vs, v[ref])
