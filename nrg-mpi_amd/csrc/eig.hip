@@ -1019,12 +1019,15 @@ __global__ __launch_bounds__(64 * SEC_ROOTS) void k_dc_secular(DcArgs a, const i
   }
 }
 
-// z-hat (Gu & Eisenstat): one lane per non-deflated pole.
-__global__ __launch_bounds__(64) void k_dc_zhat(DcArgs a, const int32_t* tasks, int ntask) {
+// z-hat of the non-deflated roots (Gu-Eisenstat): one wave per root; the
+// lanes split the k-1 ratio factors of the product and combine their
+// partial products by a butterfly, so the serial chain is k/64 factors.
+__global__ __launch_bounds__(256) void k_dc_zhat(DcArgs a, const int32_t* tasks, int ntask) {
   const int t = blockIdx.y;
-  const int j = blockIdx.x * 64 + threadIdx.x;
+  const int j = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
   const int k = a.kout[t];
-  if (j >= k) return;
+  if (j >= k) return;   // uniform per wave
   const int32_t* tk = tasks + 8 * t;
   const int m = tk[3] - tk[1];
   double* W = a.ws + join64(tk[4], tk[5]);
@@ -1033,11 +1036,16 @@ __global__ __launch_bounds__(64) void k_dc_zhat(DcArgs a, const int32_t* tasks, 
   double* zh = W + 7 * m;   // rc slot is free after the rotations
   const double rho = W[5 * m];
   const double dj = dl[j];
-  double p = (dl[org[k - 1]] - dj) + tr[k - 1];
-  for (int i = 0; i < j; ++i) p *= ((dl[org[i]] - dj) + tr[i]) / (dl[i] - dj);
-  for (int i = j; i < k - 1; ++i) p *= ((dl[org[i]] - dj) + tr[i]) / (dl[i + 1] - dj);
+  // factor i < k-1: (lambda_i - d_j) / (d_i - d_j) for i < j, over d_{i+1} - d_j for i >= j
+  double p = 1.0;
+  for (int i = lane; i < k - 1; i += 64) {
+    const double den = (i < j) ? (dl[i] - dj) : (dl[i + 1] - dj);
+    p *= ((dl[org[i]] - dj) + tr[i]) / den;
+  }
+  for (int o = 32; o > 0; o >>= 1) p *= __shfl_xor(p, o, 64);
+  p *= (dl[org[k - 1]] - dj) + tr[k - 1];
   p /= rho;
-  zh[j] = copysign(sqrt(fabs(p)), zl[j]);
+  if (lane == 0) zh[j] = copysign(sqrt(fabs(p)), zl[j]);
 }
 
 // eigenvectors of the rank-one problem: one wave per column.
@@ -1064,9 +1072,11 @@ __global__ __launch_bounds__(64) void k_dc_vectors(DcArgs a, const int32_t* task
   for (int j = threadIdx.x; j < k; j += 64) U[j + (int64_t)i * k] *= inv;
 }
 
+constexpr int FIN_T = 1024;   // threads of the final-ordering kernel (its bitonic sort)
+
 // final ordering of one merge: lambda (ascending) merged with the sorted
 // deflated values; columns from Qt (non-deflated) and Qcd (deflated).
-__global__ __launch_bounds__(256) void k_dc_finish(DcArgs a, const int32_t* tasks, int ntask) {
+__global__ __launch_bounds__(FIN_T) void k_dc_finish(DcArgs a, const int32_t* tasks, int ntask) {
   const int t = blockIdx.x;
   const int32_t* tk = tasks + 8 * t;
   const int b = tk[0], s = tk[1], e = tk[3];
@@ -1094,14 +1104,14 @@ __global__ __launch_bounds__(256) void k_dc_finish(DcArgs a, const int32_t* task
     while (P < md) P <<= 1;
     double* kv = fin_shm;
     int32_t* ki = reinterpret_cast<int32_t*>(fin_shm + P);
-    for (int i = tid; i < P; i += 256) {
+    for (int i = tid; i < P; i += FIN_T) {
       kv[i] = (i < md) ? dv[i] : __builtin_inf();
       ki[i] = (i < md) ? i : INT_MAX;
     }
     __syncthreads();
     for (int len = 2; len <= P; len <<= 1) {
       for (int st = len >> 1; st > 0; st >>= 1) {
-        for (int x = tid; x < P / 2; x += 256) {
+        for (int x = tid; x < P / 2; x += FIN_T) {
           // pair (i, i + st) with i having bit st clear; direction by bit len
           const int i = ((x & ~(st - 1)) << 1) | (x & (st - 1));
           const int j = i + st;
@@ -1117,12 +1127,12 @@ __global__ __launch_bounds__(256) void k_dc_finish(DcArgs a, const int32_t* task
         __syncthreads();
       }
     }
-    for (int r = tid; r < md; r += 256) {
+    for (int r = tid; r < md; r += FIN_T) {
       dord[r] = ki[r];
       sdv[r] = kv[r];
     }
   } else {
-    for (int i = tid; i < md; i += 256) {
+    for (int i = tid; i < md; i += FIN_T) {
       const double v = dv[i];
       int r = 0;
       for (int j = 0; j < md; ++j) {
@@ -1135,7 +1145,7 @@ __global__ __launch_bounds__(256) void k_dc_finish(DcArgs a, const int32_t* task
   }
   __syncthreads();
   // eigenvalues: lambda_i then deflated, merged by rank
-  for (int i = tid; i < m; i += 256) {
+  for (int i = tid; i < m; i += FIN_T) {
     double v;
     int fin, src;
     if (i < k) {
@@ -1764,7 +1774,7 @@ void BatchedEigensolver::divide_conquer(hipStream_t s) {
       const int32_t* tk = d_tasks_.p;
       plaunch(KF_DC_SECULAR, 0, 0, k_dc_secular, dim3(ceil_div(kmax, SEC_ROOTS), nt), dim3(64 * SEC_ROOTS),
               (uint32_t)(2 * sizeof(double) * (size_t)std::min(kmax, SEC_LDS_M)), s, da, tk, nt);
-      plaunch(KF_DC_VECTORS, 0, 0, k_dc_zhat, dim3(ceil_div(kmax, 64), nt), dim3(64), 0, s, da, tk, nt);
+      plaunch(KF_DC_VECTORS, 0, 0, k_dc_zhat, dim3(ceil_div(kmax, 4), nt), dim3(256), 0, s, da, tk, nt);
       plaunch(KF_DC_VECTORS, 0, 0, k_dc_vectors, dim3(kmax, nt), dim3(64), 0, s, da, tk, nt);
       batch_.clear();
       for (int i = 0; i < nreg[lev - 1]; ++i) {
@@ -1783,7 +1793,7 @@ void BatchedEigensolver::divide_conquer(hipStream_t s) {
     // LDS: (value, index) pairs of the deflated values, padded to a power of two
     int P = 1;
     while (P < std::min(mmax, DC_LDS_M)) P <<= 1;
-    plaunch(KF_DC_FINISH, 0, 0, k_dc_finish, dim3(nt), dim3(256),
+    plaunch(KF_DC_FINISH, 0, 0, k_dc_finish, dim3(nt), dim3(FIN_T),
             (uint32_t)((sizeof(double) + sizeof(int32_t)) * (size_t)P), s, da, tk, nt);
     if (nreg[lev - 1] > 0)
       plaunch(KF_DC_FINISH, 0, 0, k_dc_scatter, dim3(ceil_div(mmax, 64), ceil_div(mmax, DC_CCH), nreg[lev - 1]),
