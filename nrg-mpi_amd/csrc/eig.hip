@@ -981,7 +981,10 @@ __global__ __launch_bounds__(64 * SEC_ROOTS) void k_dc_secular(DcArgs a, const i
     double psi, dpsi, phi, dphi;
     sec_sums(dl, zl, k, i, dorg, tau, rho, psi, dpsi, phi, dphi);
     const double g = 1.0 + psi + phi;
-    if (g == 0.0) break;
+    // converged once g is at the level of its own rounding error (the test
+    // of LAPACK dlaed4, in this scaling: psi <= 0 <= phi, rho folded in)
+    const double errb = 8.0 * (phi - psi) + 2.0 + 3.0 * fabs(tau) * (dpsi + dphi);
+    if (fabs(g) <= eps * errb) break;
     if (g > 0.0) hi = tau; else lo = tau;
     // model: C + B1/(p-x) + B2/(q-x) = 0, u = p - x
     const double B1 = dpsi * (p - tau) * (p - tau);
