@@ -997,18 +997,39 @@ __global__ __launch_bounds__(64 * SEC_ROOTS) void k_dc_secular(DcArgs a, const i
       const double B2 = dphi * (q - tau) * (q - tau);
       C += phi - B2 / (q - tau);
       const double G = q - p;
-      const double qa = C, qb = C * G + B1 + B2, qc = B1 * G;
-      double u;
-      if (qa == 0.0) {
-        u = -qc / qb;
+      // solved for the distance to the origin pole (p = 0 when o = i, q = 0
+      // when o = i + 1), so that the new point carries no cancellation
+      if (o == i) {
+        // u = p - x: C u^2 + (C G + B1 + B2) u + B1 G = 0, u in (-G, 0)
+        const double qa = C, qb = C * G + B1 + B2, qc = B1 * G;
+        double u;
+        if (qa == 0.0) {
+          u = -qc / qb;
+        } else {
+          const double disc = fmax(qb * qb - 4.0 * qa * qc, 0.0);
+          const double qq = -0.5 * (qb + copysign(sqrt(disc), qb));
+          const double u1 = qq / qa, u2 = (qq != 0.0) ? qc / qq : u1;
+          u = (u1 < 0.0 && u1 > -G) ? u1 : u2;
+        }
+        xnew = p - u;
       } else {
-        const double disc = fmax(qb * qb - 4.0 * qa * qc, 0.0);
-        const double qq = -0.5 * (qb + copysign(sqrt(disc), qb));
-        const double u1 = qq / qa, u2 = (qq != 0.0) ? qc / qq : u1;
-        u = (u1 < 0.0 && u1 > -G) ? u1 : u2;
+        // w = q - x: C w^2 + (B1 + B2 - C G) w - B2 G = 0, w in (0, G)
+        const double qa = C, qb = B1 + B2 - C * G, qc = -B2 * G;
+        double w;
+        if (qa == 0.0) {
+          w = -qc / qb;
+        } else {
+          const double disc = fmax(qb * qb - 4.0 * qa * qc, 0.0);
+          const double qq = -0.5 * (qb + copysign(sqrt(disc), qb));
+          const double w1 = qq / qa, w2 = (qq != 0.0) ? qc / qq : w1;
+          w = (w1 > 0.0 && w1 < G) ? w1 : w2;
+        }
+        xnew = q - w;
       }
-      xnew = p - u;
     }
+    // a step below the rounding level of tau: converged (tested before the
+    // bracket, which a converged step may touch)
+    if (fabs(xnew - tau) <= 2.0 * eps * fabs(tau)) break;
     if (!(xnew > lo && xnew < hi)) xnew = lo + 0.5 * (hi - lo);
     if (xnew <= lo || xnew >= hi) break;          // bracket exhausted
     const double step = fabs(xnew - tau);
