@@ -95,8 +95,9 @@ __device__ inline double wave_sum_full(double v) {
 // A(i,j) v_j into an LDS vector with LDS atomics; that vector is written as
 // py[p][i] (rows i >= PW p) and y_i = sum_{p <= i/PW} py[p][i] is completed
 // in the next column kernel (16 waves share the sum, fixed order), as is
-// y.v from one partial per SYMV workgroup.  Every reduction has a fixed
-// order: results are reproducible bit for bit.
+// y.v from one partial per SYMV workgroup.  The reductions across
+// workgroups have a fixed order; the LDS atomics of one workgroup's waves do
+// not, so results agree from run to run only to rounding.
 // Slot q of the size order (blocks by decreasing n): everything a
 // workgroup needs to address its block, in one 64-byte record so that the
 // workgroup-to-block map and the block's pointers cost one load.
