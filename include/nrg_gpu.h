@@ -53,6 +53,17 @@ int nrg_gpu_set_model(int nv, int nops, int nchain, int ncm, int ncoef,
                       const double* coef_inv, const int32_t* conservation_block,
                       int nmat, const double* coef_spec, const int32_t* cons_diff_spec);
 
+/* Superconducting (BCS) chain of a flag_BCS model (read_initial_basis.F90:
+ * 185-201; used at diagonalization.F90:215-245): coefficient_BCS(nbcs,3)
+ * rows (op_i, op_j, channel) and the raw chain_BCS(0:nrow-1, 1:nchain),
+ * column-major (nrow = numberOfIteration + 2).  During the H assembly of
+ * iteration N, a product-state pair i < j of one block whose operations
+ * match a row in this order and whose references are equal gets
+ * H(i,j) = H(j,i) = chain_BCS(N+1, channel) of the last matching row, in
+ * place of the hopping term.  nbcs = 0 removes the BCS terms (the default
+ * after nrg_gpu_set_model). */
+int nrg_gpu_set_bcs(int nbcs, const int32_t* coef_bcs, int nrow, const double* chain_bcs);
+
 /* Initial invariant matrices of the impurity basis:
  * invariant_matrix(nold,nold,ncm) (read_initial_basis.F90:163-173) and
  * invariant_matrix_spectrum(nold,nold,nmat) (read_initial_basis.F90:271-282). */

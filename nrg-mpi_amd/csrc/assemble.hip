@@ -5,9 +5,13 @@
 //   diagonal:     E_old(ref_r) sqrt(Lambda) + sum_ch operation(op_r, nv+ch) eps_ch
 //   off-diagonal: with p = min(r,c), q = max(r,c), the (op_p, op_q) entry of the
 //                 hopping lookup (built on the host by replaying the
-//                 coefficient search, last match wins, reversed match swaps the
-//                 references) gives (type, channel, coef, swap) and
-//                 H = t_channel * coef * invariant_matrix(jref, iref, type).
+//                 coefficient search: the last match sets type, channel and
+//                 coef, any reversed match swaps the references) gives
+//                 (type, channel, coef, swap) and
+//                 H = t_channel * coef * invariant_matrix(jref, iref, type);
+//                 with a BCS chain, a forward BCS row of (op_p, op_q) and
+//                 ref_p == ref_q override it with chain_BCS(it+1, channel)
+//                 (diagonalization.F90:215-245).
 // 64x64 tiles of all blocks are flattened into one grid.
 #include "assemble.h"
 
@@ -25,11 +29,13 @@ struct AsmArgs {
   const double* inv;          // (kold, kold, ncm) column-major
   int64_t kold;
   const int32_t* op;          // [nops][nv+nchain]
-  const int32_t* hop;         // [4][nops*nops]: valid, type, chan, swap
+  const int32_t* hop;         // [5][nops*nops]: valid, type, chan, swap, BCS channel + 1
   const double* hop_coef;
   int nops, nchain;
   double cd[4];               // chain_diagonal(it+1, ch)
   double cnd[4];              // chain_nondiagonal(it, ch)
+  double cbcs[4];             // chain_BCS(it+1, ch)
+  int bcs;                    // BCS rows present
   double sqrtl;
   double* H;
 };
@@ -74,12 +80,17 @@ __global__ __launch_bounds__(256) void k_assemble(AsmArgs a) {
         if (a.hop[3 * nn + idx]) { iref = ref_q; jref = ref_p; }
         v = a.cnd[ch] * a.hop_coef[idx] * a.inv[(int64_t)jref + (int64_t)iref * a.kold + (int64_t)t * a.kold * a.kold];
       }
+      if (a.bcs) {
+        const int cb = a.hop[4 * nn + idx];
+        if (cb > 0 && ref_p == ref_q) v = a.cbcs[cb - 1];
+      }
     }
     H[(int64_t)r + (int64_t)c * n] = v;
   }
 }
 
-void assemble(Context& C, const double* chain_diag, const double* chain_nondiag, double sqrtl) {
+void assemble(Context& C, const double* chain_diag, const double* chain_nondiag, double sqrtl,
+              const double* chain_bcs) {
   const int nblk = C.nsub;
   std::vector<int32_t> ts(nblk);
   long long tot = 0;
@@ -102,7 +113,9 @@ void assemble(Context& C, const double* chain_diag, const double* chain_nondiag,
   for (int ch = 0; ch < 4; ++ch) {
     a.cd[ch] = ch < C.nchain ? chain_diag[ch] : 0.0;
     a.cnd[ch] = ch < C.nchain ? chain_nondiag[ch] : 0.0;
+    a.cbcs[ch] = (chain_bcs && ch < C.nchain) ? chain_bcs[ch] : 0.0;
   }
+  a.bcs = chain_bcs ? 1 : 0;
   a.sqrtl = sqrtl;
   a.H = C.d_H.p;
   double bytes = 0;

@@ -128,6 +128,20 @@ int nrg_gpu_finalize(void) {
 
 const char* nrg_gpu_last_error(void) { return ctx().last_error.c_str(); }
 
+// device copy of the pair tables read by the H assembly:
+// [valid, type, channel, swap, BCS channel + 1] x nops^2 and the coefficients
+static void upload_hop_table(Context& C) {
+  const int nn = C.nops * C.nops;
+  std::vector<int32_t> hop(5 * nn);
+  for (int i = 0; i < nn; ++i) {
+    hop[i] = C.hop_valid[i]; hop[nn + i] = C.hop_type[i]; hop[2 * nn + i] = C.hop_chan[i];
+    hop[3 * nn + i] = C.hop_swap[i]; hop[4 * nn + i] = C.bcs_chan[i];
+  }
+  C.d_hop.upload(hop, C.stream);
+  C.d_hop_coef.upload(C.hop_coef, C.stream);
+  NRG_HIP(hipStreamSynchronize(C.stream));
+}
+
 int nrg_gpu_set_model(int nv, int nops, int nchain, int ncm, int ncoef, const int32_t* operation,
                       const int32_t* coef_diag_type, const double* coef_diag,
                       const int32_t* coef_inv_type, const double* coef_inv,
@@ -143,8 +157,9 @@ int nrg_gpu_set_model(int nv, int nops, int nchain, int ncm, int ncoef, const in
     for (int o = 0; o < nops; ++o)
       for (int j = 0; j < nc; ++j) C.operation[o * nc + j] = operation[o + j * nops];
     // hopping lookup: replay every coefficient row in order for each
-    // (op_i, op_j); the last match wins and a reversed match swaps refs
-    // (diagonalization.F90:195-213)
+    // (op_i, op_j); the last match sets (type, channel, coef), and a
+    // reversed match swaps the references for good: a later forward match
+    // does not restore them (diagonalization.F90:195-213)
     const int nn = nops * nops;
     C.hop_valid.assign(nn, 0); C.hop_type.assign(nn, 0); C.hop_chan.assign(nn, 0);
     C.hop_swap.assign(nn, 0); C.hop_coef.assign(nn, 0.0);
@@ -159,7 +174,7 @@ int nrg_gpu_set_model(int nv, int nops, int nchain, int ncm, int ncoef, const in
           else if (oi == b && oj == a) sw = 1;
           else continue;
           NRG_REQUIRE(t >= 0 && t < ncm && ch >= 0 && ch < nchain, "set_model: bad hopping row");
-          C.hop_valid[idx] = 1; C.hop_type[idx] = t; C.hop_chan[idx] = ch; C.hop_swap[idx] = sw;
+          C.hop_valid[idx] = 1; C.hop_type[idx] = t; C.hop_chan[idx] = ch; C.hop_swap[idx] |= sw;
           C.hop_coef[idx] = coef_diag[ic];
         }
       }
@@ -187,13 +202,31 @@ int nrg_gpu_set_model(int nv, int nops, int nchain, int ncm, int ncoef, const in
         for (int i = 0; i < nv; ++i) C.cons_diff_spec[im * nv + i] = cons_diff_spec[im + i * nmat];
     }
     C.d_op.upload(C.operation, C.stream);
-    std::vector<int32_t> hop(4 * nn);
-    for (int i = 0; i < nn; ++i) {
-      hop[i] = C.hop_valid[i]; hop[nn + i] = C.hop_type[i]; hop[2 * nn + i] = C.hop_chan[i]; hop[3 * nn + i] = C.hop_swap[i];
+    C.bcs_chan.assign(nn, 0);
+    C.chain_bcs.clear();
+    C.bcs_rows = 0;
+    upload_hop_table(C);
+  });
+}
+
+int nrg_gpu_set_bcs(int nbcs, const int32_t* coef_bcs, int nrow, const double* chain_bcs) {
+  return guarded([&](Context& C) {
+    need_init(C);
+    NRG_REQUIRE(C.nops > 0, "set_bcs: call nrg_gpu_set_model first");
+    NRG_REQUIRE(nbcs >= 0 && (nbcs == 0 || (coef_bcs && chain_bcs && nrow >= 1)), "set_bcs: bad sizes");
+    const int nn = C.nops * C.nops;
+    C.bcs_chan.assign(nn, 0);
+    // a row matches in its forward order only, with equal references; the
+    // last match sets the channel (diagonalization.F90:215-223)
+    for (int ic = 0; ic < nbcs; ++ic) {
+      const int a = coef_bcs[ic] - 1, b = coef_bcs[ic + nbcs] - 1, ch = coef_bcs[ic + 2 * nbcs] - 1;
+      NRG_REQUIRE(a >= 0 && a < C.nops && b >= 0 && b < C.nops && ch >= 0 && ch < C.nchain,
+                  "set_bcs: bad coefficient_BCS row");
+      C.bcs_chan[a * C.nops + b] = ch + 1;
     }
-    C.d_hop.upload(hop, C.stream);
-    C.d_hop_coef.upload(C.hop_coef, C.stream);
-    NRG_HIP(hipStreamSynchronize(C.stream));
+    C.bcs_rows = nbcs ? nrow : 0;
+    C.chain_bcs.assign(chain_bcs, chain_bcs + (nbcs ? (size_t)nrow * C.nchain : 0));
+    upload_hop_table(C);
   });
 }
 
@@ -245,7 +278,12 @@ int nrg_gpu_diagonalize(int iteration, int nsub, const int32_t* subspace_info, i
     C.d_U.reserve(n2);
     C.d_w.reserve(nfull);
     double t0 = wall_ms();
-    assemble(C, chain_diag, chain_nondiag, sqrt_lambda);
+    double cbcs[4] = {0, 0, 0, 0};   // chain_BCS(iteration+1, ch) (diagonalization.F90:242)
+    if (C.bcs_rows) {
+      NRG_REQUIRE(iteration + 1 >= 0 && iteration + 1 < C.bcs_rows, "diagonalize: iteration outside chain_BCS");
+      for (int ch = 0; ch < C.nchain && ch < 4; ++ch) cbcs[ch] = C.chain_bcs[(size_t)ch * C.bcs_rows + iteration + 1];
+    }
+    assemble(C, chain_diag, chain_nondiag, sqrt_lambda, C.bcs_rows ? cbcs : nullptr);
     NRG_HIP(hipStreamSynchronize(C.stream));
     double t1 = wall_ms();
     C.timers.assemble = t1 - t0;

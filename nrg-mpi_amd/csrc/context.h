@@ -41,8 +41,13 @@ struct Context {
   std::vector<int32_t> cons;            // chargeStart, chargeEnd, spinStart, spinEnd
   std::vector<int32_t> cons_diff_spec;  // [nmat][nv]
   std::vector<int32_t> pos_op, neg_op;  // [nmat][nv] label shifts of the spectral operators
-  DevBuf<int32_t> d_op, d_hop;          // d_hop: [valid,type,chan,swap] x nops^2
+  DevBuf<int32_t> d_op, d_hop;          // d_hop: [valid,type,chan,swap,bcs] x nops^2
   DevBuf<double> d_hop_coef;
+  // BCS pairing: channel + 1 of the last forward match of (op_i, op_j) in
+  // coefficient_BCS (0: none), and chain_BCS(row, ch) column-major, rows 0..
+  std::vector<int32_t> bcs_chan;        // [nops*nops]
+  std::vector<double> chain_bcs;
+  int bcs_rows = 0;
 
   // ---- current iteration (device resident)
   int nsub = 0, nfull = 0;

@@ -359,11 +359,14 @@ __global__ __launch_bounds__(1024) void k_tr_col(TrArgs a, int t, int jp, int jj
 }
 
 // Reflector + lower-triangle SYMV kernel, one workgroup (16 waves) per
-// PW = 16 CPW columns of the trailing block of one block.
-template <int CPW, int NW, int U>
+// PW = 16 CPW columns of the trailing block of one block.  VL = 1 stages v
+// of rows [j0, m) in LDS next to the row parts (2 doubles of LDS per row);
+// VL = 0 reads v from the updated column in global memory (1 double per
+// row), for trailing blocks whose two LDS vectors would not fit.
+template <int CPW, int NW, int U, int VL>
 __global__ __launch_bounds__(NW * 64) void k_tr_symv(TrArgs a, int t, int jj, int act, int panel_major) {
   constexpr int PW = CPW * NW;
-  extern __shared__ double ax[];   // row parts, rows [j0, m), then v of those rows
+  extern __shared__ double ax[];   // row parts, rows [j0, m), then (VL) v of those rows
   int qs, pan;
   auto tiles = [&](int q) {
     const int cq = t - a.slot[q].coff, mq = a.slot[q].n - cq - 1;
@@ -436,11 +439,13 @@ __global__ __launch_bounds__(NW * 64) void k_tr_symv(TrArgs a, int t, int jj, in
     }
     s_u[wv][lane] = s;
   }
-  double* sv = ax + (m - j0);   // v_i of rows [j0, m), staged once per workgroup
+  double* sv = ax + (m - j0);   // VL: v_i of rows [j0, m), staged once per workgroup
   for (int i = j0 + tid; i < m; i += NW * 64) {
     ax[i - j0] = 0.0;
-    sv[i - j0] = vv(i);
+    if (VL) sv[i - j0] = vv(i);
   }
+  // v_i for a row i in [j0, m)
+  auto vr = [&](int i) { return VL ? sv[i - j0] : vv(i); };
   __syncthreads();
   if (pan == 0 && tid < 2 * jj) {
     const int col = (tid < jj) ? tid : NB + (tid - jj);
@@ -454,7 +459,7 @@ __global__ __launch_bounds__(NW * 64) void k_tr_symv(TrArgs a, int t, int jj, in
     const int j = j0 + wv * CPW + q;
     if (j >= m) break;
     const double* col = At + (int64_t)j * ld;
-    const double vj = sv[j - j0];
+    const double vj = vr(j);
     // U loads in flight per lane, all issued before the first use; two
     // accumulators keep the kernel within 64 VGPRs (two workgroups per CU)
     double s0 = 0.0, s1 = 0.0;
@@ -465,8 +470,8 @@ __global__ __launch_bounds__(NW * 64) void k_tr_symv(TrArgs a, int t, int jj, in
       for (int u = 0; u < U; ++u) x[u] = col[i + 64 * u];
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        if (u & 1) s1 += x[u] * sv[i + 64 * u - j0];
-        else s0 += x[u] * sv[i + 64 * u - j0];
+        if (u & 1) s1 += x[u] * vr(i + 64 * u);
+        else s0 += x[u] * vr(i + 64 * u);
         if (u > 0 || i > j) atomicAdd(&ax[i + 64 * u - j0], x[u] * vj);
       }
     }
@@ -481,8 +486,8 @@ __global__ __launch_bounds__(NW * 64) void k_tr_symv(TrArgs a, int t, int jj, in
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         if (i + 64 * u < m) {
-          if (u & 1) s1 += x[u] * sv[i + 64 * u - j0];
-          else s0 += x[u] * sv[i + 64 * u - j0];
+          if (u & 1) s1 += x[u] * vr(i + 64 * u);
+          else s0 += x[u] * vr(i + 64 * u);
           if (u > 0 || i > j) atomicAdd(&ax[i + 64 * u - j0], x[u] * vj);
         }
       }
@@ -496,12 +501,12 @@ __global__ __launch_bounds__(NW * 64) void k_tr_symv(TrArgs a, int t, int jj, in
   for (int i = j0 + tid; i < m; i += NW * 64) {
     const double t = ax[i - j0];
     py[i] = t;
-    yv += t * sv[i - j0];
+    yv += t * vr(i);
   }
   if (tid < PW && j0 + tid < m) {
     const int i = j0 + tid;
     const int r = c + 1 + i;
-    const double vi = sv[i - j0];
+    const double vi = vr(i);
     P[r + jj * ld] = vi;
     P[r + (3 * NB + jj) * ld] = vi;
   }
@@ -1456,6 +1461,26 @@ void BatchedEigensolver::solve(int nblk, const int* n, const int64_t* aoff, cons
   }
 }
 
+// Whether a SYMV launch over trailing blocks of up to m rows fits the
+// 160 KB LDS of a gfx950 workgroup with `per_row` doubles of dynamic LDS per
+// row (2: row parts and v, 1: row parts only), counting the kernel's static
+// LDS as the runtime reports it.
+static size_t symv_static_lds() {
+  static size_t st = 0;
+  static bool done = false;
+  if (!done) {
+    hipFuncAttributes fa{}, fb{};
+    NRG_HIP(hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(k_tr_symv<2, 16, SYMV_U, 1>)));
+    NRG_HIP(hipFuncGetAttributes(&fb, reinterpret_cast<const void*>(k_tr_symv<1, 16, SYMV_U, 0>)));
+    st = std::max(fa.sharedSizeBytes, fb.sharedSizeBytes);
+    done = true;
+  }
+  return st;
+}
+static bool symv_lds_fits(int m, int per_row) {
+  return (size_t)per_row * (size_t)std::max(m, 1) * sizeof(double) + symv_static_lds() <= 160 * 1024;
+}
+
 void BatchedEigensolver::tridiag(hipStream_t s) {
   const int nblk = nblk_;
   const int nmax = n_[order_[0]];
@@ -1479,8 +1504,8 @@ void BatchedEigensolver::tridiag(hipStream_t s) {
   // every slot sees the same panel phase t mod NB.
   std::vector<int32_t> coff(nblk);
   for (int q = 0; q < nblk; ++q) coff[q] = align_end_ ? ((nmax - n_[order_[q]]) / NB) * NB : 0;
-  const size_t shm = (size_t)std::max(nmax, 1) * sizeof(double);
-  NRG_REQUIRE(2 * shm <= 160 * 1024, "eigensolver: block too large for the SYMV stage (n > 10240)");
+  NRG_REQUIRE(symv_lds_fits(nmax - 1, 1),
+              "eigensolver: block too large for the SYMV stage (its row-part vector exceeds the LDS)");
   int T = 0;
   for (int q = 0; q < nblk; ++q) T = std::max(T, coff[q] + n_[order_[q]]);
 
@@ -1643,8 +1668,9 @@ void BatchedEigensolver::tridiag(hipStream_t s) {
       if (m > 0) tmax = std::max(tmax, ceil_div(m, pw));
     }
     if (gB > 0) {
-      // LDS: row parts and v of rows [j0, m) of the longest trailing block
-      const uint32_t shB = (uint32_t)(2 * mB * sizeof(double));
+      // LDS: row parts and (VL) v of rows [j0, m) of the longest trailing block
+      const bool vl = symv_lds_fits(mB, 2);
+      const uint32_t shB = (uint32_t)((vl ? 2 : 1) * mB * sizeof(double));
       // panel-major order when the panel counts are nearly uniform
       const int pm = (align_end_ && (int64_t)act * tmax <= gB + gB / 8) ? 1 : 0;
       const int g = pm ? act * tmax : gB;
@@ -1660,10 +1686,14 @@ void BatchedEigensolver::tridiag(hipStream_t s) {
             NRG_HIP(hipEventRecord(evf_[h], grp[h].s));
             NRG_HIP(hipStreamWaitEvent(G.s, evf_[h], 0));
           }
-      if (pw == 2 * CG)
-        plaunch(KF_TR_SYMV, bytes, flops, k_tr_symv<2, 16, SYMV_U>, dim3(g), dim3(1024), shB, G.s, ta, t, jj, act, pm);
+      if (pw == 2 * CG && vl)
+        plaunch(KF_TR_SYMV, bytes, flops, k_tr_symv<2, 16, SYMV_U, 1>, dim3(g), dim3(1024), shB, G.s, ta, t, jj, act, pm);
+      else if (vl)
+        plaunch(KF_TR_SYMV, bytes, flops, k_tr_symv<1, 16, SYMV_U, 1>, dim3(g), dim3(1024), shB, G.s, ta, t, jj, act, pm);
+      else if (pw == 2 * CG)
+        plaunch(KF_TR_SYMV, bytes, flops, k_tr_symv<2, 16, SYMV_U, 0>, dim3(g), dim3(1024), shB, G.s, ta, t, jj, act, pm);
       else
-        plaunch(KF_TR_SYMV, bytes, flops, k_tr_symv<1, 16, SYMV_U>, dim3(g), dim3(1024), shB, G.s, ta, t, jj, act, pm);
+        plaunch(KF_TR_SYMV, bytes, flops, k_tr_symv<1, 16, SYMV_U, 0>, dim3(g), dim3(1024), shB, G.s, ta, t, jj, act, pm);
       if (excl) {
         NRG_HIP(hipEventRecord(evf_[gi], G.s));
         for (int h = 0; h < ng; ++h)

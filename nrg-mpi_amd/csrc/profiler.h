@@ -135,6 +135,9 @@ inline void plaunch(int fam, double bytes, double flops, F kernel, const dim3& g
   } else {
     hipLaunchKernelGGL(kernel, grid, block, shm, s, args...);
   }
+  // a launch the runtime rejects (resources, configuration) is reported at
+  // its own call site
+  NRG_HIP(hipGetLastError());
 }
 
 }  // namespace nrg

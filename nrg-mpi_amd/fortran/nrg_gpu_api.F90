@@ -22,6 +22,12 @@ module nrg_gpu_api
        real(c_double), intent(in) :: coef_diag(*), coef_inv(*)
        type(c_ptr), value :: coef_spec, cons_diff_spec
      end function
+     integer(c_int) function nrg_gpu_set_bcs(nbcs, coef_bcs, nrow, chain_bcs) bind(C, name="nrg_gpu_set_bcs")
+       import :: c_int, c_double
+       integer(c_int), value :: nbcs, nrow
+       integer(c_int), intent(in) :: coef_bcs(*)
+       real(c_double), intent(in) :: chain_bcs(*)
+     end function
      integer(c_int) function nrg_gpu_set_invariant(nold, inv) bind(C, name="nrg_gpu_set_invariant")
        import :: c_int, c_double
        integer(c_int), value :: nold

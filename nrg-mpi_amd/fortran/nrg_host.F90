@@ -36,6 +36,12 @@ module nrg_host
   double precision, allocatable :: chain_diag(:, :), chain_nondiag(:, :)
   double precision, allocatable, target :: coef_spec(:, :), inv0(:, :, :), spec0(:, :, :)
   integer, allocatable, target :: cons_diff_spec(:, :), pos_op(:, :), neg_op(:, :)
+  ! superconducting (BCS) chain: coefficient_BCS(nbcs,3) and the raw
+  ! chain_BCS(0:n_iter+1, nchain) (read_initial_basis.F90:185-201)
+  logical :: flag_bcs = .false.
+  integer :: nbcs = 0
+  integer, allocatable, target :: coef_bcs(:, :)
+  double precision, allocatable, target :: chain_bcs(:, :)
 
   ! ---- iteration state
   integer :: nold = 0, nsub = 0, nfull = 0, nkept = 0
@@ -70,9 +76,10 @@ contains
   ! namelist /hamiltonianInfo/ (module.F90:846-861); the locals carry the
   ! namelist names and hide the module variables of the same name
   subroutine read_hamiltonian_nml(path, ierr, o_iter, o_nchain, o_nv, o_cons, o_nmat, o_ncm, o_nb, &
-       o_ncoef, o_nops, o_i0, o_spectrum, o_bcs)
+       o_ncoef, o_nops, o_i0, o_spectrum, o_bcs, o_nbcs)
     character(len=*), intent(in) :: path
     integer, intent(out) :: ierr, o_iter, o_nchain, o_nv, o_cons(2, 2), o_nmat, o_ncm, o_nb, o_ncoef, o_nops, o_i0
+    integer, intent(out) :: o_nbcs
     logical, intent(out) :: o_spectrum, o_bcs
     integer :: numberOfIteration, numberOfChain, numberOfVariation, conservationBlock(2, 2)
     integer :: numberOfMatrix, numberOfConductionMatrix, numberOfBasis, numberOfCoefficient
@@ -95,6 +102,7 @@ contains
     o_iter = numberOfIteration; o_nchain = numberOfChain; o_nv = numberOfVariation; o_cons = conservationBlock
     o_nmat = numberOfMatrix; o_ncm = numberOfConductionMatrix; o_nb = numberOfBasis; o_ncoef = numberOfCoefficient
     o_nops = numberOfOperation; o_i0 = indexOfFirstIteration; o_spectrum = flag_spectrum; o_bcs = flag_BCS
+    o_nbcs = numberOfBCSCoefficient
   end subroutine read_hamiltonian_nml
 
   ! namelist /nrg/ (module.F90:665-675)
@@ -127,14 +135,12 @@ contains
     logical :: bcs
 
     call read_hamiltonian_nml(dir // "/hamiltonianInfo.dat", ierr, n_iter, nchain, nv, cons_block, nmat, ncm, &
-         nb0, ncoef, nops, i_first, flag_spectrum, bcs)
+         nb0, ncoef, nops, i_first, flag_spectrum, bcs, nbcs)
     if (ierr /= 0) return
     call read_parameter_nml(dir // "/parameter.txt", ierr, band_width, ztrick, lambda, scaling_type, nkeep)
     if (ierr /= 0) return
-    if (bcs) then
-       ierr = 2   ! BCS chains are outside the accelerated path
-       return
-    end if
+    flag_bcs = bcs
+    if (.not. flag_bcs) nbcs = 0
 
     if (allocated(operation)) deallocate(operation, coef_inv_type, coef_diag_type, coef_inv, coef_diag, &
          chain_diag, chain_nondiag, inv0)
@@ -232,6 +238,23 @@ contains
        read(u, *) hdr
        do i = 1, nmat
           read(u, *) neg_op(i, 1:nv)
+       end do
+    end if
+    if (flag_bcs) then
+       ! chain_BCS per chain, not rescaled, then the pair rows
+       ! (read_initial_basis.F90:185-201)
+       if (allocated(coef_bcs)) deallocate(coef_bcs, chain_bcs)
+       allocate(chain_bcs(0:n_iter + 1, nchain), coef_bcs(max(nbcs, 1), 3))
+       coef_bcs = 0
+       do c = 1, nchain
+          read(u, *) hdr
+          do j = 0, n_iter + 1
+             read(u, *) chain_bcs(j, c)
+          end do
+       end do
+       read(u, *) hdr
+       do i = 1, nbcs
+          read(u, *) coef_bcs(i, 1:3)
        end do
     end if
     close(u)
@@ -927,6 +950,8 @@ contains
     integer, intent(in) :: device
     integer, intent(out) :: ierr
     double precision :: emax, log10emax
+    double precision :: lambda_dummy(1)
+    lambda_dummy = 0
     t_stock = 0; t_rho = 0; t_cfs = 0
     call read_inputs(dir, ierr)
     if (ierr /= 0) then
@@ -943,6 +968,12 @@ contains
        ierr = nrg_gpu_set_model(int(nv, c_int), int(nops, c_int), int(nchain, c_int), int(ncm, c_int), &
             int(ncoef, c_int), operation, coef_diag_type, coef_diag, coef_inv_type, coef_inv, cons_block, &
             0_c_int, c_null_ptr, c_null_ptr)
+    end if
+    if (ierr /= 0) return
+    if (flag_bcs) then
+       ierr = nrg_gpu_set_bcs(int(nbcs, c_int), coef_bcs, int(n_iter + 2, c_int), chain_bcs)
+    else
+       ierr = nrg_gpu_set_bcs(0_c_int, cons_block, 0_c_int, lambda_dummy)
     end if
     if (ierr /= 0) return
     ierr = nrg_gpu_set_invariant(int(nb0, c_int), inv0)

@@ -28,6 +28,7 @@ SIGNATURES = {
     "nrg_gpu_last_error": (ctypes.c_char_p, []),
     "nrg_gpu_set_model": (ctypes.c_int, [ctypes.c_int] * 5 + [_i32p, _i32p, _f64p, _i32p, _f64p, _i32p,
                                                               ctypes.c_int, _f64p, _i32p]),
+    "nrg_gpu_set_bcs": (ctypes.c_int, [ctypes.c_int, _i32p, ctypes.c_int, _f64p]),
     "nrg_gpu_set_invariant": (ctypes.c_int, [ctypes.c_int, _f64p]),
     "nrg_gpu_set_invariant_spectrum": (ctypes.c_int, [ctypes.c_int, _f64p]),
     "nrg_gpu_diagonalize": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _i32p, ctypes.c_int, _i32p, ctypes.c_int,

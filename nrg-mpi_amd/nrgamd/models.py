@@ -120,11 +120,12 @@ class LocalSite:
     def nchan(self, s: int, c: int) -> int:
         return sum(1 for m, (ch, _) in enumerate(self.modes) if ch == c and s >> m & 1)
 
-    def operation(self) -> np.ndarray:
+    def operation(self, sz_only: bool = False) -> np.ndarray:
         """operation(nops, nv+nchain), op index = mask+1."""
         rows = []
         for s in range(self.nops):
-            rows.append([-self.q(s), -self.twosz(s)] + [self.nchan(s, c) for c in range(self.nchain)])
+            lab = [-self.twosz(s)] if sz_only else [-self.q(s), -self.twosz(s)]
+            rows.append(lab + [self.nchan(s, c) for c in range(self.nchain)])
         return np.array(rows, dtype=np.int64)
 
     def hopping_table(self):
@@ -192,6 +193,16 @@ class ModelSpec:
     niter: int = 40                 # total number of NRG iterations (loop i0..N)
     first_iteration: int = -1
     spectrum: bool = False
+    # on-site pairing Delta of a superconducting (BCS) Wilson chain; nonzero
+    # writes a flag_BCS deck whose blocks carry 2Sz only (charge is not
+    # conserved), chain_bcs(n) = Delta / scale(n-1) and BCS coefficient rows
+    # joining the empty and doubly occupied configurations of the new site
+    bcs_delta: float = 0.0
+    # coefficient-table probe: a reversed copy (half the coefficient) of the
+    # first hopping row is written after it (and a duplicate of the first
+    # invariant row, the tables share numberOfCoefficient), so that for the
+    # pair that occurs a reversed match precedes a forward one
+    h3_reversed_row: bool = False
 
     @property
     def last_iteration(self) -> int:
@@ -211,12 +222,26 @@ def two_orbital(nkeep=4000, niter=45, spectrum=False, U=0.3, J=0.05, Gamma=0.01,
     return ModelSpec("two_orbital", 2, U, Gamma, eps_d, J=J, nkeep=nkeep, niter=niter, spectrum=spectrum)
 
 
+def siam_bcs(nkeep=128, niter=12, U=0.2, Gamma=0.01, delta=0.02) -> ModelSpec:
+    """SIAM coupled to a superconducting Wilson chain (on-site pairing delta)."""
+    return ModelSpec("siam", 1, U, Gamma, -U / 2, nkeep=nkeep, niter=niter, spectrum=False, bcs_delta=delta)
+
+
+def siam_h3(nkeep=128, niter=12) -> ModelSpec:
+    """SIAM deck in which a pair's reversed hopping row precedes a forward row."""
+    return ModelSpec("siam", 1, 0.2, 0.01, -0.1, nkeep=nkeep, niter=niter, spectrum=False, h3_reversed_row=True)
+
+
 WORKLOADS = {
     # name: (factory, kwargs) -- sizes are the ones the workloads are quoted on
     "siam_nk256_it40": (siam, dict(nkeep=256, niter=40, spectrum=True)),
     "siam_nk2048_it50": (siam, dict(nkeep=2048, niter=50, spectrum=False)),
     "two_orbital_nk4000_it45": (two_orbital, dict(nkeep=4000, niter=45, spectrum=False)),
     "two_orbital_nk8000_it45": (two_orbital, dict(nkeep=8000, niter=45, spectrum=True)),
+    # decks exercising input features of the reference the workloads above
+    # leave unused: the BCS pairing override and the coefficient-search order
+    "siam_bcs_nk128_it12": (siam_bcs, dict(nkeep=128, niter=12)),
+    "siam_h3_nk128_it12": (siam_h3, dict(nkeep=128, niter=12)),
 }
 
 
@@ -279,7 +304,8 @@ def impurity_basis(spec: ModelSpec):
     dim = 1 << nm
     labels = {}
     for s in range(dim):
-        labels.setdefault((site.q(s), site.twosz(s)), []).append(s)
+        lab = (site.twosz(s),) if spec.bcs_delta else (site.q(s), site.twosz(s))
+        labels.setdefault(lab, []).append(s)
     states = []
     vecs = []
     for lab in sorted(labels):
@@ -322,26 +348,43 @@ def write_inputs(spec: ModelSpec, outdir: str) -> dict:
     hop = site.hopping_table()
     inv = site.invariant_table()
     assert len(hop) == len(inv)
+    if spec.h3_reversed_row:
+        # inside a block the product state made by op a (one electron less
+        # on the new site than op b) always has the larger reference, so the
+        # pair that occurs is (op_i, op_j) = (b, a): row (a, b) is a reversed
+        # match for it; the copy (b, a) written after it is a forward match
+        # (last match: coef/2) and must leave the references swapped
+        a, b, t, ch, c = hop[0]
+        hop = [hop[0], (b, a, t, ch, 0.5 * c)] + hop[1:]
+        inv = [inv[0], inv[0]] + inv[1:]
     ncoef = len(hop)
     i0, N = spec.first_iteration, spec.last_iteration
+    bcs = bool(spec.bcs_delta)
+    nv = 1 if bcs else 2
+    if bcs:
+        # pair (empty, doubly occupied) of the new site, both orders: the
+        # reference tests the forward order only (diagonalization.F90:217)
+        full = nops
+        bcs_rows = [(1, full, 1), (full, 1, 1)]
 
     with open(os.path.join(outdir, "hamiltonianInfo.dat"), "w") as f:
         f.write("&hamiltonianInfo\n")
         f.write(f" numberOfIteration = {N}\n")
         f.write(f" numberOfChain = {spec.nchain}\n")
-        f.write(" numberOfVariation = 2\n")
-        f.write(" conservationBlock = 1, 2, 1, 2\n")
+        f.write(f" numberOfVariation = {nv}\n")
+        # charge range 0..0 selects the 2Sz-only pair rule (invariantMatrixDGEMM.F90:168-176)
+        f.write(" conservationBlock = 0, 1, 0, 1\n" if bcs else " conservationBlock = 1, 2, 1, 2\n")
         f.write(f" numberOfMatrix = {nmat}\n")
         f.write(f" numberOfConductionMatrix = {ncm}\n")
         f.write(f" numberOfBasis = {nb}\n")
         f.write(f" numberOfCoefficient = {ncoef}\n")
-        f.write(" numberOfBCSCoefficient = 0\n")
+        f.write(f" numberOfBCSCoefficient = {len(bcs_rows) if bcs else 0}\n")
         f.write(" numberOfCoefficientImpurity = 0\n")
         f.write(f" numberOfOperation = {nops}\n")
         f.write(f" indexOfFirstIteration = {i0}\n")
         f.write(f" flag_spectrum = {'.true.' if spec.spectrum else '.false.'}\n")
         f.write(" flag_local = .false.\n")
-        f.write(" flag_BCS = .false.\n")
+        f.write(f" flag_BCS = {'.true.' if bcs else '.false.'}\n")
         f.write("/\n")
 
     with open(os.path.join(outdir, "parameter.txt"), "w") as f:
@@ -363,7 +406,7 @@ def write_inputs(spec: ModelSpec, outdir: str) -> dict:
     L = []
     L.append("basis")
     for (lab, ref, e) in states:
-        L.append(f"{lab[0]} {lab[1]} {ref} {_fmt(e / s_m2)}")
+        L.append(" ".join(str(x) for x in lab) + f" {ref} {_fmt(e / s_m2)}")
     L.append("invariant_matrix")
     for t in range(ncm):
         for j in range(nb):
@@ -376,7 +419,7 @@ def write_inputs(spec: ModelSpec, outdir: str) -> dict:
     for (a, b, t, ch, c) in hop:
         L.append(f"{a} {b} {t} {ch} {_fmt(c)}")
     L.append("operation")
-    for row in site.operation():
+    for row in site.operation(sz_only=bcs):
         L.append(" ".join(str(int(x)) for x in row))
     L.append("chain")
     for c in range(spec.nchain):
@@ -410,6 +453,16 @@ def write_inputs(spec: ModelSpec, outdir: str) -> dict:
         L.append("negative_spectrum_operation")
         for (ch, sp) in modes:
             L.append(f"-1 {-sp}")
+    if bcs:
+        assert not spec.spectrum, "spectra of BCS decks are not generated"
+        # read raw, without the chain rescaling (read_initial_basis.F90:185-201)
+        for c in range(spec.nchain):
+            L.append(f"chain_bcs{c + 1}")
+            for n in range(0, N + 2):
+                L.append(_fmt(spec.bcs_delta / scale_wilson(n - 1, spec.Lam, spec.D, spec.z)))
+        L.append("coefficient_BCS")
+        for (a, b, ch) in bcs_rows:
+            L.append(f"{a} {b} {ch}")
     with open(os.path.join(outdir, "initial.dat"), "w") as f:
         f.write("\n".join(L) + "\n")
     return dict(nb=nb, nops=nops, ncm=ncm, nmat=nmat, ncoef=ncoef, i0=i0, N=N)

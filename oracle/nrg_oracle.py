@@ -108,6 +108,8 @@ class Model:
     cons_diff_spec: Optional[np.ndarray] = None
     pos_op: Optional[np.ndarray] = None
     neg_op: Optional[np.ndarray] = None
+    coef_bcs: Optional[np.ndarray] = None    # [nbcs, 3] (op_i, op_j, channel), 1-based
+    chain_bcs: Optional[np.ndarray] = None   # [N+2, nchain], index n, raw
 
     def scale(self, n: int) -> float:
         """module.F90:690-711"""
@@ -136,6 +138,8 @@ def read_inputs(d: str) -> Model:
     nops = int(hi["numberofoperation"])
     i0 = int(hi["indexoffirstiteration"])
     spectrum = _bool(hi["flag_spectrum"])
+    bcs = _bool(hi.get("flag_bcs", ".false."))
+    nbcs = int(hi.get("numberofbcscoefficient", "0"))
     D, z, Lam = _flt(pa["bandwidth"]), _flt(pa["ztrick"]), _flt(pa["lambda"])
     scaling, nkeep = int(pa["scalingtype"]), int(pa["truncation"])
 
@@ -231,6 +235,17 @@ def read_inputs(d: str) -> Model:
         hdr()
         no = np.array([[int(x) for x in row()[:nv]] for _ in range(nmat)])
         m.coef_spec, m.spec0, m.cons_diff_spec, m.pos_op, m.neg_op = cs, sp, cds, po, no
+    if bcs:
+        # read_initial_basis.F90:185-201: chain_bcs(0:N+1) per chain, not
+        # rescaled, then the coefficient_BCS rows
+        cb = np.zeros((N + 2, nchain))
+        for c in range(nchain):
+            hdr()
+            for n in range(0, N + 2):
+                cb[n, c] = _flt(row()[0])
+        hdr()
+        m.coef_bcs = np.array([[int(x) for x in row()[:3]] for _ in range(nbcs)], dtype=np.int64).reshape(nbcs, 3)
+        m.chain_bcs = cb
     return m
 
 
@@ -375,12 +390,27 @@ class NRG:
         v = cnd[ch] * T["coef"][oi, oj] * self.inv[t, ir, jr]
         H[iu, ju] = v
         H[ju, iu] = v
+        if m.coef_bcs is not None and len(m.coef_bcs):
+            # BCS pairing (diagonalization.F90:215-245): a forward row of
+            # (op_i, op_j) with equal references; the last match sets the
+            # channel and the value replaces the hopping term
+            iu, ju = np.triu_indices(dim, 1)
+            bch = np.full((m.nops, m.nops), -1, dtype=np.int64)
+            for a, b, c in m.coef_bcs:
+                bch[a - 1, b - 1] = c - 1
+            cb = bch[ops[iu], ops[ju]]
+            hit = (cb >= 0) & (refs[iu] == refs[ju])
+            val = m.chain_bcs[it + 1, np.maximum(cb[hit], 0)]
+            H[iu[hit], ju[hit]] = val
+            H[ju[hit], iu[hit]] = val
         return H
 
     def hopping_lookup(self):
         """(op_i, op_j) -> (type, channel, coef, swap) replaying the coefficient
         search of ``diagonalization.F90:195-213``: every row is tested, the last
-        match wins, a reversed match also swaps the references."""
+        match sets (type, channel, coef), and a reversed match swaps the
+        references for good (``:208-209`` reassign ireference/jreference and a
+        later forward match does not restore them)."""
         if getattr(self, "_hop", None) is not None:
             return self._hop
         m = self.m
@@ -398,7 +428,7 @@ class NRG:
                     else:
                         continue
                     T["valid"][oi, oj] = True
-                    T["swap"][oi, oj] = hit[0]
+                    T["swap"][oi, oj] |= hit[0]
                     T["type"][oi, oj] = t
                     T["chan"][oi, oj] = ch
                     T["coef"][oi, oj] = m.coef_diag[ic]

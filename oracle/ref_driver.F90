@@ -14,9 +14,13 @@
 ! cfsSpectrumCalculation(startTrancation..N) -> showCount.
 !
 ! Usage (inside a directory holding the three input files):
-!   nrg_ref [full_dump_upto]
+!   nrg_ref [full_dump_upto [stop_after]]
 ! full_dump_upto (default -100): iterations <= this value also dump the
 ! eigenvector array and the invariant matrices.
+! stop_after (default: the deck's last iteration): the run ends after this
+! iteration, skipping the spectrum phase.  Iterations up to it are exactly
+! those of the complete run (the last-iteration rule of postprocess.F90:79-81
+! looks at the deck's numberOfIteration, not at stop_after).
 program ref_driver
   use parallel
   use system
@@ -26,7 +30,7 @@ program ref_driver
   implicit none
   include 'mpif.h'
 
-  integer :: ierr, it, imat, ib, full_upto, nargs
+  integer :: ierr, it, imat, ib, full_upto, nargs, stop_after
   character(64) :: arg
 
   call MPI_INIT(ierr)
@@ -41,11 +45,17 @@ program ref_driver
      call get_command_argument(1, arg)
      read(arg, *) full_upto
   end if
+  stop_after = huge(stop_after)
+  if (nargs >= 2) then
+     call get_command_argument(2, arg)
+     read(arg, *) stop_after
+  end if
 
   call setupHamiltonian
   if (hami%flag_spectrum) call setSpectrumInfo
   allocate(chain_diagonal(0:hami%numberOfIteration+1, hami%numberOfChain))
   allocate(chain_nondiagonal(hami%indexOfFirstIteration:hami%numberOfIteration+1, hami%numberOfChain))
+  if (hami%flag_BCS) allocate(chain_BCS(0:hami%numberOfIteration+1, hami%numberOfChain))
   call read_parameter
   open(errorlog, file=errorfile)
   open(eigeneven, file=fileEigenEven)
@@ -77,10 +87,11 @@ program ref_driver
   do it = hami%indexOfFirstIteration, hami%numberOfIteration
      call nrgIteration(it)
      call dump_iteration(it, it <= full_upto)
+     if (it >= stop_after) exit
   end do
   call stopCount
 
-  if (hami%flag_spectrum) then
+  if (hami%flag_spectrum .and. stop_after >= hami%numberOfIteration) then
      call startCount("main:spec")
      if (hami%flag_truncation) call initializeReducedDensity
      elementNumFullBasisInput  = pointFullBasisInput - 1

@@ -76,3 +76,66 @@ def test_syev_large_block(gpu):
     H = (a + a.T) / np.sqrt(n)
     ws, Us = gpu.syev_batched([H])
     _check_eig(H, ws[0], Us[0])
+
+
+def _fast_rotation(rng, n):
+    """A dense random orthogonal matrix applied in O(n^2 log n): random
+    signs, an orthonormal DCT-II and a random permutation.  Returns
+    apply(X) = Q X and apply_t(X) = Q^T X for column blocks X."""
+    import scipy.fft as sf
+    s1 = rng.choice([-1.0, 1.0], size=n)
+    s2 = rng.choice([-1.0, 1.0], size=n)
+    p = rng.permutation(n)
+
+    def apply(X):   # Q = S2 C S1 P
+        return s2[:, None] * sf.dct(s1[:, None] * X[p], type=2, norm="ortho", axis=0, workers=-1)
+
+    def apply_t(X):  # Q^T = P^T S1 C^T S2
+        Y = s1[:, None] * sf.idct(s2[:, None] * X, type=2, norm="ortho", axis=0, workers=-1)
+        out = np.empty_like(Y)
+        out[p] = Y
+        return out
+    return apply, apply_t
+
+
+def _planted(rng, n, lam):
+    """H = Q diag(lam) Q^T with a dense random orthogonal Q."""
+    apply, apply_t = _fast_rotation(rng, n)
+    Y = apply_t(np.eye(n)) * lam[:, None]   # diag(lam) Q^T
+    H = apply(Y)
+    return 0.5 * (H + H.T)
+
+
+def _check_planted(H, w, U, lam, rng, tol):
+    """Eigenvalues against the planted spectrum (a closed form), residual
+    and orthogonality by random probes (matrix-vector products only)."""
+    n = H.shape[0]
+    nrm = np.abs(lam).max()
+    assert np.all(np.diff(w) >= -1e-14 * nrm), "eigenvalues not ascending"
+    assert np.max(np.abs(w - np.sort(lam))) <= tol * nrm
+    X = rng.standard_normal((n, 4))
+    UX = U @ X
+    res = np.linalg.norm(H @ UX - U @ (w[:, None] * X)) / (nrm * np.linalg.norm(X))
+    orth = np.linalg.norm(U.T @ UX - X) / np.linalg.norm(X)
+    assert res <= tol, res
+    assert orth <= tol, orth
+    return res, orth
+
+
+@pytest.mark.parametrize("n,ladder", [(8500, "integer"), (10000, "continuous")])
+def test_syev_blocks_above_lds_limits(gpu, n, ladder):
+    """Blocks above the LDS-resident paths: the top divide-and-conquer merge
+    of m > 8192 runs its deflation scan from global memory, the continuous
+    ladder leaves > 4000 secular roots (poles from global memory), and at
+    n = 10000 the first ~280 SYMV launches keep v in global memory (their two
+    LDS vectors would exceed 160 KB).  Spectra Lambda^(-k/2), Lambda = 2,
+    k uniform on 0..60 (integer: exact degeneracies, heavy deflation)."""
+    rng = np.random.default_rng(n)
+    k = rng.integers(0, 61, size=n).astype(float) if ladder == "integer" else rng.uniform(0, 60, size=n)
+    lam = 2.0 ** (-k / 2.0)
+    H = _planted(rng, n, lam)
+    print(f"syev n={n} {ladder} ladder", flush=True)
+    ws, Us = gpu.syev_batched([H])
+    # eigenvalue error ~ n eps |H|, residual and orthogonality ~ n eps
+    res, orth = _check_planted(H, ws[0], Us[0], lam, rng, tol=2e-16 * n * 8)
+    print("residual", res, "orthogonality", orth)

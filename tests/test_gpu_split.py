@@ -97,6 +97,6 @@ def test_nrg_iterations_split(tmp_path, gpu):
     gpu.load().nrg_gpu_set_gemm_split(8)
     try:
         run_and_compare(tmp_path, "siam_nk256_it40", {})
-        run_and_compare(tmp_path, "two_orbital_nk4000_it45", {"niter": 6}, last=2)
+        run_and_compare(tmp_path, "two_orbital_nk4000_it45", {}, last=2)
     finally:
         gpu.load().nrg_gpu_set_gemm_split(0)

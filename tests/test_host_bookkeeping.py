@@ -36,7 +36,8 @@ def kept_basis(sub, kept, nv):
 
 
 CASES = [("siam_nk256_it40", {}, None), ("siam_nk2048_it50", {}, 12),
-         ("two_orbital_nk4000_it45", {"niter": 6}, None)]
+         ("two_orbital_nk4000_it45", {}, None), ("siam_bcs_nk128_it12", {}, None),
+         ("siam_h3_nk128_it12", {}, None)]
 
 
 @pytest.mark.parametrize("name,ov,limit", CASES)
@@ -64,7 +65,7 @@ def test_canonical_sort_same_blocks(tmp_path):
     """Canonical mode: same block table, records equal as a multiset per block."""
     from nrgamd.host import Host
     fx = np.load(os.path.join(GOLD, "ref_two_orbital_nk4000_it45.npz"))
-    h = Host(deck(tmp_path, "two_orbital_nk4000_it45", niter=6), device=None)
+    h = Host(deck(tmp_path, "two_orbital_nk4000_it45"), device=None)
     nv = h.i["nv"]
     sub = fx["it1_subspace"]
     bo = kept_basis(sub, fx["it1_kept_blk"], nv)

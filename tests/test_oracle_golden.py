@@ -115,7 +115,7 @@ def test_siam_nk2048_first_iterations(tmp_path):
 
 def test_two_orbital_nk4000_first_iterations(tmp_path):
     fx = np.load(os.path.join(GOLD, "ref_two_orbital_nk4000_it45.npz"))
-    nrg = run_oracle(tmp_path, "two_orbital_nk4000_it45", niter=6, upto=0)
+    nrg = run_oracle(tmp_path, "two_orbital_nk4000_it45", upto=0)
     compare_forward(nrg, fx)
 
 
@@ -185,3 +185,53 @@ def test_smoke_workload_is_well_conditioned(tmp_path):
     bad = list(zip(_siam_eigenvalues(tmp_path, 64, np.linalg.eigh, "c"), _siam_eigenvalues(tmp_path, 64, alt, "d")))
     worst = max(np.max(np.abs(e0 - e1)) / max(1.0, np.max(np.abs(e0))) for e0, e1 in bad if len(e0) == len(e1))
     assert worst > 1e-11
+
+
+@pytest.mark.parametrize("name", ["siam_h3_nk128_it12", "siam_bcs_nk128_it12"])
+def test_input_feature_decks(tmp_path, name):
+    """Decks that exercise reference input features the physical workloads
+    leave unused, against reference runs of the same decks:
+    siam_h3 lists a reversed hopping row before its forward row (the
+    reversed match swaps the references for good, diagonalization.F90:195-211);
+    siam_bcs is a flag_BCS deck with 2Sz-only blocks whose pairing rows
+    replace the hopping term (:215-245)."""
+    fx = np.load(os.path.join(GOLD, f"ref_{name}.npz"))
+    nrg = run_oracle(tmp_path, name)
+    assert nrg.start_trunc == int(fx["start_trunc"])
+    compare_forward(nrg, fx)
+
+
+def test_input_feature_decks_reject_planted_mistakes(tmp_path, monkeypatch):
+    """The two decks catch the mistakes they are made for: a swap flag taken
+    from the last matching row only, and BCS rows that are ignored."""
+    fx = np.load(os.path.join(GOLD, "ref_siam_h3_nk128_it12.npz"))
+    orig = O.NRG.hopping_lookup
+
+    def last_match(self):
+        T = dict(orig(self))
+        sw = np.zeros_like(T["swap"])
+        for oi in range(self.m.nops):
+            for oj in range(self.m.nops):
+                for a, b, t, ch in self.m.coef_diag_type:
+                    if (oi, oj) == (a - 1, b - 1):
+                        sw[oi, oj] = False
+                    elif (oi, oj) == (b - 1, a - 1):
+                        sw[oi, oj] = True
+        T["swap"] = sw
+        self._hop = T
+        return T
+    monkeypatch.setattr(O.NRG, "hopping_lookup", last_match)
+    nrg = run_oracle(tmp_path, "siam_h3_nk128_it12")
+    with pytest.raises(AssertionError):
+        compare_forward(nrg, fx)
+    monkeypatch.setattr(O.NRG, "hopping_lookup", orig)
+
+    fx = np.load(os.path.join(GOLD, "ref_siam_bcs_nk128_it12.npz"))
+    d = str(tmp_path / "bcs_off")
+    models.write_inputs(models.workload("siam_bcs_nk128_it12"), d)
+    m = O.read_inputs(d)
+    m.coef_bcs = m.coef_bcs[:0]
+    nrg = O.NRG(m)
+    nrg.run_forward(None)
+    with pytest.raises(AssertionError):
+        compare_forward(nrg, fx)
