@@ -3,18 +3,24 @@
 
 Workload: two-orbital Anderson model with Kanamori interaction (U=0.3,
 J=0.05, Gamma=0.01 per orbital, two Wilson chains, Lambda=2, local site of
-dimension 16), Nkeep=4000.  One step is one NRG iteration of the forward
-pass: prepareBasis + block-Hamiltonian assembly + batched FP64 eigensolve of
-all (Q,Sz) blocks + truncation + invariant-matrix contractions, driven by the
-Fortran host (nrg-mpi_amd/lib/libnrg_host.so) through the HIP library.
-Warmup iterations run the growth phase (-1, 0, 1, ...); the timed steps are
-steady-state truncating iterations (~64k states, ~90 blocks up to ~4700).
+dimension 16), Nkeep=4000, 45 iterations.  One step is one NRG iteration of
+the forward pass: prepareBasis + block-Hamiltonian assembly + batched FP64
+eigensolve of all (Q,Sz) blocks + truncation + invariant-matrix
+contractions, driven by the Fortran host (nrg-mpi_amd/lib/libnrg_host.so)
+through the HIP library.  Warmup iterations run the growth phase (-1, 0, 1,
+...); the timed steps are steady-state truncating iterations (~64k states,
+~99 blocks up to ~4100).
 
 Prints one JSON line: metric, value (iter/s), ms_per_step, config, the
-per-stage seconds, `roofline` of the dominant kernel family (algorithmic
-bytes or flops over its HIP-event time, against 8 TB/s HBM or 78.6 TFLOP/s
-FP64 MFMA) and `cpu_baseline` (LAPACK dsyev on one host core over a sample
-of one iteration's blocks).
+per-stage seconds, the executed FP64 flops and the percentage of the FP64
+matrix-core peak they reach over the step, `roofline` of the dominant
+kernel family (algorithmic bytes over its HIP-event time against 8 TB/s),
+three-iteration comparison legs (plain FP64: one block per eigensolver
+pass; batched with split-precision contraction products), `cpu_baseline`
+(one whole steady-state iteration of the oracle measured on the host
+cores, next to the reference's own timing from its fixture run), the
+synthetic 256-block batch, and the Nkeep=8000 pipeline with spectra in FP64
+and split precision.
 """
 import argparse
 import json
@@ -34,8 +40,8 @@ FP64_PEAK_TFS = 78.6
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=6)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="two_orbital_nk4000_it45")
     ap.add_argument("--sort", choices=["exact", "canonical"], default="exact")
     ap.add_argument("--no-profile", action="store_true", help="skip the HIP-event kernel timing")
@@ -50,7 +56,8 @@ def parse():
                     help="write a CSV of every launch of the profiled families (ordinal per kernel function "
                          "from the first iteration, algorithmic bytes and flops; phase warmup/timed)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--leg-budget", type=float, default=240.0,
+                    help="start the Nkeep=8000 leg only if the run so far took less than this many seconds")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a plain device-to-device copy of BYTES bytes five times (the reference "
                          "kernel that scales the FETCH_SIZE/WRITE_SIZE counters to bytes) and exit")
@@ -60,54 +67,88 @@ def parse():
                     help="slices of the split-precision contraction GEMMs in the timed steps (0: FP64 matrix "
                          "cores, the faster form on MI355X; the split form is measured in its own iteration)")
     ap.add_argument("--split-leg", type=int, default=8,
-                    help="slices of the one-iteration split-precision comparison (0: skip)")
+                    help="slices of the three-iteration split-precision comparison (0: skip)")
     ap.add_argument("--no-spectrum-leg", dest="spectrum_leg", action="store_false",
-                    help="skip the full-pipeline comparison with spectra (forward + rho sweep + CFS)")
+                    help="skip the Nkeep=8000 full pipeline with spectra (forward + rho sweep + CFS, FP64 and "
+                         "split contraction products)")
     ap.add_argument("--no-synthetic", dest="synthetic", action="store_false",
-                    help="skip the synthetic block-batch GEMM comparison (FP64 vs split precision)")
+                    help="skip the synthetic 256-block batch (eigensolve, FP64 vs split-precision U^T O U)")
     ap.add_argument("--no-plain", dest="plain", action="store_false",
-                    help="skip the one-iteration plain (one block per pass) comparison")
+                    help="skip the three-iteration plain (one block per pass) comparison")
     return ap.parse_args()
 
 
-def cpu_baseline(block_sizes, budget_s):
-    """Reference-equivalent CPU eigensolve (LAPACK dsyev, the routine the
-    reference calls per block, diagonalization.F90:267) on one host core, on a
-    bounded sample of the iteration's blocks; the remaining blocks are
-    extrapolated with the n^3 cost of the largest measured block."""
-    os.environ.setdefault("OPENBLAS_NUM_THREADS", "1")
+MFMA_FAMILIES = ("tridiag_gemm", "dc_gemm", "back_gemm", "invariant_gemm", "spectral_gemm", "cfs_products",
+                 "cfs_rho_weight", "rho_gemm")
+EIG_FAMILIES = ("tridiag_symv", "tridiag_gemm", "dc_gemm", "back_gemm")
+
+
+def reference_count_csv(workload):
+    """The reference's own per-iteration cost in this build's fixture run
+    (count.f90's nrgIteration row: wall seconds over the run's iterations)."""
     import numpy as np
-    from scipy.linalg import eigh
-    try:
-        from threadpoolctl import threadpool_limits
-        lim = threadpool_limits(1)
-    except Exception:  # pragma: no cover
-        lim = None
-    rng = np.random.default_rng(0)
-    sizes = sorted(int(n) for n in block_sizes)
-    t_meas, n_meas, last_rate = 0.0, [], None
-    for n in sizes:
-        if t_meas > budget_s:
-            break
-        a = rng.standard_normal((n, n))
-        a = a + a.T
-        t0 = time.perf_counter()
-        eigh(a, driver="ev", overwrite_a=True, check_finite=False)
-        dt = time.perf_counter() - t0
-        t_meas += dt
-        n_meas.append(n)
-        if n >= 64:
-            last_rate = dt / float(n) ** 3
-    rest = sizes[len(n_meas):]
-    t_rest = sum(last_rate * float(n) ** 3 for n in rest) if rest else 0.0
-    total = t_meas + t_rest
-    if lim is not None:
-        lim.unregister() if hasattr(lim, "unregister") else None
-    sample = (f"dsyev (scipy/OpenBLAS, 1 thread) on {len(n_meas)} of {len(sizes)} blocks of one steady-state "
-              f"iteration (n <= {max(n_meas) if n_meas else 0}, {t_meas:.1f} s measured), the other "
-              f"{len(rest)} blocks extrapolated by n^3; eigensolve stage only, host bookkeeping excluded")
-    return dict(value=1.0 / total, unit="iter/s", cores=1, kind="port", sample=sample,
-                seconds_per_iteration=total)
+    path = os.path.join(ROOT, "tests", "golden", f"ref_{workload}.npz")
+    if not os.path.exists(path):
+        return None
+    fx = np.load(path)
+    if "count_csv" not in fx:
+        return None
+    rows = [l.split(",") for l in str(fx["count_csv"]).splitlines()]
+    for r in rows:
+        if r and r[0].strip() == "nrgIteration":
+            tot = float(r[1])
+            n = int(r[2])
+            out = {"seconds_total": tot, "iterations": n, "mpi_ranks": int(fx["mpi_ranks"])}
+            if "iteration_wall_s" in fx:
+                w = fx["iteration_wall_s"]
+                steady = [float(x) for it, x in zip(fx["iterations"], w) if it >= 4]
+                out["steady_state_s_per_iteration"] = float(np.mean(steady)) if steady else None
+            out.update({
+                    "note": "reference build (amdflang -O2, MKL sequential) in the build container, "
+                            "iterations -1..%d of the 45-iteration deck, count.csv nrgIteration row" %
+                            int(fx["iterations"][-1])})
+            return out
+    return None
+
+
+def cpu_baseline(h, gpu, deck, it, workers):
+    """One whole steady-state NRG iteration of the oracle (the numpy
+    restatement of the reference path) on the host cores, measured: it
+    continues from the state the GPU run reached before iteration `it`
+    (kept basis records, energies, invariant matrix), runs prepareBasis with
+    the reference's exchange-sort permutation, the block assembly, one LAPACK
+    eigensolve per block on a pool of `workers` threads (the reference's
+    OpenMP loop over blocks), the truncation and the invariant matrices
+    (dense coefficient matrices and two GEMMs per block pair, the
+    reference's form).  Returns the baseline and the oracle's record of the
+    iteration for comparison with the GPU's."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import nrg_oracle as O
+    m = O.read_inputs(deck)
+    nrg = O.NRG(m, workers=workers)
+    k = h.i["nkept"]
+    nrg.seed(h.basis_output(), h.eigenvalues(), gpu.invariant_get(k, h.i["ncm"]), k, h.i["start_trunc"])
+    t = [time.perf_counter()]
+    nrg.prepare_basis()
+    t.append(time.perf_counter())
+    nrg.diagonalize(it)
+    t.append(time.perf_counter())
+    nrg.calc_thermo()
+    nrg.truncate(it)
+    t.append(time.perf_counter())
+    nrg.invariant_matrix()
+    t.append(time.perf_counter())
+    total = t[-1] - t[0]
+    stages = dict(prepare=t[1] - t[0], diagonalize=t[2] - t[1], truncate=t[3] - t[2], invariant=t[4] - t[3])
+    rec = dict(nkept=nrg.nkept, kept_blk=nrg.kept_per_block(), E=np.sort(nrg.eigenvalue))
+    return dict(value=1.0 / total, unit="iter/s", cores=workers, kind="port",
+                sample=f"one whole steady-state iteration ({it}) of the two-orbital Nkeep=4000 workload, "
+                       f"measured, nothing extrapolated: the oracle's prepareBasis (exact exchange-sort "
+                       f"permutation), block assembly, LAPACK eigensolve per block on {workers} threads, "
+                       f"truncation and invariant matrices, continued from the GPU run's state after "
+                       f"iteration {it - 1}",
+                seconds_per_iteration=total, stages_s=stages, nproc=os.cpu_count()), rec
 
 
 def profiled_traffic(family, workload):
@@ -129,64 +170,65 @@ def profiled_traffic(family, workload):
     return vals
 
 
-def synthetic_gemm(gpu, slices, nblk=32, median=600, seed=0):
-    """FP64 vs split-precision matrix-core products on a synthetic block batch:
-    block sizes log-normal (median `median`, sigma 1 in ln) clipped to
-    [16, 4096], one dense N(0,1) operator O and a dense N(0,1) U per block,
-    the product O U of every block in one batched launch, device-timed."""
-    import numpy as np
-    gpu.init(0)
-    rng = np.random.default_rng(seed)
-    sizes = np.clip(np.exp(rng.normal(np.log(median), 1.0, size=nblk)), 16, 4096).astype(int)
-    As = [rng.standard_normal((n, n)) for n in sizes]
-    Bs = [rng.standard_normal((n, n)) for n in sizes]
-    flops = float(sum(2.0 * n ** 3 for n in sizes))
-    C0, t0 = gpu.dgemm_batched(As, Bs, slices=0, repeat=3)
-    out = {"config": f"{nblk} blocks, n log-normal(median {median}, sigma 1) in [16, 4096], seed {seed}, "
-                     f"one product O U per block (n <= {int(sizes.max())}, sum 2n^3 = {flops:.3e})",
-           "fp64": {"ms": t0, "tflops": flops / (t0 * 1e-3) / 1e12}}
-    for s in slices:
-        Cs, ts = gpu.dgemm_batched(As, Bs, slices=s, repeat=3)
-        err = max(float(np.abs(c - r).max() / np.abs(r).max()) for c, r in zip(Cs, C0))
-        pairs = s * (s + 1) // 2
-        out[f"split{s}"] = {"ms": ts, "tflops_fp64_equiv": flops / (ts * 1e-3) / 1e12,
-                            "fp16_mfma_tflops": pairs * flops / (ts * 1e-3) / 1e12,
-                            "max_rel_err_vs_fp64": err, "digit_products": pairs}
-    return out
+def synthetic_batch(log):
+    """The synthetic block batch at its stated size: 256 blocks, sizes
+    log-normal (median 600, sigma 1.0 in ln-space, seed 20240) clipped to
+    16..4096, eigenvalues 2^(-k/2), k uniform on 0..60, Haar rotations, four
+    N(0,1) operators per block.  Batched FP64 eigensolve checked against the
+    planted ladder; U^T O U in FP64 and in split precision (8 and 6 slices),
+    device-timed (mean of one repeat), errors against numpy (FP64, sampled
+    blocks) and against the FP64 products (split, every block)."""
+    from nrgamd import synthetic as S
+    t0 = time.perf_counter()
+    b = S.make_batch(seed=20240, nblk=256, median=600.0, sigma=1.0)
+    gen = time.perf_counter() - t0
+    r = S.run(b, nops=4, split=(8, 6), repeat=1, sample=8, log=log)
+    r["generation_s"] = gen
+    r["config"] = ("256 blocks, n ~ log-normal(median 600, sigma 1.0) clipped to [16, 4096], seed 20240, "
+                   "eigenvalues 2^(-k/2) k in 0..60 (integers), Haar rotations, 4 N(0,1) operators per block")
+    return r
 
 
-def spectrum_pipeline(gpu, models, Host, nkeep=1000, niter=25):
-    """Full pipeline with spectra (forward with stocks, rho sweep, CFS
-    spikes and binning) on the two-orbital Kanamori model at a reduced
-    Nkeep, once with FP64 contraction products and once split (8 slices)."""
+def workload_nk8000(gpu, models, Host, log):
+    """Two-orbital Kanamori model at Nkeep=8000 with spectra (45 iterations,
+    4 spectral types): forward pass with stocks, rho sweep and CFS spikes,
+    with FP64 contraction products and with split-precision (8-slice)
+    products; the FP64 form is the oracle of the split one."""
     import numpy as np
-    out = {"config": f"two-orbital Kanamori, Nkeep={nkeep}, {niter} iterations, 4 spectral types"}
+    out = {"config": "two-orbital Kanamori U=0.3 J=0.05 Gamma=0.01/orbital, Nkeep=8000, 45 iterations, "
+                     "4 spectral types"}
     bins = {}
     for split in (0, 8):
-        tmp = tempfile.mkdtemp(prefix="nrgspec_")
-        models.write_inputs(models.workload("two_orbital_nk4000_it45", nkeep=nkeep, niter=niter, spectrum=True), tmp)
+        tmp = tempfile.mkdtemp(prefix="nrg8000_")
+        models.write_inputs(models.workload("two_orbital_nk8000_it45"), tmp)
         h = Host(tmp, device=0)
         gpu.load().nrg_gpu_set_gemm_split(split)
         try:
             t0 = time.perf_counter()
-            h.run_forward()
+            steady = []
+            for it in h.iterations():
+                ti = time.perf_counter()
+                h.iterate(it)
+                if 4 <= it < h.i["n_iter"]:
+                    steady.append(time.perf_counter() - ti)
             t1 = time.perf_counter()
             h.spectrum()
             t2 = time.perf_counter()
             st = h.spectrum_times()
             neg, pos = h.bins()
-            i = h.info()
-            nit = i["n_iter"] - i["i_first"] + 1
-            out["split8" if split else "fp64"] = {
-                "forward_s": t1 - t0, "forward_iter_per_s": nit / (t1 - t0),
-                "stock_total_s": st["stock"], "rho_sweep_s": st["rho"], "cfs_s": st["cfs"],
-                "spectrum_phase_s": t2 - t1, "pipeline_s": t2 - t0,
-                "sum_rule_max_dev": float(np.max(np.abs(neg.sum(1) + pos.sum(1) - 1.0)))}
+            tr = h.rho_traces()
+            key = "split8" if split else "fp64"
+            out[key] = {"forward_s": t1 - t0, "steady_state_iter_per_s": len(steady) / sum(steady),
+                        "stock_total_s": st["stock"], "rho_sweep_s": st["rho"], "cfs_s": st["cfs"],
+                        "pipeline_s": t2 - t0,
+                        "sum_rule_max_dev": float(np.max(np.abs(neg.sum(1) + pos.sum(1) - 1.0))),
+                        "trace_rho_max_dev": float(max(abs(v - 1.0) for v in tr.values()))}
             bins[split] = (neg, pos)
             if split:
                 peak = max(np.abs(bins[0][0]).max(), np.abs(bins[0][1]).max())
                 diff = max(np.abs(neg - bins[0][0]).max(), np.abs(pos - bins[0][1]).max())
-                out["split8"]["max_bin_diff_vs_fp64_rel_peak"] = float(diff / peak)
+                out[key]["max_bin_diff_vs_fp64_rel_peak"] = float(diff / peak)
+            log(f"Nkeep=8000 {key}: pipeline {t2 - t0:.1f} s")
         finally:
             gpu.load().nrg_gpu_set_gemm_split(0)
             h.close()
@@ -222,11 +264,16 @@ def main():
     import numpy as np
     from nrgamd import gpu, models
     from nrgamd.host import Host
+    t_start = time.perf_counter()
+
+    def log(msg):
+        print(f"[{time.perf_counter() - t_start:7.1f} s] {msg}", file=sys.stderr, flush=True)
 
     spec = models.workload(a.workload)
     need = a.warmup + a.steps
-    if need > spec.niter:
-        raise SystemExit(f"warmup+steps={need} exceeds the {spec.niter} iterations of {a.workload}")
+    legs = 3 * (int(a.plain) + int(a.split_leg > 0 or a.split > 0))
+    if need + legs > spec.niter:
+        raise SystemExit(f"warmup+steps+legs={need + legs} exceeds the {spec.niter} iterations of {a.workload}")
     tmp = tempfile.mkdtemp(prefix="nrgbench_")
     models.write_inputs(spec, tmp)
     h = Host(tmp, device=0, sort_mode=0 if a.sort == "exact" else 1)
@@ -238,29 +285,27 @@ def main():
     its = list(range(spec.first_iteration, spec.first_iteration + need))
     for it in its[:a.warmup]:
         h.iterate(it)
-        print(f"warmup iteration {it}: nfull={h.i['nfull']} nsub={h.i['nsub']} kept={h.i['nkept']}",
-              file=sys.stderr, flush=True)
+        log(f"warmup iteration {it}: nfull={h.i['nfull']} nsub={h.i['nsub']} kept={h.i['nkept']}")
     if not a.no_profile:
         fams = None if a.profile_families == "all" else [f for f in a.profile_families.split(",") if f]
         gpu.profile(True, families=fams, every=a.profile_every)
     stages = {}
-    flops_eig = 0.0
     n_warm_log = len(gpu.launch_log_get()) if a.launch_log else 0
+    gpu.work_counters(reset=True)
     t0 = time.perf_counter()
     for it in its[a.warmup:]:
         h.iterate(it)
         t = h.times()
         st = gpu.stage_times()
-        flops_eig += st["eig_flops"]
         for k in ("prepare", "diagonalization", "postprocess", "invariant"):
             stages[k] = stages.get(k, 0.0) + t[k]
         for k in ("assemble", "eig", "tridiag", "dc", "back"):
             stages["gpu_" + k] = stages.get("gpu_" + k, 0.0) + st[k] * 1e-3
-        print(f"timed iteration {it}: {t['iteration']:.3f} s  (prepare {t['prepare']:.3f}, diag "
-              f"{t['diagonalization']:.3f} [tri {st['tridiag']:.0f} ms, dc {st['dc']:.0f} ms, back "
-              f"{st['back']:.0f} ms], post {t['postprocess']:.3f}) nfull={h.i['nfull']} nsub={h.i['nsub']}",
-              file=sys.stderr, flush=True)
+        log(f"timed iteration {it}: {t['iteration']:.3f} s  (prepare {t['prepare']:.3f}, diag "
+            f"{t['diagonalization']:.3f} [tri {st['tridiag']:.0f} ms, dc {st['dc']:.0f} ms, back "
+            f"{st['back']:.0f} ms], post {t['postprocess']:.3f}) nfull={h.i['nfull']} nsub={h.i['nsub']}")
     elapsed = time.perf_counter() - t0
+    work = gpu.work_counters(reset=True)
     if a.launch_log:
         L = gpu.launch_log_get()
         names = {int(k): gpu.launch_log_kernel(int(k)) for k in set(L[:, 2].tolist())}
@@ -273,47 +318,81 @@ def main():
     ks = gpu.kernel_stats() if not a.no_profile else {}
     if not a.no_profile:
         gpu.profile(False)
-    plain = batched64 = batched_split = None
-    nxt = its[-1] + 1
-    if a.plain and a.split > 0 and nxt <= spec.last_iteration:
-        # the batched path with FP64 contraction products, one iteration
-        gpu.load().nrg_gpu_set_gemm_split(0)
-        tb = time.perf_counter()
-        h.iterate(nxt)
-        tb = time.perf_counter() - tb
-        batched64 = {"value": 1.0 / tb, "unit": "iter/s", "iteration": nxt,
-                     "form": "batched FP64: all blocks per eigensolver pass, FP64 contraction products"}
-        print(f"batched FP64 iteration {nxt}: {tb:.3f} s", file=sys.stderr, flush=True)
-        nxt += 1
-    if a.plain and a.split == 0 and a.split_leg > 0 and nxt <= spec.last_iteration:
-        # the batched path with split-precision contraction products, one iteration
-        gpu.load().nrg_gpu_set_gemm_split(a.split_leg)
-        tb = time.perf_counter()
-        h.iterate(nxt)
-        tb = time.perf_counter() - tb
-        batched_split = {"value": 1.0 / tb, "unit": "iter/s", "iteration": nxt,
-                         "form": f"batched eigensolve + split-precision contraction products ({a.split_leg} "
-                                 f"slices, FP16 matrix cores)"}
-        print(f"batched split iteration {nxt}: {tb:.3f} s", file=sys.stderr, flush=True)
-        nxt += 1
-    gpu.load().nrg_gpu_set_gemm_split(0)
-    if a.plain and nxt <= spec.last_iteration:
-        # one block per eigensolver pass (the per-block schedule of the
-        # reference's dsyev loop) and FP64 products, one iteration
-        gpu.load().nrg_gpu_set_batch_limit(1)
-        tp = time.perf_counter()
-        h.iterate(nxt)
-        tp = time.perf_counter() - tp
-        gpu.load().nrg_gpu_set_batch_limit(0)
-        st = gpu.stage_times()
-        plain = {"value": 1.0 / tp, "unit": "iter/s", "iteration": nxt,
-                 "eig_s": st["eig"] * 1e-3, "form": "plain FP64: one block per eigensolver pass, FP64 products"}
-        print(f"plain iteration {nxt}: {tp:.3f} s", file=sys.stderr, flush=True)
     sizes = h.subspace()[:, h.i["nv"]].tolist()
     nfull, nsub, nmax = h.i["nfull"], h.i["nsub"], int(max(sizes))
+    value = a.steps / elapsed
+    ms_step = 1e3 * elapsed / a.steps
+
+    # ---- CPU baseline: the oracle continues from the GPU run's state and
+    # runs the next iteration on the host cores; the first comparison leg
+    # below computes the same iteration on the GPU
+    nxt = its[-1] + 1
+    cb = None
+    cpu_rec = None
+    if not a.no_cpu_baseline:
+        workers = min(16, os.cpu_count() or 1)
+        log(f"CPU baseline: oracle iteration {nxt} on {workers} threads")
+        cb, cpu_rec = cpu_baseline(h, gpu, tmp, nxt, workers)
+        cb["reference_run"] = reference_count_csv(a.workload)
+        log(f"CPU baseline: {cb['seconds_per_iteration']:.1f} s per iteration")
+
+    def leg(split, batch_limit, label):
+        nonlocal nxt
+        gpu.load().nrg_gpu_set_gemm_split(split)
+        gpu.load().nrg_gpu_set_batch_limit(batch_limit)
+        dts, itl = [], []
+        try:
+            for _ in range(3):
+                tb = time.perf_counter()
+                h.iterate(nxt)
+                dts.append(time.perf_counter() - tb)
+                itl.append(nxt)
+                if cpu_rec is not None and nxt == its[-1] + 1:
+                    E = np.sort(h.eigenvalues())
+                    sc = max(1.0, float(E.max()))
+                    n = min(len(E), len(cpu_rec["E"]))
+                    cb["agrees_with_gpu"] = {
+                        "iteration": nxt, "kept_equal": int(h.i["nkept"]) == int(cpu_rec["nkept"]),
+                        "kept_per_block_equal": bool(np.array_equal(h.kept_per_block(), cpu_rec["kept_blk"])),
+                        "max_eigenvalue_diff_rel": float(np.max(np.abs(E[:n] - cpu_rec["E"][:n])) / sc)}
+                nxt += 1
+        finally:
+            gpu.load().nrg_gpu_set_batch_limit(0)
+            gpu.load().nrg_gpu_set_gemm_split(0)
+        log(f"{label} iterations {itl}: {[round(x, 3) for x in dts]} s")
+        return {"value": len(dts) / sum(dts), "unit": "iter/s", "iterations": itl, "seconds": dts}
+
+    plain = batched_split = None
+    if a.plain:
+        # one block per eigensolver pass (the per-block schedule of the
+        # reference's dsyev loop) with FP64 contraction products
+        plain = leg(0, 1, "plain FP64")
+        plain["form"] = "plain FP64: one block per eigensolver pass, FP64 contraction products"
+    if a.split == 0 and a.split_leg > 0:
+        batched_split = leg(a.split_leg, 0, "batched split")
+        batched_split["form"] = (f"batched eigensolve (FP64) + split-precision contraction products "
+                                 f"({a.split_leg} slices)")
     h.close()
 
-    value = a.steps / elapsed
+    # executed work of the timed steps, per step
+    mfma = sum(work[f]["flops"] for f in MFMA_FAMILIES) / a.steps
+    vec = work["tridiag_symv"]["flops"] / a.steps
+    eig_flops = sum(work[f]["flops"] for f in EIG_FAMILIES) / a.steps
+    eig_s = (stages.get("gpu_tridiag", 0) + stages.get("gpu_dc", 0) + stages.get("gpu_back", 0)) / a.steps
+    executed = {
+        "mfma_fp64_flops_per_step": mfma,
+        "vector_fp64_flops_per_step": vec,
+        "mfma_fp64_tflops_over_step": mfma / (ms_step * 1e-3) / 1e12,
+        "pct_fp64_mfma_peak_over_step": 100.0 * mfma / (ms_step * 1e-3) / (FP64_PEAK_TFS * 1e12),
+        "eigensolver_fp64_tflops": eig_flops / max(eig_s, 1e-12) / 1e12,
+        "eigensolver_pct_fp64_peak": 100.0 * eig_flops / max(eig_s, 1e-12) / (FP64_PEAK_TFS * 1e12),
+        "note": "flops actually issued in the timed steps (every launch counted): GEMM products 2mnk "
+                "(lower-triangle SYR2K halved) on FP64 matrix cores, the tridiagonal SYMV's 2m^2 per column "
+                "on the vector units; eigensolver rate = (SYMV + SYR2K + D&C merge + back-transform flops) "
+                "over the tridiagonalisation + D&C + back-transform time",
+        "per_family": {f: {"launches": v["launches"], "flops": v["flops"] / a.steps,
+                           "bytes": v["bytes"] / a.steps} for f, v in work.items() if v["launches"]},
+    }
     out = {
         "metric": METRIC,
         "value": value,
@@ -321,7 +400,7 @@ def main():
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": 1e3 * elapsed / a.steps,
+        "ms_per_step": ms_step,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -337,16 +416,13 @@ def main():
                             f"({a.split} slices on FP16 matrix cores)") if a.split > 0 else
                            "batched eigensolve (FP64) + FP64 contraction GEMMs"},
         "stages_s_per_step": {k: v / a.steps for k, v in stages.items()},
-        "eig_alg_tflops": flops_eig / max(stages.get("gpu_eig", 1e-9), 1e-9) / 1e12,
+        "executed_flops": executed,
+        "pct_mfma_peak": executed["pct_fp64_mfma_peak_over_step"],
         "plain_fp64": plain,
-        "batched_fp64": batched64,
         "batched_split": batched_split,
         "batched_over_plain": (value / plain["value"]) if plain else None,
+        "batched_split_over_plain": (batched_split["value"] / plain["value"]) if (plain and batched_split) else None,
     }
-    if a.spectrum_leg:
-        out["spectrum_pipeline"] = spectrum_pipeline(gpu, models, Host)
-    if a.synthetic:
-        out["synthetic_split_gemm"] = synthetic_gemm(gpu, (8, 6))
     if ks:
         fam = max(ks, key=lambda f: ks[f]["ms"])
         k = ks[fam]
@@ -380,11 +456,16 @@ def main():
                                          if v["launches"]}
     else:
         out["roofline"] = None
-    if not a.no_cpu_baseline:
-        cb = cpu_baseline(sizes, a.cpu_seconds)
-        out["cpu_baseline"] = cb
-    else:
-        out["cpu_baseline"] = None
+    out["cpu_baseline"] = cb
+    if a.synthetic:
+        log("synthetic 256-block batch")
+        out["synthetic_batch"] = synthetic_batch(log)
+    if a.spectrum_leg:
+        if time.perf_counter() - t_start < a.leg_budget:
+            out["workload_nk8000"] = workload_nk8000(gpu, models, Host, log)
+        else:
+            out["workload_nk8000"] = {"skipped": f"over the {a.leg_budget:.0f} s budget for starting this leg"}
+    log("done")
     print(json.dumps(out))
 
 

@@ -101,6 +101,13 @@ int nrg_gpu_diagonalize(int iteration, int nsub, const int32_t* subspace_info, i
  * later call of the library is ordered after it). */
 int nrg_gpu_invariant(const int32_t* kept_per_block, double* invariant_matrix_out);
 
+/* Copy the device-resident invariant matrix of the current kept basis (the
+ * reference's module global invariant_matrix(k,k,numberOfConductionMatrix),
+ * module.F90:809, column-major) to the host: the last nrg_gpu_invariant
+ * result, or the nrg_gpu_set_invariant input before the first iteration.
+ * size = k*k*ncm (checked). */
+int nrg_gpu_invariant_get(double* invariant_matrix, int64_t size);
+
 /* invariantMatrixForSpectrumDGEMM (invariantMatrixForSpectrumDGEMM.F90:1-392). */
 int nrg_gpu_invariant_spectrum(const int32_t* kept_per_block, double* invariant_matrix_spectrum_out);
 
@@ -143,6 +150,21 @@ int nrg_gpu_spectrum_setup(int nbin);
  * scale(iteration) (module.F90:690-711); spikes are binned on the device. */
 int nrg_gpu_spike_cfs(int iteration, double scale);
 int nrg_gpu_spike_zero_temperature(int iteration, double scale);
+
+/* The spectral products behind spikeZeroTemperature's spikes
+ * (calculateSpectrumDGEMM.F90:114-210) after nrg_gpu_stock of the last
+ * iteration, for the step_* spike files the reference writes there
+ * (:80-90,157-159,203-205): for the ground-state block gs_block (1-based,
+ * last iteration) and spectral type `type` (1-based), on the positive
+ * (positive != 0) or negative side, *excited_block receives the 1-based
+ * block whose labels are the ground-state labels plus the type's
+ * positive/negative operation (0 when there is none; then *n = 0), *n its
+ * dimension, and w(k) the product of its state k with column 1 of the
+ * ground-state block (zeros when the two blocks share no operation).  A
+ * ground state's spike of state k has weight w(k)^2 / g (g ground states).
+ * cap >= the excited block's dimension (checked). */
+int nrg_gpu_zero_temperature_weights(int gs_block, int type, int positive, double* w, int cap,
+                                     int* excited_block, int* n);
 
 /* Copy the bin weights: negative and positive, each (nbin*nmat) with bin i
  * of type t at i + t*nbin (the weight column of binnedSpike*, main.F90:146). */
@@ -221,6 +243,13 @@ int nrg_gpu_profile_sample(int every);
  * events (nrg_gpu_profile) else 0} and the record count to *count;
  * nrg_gpu_launch_log_kernel gives the (mangled) name of a kernel id. */
 int nrg_gpu_launch_log(int fam_mask);
+/* Work issued per kernel family (no counterpart in the reference), counted
+ * for every launch whether timed or not: out[3f..3f+2] = {launches,
+ * algorithmic bytes, algorithmic flops} since the last reset, for family
+ * f < nfam (families as for nrg_gpu_kernel_stats; GEMM flops are 2mnk per
+ * product, halved for lower-triangle-only products); reset != 0 clears the
+ * counters after reading. */
+int nrg_gpu_work_counters(double* out, int nfam, int reset);
 int nrg_gpu_launch_log_get(double* out, int64_t cap, int64_t* count);
 int nrg_gpu_launch_log_kernel(int kid, char* name, int cap);
 

@@ -429,6 +429,17 @@ int nrg_gpu_spike_zero_temperature(int iteration, double scale) {
   });
 }
 
+int nrg_gpu_zero_temperature_weights(int gs_block, int type, int positive, double* w, int cap,
+                                     int* excited_block, int* n) {
+  return guarded([&](Context& C) {
+    need_init(C);
+    NRG_REQUIRE(w && excited_block && n, "zero_temperature_weights: null output");
+    int K = -1;
+    zero_temperature_weights(C, gs_block - 1, type - 1, positive ? 1 : 0, w, cap, &K, n);
+    *excited_block = K + 1;
+  });
+}
+
 int nrg_gpu_spectrum_bins(double* negative, double* positive, int64_t size) {
   return guarded([&](Context& C) {
     need_init(C);
@@ -461,6 +472,17 @@ int nrg_gpu_invariant(const int32_t* kept_per_block, double* out) {
     invariant(C, kept_per_block, out);
     NRG_HIP(hipEventRecord(C.ev_inv[1], C.stream));
     C.inv_timed = true;
+  });
+}
+
+int nrg_gpu_invariant_get(double* out, int64_t size) {
+  return guarded([&](Context& C) {
+    need_init(C);
+    const int64_t k = C.kold;
+    NRG_REQUIRE(C.ncm > 0 && k > 0, "invariant_get: no invariant matrix on the device");
+    NRG_REQUIRE(out && size == k * k * C.ncm, "invariant_get: size differs from k*k*numberOfConductionMatrix");
+    C.d_inv.download(out, (size_t)size, C.stream);
+    NRG_HIP(hipStreamSynchronize(C.stream));
   });
 }
 
@@ -625,6 +647,18 @@ int nrg_gpu_profile_sample(int every) {
   return guarded([&](Context&) {
     NRG_REQUIRE(every >= 1, "profile_sample: every must be >= 1");
     profiler().every = (uint32_t)every;
+  });
+}
+
+int nrg_gpu_work_counters(double* out, int nfam, int reset) {
+  return guarded([&](Context&) {
+    Profiler& P = profiler();
+    for (int f = 0; f < nfam && f < KF_COUNT && out; ++f) {
+      out[3 * f] = P.w_launches[f];
+      out[3 * f + 1] = P.w_bytes[f];
+      out[3 * f + 2] = P.w_flops[f];
+    }
+    if (reset) { P.w_launches = {}; P.w_bytes = {}; P.w_flops = {}; }
   });
 }
 

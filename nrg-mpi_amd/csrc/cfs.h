@@ -78,6 +78,13 @@ void rho_init(Context& C, int iteration);
 double rho_shell(Context& C, int iteration);
 // calculateSpectrumDGEMM.F90:1-646 for one iteration (both spike routines)
 void spikes(Context& C, int iteration, double scale, int zero_temperature);
+// Spectral products of spikeZeroTemperature for one ground-state block Gb
+// (0-based, last iteration), type im (0-based) and side (1 positive,
+// 0 negative): W(k) over the states k of the excited block into w (zeros
+// where the two blocks share no operation), its index into *K and its
+// dimension into *n; *K = -1, *n = 0 when no excited block has the shifted
+// labels (calculateSpectrumDGEMM.F90:114-136,166-178).
+void zero_temperature_weights(Context& C, int Gb, int im, int side, double* w, int cap, int* K, int* n);
 void spectrum_setup(Context& C, int nbin);
 
 }  // namespace nrg

@@ -467,6 +467,30 @@ void spectrum_setup(Context& C, int nbin) {
   NRG_HIP(hipStreamSynchronize(C.stream));
 }
 
+void zero_temperature_weights(Context& C, int Gb, int im, int side, double* w, int cap, int* K, int* n) {
+  Cfs& F = cfs();
+  NRG_REQUIRE(!F.hist.empty(), "zero_temperature_weights: nothing stocked");
+  const HistEntry* h = F.hist.back();
+  const int nv = C.nv;
+  NRG_REQUIRE(Gb >= 0 && Gb < h->nsub && im >= 0 && im < C.nmat, "zero_temperature_weights: bad block or type");
+  const std::vector<int32_t>& op = side ? C.pos_op : C.neg_op;
+  std::vector<int32_t> lab(nv);
+  for (int i = 0; i < nv; ++i) lab[i] = h->sub[(size_t)Gb * (nv + 3) + i] + op[im * nv + i];
+  *K = h->find(lab.data(), nv);
+  *n = 0;
+  if (*K < 0) return;
+  const int dim = h->bn[*K];
+  NRG_REQUIRE(cap >= dim, "zero_temperature_weights: output too short for the excited block");
+  *n = dim;
+  for (int k = 0; k < dim; ++k) w[k] = 0.0;
+  for (const SpikeGroup& g : h->groups)
+    if (g.zero_t && g.L == Gb && g.K == *K && g.im == im && g.side == side) {
+      NRG_HIP(hipMemcpyAsync(w, h->W.p + g.woff, sizeof(double) * dim, hipMemcpyDeviceToHost, C.stream));
+      NRG_HIP(hipStreamSynchronize(C.stream));
+      break;
+    }
+}
+
 void spikes(Context& C, int iteration, double scale, int zero_temperature) {
   Cfs& F = cfs();
   HistEntry* h = F.at(iteration);

@@ -65,6 +65,10 @@ struct Profiler {
     return 1;
   }
   std::array<KStat, KF_COUNT> st{};
+  // work issued per family since the last reset, counted for every launch
+  // (timed or not): launches, algorithmic bytes and flops as declared at the
+  // launch sites
+  std::array<double, KF_COUNT> w_launches{}, w_bytes{}, w_flops{};
   struct Pending { hipEvent_t a, b; int fam; double bytes, flops; };
   std::vector<Pending> pend;
   std::vector<hipEvent_t> pool;
@@ -122,6 +126,9 @@ inline void plaunch(int fam, double bytes, double flops, F kernel, const dim3& g
                     uint32_t shm, hipStream_t s, Args... args) {
   Profiler& P = profiler();
   const bool tm = P.timed(fam);
+  P.w_launches[fam] += 1.0;
+  P.w_bytes[fam] += bytes;
+  P.w_flops[fam] += flops;
   if (P.log_mask) {
     int kid = 0;
     const uint64_t k = P.next_nth((const void*)kernel, s, kid);   // counted for every kernel

@@ -255,7 +255,25 @@ contains
     nrg_host_get_spectrum_times = 0
   end function nrg_host_get_spectrum_times
 
+  ! Write the reference's output files into dir (n > 0), or stop writing
+  ! them (n = 0): Xn, Sn, Cn, eigenodd.txt, eigeneven.txt, keepingState.txt
+  ! and error.log per iteration, the CFS-spectrum-* files after the
+  ! spectrum phase.
+  integer(c_int) function nrg_host_set_output(dir, n) bind(C, name="nrg_host_set_output")
+    type(c_ptr), value :: dir
+    integer(c_int), value :: n
+    integer :: ierr
+    ierr = 0
+    if (n > 0) then
+       call open_outputs(c_string(dir, n), ierr)
+    else
+       call close_outputs()
+    end if
+    nrg_host_set_output = ierr
+  end function nrg_host_set_output
+
   integer(c_int) function nrg_host_close() bind(C, name="nrg_host_close")
+    call close_outputs()
     nrg_host_close = nrg_gpu_finalize()
   end function nrg_host_close
 

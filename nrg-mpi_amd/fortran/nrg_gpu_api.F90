@@ -60,6 +60,13 @@ module nrg_gpu_api
        integer(c_int), intent(in) :: kept(*)
        type(c_ptr), value :: out
      end function
+     integer(c_int) function nrg_gpu_zero_temperature_weights(gs_block, type, positive, w, cap, excited, n) &
+          bind(C, name="nrg_gpu_zero_temperature_weights")
+       import :: c_int, c_double
+       integer(c_int), value :: gs_block, type, positive, cap
+       real(c_double), intent(out) :: w(*)
+       integer(c_int), intent(out) :: excited, n
+     end function
      integer(c_int) function nrg_gpu_set_spectrum_operators(nmat, pos_op, neg_op) &
           bind(C, name="nrg_gpu_set_spectrum_operators")
        import :: c_int

@@ -64,6 +64,17 @@ module nrg_host
   double precision, allocatable, target :: binned_pos(:), binned_neg(:)
   double precision, allocatable :: rho_trace(:)   ! Tr rho per shell, indexed by iteration
 
+  ! ---- the reference's output files, written when write_files is on
+  ! (open_outputs): Xn/Sn/Cn (postprocess.F90:60-67), eigenodd.txt /
+  ! eigeneven.txt (:128-137), keepingState.txt (:163-167), error.log, and
+  ! after the spectrum phase CFS-spectrum-{positive,negative}-typeNNN
+  ! (main.F90:228-252) and the zero-temperature spikes step_ 02_itr_NNN.*
+  ! (calculateSpectrumDGEMM.F90:80-90,157-159,203-205).  The records are
+  ! written by the same list-directed statements as the reference's.
+  logical :: write_files = .false.
+  integer :: u_xn = -1, u_sn = -1, u_cn = -1, u_odd = -1, u_even = -1, u_info = -1, u_err = -1
+  character(len=:), allocatable :: out_dir
+
   ! ---- timing (seconds)
   double precision :: t_prepare = 0, t_diag = 0, t_post = 0, t_invariant = 0, t_iter = 0
   double precision :: t_stock = 0, t_rho = 0, t_cfs = 0
@@ -766,6 +777,16 @@ contains
        nb = cutoff
        trunc_margin = minval(abs(es - ecut))
     end if
+    if (write_files) then
+       ! kept energies in ascending order (postprocess.F90:128-137)
+       do i = 1, nb
+          if (mod(it + 1, 2) == 1) then
+             write(u_odd, *) it + 1, real(es(i))
+          else
+             write(u_even, *) it + 1, real(es(i))
+          end if
+       end do
+    end if
     ! kept prefix back to index order (sortByIndex on 1..nb)
     allocate(kidx(nb), order(nfull), kk(nb), wk(nb))
     do i = 1, nb
@@ -784,6 +805,12 @@ contains
        call move_alloc(tmp, basis_out)
     end if
     nkept = nb
+    if (write_files) then
+       ! kept states in index order with their labels (postprocess.F90:163-167)
+       do i = 1, nb
+          write(u_info, *) it + 1, real(eigval(i)), basis_out(1:nv, i)
+       end do
+    end if
   end subroutine truncate
 
   ! kept states per block (invariantMatrixDGEMM.F90:84-100): blocks are
@@ -816,6 +843,11 @@ contains
     double precision :: t0, t1
     ierr = 0
     call calc_thermo()
+    if (write_files) then
+       write(u_xn, *) thermo_x
+       write(u_sn, *) thermo_s
+       write(u_cn, *) thermo_c
+    end if
     call truncate(it)
     call kept_per_block()
     ! stock of the truncated iterations (postprocess.F90:213-374); the
@@ -936,11 +968,120 @@ contains
     allocate(binned_pos(number_of_bin * nmat), binned_neg(number_of_bin * nmat))
     ierr = nrg_gpu_spectrum_bins(binned_neg, binned_pos, int(number_of_bin * nmat, c_int64_t))
     if (ierr /= 0) return
+    if (write_files) call write_spectrum_files()
     if (.not. keep_history) ierr = nrg_gpu_history_clear()
     call cpu_time_wall(t2)
     t_rho = t1 - t0
     t_cfs = t2 - t1
   end subroutine spectrum_phase
+
+  ! =====================================================================
+  ! output files
+  ! =====================================================================
+  subroutine open_outputs(dir, ierr)
+    character(len=*), intent(in) :: dir
+    integer, intent(out) :: ierr
+    call close_outputs()
+    out_dir = dir
+    open(newunit=u_err, file=dir // "/error.log", status="replace", iostat=ierr)
+    if (ierr == 0) open(newunit=u_even, file=dir // "/eigeneven.txt", status="replace", iostat=ierr)
+    if (ierr == 0) open(newunit=u_odd, file=dir // "/eigenodd.txt", status="replace", iostat=ierr)
+    if (ierr == 0) open(newunit=u_xn, file=dir // "/Xn", status="replace", iostat=ierr)
+    if (ierr == 0) open(newunit=u_sn, file=dir // "/Sn", status="replace", iostat=ierr)
+    if (ierr == 0) open(newunit=u_cn, file=dir // "/Cn", status="replace", iostat=ierr)
+    if (ierr == 0) open(newunit=u_info, file=dir // "/keepingState.txt", status="replace", iostat=ierr)
+    write_files = (ierr == 0)
+  end subroutine open_outputs
+
+  subroutine close_outputs()
+    if (u_err >= 0) close(u_err)
+    if (u_even >= 0) close(u_even)
+    if (u_odd >= 0) close(u_odd)
+    if (u_xn >= 0) close(u_xn)
+    if (u_sn >= 0) close(u_sn)
+    if (u_cn >= 0) close(u_cn)
+    if (u_info >= 0) close(u_info)
+    u_err = -1; u_even = -1; u_odd = -1; u_xn = -1; u_sn = -1; u_cn = -1; u_info = -1
+    write_files = .false.
+  end subroutine close_outputs
+
+  ! binned spectra of every type with |weight| > 1e-16 scale(N), frequency
+  ! and weight per record (main.F90:228-252); the frequency grid is the
+  ! one main.F90:145-150 builds, 1000 bins per decade from 1e-15
+  subroutine write_spectrum_files()
+    character(len=64) :: fname
+    integer :: u, imat, i, side
+    double precision :: thr, rec(2)
+    thr = 1.0d-16 * energy_scale(n_iter)
+    do side = 1, 2
+       do imat = 1, nmat
+          if (side == 1) then
+             write(fname, '("CFS-spectrum-positive-type",i3.3)') imat
+          else
+             write(fname, '("CFS-spectrum-negative-type",i3.3)') imat
+          end if
+          open(newunit=u, file=out_dir // "/" // trim(fname), status="replace")
+          do i = 1, number_of_bin
+             if (side == 1) then
+                rec(1) = +10.0d0**(log10_emin + dble(i - 1) / dble(mesh_of_bin))
+                rec(2) = binned_pos(i + (imat - 1) * number_of_bin)
+             else
+                rec(1) = -10.0d0**(log10_emin + dble(i - 1) / dble(mesh_of_bin))
+                rec(2) = binned_neg(i + (imat - 1) * number_of_bin)
+             end if
+             if (abs(rec(2)) > thr) write(u, *) rec(1:2)
+          end do
+          close(u)
+       end do
+    end do
+    call write_step_files()
+  end subroutine write_spectrum_files
+
+  ! zero-temperature spikes of the last iteration, one record per ground
+  ! state and state of the excited block, ground states in index order
+  ! (calculateSpectrumDGEMM.F90:80-90,114-210): frequency E_k scale(N)
+  ! (negated on the negative side) and weight w(k)^2 / g, for |frequency|
+  ! > 1e-16; the file names keep the reference's i3.2 fields
+  subroutine write_step_files()
+    integer :: ip, side, i, b, k, kb, n, g, ierr
+    integer, allocatable :: uf(:, :)
+    double precision, allocatable :: w(:)
+    double precision :: rec(2), sc
+    character(len=64) :: fname
+    allocate(uf(nmat, 2), w(max(1, maxval(subspace(nv + 1, 1:nsub)))))
+    do ip = 1, nmat
+       write(fname, '("step_",i3.2,"_itr_",i3.2,".positive_type",i3.3)') 2, n_iter, ip
+       open(newunit=uf(ip, 1), file=out_dir // "/" // trim(fname), status="replace")
+       write(fname, '("step_",i3.2,"_itr_",i3.2,".negative_type",i3.3)') 2, n_iter, ip
+       open(newunit=uf(ip, 2), file=out_dir // "/" // trim(fname), status="replace")
+    end do
+    g = count(eigval(1:nfull) < degeneracy_threshold)
+    sc = energy_scale(n_iter)
+    b = 1
+    do i = 1, nfull
+       if (eigval(i) >= degeneracy_threshold) cycle
+       do while (i >= subspace(nv + 3, b) + subspace(nv + 1, b))
+          b = b + 1
+       end do
+       do side = 1, 2
+          do ip = 1, nmat
+             ierr = nrg_gpu_zero_temperature_weights(int(b, c_int), int(ip, c_int), &
+                  merge(1_c_int, 0_c_int, side == 1), w, int(size(w), c_int), kb, n)
+             if (ierr /= 0) exit
+             do k = 1, n
+                rec(1) = eigval(subspace(nv + 3, kb) + k - 1) * sc
+                if (side == 2) rec(1) = -rec(1)
+                rec(2) = w(k) * w(k) / dble(g)
+                if (abs(rec(1)) > 1.0d-16) write(uf(ip, side), *) rec(1:2)
+             end do
+          end do
+       end do
+    end do
+    do ip = 1, nmat
+       close(uf(ip, 1))
+       close(uf(ip, 2))
+    end do
+  end subroutine write_step_files
 
   ! =====================================================================
   ! setup: inputs + device

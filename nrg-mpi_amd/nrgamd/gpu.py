@@ -35,6 +35,7 @@ SIGNATURES = {
                                            _f64p, _f64p, _f64p, ctypes.c_double, _f64p, _f64p, ctypes.c_int64]),
     "nrg_gpu_invariant": (ctypes.c_int, [_i32p, _f64p]),
     "nrg_gpu_invariant_spectrum": (ctypes.c_int, [_i32p, _f64p]),
+    "nrg_gpu_invariant_get": (ctypes.c_int, [_f64p, ctypes.c_int64]),
     "nrg_gpu_stage_times": (ctypes.c_int, [_f64p, ctypes.c_int]),
     "nrg_gpu_syev_batched": (ctypes.c_int, [ctypes.c_int, _i32p, _f64p, _f64p, _f64p]),
     "nrg_gpu_dgemm": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
@@ -45,6 +46,7 @@ SIGNATURES = {
     "nrg_gpu_kernel_stats": (ctypes.c_int, [_f64p, ctypes.c_int]),
     "nrg_gpu_profile_sample": (ctypes.c_int, [ctypes.c_int]),
     "nrg_gpu_launch_log": (ctypes.c_int, [ctypes.c_int]),
+    "nrg_gpu_work_counters": (ctypes.c_int, [_f64p, ctypes.c_int, ctypes.c_int]),
     "nrg_gpu_launch_log_get": (ctypes.c_int, [_f64p, ctypes.c_int64, _i64p]),
     "nrg_gpu_launch_log_kernel": (ctypes.c_int, [ctypes.c_int, ctypes.c_char_p, ctypes.c_int]),
     "nrg_gpu_set_spectrum_operators": (ctypes.c_int, [ctypes.c_int, _i32p, _i32p]),
@@ -56,6 +58,8 @@ SIGNATURES = {
     "nrg_gpu_spike_cfs": (ctypes.c_int, [ctypes.c_int, ctypes.c_double]),
     "nrg_gpu_spike_zero_temperature": (ctypes.c_int, [ctypes.c_int, ctypes.c_double]),
     "nrg_gpu_spectrum_bins": (ctypes.c_int, [_f64p, _f64p, ctypes.c_int64]),
+    "nrg_gpu_zero_temperature_weights": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, _f64p, ctypes.c_int,
+                                                        _i32p, _i32p]),
     "nrg_gpu_history_clear": (ctypes.c_int, []),
     "nrg_gpu_set_batch_limit": (ctypes.c_int, [ctypes.c_int]),
     "nrg_gpu_set_tridiag_schedule": (ctypes.c_int, [ctypes.c_int]),
@@ -178,6 +182,14 @@ def syev_batched(blocks):
     return ws, Us
 
 
+def invariant_get(k: int, ncm: int) -> np.ndarray:
+    """The device-resident invariant matrix of the current kept basis as
+    [t, l, r] = invariant_matrix(r, l, t) (k x k per conduction type)."""
+    a = np.zeros(k * k * ncm)
+    _check(load().nrg_gpu_invariant_get(_p(a, _f64p), a.size))
+    return a.reshape(ncm, k, k)
+
+
 def rho_get(iteration: int, kept_per_block):
     """Reduced density matrix of a swept iteration: list of k_b x k_b blocks
     (stocked blocks in block order; row-major as the reference stores it)."""
@@ -230,6 +242,15 @@ def launch_log_kernel(kid):
     buf = ctypes.create_string_buffer(512)
     _check(load().nrg_gpu_launch_log_kernel(int(kid), buf, 512))
     return buf.value.decode()
+
+
+def work_counters(reset=False):
+    """{family: {launches, bytes, flops}} issued since the last reset."""
+    n = len(KERNEL_FAMILIES)
+    a = np.zeros(3 * n)
+    _check(load().nrg_gpu_work_counters(_p(a, _f64p), n, int(reset)))
+    a = a.reshape(n, 3)
+    return {f: dict(launches=int(a[i, 0]), bytes=a[i, 1], flops=a[i, 2]) for i, f in enumerate(KERNEL_FAMILIES)}
 
 
 def kernel_stats():

@@ -35,6 +35,7 @@ SIGNATURES = {
     "nrg_host_get_eigenvectors": (ctypes.c_int, [_f64p, ctypes.c_int64]),
     "nrg_host_get_times": (ctypes.c_int, [_f64p]),
     "nrg_host_close": (ctypes.c_int, []),
+    "nrg_host_set_output": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int]),
     "nrg_host_spectrum": (ctypes.c_int, [ctypes.c_int]),
     "nrg_host_get_bins": (ctypes.c_int, [_f64p, _f64p, ctypes.c_int]),
     "nrg_host_get_rho_traces": (ctypes.c_int, [_f64p, ctypes.c_int]),
@@ -201,6 +202,14 @@ class Host:
         t = np.zeros(3)
         self.lib.nrg_host_get_spectrum_times(t.ctypes.data_as(_f64p))
         return dict(zip(["stock", "rho", "cfs"], t.tolist()))
+
+    def set_output(self, directory: Optional[str]):
+        """Write the reference's output files (Xn, Sn, Cn, eigenodd.txt,
+        eigeneven.txt, keepingState.txt, error.log, and after the spectrum
+        phase CFS-spectrum-* and the zero-temperature step_* files) into
+        `directory` from now on; None stops writing them."""
+        d = (directory or "").encode()
+        _check(self.lib.nrg_host_set_output(d, len(d)), "nrg_host_set_output")
 
     def close(self):
         return self.lib.nrg_host_close()

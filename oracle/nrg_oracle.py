@@ -318,10 +318,14 @@ class IterationRecord:
 class NRG:
     """Forward NRG + CFS in the reference's semantics."""
 
-    def __init__(self, model: Model, keep_history: Optional[bool] = None, eigh=None):
+    def __init__(self, model: Model, keep_history: Optional[bool] = None, eigh=None, workers: int = 1):
         self.m = model
         self.keep = model.spectrum if keep_history is None else keep_history
         self.eigh = eigh or np.linalg.eigh
+        # blocks solved by a pool of `workers` threads, one single-threaded
+        # LAPACK call each (the reference's OpenMP loop over blocks); the
+        # results do not depend on the count
+        self.workers = max(1, int(workers))
         self.basis_output = model.basis_output0.copy()
         self.eigenvalue = model.eigenvalue0.copy()
         self.inv = model.invariant0.copy()
@@ -441,11 +445,21 @@ class NRG:
         nsub = len(self.subspace)
         nfull = len(self.basis_input)
         E = np.zeros(nfull)
+
+        def solve(b):
+            w, U = self.eigh(self.hamiltonian_block(b, it))
+            return w, np.where(np.abs(U) < EIGENVECTOR_THRESHOLD, 0.0, U)
+        if self.workers > 1:
+            from concurrent.futures import ThreadPoolExecutor
+            from threadpoolctl import threadpool_limits
+            order = sorted(range(nsub), key=lambda b: -int(self.subspace[b, nv]))   # largest first
+            with threadpool_limits(limits=1), ThreadPoolExecutor(self.workers) as ex:
+                done = dict(zip(order, ex.map(solve, order)))
+            res = [done[b] for b in range(nsub)]
+        else:
+            res = [solve(b) for b in range(nsub)]
         Us = []
-        for b in range(nsub):
-            H = self.hamiltonian_block(b, it)
-            w, U = self.eigh(H)
-            U = np.where(np.abs(U) < EIGENVECTOR_THRESHOLD, 0.0, U)
+        for b, (w, U) in enumerate(res):
             s0 = int(self.subspace[b, nv + 2]) - 1
             E[s0:s0 + len(w)] = w
             Us.append(U)
@@ -650,6 +664,17 @@ class NRG:
         if self.m.spectrum and it >= self.start_trunc:
             self.stock(it)
         return rec
+
+    def seed(self, basis_output, eigenvalue, invariant, nkept: int, start_trunc: int):
+        """Continue a run from the state another implementation reached:
+        the kept basis records (first nkept rows of basis_output), their
+        energies and the invariant matrix [t, l, r] of the kept basis."""
+        self.basis_output = np.asarray(basis_output, dtype=np.int64).copy()
+        self.eigenvalue = np.asarray(eigenvalue, dtype=np.float64).copy()
+        self.inv = np.asarray(invariant, dtype=np.float64)
+        self.nold = int(nkept)
+        self.start_trunc = int(start_trunc)
+        self.flag_trunc = self.start_trunc <= self.m.N
 
     def run_forward(self, upto: Optional[int] = None):
         last = self.m.N if upto is None else upto

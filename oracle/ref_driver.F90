@@ -105,6 +105,7 @@ program ref_driver
      end do
      call stopCount
      call dump_spectrum()
+     call write_spectrum_files()
   end if
 
   call stopCount
@@ -148,6 +149,32 @@ contains
     end if
     close(u)
   end subroutine dump_iteration
+
+  ! the binned-spectrum files main.F90:228-252 writes: per type and side,
+  ! the (frequency, weight) records of the bins above 1e-16 scale(N)
+  subroutine write_spectrum_files()
+    character(64) :: fname
+    integer :: u, imat, i
+    if (my_rank /= 0) return
+    do imat = 1, hami%numberOfMatrix
+       write(fname, '("CFS-spectrum-positive-type",i3.3)') imat
+       open(newunit=u, file=trim(fname), status='replace')
+       do i = 1, numberOfBin
+          if (abs(binnedSpikePositive(i+(imat-1)*numberOfBin, 2)) > 1.0d-16*scale(hami%numberOfIteration)) &
+               write(u, *) binnedSpikePositive(i+(imat-1)*numberOfBin, 1:2)
+       end do
+       close(u)
+    end do
+    do imat = 1, hami%numberOfMatrix
+       write(fname, '("CFS-spectrum-negative-type",i3.3)') imat
+       open(newunit=u, file=trim(fname), status='replace')
+       do i = 1, numberOfBin
+          if (abs(binnedSpikeNegative(i+(imat-1)*numberOfBin, 2)) > 1.0d-16*scale(hami%numberOfIteration)) &
+               write(u, *) binnedSpikeNegative(i+(imat-1)*numberOfBin, 1:2)
+       end do
+       close(u)
+    end do
+  end subroutine write_spectrum_files
 
   subroutine dump_spectrum()
     integer :: u, nv

@@ -40,6 +40,11 @@ RUNS = {
     # iterations -1..29 are exactly those of the complete run, including the
     # truncating iterations 4..29 that the benchmark times
     "two_orbital_nk4000_it45": (dict(), -100, 7, 29),
+    # the Nkeep=8000 deck's forward pass over its first iterations (-1..2,
+    # blocks up to 8620): the reference cannot run this workload with
+    # spectra, and a steady-state iteration costs it ~10 minutes on 4 ranks
+    # (4 ranks: each holds the full eigenvector array, ~5 GB here)
+    "two_orbital_nk8000_it45": (dict(spectrum=False), -100, 4, 2),
     # two-orbital model with spectra at a size the reference finishes in
     # seconds: four spectral types, spin-triplet ground state
     "two_orbital_nk300_it20_spec": (dict(nkeep=300, niter=20, spectrum=True), -100, 8),
@@ -83,6 +88,12 @@ def fixture(name, workdir, wall, nrank):
     out["nfull"] = np.array([d["nfull"] for d in its])
     out["nkept"] = np.array([d["nkept"] for d in its])
     out["start_trunc"] = np.array(its[-1]["start_trunc"])
+    # wall seconds of each iteration on the reference run: rank 0 writes the
+    # dump of an iteration as soon as the iteration ends
+    import glob
+    mt = [os.path.getmtime(p) for p in sorted(glob.glob(os.path.join(workdir, "dump_it*.bin")))]
+    st = os.path.getmtime(os.path.join(workdir, "initial.dat"))
+    out["iteration_wall_s"] = np.diff(np.array([st] + mt))
     for d in its:
         k = f"it{d['it']}"
         out[k + "_subspace"] = d["subspace"].astype(np.int32)
@@ -130,6 +141,16 @@ def fixture(name, workdir, wall, nrank):
             rho_eigs.append(np.linalg.eigvalsh(0.5 * (blk + blk.T)))
         out["rho_block_trace"] = np.array(rho_tr)
         out["rho_block_eigs"] = np.concatenate(rho_eigs)
+    if small or spec is not None:
+        # the reference's own output files, verbatim (text)
+        import glob
+        names = ["Xn", "Sn", "Cn", "eigenodd.txt", "eigeneven.txt", "keepingState.txt"]
+        names += sorted(os.path.basename(p) for p in glob.glob(os.path.join(workdir, "CFS-spectrum-*")))
+        names += sorted(os.path.basename(p) for p in glob.glob(os.path.join(workdir, "step_*")))
+        out["output_files"] = np.array(names)
+        for nm in names:
+            with open(os.path.join(workdir, nm)) as f:
+                out["file:" + nm] = np.array(f.read())
     csv = os.path.join(workdir, "count.csv")
     if os.path.exists(csv):
         out["count_csv"] = np.array(open(csv).read())

@@ -235,3 +235,15 @@ def test_input_feature_decks_reject_planted_mistakes(tmp_path, monkeypatch):
     nrg.run_forward(None)
     with pytest.raises(AssertionError):
         compare_forward(nrg, fx)
+
+
+@pytest.mark.parametrize("name", ["siam_nk256_it40", "siam_bcs_nk128_it12"])
+def test_thermodynamics_against_reference_files(tmp_path, name):
+    """calcThermo (module.F90:1931-1982) of every iteration against the
+    Xn, Sn, Cn files the reference run wrote (postprocess.F90:60-67)."""
+    fx = np.load(os.path.join(GOLD, f"ref_{name}.npz"))
+    nrg = run_oracle(tmp_path, name)
+    ref = np.array([[float(x) for x in str(fx["file:" + f]).split()] for f in ("Xn", "Sn", "Cn")]).T
+    got = np.array(nrg.thermo)
+    assert got.shape == ref.shape
+    np.testing.assert_allclose(got, ref, rtol=1e-10, atol=1e-13)
