@@ -170,6 +170,21 @@ int nrg_gpu_zero_temperature_weights(int gs_block, int type, int positive, doubl
  * of type t at i + t*nbin (the weight column of binnedSpike*, main.F90:146). */
 int nrg_gpu_spectrum_bins(double* negative, double* positive, int64_t size);
 
+/* Restart file of the CFS phase (saveRestart / loadRestart, module.F90:
+ * 1723-1890, used at main.F90:165-221).  Save before the CFS step of
+ * `iteration`: the bins so far, the device-resident history of the
+ * iterations >= `iteration` (kept columns, spectral products, reduced
+ * density) and nstate integers of the caller's own state, into one stream
+ * file led by the reference's mode word MODE_RESTART_CFS (1002) and the
+ * iteration; the file is replaced only once complete.  Load restores the
+ * history and bins after nrg_gpu_init / set_model / spectrum set-up and
+ * returns the iteration at which the CFS steps resume and the caller's
+ * state; the forward pass and the reduced-density sweep are not needed.
+ * Errors: a missing file, a mode word other than 1002 ("broken restart
+ * file"), another model or a different nstate. */
+int nrg_gpu_save_restart(const char* path, int iteration, int nstate, const int32_t* state);
+int nrg_gpu_load_restart(const char* path, int* iteration, int nstate, int32_t* state);
+
 /* Free the stocked history (no counterpart; the reference frees at exit). */
 int nrg_gpu_history_clear(void);
 

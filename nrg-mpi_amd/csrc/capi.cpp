@@ -429,6 +429,21 @@ int nrg_gpu_spike_zero_temperature(int iteration, double scale) {
   });
 }
 
+int nrg_gpu_save_restart(const char* path, int iteration, int nstate, const int32_t* state) {
+  return guarded([&](Context& C) {
+    need_init(C);
+    save_restart(C, path, iteration, nstate, state);
+  });
+}
+
+int nrg_gpu_load_restart(const char* path, int* iteration, int nstate, int32_t* state) {
+  return guarded([&](Context& C) {
+    need_init(C);
+    NRG_REQUIRE(iteration != nullptr, "load_restart: null iteration");
+    load_restart(C, path, iteration, nstate, state);
+  });
+}
+
 int nrg_gpu_zero_temperature_weights(int gs_block, int type, int positive, double* w, int cap,
                                      int* excited_block, int* n) {
   return guarded([&](Context& C) {

@@ -85,6 +85,10 @@ void spikes(Context& C, int iteration, double scale, int zero_temperature);
 // dimension into *n; *K = -1, *n = 0 when no excited block has the shifted
 // labels (calculateSpectrumDGEMM.F90:114-136,166-178).
 void zero_temperature_weights(Context& C, int Gb, int im, int side, double* w, int cap, int* K, int* n);
+// Restart file of the CFS phase (restart.cpp): the bins, the history entries
+// from `iteration` on and an opaque host state of nstate integers.
+void save_restart(Context& C, const char* path, int iteration, int nstate, const int32_t* state);
+void load_restart(Context& C, const char* path, int* iteration, int nstate, int32_t* state);
 void spectrum_setup(Context& C, int nbin);
 
 }  // namespace nrg

@@ -272,6 +272,23 @@ contains
     nrg_host_set_output = ierr
   end function nrg_host_set_output
 
+  ! Restart of the CFS phase: save (before every CFS step) and load
+  ! (resume at the saved step, skipping the forward pass and the reduced-
+  ! density sweep), overriding saveRestart / loadRestart of parameter.txt;
+  ! a file name of length n > 0 replaces restart.dat beside the inputs;
+  ! stop_at >= 0 ends the spectrum phase right after the save at that
+  ! iteration (an interrupted run).
+  integer(c_int) function nrg_host_set_restart(save, load, stop_at, path, n) bind(C, name="nrg_host_set_restart")
+    integer(c_int), value :: save, load, stop_at, n
+    type(c_ptr), value :: path
+    save_restart = (save /= 0)
+    load_restart = (load /= 0)
+    restart_stop = huge(restart_stop)
+    if (stop_at >= 0) restart_stop = stop_at
+    if (n > 0) restart_path = c_string(path, n)
+    nrg_host_set_restart = 0
+  end function nrg_host_set_restart
+
   integer(c_int) function nrg_host_close() bind(C, name="nrg_host_close")
     call close_outputs()
     nrg_host_close = nrg_gpu_finalize()

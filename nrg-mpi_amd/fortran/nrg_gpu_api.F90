@@ -60,6 +60,21 @@ module nrg_gpu_api
        integer(c_int), intent(in) :: kept(*)
        type(c_ptr), value :: out
      end function
+     integer(c_int) function nrg_gpu_save_restart(path, iteration, nstate, state) &
+          bind(C, name="nrg_gpu_save_restart")
+       import :: c_int, c_char
+       character(kind=c_char), intent(in) :: path(*)
+       integer(c_int), value :: iteration, nstate
+       integer(c_int), intent(in) :: state(*)
+     end function
+     integer(c_int) function nrg_gpu_load_restart(path, iteration, nstate, state) &
+          bind(C, name="nrg_gpu_load_restart")
+       import :: c_int, c_char
+       character(kind=c_char), intent(in) :: path(*)
+       integer(c_int), intent(out) :: iteration
+       integer(c_int), value :: nstate
+       integer(c_int), intent(out) :: state(*)
+     end function
      integer(c_int) function nrg_gpu_zero_temperature_weights(gs_block, type, positive, w, cap, excited, n) &
           bind(C, name="nrg_gpu_zero_temperature_weights")
        import :: c_int, c_double

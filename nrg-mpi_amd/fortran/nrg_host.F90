@@ -72,7 +72,16 @@ module nrg_host
   ! (calculateSpectrumDGEMM.F90:80-90,157-159,203-205).  The records are
   ! written by the same list-directed statements as the reference's.
   logical :: write_files = .false.
-  integer :: u_xn = -1, u_sn = -1, u_cn = -1, u_odd = -1, u_even = -1, u_info = -1, u_err = -1
+  ! ---- restart of the CFS phase (main.F90:165-221): loadRestart /
+  ! saveRestart of parameter.txt; the file sits beside the inputs (the
+  ! reference writes restart.dat in its working directory); restart_stop
+  ! ends the spectrum phase right after the save at that iteration (an
+  ! interrupted run, for tests)
+  logical :: load_restart = .false., save_restart = .false.
+  character(len=:), allocatable :: restart_path
+  integer :: restart_stop
+  ! open units (newunit= numbers are negative; 0 marks a closed file)
+  integer :: u_xn = 0, u_sn = 0, u_cn = 0, u_odd = 0, u_even = 0, u_info = 0, u_err = 0
   character(len=:), allocatable :: out_dir
 
   ! ---- timing (seconds)
@@ -117,10 +126,11 @@ contains
   end subroutine read_hamiltonian_nml
 
   ! namelist /nrg/ (module.F90:665-675)
-  subroutine read_parameter_nml(path, ierr, o_band, o_z, o_lambda, o_scaling, o_nkeep)
+  subroutine read_parameter_nml(path, ierr, o_band, o_z, o_lambda, o_scaling, o_nkeep, o_load, o_save)
     character(len=*), intent(in) :: path
     integer, intent(out) :: ierr, o_scaling, o_nkeep
     double precision, intent(out) :: o_band, o_z, o_lambda
+    logical, intent(out) :: o_load, o_save
     double precision :: bandWidth, ztrick, Lambda
     integer :: scalingType, truncation, u
     logical :: cfsnrg, noimpurity, loadRestart, saveRestart
@@ -132,6 +142,7 @@ contains
     close(u)
     if (ierr /= 0) return
     o_band = bandWidth; o_z = ztrick; o_lambda = Lambda; o_scaling = scalingType; o_nkeep = truncation
+    o_load = loadRestart; o_save = saveRestart
   end subroutine read_parameter_nml
 
   ! =====================================================================
@@ -148,7 +159,10 @@ contains
     call read_hamiltonian_nml(dir // "/hamiltonianInfo.dat", ierr, n_iter, nchain, nv, cons_block, nmat, ncm, &
          nb0, ncoef, nops, i_first, flag_spectrum, bcs, nbcs)
     if (ierr /= 0) return
-    call read_parameter_nml(dir // "/parameter.txt", ierr, band_width, ztrick, lambda, scaling_type, nkeep)
+    call read_parameter_nml(dir // "/parameter.txt", ierr, band_width, ztrick, lambda, scaling_type, nkeep, &
+         load_restart, save_restart)
+    restart_path = dir // "/restart.dat"
+    restart_stop = huge(restart_stop)
     if (ierr /= 0) return
     flag_bcs = bcs
     if (.not. flag_bcs) nbcs = 0
@@ -948,19 +962,44 @@ contains
   subroutine spectrum_phase(ierr, keep_history)
     integer, intent(out) :: ierr
     logical, intent(in) :: keep_history
-    integer :: it
+    integer :: it, it_restart
+    integer(c_int) :: st(2), itc
     double precision :: t0, t1, t2
     ierr = 0
     if (.not. flag_spectrum) return
     call cpu_time_wall(t0)
-    if (flag_trunc) then
-       call initialize_reduced_density(ierr)
+    if (load_restart) then
+       ! resume: the history, reduced density and bins come from the file
+       ! (the NRG loop and the sweep are skipped, main.F90:171-199)
+       ierr = nrg_gpu_load_restart(restart_path // c_null_char, itc, 2_c_int, st)
        if (ierr /= 0) return
+       it_restart = itc
+       start_trunc = st(1)
+       flag_trunc = (st(2) /= 0)
+       if (allocated(rho_trace)) deallocate(rho_trace)
+       allocate(rho_trace(i_first:n_iter))
+       rho_trace = 0.0d0   ! the traces of the swept shells are not in the file
+    else
+       if (flag_trunc) then
+          call initialize_reduced_density(ierr)
+          if (ierr /= 0) return
+       end if
+       call reduced_density_matrix_calculation(ierr)
+       if (ierr /= 0) return
+       it_restart = start_trunc
     end if
-    call reduced_density_matrix_calculation(ierr)
-    if (ierr /= 0) return
     call cpu_time_wall(t1)
+    st(1) = start_trunc
+    st(2) = merge(1, 0, flag_trunc)
     do it = start_trunc, n_iter
+       if (load_restart .and. it < it_restart) cycle
+       ! the file is rewritten before every step, after the restart step
+       ! when resuming (main.F90:204-218)
+       if (save_restart .and. (.not. load_restart .or. it > it_restart)) then
+          ierr = nrg_gpu_save_restart(restart_path // c_null_char, int(it, c_int), 2_c_int, st)
+          if (ierr /= 0) return
+          if (it == restart_stop) return
+       end if
        call cfs_spectrum_calculation(it, ierr)
        if (ierr /= 0) return
     end do
@@ -994,14 +1033,14 @@ contains
   end subroutine open_outputs
 
   subroutine close_outputs()
-    if (u_err >= 0) close(u_err)
-    if (u_even >= 0) close(u_even)
-    if (u_odd >= 0) close(u_odd)
-    if (u_xn >= 0) close(u_xn)
-    if (u_sn >= 0) close(u_sn)
-    if (u_cn >= 0) close(u_cn)
-    if (u_info >= 0) close(u_info)
-    u_err = -1; u_even = -1; u_odd = -1; u_xn = -1; u_sn = -1; u_cn = -1; u_info = -1
+    if (u_err /= 0) close(u_err)
+    if (u_even /= 0) close(u_even)
+    if (u_odd /= 0) close(u_odd)
+    if (u_xn /= 0) close(u_xn)
+    if (u_sn /= 0) close(u_sn)
+    if (u_cn /= 0) close(u_cn)
+    if (u_info /= 0) close(u_info)
+    u_err = 0; u_even = 0; u_odd = 0; u_xn = 0; u_sn = 0; u_cn = 0; u_info = 0
     write_files = .false.
   end subroutine close_outputs
 
@@ -1034,7 +1073,9 @@ contains
           close(u)
        end do
     end do
-    call write_step_files()
+    ! the spike files need the last iteration's block table and energies,
+    ! which a run resumed from a restart file does not have on the host
+    if (nsub > 0 .and. cur_iteration == n_iter) call write_step_files()
   end subroutine write_spectrum_files
 
   ! zero-temperature spikes of the last iteration, one record per ground

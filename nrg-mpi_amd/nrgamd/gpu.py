@@ -61,6 +61,8 @@ SIGNATURES = {
     "nrg_gpu_zero_temperature_weights": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, _f64p, ctypes.c_int,
                                                         _i32p, _i32p]),
     "nrg_gpu_history_clear": (ctypes.c_int, []),
+    "nrg_gpu_save_restart": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, _i32p]),
+    "nrg_gpu_load_restart": (ctypes.c_int, [ctypes.c_char_p, _i32p, ctypes.c_int, _i32p]),
     "nrg_gpu_set_batch_limit": (ctypes.c_int, [ctypes.c_int]),
     "nrg_gpu_set_tridiag_schedule": (ctypes.c_int, [ctypes.c_int]),
     "nrg_gpu_set_gemm_split": (ctypes.c_int, [ctypes.c_int]),

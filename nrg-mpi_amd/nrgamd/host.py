@@ -36,6 +36,8 @@ SIGNATURES = {
     "nrg_host_get_times": (ctypes.c_int, [_f64p]),
     "nrg_host_close": (ctypes.c_int, []),
     "nrg_host_set_output": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int]),
+    "nrg_host_set_restart": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_char_p,
+                                            ctypes.c_int]),
     "nrg_host_spectrum": (ctypes.c_int, [ctypes.c_int]),
     "nrg_host_get_bins": (ctypes.c_int, [_f64p, _f64p, ctypes.c_int]),
     "nrg_host_get_rho_traces": (ctypes.c_int, [_f64p, ctypes.c_int]),
@@ -210,6 +212,15 @@ class Host:
         `directory` from now on; None stops writing them."""
         d = (directory or "").encode()
         _check(self.lib.nrg_host_set_output(d, len(d)), "nrg_host_set_output")
+
+    def set_restart(self, save: bool = False, load: bool = False, stop_at: Optional[int] = None,
+                    path: Optional[str] = None):
+        """Restart of the CFS phase (saveRestart / loadRestart): save the
+        file before every CFS step, or resume from it; stop_at ends the
+        spectrum phase right after the save at that iteration."""
+        p = (path or "").encode()
+        _check(self.lib.nrg_host_set_restart(int(save), int(load), -1 if stop_at is None else int(stop_at), p,
+                                             len(p)), "nrg_host_set_restart")
 
     def close(self):
         return self.lib.nrg_host_close()
