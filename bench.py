@@ -178,7 +178,8 @@ def synthetic_batch(log):
     planted ladder; U^T O U in FP64 and in split precision (8 and 6 slices),
     device-timed (mean of one repeat), errors against numpy (FP64, sampled
     blocks) and against the FP64 products (split, every block)."""
-    from nrgamd import synthetic as S
+    from nrgamd import gpu, synthetic as S
+    gpu.init(0)
     t0 = time.perf_counter()
     b = S.make_batch(seed=20240, nblk=256, median=600.0, sigma=1.0)
     gen = time.perf_counter() - t0
