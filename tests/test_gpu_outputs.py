@@ -53,7 +53,8 @@ def compare_records(name, mine, ref, rtol_real):
 
 def compare_step_file(name, mine, ref):
     """Zero-temperature spike records: same records in the same order,
-    frequencies to 1e-12 relative; weights summed over each run of equal
+    frequencies to 1e-12 of the file's largest (the eigenvalue parity is
+    relative to the largest eigenvalue); weights summed over each run of equal
     frequencies (a degenerate multiplet of the excited block, whose
     individual eigenvectors are any rotation of each other, so only the
     multiplet's total is solver independent) to 1e-10 of the file's largest."""
@@ -62,7 +63,7 @@ def compare_step_file(name, mine, ref):
     assert a.shape == b.shape, (name, a.shape, b.shape)
     if not len(b):
         return 0.0
-    np.testing.assert_allclose(a[:, 0], b[:, 0], rtol=1e-12, atol=0)
+    np.testing.assert_allclose(a[:, 0], b[:, 0], rtol=0, atol=1e-12 * np.abs(b[:, 0]).max())
     brk = np.ones(len(b), dtype=bool)
     brk[1:] = np.abs(np.diff(b[:, 0])) > 1e-9 * np.abs(b[1:, 0])
     run = np.cumsum(brk) - 1
