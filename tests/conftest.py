@@ -17,8 +17,10 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "slow: long CPU test")
 
 
-@pytest.fixture(scope="session")
+@pytest.fixture
 def gpu():
+    # per test: a test that closes a Fortran host finalises the library, and
+    # init is a no-op on an initialised library
     from nrgamd import gpu as g
     g.init(0)
     yield g

@@ -17,7 +17,7 @@ from nrgamd import models
 pytestmark = pytest.mark.gpu
 
 
-@pytest.fixture(scope="module")
+@pytest.fixture
 def gpu():
     from nrgamd import gpu as g
     g.init(0)
