@@ -229,6 +229,21 @@ int nrg_gpu_dgemm_batched(int ntask, const int32_t* mnk, int transa, int transb,
                           const int64_t* offA, const double* B, const int64_t* offB, double* C,
                           const int64_t* offC, int slices, int repeat, double* ms);
 
+/* No counterpart in the reference: the FP64 form of nrg_gpu_dgemm_batched
+ * for callers that cannot allocate device memory (code under stream capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  A, B and C are
+ * device arrays owned by the caller (same task layout and offsets as
+ * nrg_gpu_dgemm_batched, C_i = op(A_i) op(B_i)); the task tables and split-K
+ * partial products go to a statically allocated per-device scratch array of
+ * 128 MiB, so the call allocates nothing.  At most 4096 products per call,
+ * and their tables and partials must fit the scratch (both checked, status
+ * 2 with a message).  Returns after the products are done.  Every call on a
+ * device shares that device's scratch: calls that overlap on one device
+ * race. */
+int nrg_gpu_dgemm_batched_noalloc(int ntask, const int32_t* mnk, int transa, int transb, const double* A,
+                                  const int64_t* offA, const double* B, const int64_t* offB, double* C,
+                                  const int64_t* offC);
+
 /* Per-kernel-family HIP-event timing (no counterpart in the reference; the
  * reference's count.f90 timers measure host wall clock only).  enable = 0
  * stops, -1 times every family, any other value is a bitmask of the

@@ -1,6 +1,7 @@
 // extern "C" entry points of libnrg_gpu.so (declared in include/nrg_gpu.h).
 #include "../../include/nrg_gpu.h"
 #include "assemble.h"
+#include "noalloc.h"
 #include "cfs.h"
 #include "context.h"
 #include <algorithm>
@@ -87,6 +88,7 @@ int nrg_gpu_init(int device) {
     NRG_REQUIRE(device >= 0 && device < nd, "nrg_gpu_init: no such device");
     NRG_HIP(hipSetDevice(device));
     C.device = device;
+    scratch_setup(C);
     NRG_HIP(hipStreamCreateWithFlags(&C.stream, hipStreamNonBlocking));
     NRG_HIP(hipEventCreate(&C.ev0));
     NRG_HIP(hipEventCreate(&C.ev1));
@@ -572,6 +574,15 @@ int nrg_gpu_dgemm(int transa, int transb, int m, int n, int k, double alpha, con
     dC.download(C_, nc, C.stream);
     NRG_HIP(hipStreamSynchronize(C.stream));
     dA.release(); dB.release(); dC.release();
+  });
+}
+
+int nrg_gpu_dgemm_batched_noalloc(int ntask, const int32_t* mnk, int transa, int transb, const double* A,
+                                  const int64_t* offA, const double* B, const int64_t* offB, double* C_,
+                                  const int64_t* offC) {
+  return guarded([&](Context& C) {
+    need_init(C);
+    dgemm_batched_noalloc(C, ntask, mnk, transa, transb, A, offA, B, offB, C_, offC);
   });
 }
 

@@ -29,6 +29,9 @@ struct Context {
   // pairs, read when the stage times are asked for
   hipEvent_t ev_back[2] = {nullptr, nullptr}, ev_inv[2] = {nullptr, nullptr};
   bool back_timed = false, inv_timed = false;
+  // this device's copy of the no-allocation entry point's scratch array
+  // (hipGetSymbolAddress at init)
+  char* scratch = nullptr;
 
   // ---- model constants (set once)
   int nv = 0, nops = 0, nchain = 0, ncm = 0, ncoef = 0, nmat = 0;

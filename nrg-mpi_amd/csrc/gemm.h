@@ -122,6 +122,19 @@ struct GemmRunner {
   void release() {
     d_pack[0].release(); d_pack[1].release(); d_red_.release(); d_splitw_.release(); r_dev_.release();
   }
+  // Split of a batch into plain tasks and split-K parts (plain_, part_,
+  // h_red_); returns the partial-product doubles the parts need.
+  int64_t decompose(const GemmBatch& b);
+  // run() (FP64 form) with every device-side table and the split-K partials
+  // in a caller-provided device workspace: the same decomposition and
+  // launches, no device allocation.  workspace_bytes(b) is what it needs.
+  size_t workspace_bytes(const GemmBatch& b);
+  void run_into(const GemmBatch& b, const double* A, const double* B, double* C, char* ws, size_t ws_bytes,
+                hipStream_t s, int family = KF_OTHER);
+  char* ws_ = nullptr;           // set during run_into: tables go here instead of the DevBufs
+  size_t ws_cap_ = 0, ws_used_ = 0;
+  hipStream_t ws_stream_ = nullptr;
+  char* ws_stage(const void* p, size_t n);
 };
 
 inline GemmTask gemm_task(int m, int n, int k, int transA, int transB,

@@ -334,19 +334,7 @@ __global__ void k_splitk_reduce(const SplitRed* __restrict__ recs, const double*
   }
 }
 
-void GemmRunner::run(const GemmBatch& b, const double* A, const double* B, double* C,
-                     const int32_t* gatherA, const int32_t* gatherB, const double* kscale,
-                     hipStream_t s, int family) {
-  const int nt = (int)b.tasks.size();
-  if (nt == 0) return;
-  if (split > 0 && (family == KF_GEMM_INV || family == KF_GEMM_SPEC || family == KF_GEMM_CFSW ||
-                    family == KF_GEMM_CFSX || family == KF_GEMM_RHO)) {
-    run_split(b, A, B, C, gatherA, gatherB, kscale, split, s, family);
-    return;
-  }
-  // Products with a long K and few output tiles (the back-transform's
-  // V^T V and V^T Z) are split along K into partial products written to a
-  // workspace and summed in a fixed order, so that they fill the device.
+int64_t GemmRunner::decompose(const GemmBatch& b) {
   plain_.clear();
   part_.clear();
   h_red_.clear();
@@ -381,6 +369,23 @@ void GemmRunner::run(const GemmBatch& b, const double* A, const double* B, doubl
     }
     wsz += (int64_t)parts * t.m * t.n;
   }
+  return wsz;
+}
+
+void GemmRunner::run(const GemmBatch& b, const double* A, const double* B, double* C,
+                     const int32_t* gatherA, const int32_t* gatherB, const double* kscale,
+                     hipStream_t s, int family) {
+  const int nt = (int)b.tasks.size();
+  if (nt == 0) return;
+  if (split > 0 && (family == KF_GEMM_INV || family == KF_GEMM_SPEC || family == KF_GEMM_CFSW ||
+                    family == KF_GEMM_CFSX || family == KF_GEMM_RHO)) {
+    run_split(b, A, B, C, gatherA, gatherB, kscale, split, s, family);
+    return;
+  }
+  // Products with a long K and few output tiles (the back-transform's
+  // V^T V and V^T Z) are split along K into partial products written to a
+  // workspace and summed in a fixed order, so that they fill the device.
+  const int64_t wsz = decompose(b);
   if (!plain_.tasks.empty()) launch(plain_, A, B, C, gatherA, gatherB, kscale, s, family, 0);
   if (!part_.tasks.empty()) {
     int64_t mx = 0;
@@ -444,7 +449,14 @@ void GemmRunner::launch(const GemmBatch& b, const double* A, const double* B, do
   }
   const GemmTask* tk;
   const int32_t* ts;
-  upload_tables(b, s, tk, ts, slot);
+  if (ws_) {
+    // tables in the caller's workspace (run_into)
+    const size_t tb = (size_t)nt * sizeof(GemmTask), sb = (size_t)nt * sizeof(int32_t);
+    tk = reinterpret_cast<const GemmTask*>(ws_stage(b.tasks.data(), tb));
+    ts = reinterpret_cast<const int32_t*>(ws_stage(h_tile_start.data(), sb));
+  } else {
+    upload_tables(b, s, tk, ts, slot);
+  }
   const dim3 g((unsigned)total), blk(256);
   if (fast) {
     if (tv == 0) plaunch(family, bytes, flops, k_gemm_f64_fast<0, 0>, g, blk, 0, s, tk, ts, nt, A, B, C);
@@ -454,6 +466,68 @@ void GemmRunner::launch(const GemmBatch& b, const double* A, const double* B, do
   } else {
     plaunch(family, bytes, flops, k_gemm_f64, g, blk, 0, s, tk, ts, nt, A, B, C, gatherA, gatherB, kscale);
   }
+}
+
+char* GemmRunner::ws_stage(const void* p, size_t n) {
+  const size_t o = (ws_used_ + 255) / 256 * 256;
+  NRG_REQUIRE(o + n <= ws_cap_, "gemm: workspace too small for the task tables");
+  // the copy has consumed the host table when this returns (the caller's
+  // staging vectors are reused by the next launch); each table of one call
+  // has its own offset, so no copy overwrites a table a queued kernel reads
+  if (n) {
+    NRG_HIP(hipMemcpyAsync(ws_ + o, p, n, hipMemcpyHostToDevice, ws_stream_));
+    NRG_HIP(hipStreamSynchronize(ws_stream_));
+  }
+  ws_used_ = o + n;
+  return ws_ + o;
+}
+
+size_t GemmRunner::workspace_bytes(const GemmBatch& b) {
+  const int64_t wsz = decompose(b);
+  auto al = [](size_t n) { return (n + 255) / 256 * 256; };
+  size_t n = 0;
+  for (const GemmBatch* x : {&plain_, &part_})
+    if (!x->tasks.empty()) n += al(x->tasks.size() * sizeof(GemmTask)) + al(x->tasks.size() * sizeof(int32_t));
+  if (!part_.tasks.empty()) n += al(h_red_.size() * sizeof(SplitRed)) + al((size_t)wsz * sizeof(double));
+  return n;
+}
+
+void GemmRunner::run_into(const GemmBatch& b, const double* A, const double* B, double* C, char* ws,
+                          size_t ws_bytes, hipStream_t s, int family) {
+  NRG_REQUIRE(!rec_, "gemm: run_into while recording");
+  if (b.tasks.empty()) return;
+  const size_t need = workspace_bytes(b);
+  NRG_REQUIRE(need <= ws_bytes, "gemm: the products need " + std::to_string(need) +
+                                    " bytes of workspace, more than the " + std::to_string(ws_bytes) + " given");
+  const int64_t wsz = decompose(b);
+  ws_ = ws; ws_cap_ = ws_bytes; ws_used_ = 0; ws_stream_ = s;
+  try {
+    if (!plain_.tasks.empty()) launch(plain_, A, B, C, nullptr, nullptr, nullptr, s, family, 0);
+    if (!part_.tasks.empty()) {
+      // partial products after the tables, then the fixed-order reduction
+      double* W = nullptr;
+      {
+        const GemmBatch parts = part_;
+        const std::vector<SplitRed> red = h_red_;
+        const SplitRed* dred = reinterpret_cast<const SplitRed*>(ws_stage(red.data(), red.size() * sizeof(SplitRed)));
+        const size_t o = (ws_used_ + 255) / 256 * 256;
+        NRG_REQUIRE(o + (size_t)wsz * sizeof(double) <= ws_cap_, "gemm: workspace too small for split-K partials");
+        W = reinterpret_cast<double*>(ws_ + o);
+        ws_used_ = o + (size_t)wsz * sizeof(double);
+        launch(parts, A, B, W, nullptr, nullptr, nullptr, s, family, 1);
+        int64_t mx = 0;
+        for (const SplitRed& r : red) mx = std::max<int64_t>(mx, (int64_t)r.m * r.n);
+        const unsigned gx = (unsigned)std::min<int64_t>(ceil_div(mx, 256), 256);
+        plaunch(family, 0, 0, k_splitk_reduce, dim3(gx, (unsigned)red.size()), dim3(256), 0, s, dred,
+                (const double*)W, C);
+      }
+    }
+  } catch (...) {
+    ws_ = nullptr;
+    throw;
+  }
+  ws_ = nullptr;
+  NRG_HIP(hipGetLastError());
 }
 
 void GemmRunner::flush(hipStream_t s) {

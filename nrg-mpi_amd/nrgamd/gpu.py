@@ -68,6 +68,8 @@ SIGNATURES = {
     "nrg_gpu_set_gemm_split": (ctypes.c_int, [ctypes.c_int]),
     "nrg_gpu_dgemm_batched": (ctypes.c_int, [ctypes.c_int, _i32p, ctypes.c_int, ctypes.c_int, _f64p, _i64p, _f64p,
                                              _i64p, _f64p, _i64p, ctypes.c_int, ctypes.c_int, _f64p]),
+    "nrg_gpu_dgemm_batched_noalloc": (ctypes.c_int, [ctypes.c_int, _i32p, ctypes.c_int, ctypes.c_int, _f64p, _i64p,
+                                                     _f64p, _i64p, _f64p, _i64p]),
 }
 
 KERNEL_FAMILIES = ["assemble", "tridiag_col", "tridiag_symv", "tridiag_gemm", "dc_leaf", "dc_prep",
