@@ -76,8 +76,12 @@ struct GemmRunner {
   void run(const GemmBatch& b, const double* A, const double* B, double* C,
            const int32_t* gatherA, const int32_t* gatherB, const double* kscale,
            hipStream_t s, int family = KF_OTHER);
-  // FP64 product from FP16 matrix-core products of `slices` (3..8) signed
-  // 7-bit digits per element (gemm_split.hip)
+  // FP64 product from int8 matrix-core products of `slices` (3..8) signed
+  // 7-bit digits per element (gemm_split.hip); digit planes and exponents
+  // in the workspaces below
+  DevBuf<int8_t> d_planes_;
+  DevBuf<int32_t> d_exp_;
+  double* oz_C_ = nullptr;
   void run_split(const GemmBatch& b, const double* A, const double* B, double* C,
                  const int32_t* gatherA, const int32_t* gatherB, const double* kscale,
                  int slices, hipStream_t s, int family = KF_OTHER);
@@ -121,6 +125,7 @@ struct GemmRunner {
   }
   void release() {
     d_pack[0].release(); d_pack[1].release(); d_red_.release(); d_splitw_.release(); r_dev_.release();
+    d_planes_.release(); d_exp_.release();
   }
   // Split of a batch into plain tasks and split-K parts (plain_, part_,
   // h_red_); returns the partial-product doubles the parts need.

@@ -204,6 +204,12 @@ def run(batch: Batch, nops: int = 4, split: tuple = (8, 6), repeat: int = 3, sam
         log(f"operator {t}: products done")
     for f, r in res.items():
         r["fp64_equiv_tflops"] = flops / (r["ms"] * 1e-3) / 1e12 if r["ms"] > 0 else None
+        s = forms[f]
+        if s and r["ms"] > 0:
+            # S(S+1)/2 digit products of 2mnk int8 operations each
+            pairs = s * (s + 1) // 2
+            r["int8_mfma_tops"] = pairs * flops / (r["ms"] * 1e-3) / 1e12
+            r["frac_int8_peak"] = r["int8_mfma_tops"] / 5000.0
     out["contraction"] = res
     out["contraction_flops"] = flops
     return out

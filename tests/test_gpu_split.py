@@ -1,8 +1,9 @@
-"""Split-precision (Ozaki slicing) GEMM on FP16 matrix cores against FP64.
+"""Split-precision (Ozaki slicing) GEMM on int8 matrix cores against FP64.
 
-Pinned by: exact products of small-integer matrices (every digit product and
-partial sum is an integer below 2^24, so the split result must equal the
-exact product bit for bit); normwise agreement with numpy's FP64 product on
+Pinned by: exact products of small-integer matrices (two digits hold each
+entry exactly, every digit product and diagonal sum is an exact int32 and
+the weighted sum fits 53 bits, so the split result must equal the exact
+product bit for bit); normwise agreement with numpy's FP64 product on
 random and transposed operands of uneven sizes; the synthetic block batch of
 U^T O U contractions (Haar-rotated NRG-like spectra, four dense operators
 per block); the NRG iterations with every contraction product in split form
