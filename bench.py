@@ -56,6 +56,15 @@ def parse():
                     help="write a CSV of every launch of the profiled families (ordinal per kernel function "
                          "from the first iteration, algorithmic bytes and flops; phase warmup/timed)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--graphs", choices=["on", "off"], default="on",
+                    help="replay repeated tridiagonalisations from HIP graphs in the timed steps")
+    ap.add_argument("--tuning", choices=["cache", "off"], default="cache",
+                    help="launch configuration of the tridiagonalisation from the tuner's cache (per block-size "
+                         "histogram) or the defaults")
+    ap.add_argument("--tuning-cache", default=os.path.join(ROOT, "nrg-mpi_amd", "tuning", "tridiag_mi355x.txt"))
+    ap.add_argument("--tune", action="store_true",
+                    help="only run the tuner over the workload (two passes, exploring and updating the cache "
+                         "file) and exit")
     ap.add_argument("--leg-budget", type=float, default=240.0,
                     help="start the Nkeep=8000 leg only if the run so far took less than this many seconds")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -271,8 +280,21 @@ def main():
         print(f"[{time.perf_counter() - t_start:7.1f} s] {msg}", file=sys.stderr, flush=True)
 
     spec = models.workload(a.workload)
+    if a.tune:
+        os.makedirs(os.path.dirname(a.tuning_cache), exist_ok=True)
+        for p in range(2):
+            tmp = tempfile.mkdtemp(prefix="nrgtune_")
+            models.write_inputs(spec, tmp)
+            h = Host(tmp, device=0)
+            gpu.load().nrg_gpu_set_tuning(1, a.tuning_cache.encode())
+            for it in h.iterations():
+                h.iterate(it)
+            h.close()
+            log(f"tuning pass {p}: cache {a.tuning_cache}")
+        print(open(a.tuning_cache).read())
+        return
     need = a.warmup + a.steps
-    legs = 3 * (int(a.plain) + int(a.split_leg > 0 or a.split > 0))
+    legs = 3 * (int(a.plain) + int(a.split_leg > 0 or a.split > 0) + 1)
     if need + legs > spec.niter:
         raise SystemExit(f"warmup+steps+legs={need + legs} exceeds the {spec.niter} iterations of {a.workload}")
     tmp = tempfile.mkdtemp(prefix="nrgbench_")
@@ -280,6 +302,9 @@ def main():
     h = Host(tmp, device=0, sort_mode=0 if a.sort == "exact" else 1)
     gpu.load().nrg_gpu_set_tridiag_schedule(1 if a.tridiag_align == "end" else 0)
     gpu.load().nrg_gpu_set_gemm_split(a.split)
+    tuned = a.tuning == "cache" and os.path.exists(a.tuning_cache)
+    gpu.load().nrg_gpu_set_tuning(2 if tuned else 0, a.tuning_cache.encode() if tuned else None)
+    gpu.load().nrg_gpu_set_graphs(1 if a.graphs == "on" else 0)
     log_fams = [f for f in a.profile_families.split(",") if f and f != "all"] or list(gpu.KERNEL_FAMILIES)
     if a.launch_log:
         gpu.launch_log(log_fams)
@@ -307,6 +332,8 @@ def main():
             f"{st['back']:.0f} ms], post {t['postprocess']:.3f}) nfull={h.i['nfull']} nsub={h.i['nsub']}")
     elapsed = time.perf_counter() - t0
     work = gpu.work_counters(reset=True)
+    stg = gpu.stage_times()
+    gpu_graphs = (int(stg["graph_captures"]), int(stg["graph_replays"]))
     if a.launch_log:
         L = gpu.launch_log_get()
         names = {int(k): gpu.launch_log_kernel(int(k)) for k in set(L[:, 2].tolist())}
@@ -337,10 +364,15 @@ def main():
         cb["reference_run"] = reference_count_csv(a.workload)
         log(f"CPU baseline: {cb['seconds_per_iteration']:.1f} s per iteration")
 
-    def leg(split, batch_limit, label):
+    def leg(split, batch_limit, label, plain_launch=True):
         nonlocal nxt
         gpu.load().nrg_gpu_set_gemm_split(split)
         gpu.load().nrg_gpu_set_batch_limit(batch_limit)
+        if plain_launch:
+            # the comparison legs run without graphs and with the default
+            # launch configuration
+            gpu.load().nrg_gpu_set_graphs(0)
+            gpu.load().nrg_gpu_set_tuning(0, None)
         dts, itl = [], []
         try:
             for _ in range(3):
@@ -364,6 +396,9 @@ def main():
         return {"value": len(dts) / sum(dts), "unit": "iter/s", "iterations": itl, "seconds": dts}
 
     plain = batched_split = None
+    # the timed form without graph replay and tuned launch configuration
+    untuned = leg(0, 0, "batched FP64, no graphs, default launch configuration")
+    untuned["form"] = "batched FP64 eigensolve and contractions, launches issued one by one, default configuration"
     if a.plain:
         # one block per eigensolver pass (the per-block schedule of the
         # reference's dsyev loop) with FP64 contraction products
@@ -414,7 +449,7 @@ def main():
                    "states_per_iteration": nfull, "blocks_per_iteration": nsub, "largest_block": nmax,
                    "sort": a.sort, "tridiag_schedule": a.tridiag_align,
                    "form": ("batched eigensolve (FP64) + split-precision contraction GEMMs "
-                            f"({a.split} slices on FP16 matrix cores)") if a.split > 0 else
+                            f"({a.split} slices on int8 matrix cores)") if a.split > 0 else
                            "batched eigensolve (FP64) + FP64 contraction GEMMs"},
         "stages_s_per_step": {k: v / a.steps for k, v in stages.items()},
         "executed_flops": executed,
@@ -423,6 +458,9 @@ def main():
         "batched_split": batched_split,
         "batched_over_plain": (value / plain["value"]) if plain else None,
         "batched_split_over_plain": (batched_split["value"] / plain["value"]) if (plain and batched_split) else None,
+        "launch_options": {"graphs": a.graphs, "tuning": ("cache " + os.path.relpath(a.tuning_cache, ROOT))
+                           if tuned else "off", "graph_captures": gpu_graphs[0], "graph_replays": gpu_graphs[1]},
+        "batched_no_graphs_default_config": untuned,
     }
     if ks:
         fam = max(ks, key=lambda f: ks[f]["ms"])

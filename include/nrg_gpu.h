@@ -192,7 +192,9 @@ int nrg_gpu_history_clear(void);
  * t[0] H assembly, t[1] eigensolver total, t[2] tridiagonalisation,
  * t[3] divide and conquer, t[4] back-transform, t[5] invariant,
  * t[6] spectral invariant, t[7] rho sweep, t[8] CFS.  Also returns the
- * algorithmic FP64 flops of the last eigensolve (sum 10/3 n^3) in t[9]. */
+ * algorithmic FP64 flops of the last eigensolve (sum 10/3 n^3) in t[9] and,
+ * for n >= 12, the tridiagonalisation graphs captured (t[10]) and replayed
+ * (t[11]) since init (nrg_gpu_set_graphs). */
 int nrg_gpu_stage_times(double* t, int n);
 
 /* ---- batched eigensolver on host arrays (kernel-level interface) -------
@@ -213,10 +215,11 @@ int nrg_gpu_dgemm(int transa, int transb, int m, int n, int k, double alpha,
  * invariant (invariantMatrixDGEMM.F90:300,316), spectral invariant
  * (invariantMatrixForSpectrumDGEMM.F90:296,312), CFS weight
  * (module.F90:1100,1113) and reduced-density (reducedDensityMatrixDGEMM.F90:
- * 280,296) call sites run as FP64 products recovered from FP16 matrix-core
- * products of `slices` signed 7-bit digits per element (Ozaki slicing, FP32
- * accumulation flushed exactly to FP64).  0 (the default) keeps FP64 matrix
- * cores; 3..8 slices.  The eigensolver always stays FP64. */
+ * 280,296) call sites run as FP64 products recovered from int8 matrix-core
+ * products of `slices` signed 7-bit digits per element (Ozaki slicing, exact
+ * int32 accumulation flushed to FP64; 7*slices - 1 bits of each operand).
+ * 0 (the default) keeps FP64 matrix cores; 3..8 slices.  The eigensolver
+ * always stays FP64. */
 int nrg_gpu_set_gemm_split(int slices);
 
 /* Batched GEMM on host arrays (kernel-level interface for the synthetic
@@ -288,6 +291,28 @@ int nrg_gpu_launch_log_kernel(int kid, char* name, int cap);
  * pass, the default; 1 = one block at a time, the per-block schedule of the
  * reference's dsyev loop, used as the plain-FP64 comparison). */
 int nrg_gpu_set_batch_limit(int max_blocks);
+
+/* Launch option with no counterpart in the reference: on != 0 captures the
+ * launch sequence of each batched tridiagonalisation (the column kernels,
+ * SYMVs and trailing updates of all blocks on both stream groups) as a HIP
+ * graph, keyed on the block set (sizes, offsets, schedule, buffers); a
+ * later iteration with the same block set -- the NRG steady state repeats
+ * its block tables -- replays the graph instead of issuing the launches
+ * (at most 4 graphs are kept, least recently used replaced).  Off by
+ * default; kernel timing (nrg_gpu_profile) and the launch log disable it. */
+int nrg_gpu_set_graphs(int on);
+
+/* Tuning option with no counterpart in the reference (the role the
+ * reference's n^3 cost model plays in loadbalance.F90:102-186): the launch
+ * configuration of the batched tridiagonalisation -- the number of stream
+ * groups sharing the device and the launch size from which the SYMV uses
+ * 32-column panels -- chosen per block-size histogram (block count, largest
+ * block, sum n^3).  mode 0: defaults (2 groups, 512 workgroups); 1: time the
+ * candidate configurations on successive iterations with the same
+ * histogram, then keep the fastest, updating the cache file; 2: use the
+ * cache file's fastest only.  cache_path: a text file of measurements (NULL
+ * or "" keeps them in memory). */
+int nrg_gpu_set_tuning(int mode, const char* cache_path);
 
 /* Scheduling option with no counterpart in the reference: the column order
  * of the batched tridiagonalisation.  align_end != 0 (the default) starts

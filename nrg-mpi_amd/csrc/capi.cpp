@@ -294,8 +294,22 @@ int nrg_gpu_diagonalize(int iteration, int nsub, const int32_t* subspace_info, i
       // without an eigenvector output the back-transform waits for the
       // columns the invariant / stock calls read (ensure_columns)
       const bool defer = (eigenvector == nullptr);
+      // launch configuration of the tridiagonalisation: the tuner's choice
+      // for this block-size histogram (the defaults with tuning off); no
+      // graph capture while a candidate is being timed
+      bool explore = false;
+      std::string tkey;
+      TuneConfig cfg = Tuner::candidates()[0];
+      if (C.tuner.mode()) {
+        tkey = Tuner::key(C.bn.data(), nsub);
+        cfg = C.tuner.choose(tkey, &explore);
+      }
+      C.eig.set_groups(cfg.groups);
+      C.eig.set_symv_wide(cfg.symv_wide_wg, 64.0e6);
+      C.eig.set_graphs(C.graphs_on && !explore);
       C.eig.solve(nsub, C.bn.data(), C.boff.data(), C.bvoff.data(), C.d_H.p, C.d_U.p, C.d_w.p, defer,
                   C.stream, &C.eig_stats);
+      if (C.tuner.mode() == 1) C.tuner.record(tkey, cfg, C.eig_stats.ms_tridiag);
       C.u_lazy = defer;
     } else {
       // groups of at most batch_limit blocks, one solve each (batch_limit 1:
@@ -525,10 +539,10 @@ int nrg_gpu_stage_times(double* t, int n) {
     double back = C.timers.back;
     if (C.back_timed) back += span(C.ev_back);
     const double inv = C.inv_timed ? span(C.ev_inv) : 0.0;
-    const double v[10] = {C.timers.assemble, C.timers.eig, C.timers.tridiag, C.timers.dc, back,
+    const double v[12] = {C.timers.assemble, C.timers.eig, C.timers.tridiag, C.timers.dc, back,
                           inv, C.timers.spectral, C.timers.rho, C.timers.cfs,
-                          C.eig_stats.flops_alg};
-    for (int i = 0; i < n && i < 10; ++i) t[i] = v[i];
+                          C.eig_stats.flops_alg, (double)C.eig.graph_captures(), (double)C.eig.graph_replays()};
+    for (int i = 0; i < n && i < 12; ++i) t[i] = v[i];
   });
 }
 
@@ -645,6 +659,21 @@ int nrg_gpu_set_batch_limit(int max_blocks) {
   return guarded([&](Context& C) {
     NRG_REQUIRE(max_blocks >= 0, "set_batch_limit: negative limit");
     C.batch_limit = max_blocks;
+  });
+}
+
+int nrg_gpu_set_graphs(int on) {
+  return guarded([&](Context& C) {
+    need_init(C);
+    C.graphs_on = (on != 0);
+  });
+}
+
+int nrg_gpu_set_tuning(int mode, const char* cache_path) {
+  return guarded([&](Context& C) {
+    need_init(C);
+    NRG_REQUIRE(mode >= 0 && mode <= 2, "set_tuning: mode must be 0, 1 or 2");
+    C.tuner.set(mode, cache_path ? std::string(cache_path) : std::string());
   });
 }
 

@@ -3,6 +3,7 @@
 #include "common.h"
 #include "eig.h"
 #include "gemm.h"
+#include "tuner.h"
 
 namespace nrg {
 
@@ -32,6 +33,10 @@ struct Context {
   // this device's copy of the no-allocation entry point's scratch array
   // (hipGetSymbolAddress at init)
   char* scratch = nullptr;
+  // tridiagonalisation launch options: graphs (nrg_gpu_set_graphs) and the
+  // histogram-keyed tuner (nrg_gpu_set_tuning)
+  bool graphs_on = false;
+  Tuner tuner;
 
   // ---- model constants (set once)
   int nv = 0, nops = 0, nchain = 0, ncm = 0, ncoef = 0, nmat = 0;

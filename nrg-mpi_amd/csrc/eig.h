@@ -15,6 +15,8 @@
 #pragma once
 #include "common.h"
 #include "gemm.h"
+#include "profiler.h"
+#include <array>
 
 namespace nrg {
 
@@ -48,6 +50,16 @@ class BatchedEigensolver {
   void set_align_end(bool on) { align_end_ = on; }
   // Blocks split over this many streams in the tridiagonalisation (1..4).
   void set_groups(int g) { groups_ = std::max(1, std::min(g, MAX_TR_GROUPS)); }
+  int groups() const { return groups_; }
+  // 32-column SYMV panels only for launches with at least `wg` of them and
+  // `bytes` of traffic (both per group count); 16-column panels otherwise.
+  void set_symv_wide(int wg, double bytes) { symv_wide_wg_ = std::max(1, wg); symv_wide_bytes_ = bytes; }
+  int symv_wide_wg() const { return symv_wide_wg_; }
+  // Capture the tridiagonalisation's launch sequence as a HIP graph and
+  // replay it for a repeated block set (off by default).
+  void set_graphs(bool on) { use_graphs_ = on; }
+  long long graph_captures() const { return graph_captures_; }
+  long long graph_replays() const { return graph_replays_; }
 
  private:
   void tridiag(hipStream_t s);
@@ -106,6 +118,20 @@ class BatchedEigensolver {
   DevBuf<char> d_slotg_[MAX_TR_GROUPS];
   GemmRunner gemmg_[MAX_TR_GROUPS];
   GemmBatch batchg_[MAX_TR_GROUPS];
+  int symv_wide_wg_ = 512;
+  double symv_wide_bytes_ = 64.0e6;
+  // captured tridiagonalisations: launch sequence of one block set, with the
+  // work its launches declare (added to the issued-work counters on replay)
+  struct TrGraph {
+    std::vector<int64_t> key;
+    hipGraphExec_t exec = nullptr;
+    std::array<double, KF_COUNT> launches{}, bytes{}, flops{};
+    long long used = 0;
+  };
+  static constexpr size_t kMaxGraphs = 4;
+  std::vector<TrGraph> graphs_;
+  bool use_graphs_ = false;
+  long long graph_clock_ = 0, graph_captures_ = 0, graph_replays_ = 0;
 };
 
 }  // namespace nrg

@@ -26,8 +26,6 @@ constexpr int DC_LDS_M = 8192;  // merges up to this size run the deflation scan
 constexpr int SEC_LDS_M = 4000; // secular solves stage poles and weights in LDS up to this size
 constexpr int NBB = 128;   // back-transform panel width (Z is streamed once per panel)
 constexpr int MAXN = 16384;
-constexpr int SYMV_WIDE_MIN_WG = 512;   // 32-column SYMV panels only for launches of at least this many
-constexpr double SYMV_WIDE_MIN_BYTES = 64.0e6;   // ... and of at least this many bytes (both per group count)
 #ifndef SYMV_U
 #define SYMV_U 4   // column loads in flight per lane in the SYMV
 #endif
@@ -1378,6 +1376,8 @@ static double now_ms() {
 }
 
 void BatchedEigensolver::release() {
+  for (TrGraph& e : graphs_) (void)hipGraphExecDestroy(e.exec);
+  graphs_.clear();
   d_n_.release(); d_order_.release();
   for (int g = 0; g < MAX_TR_GROUPS; ++g) {
     d_slotg_[g].release();
@@ -1555,11 +1555,6 @@ void BatchedEigensolver::tridiag(hipStream_t s) {
     G.ta = TrArgs{reinterpret_cast<const TrSlot*>(G.d_slot->p), A_, d_d_.p, d_e_.p, d_tau_.p, d_y_.p, d_P_.p,
                   d_part_.p};
   }
-  if (ng > 1) {
-    // the other streams start after everything queued so far (inputs, tables)
-    NRG_HIP(hipEventRecord(ev_fork_, s));
-    for (int g = 1; g < ng; ++g) NRG_HIP(hipStreamWaitEvent(sg_[g], ev_fork_, 0));
-  }
   // trailing updates [V W][W V]^T of panel boundary t (t = NB, 2 NB, ...):
   // every group's products are recorded up front and their task tables
   // uploaded once; the step loop issues each panel's launches in place
@@ -1588,7 +1583,53 @@ void BatchedEigensolver::tridiag(hipStream_t s) {
       G.gemm->run(*G.batch, d_P_.p, d_P_.p, A_, nullptr, nullptr, nullptr, G.s, KF_GEMM_TRIDIAG);
       G.syr2k.push_back(G.gemm->recorded());
     }
-    G.gemm->upload_recorded(G.s);
+    G.gemm->upload_recorded(s);   // on the caller's stream, before the fork
+  }
+
+  // From here on the work is launches only (every table is on the device):
+  // with graphs on, the fork, the column steps and the join of this block
+  // set are captured once and replayed when the same block set comes again
+  // (the same sizes, offsets, schedule and buffer addresses, so the same
+  // tables and launch arguments).
+  const bool graph = use_graphs_ && !profiler().on && !profiler().log_mask;
+  std::vector<int64_t> key;
+  if (graph) {
+    key.reserve(3 * nblk + 32);
+    key.push_back(nblk);
+    key.push_back(align_end_);
+    key.push_back(ng);
+    key.push_back(symv_wide_wg_);
+    key.push_back((int64_t)symv_wide_bytes_);
+    for (int b = 0; b < nblk; ++b) { key.push_back(n_[b]); key.push_back(aoff_[b]); key.push_back(voff_[b]); }
+    for (const void* p : {(const void*)A_, (const void*)d_d_.p, (const void*)d_e_.p, (const void*)d_tau_.p,
+                          (const void*)d_y_.p, (const void*)d_P_.p, (const void*)d_part_.p})
+      key.push_back((int64_t)(intptr_t)p);
+    for (int g = 0; g < ng; ++g) {
+      key.push_back((int64_t)(intptr_t)d_slotg_[g].p);
+      key.push_back((int64_t)(intptr_t)gemmg_[g].r_dev_.p);
+      key.push_back((int64_t)(intptr_t)(g ? sg_[g] : s));
+    }
+    for (TrGraph& e : graphs_)
+      if (e.key == key) {
+        NRG_HIP(hipGraphLaunch(e.exec, s));
+        Profiler& P = profiler();
+        for (int f = 0; f < KF_COUNT; ++f) {
+          P.w_launches[f] += e.launches[f]; P.w_bytes[f] += e.bytes[f]; P.w_flops[f] += e.flops[f];
+        }
+        e.used = ++graph_clock_;
+        ++graph_replays_;
+        for (int g = 0; g < ng; ++g) grp[g].gemm->end_replay();
+        NRG_HIP(hipGetLastError());
+        return;
+      }
+    NRG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  }
+  Profiler& Pw = profiler();
+  const auto w0 = Pw.w_launches, b0 = Pw.w_bytes, f0 = Pw.w_flops;
+  if (ng > 1) {
+    // the other streams start after everything queued so far (inputs, tables)
+    NRG_HIP(hipEventRecord(ev_fork_, s));
+    for (int g = 1; g < ng; ++g) NRG_HIP(hipStreamWaitEvent(sg_[g], ev_fork_, 0));
   }
 
   auto step = [&](Group& G, int t) {
@@ -1660,7 +1701,7 @@ void BatchedEigensolver::tridiag(hipStream_t s) {
       const int m = rows_at(i, t) - 1;
       if (m > 0) g32 += ceil_div(m, 2 * CG);
     }
-    const int pw = (bytes >= SYMV_WIDE_MIN_BYTES / ng && g32 >= SYMV_WIDE_MIN_WG / ng) ? 2 * CG : CG;
+    const int pw = (bytes >= symv_wide_bytes_ / ng && g32 >= symv_wide_wg_ / ng) ? 2 * CG : CG;
     int gB = 0, tmax = 0;
     for (int i = 0; i < act; ++i) {
       const int m = rows_at(i, t) - 1;
@@ -1708,6 +1749,32 @@ void BatchedEigensolver::tridiag(hipStream_t s) {
   for (int g = 1; g < ng; ++g) {
     NRG_HIP(hipEventRecord(evg_[g], sg_[g]));
     NRG_HIP(hipStreamWaitEvent(s, evg_[g], 0));
+  }
+  if (graph) {
+    hipGraph_t gr = nullptr;
+    NRG_HIP(hipStreamEndCapture(s, &gr));
+    TrGraph e;
+    e.key = std::move(key);
+    const hipError_t ie = hipGraphInstantiate(&e.exec, gr, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(gr);
+    NRG_HIP(ie);
+    for (int f = 0; f < KF_COUNT; ++f) {
+      e.launches[f] = Pw.w_launches[f] - w0[f];
+      e.bytes[f] = Pw.w_bytes[f] - b0[f];
+      e.flops[f] = Pw.w_flops[f] - f0[f];
+    }
+    e.used = ++graph_clock_;
+    NRG_HIP(hipGraphLaunch(e.exec, s));
+    ++graph_captures_;
+    if (graphs_.size() >= kMaxGraphs) {
+      size_t old = 0;
+      for (size_t i = 1; i < graphs_.size(); ++i)
+        if (graphs_[i].used < graphs_[old].used) old = i;
+      NRG_HIP(hipStreamSynchronize(s));   // the evicted graph may still run
+      (void)hipGraphExecDestroy(graphs_[old].exec);
+      graphs_.erase(graphs_.begin() + old);
+    }
+    graphs_.push_back(std::move(e));
   }
   NRG_HIP(hipGetLastError());
 }

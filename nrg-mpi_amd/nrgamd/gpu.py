@@ -65,6 +65,8 @@ SIGNATURES = {
     "nrg_gpu_load_restart": (ctypes.c_int, [ctypes.c_char_p, _i32p, ctypes.c_int, _i32p]),
     "nrg_gpu_set_batch_limit": (ctypes.c_int, [ctypes.c_int]),
     "nrg_gpu_set_tridiag_schedule": (ctypes.c_int, [ctypes.c_int]),
+    "nrg_gpu_set_graphs": (ctypes.c_int, [ctypes.c_int]),
+    "nrg_gpu_set_tuning": (ctypes.c_int, [ctypes.c_int, ctypes.c_char_p]),
     "nrg_gpu_set_gemm_split": (ctypes.c_int, [ctypes.c_int]),
     "nrg_gpu_dgemm_batched": (ctypes.c_int, [ctypes.c_int, _i32p, ctypes.c_int, ctypes.c_int, _f64p, _i64p, _f64p,
                                              _i64p, _f64p, _i64p, ctypes.c_int, ctypes.c_int, _f64p]),
@@ -267,7 +269,8 @@ def kernel_stats():
 
 
 def stage_times():
-    t = np.zeros(10)
-    _check(load().nrg_gpu_stage_times(_p(t, _f64p), 10))
-    keys = ["assemble", "eig", "tridiag", "dc", "back", "invariant", "spectral", "rho", "cfs", "eig_flops"]
+    t = np.zeros(12)
+    _check(load().nrg_gpu_stage_times(_p(t, _f64p), 12))
+    keys = ["assemble", "eig", "tridiag", "dc", "back", "invariant", "spectral", "rho", "cfs", "eig_flops",
+            "graph_captures", "graph_replays"]
     return dict(zip(keys, t.tolist()))

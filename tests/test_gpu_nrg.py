@@ -159,3 +159,26 @@ def test_solver_usable_after_host_close(tmp_path):
     for H, w, U in zip(blocks, ws, Us):
         np.testing.assert_allclose(np.sort(w), np.linalg.eigvalsh(H), rtol=0, atol=1e-12 * np.abs(H).max() * len(H))
         assert np.max(np.abs(H @ U - U * w)) <= 1e-11 * np.abs(H).max() * len(H)
+
+
+def test_tridiagonal_graphs_replay(tmp_path):
+    # tridiagonalisations captured as HIP graphs and replayed when a block
+    # set repeats (the steady state alternates between two block tables):
+    # the reference-pinned window, with the eager work of every iteration
+    # (divide and conquer, back-transform, invariant products, host
+    # bookkeeping) and its synchronisations between the replays; every
+    # iteration is checked against the fixture
+    from nrgamd import gpu
+    gpu.init(0)
+    st0 = gpu.stage_times()
+    gpu.load().nrg_gpu_set_graphs(1)
+    try:
+        run_and_compare(tmp_path, "two_orbital_nk4000_it45", {})
+        st1 = gpu.stage_times()
+    finally:
+        gpu.init(0)
+        gpu.load().nrg_gpu_set_graphs(0)
+    cap = st1["graph_captures"] - st0["graph_captures"]
+    rep = st1["graph_replays"] - st0["graph_replays"]
+    print(f"graphs captured {cap:.0f}, replayed {rep:.0f}")
+    assert rep >= 10
