@@ -56,8 +56,12 @@ def parse():
                     help="write a CSV of every launch of the profiled families (ordinal per kernel function "
                          "from the first iteration, algorithmic bytes and flops; phase warmup/timed)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--graphs", choices=["on", "off"], default="on",
-                    help="replay repeated tridiagonalisations from HIP graphs in the timed steps")
+    ap.add_argument("--graphs", choices=["on", "off"], default="off",
+                    help="capture the tridiagonalisation's launch sequence in a HIP graph per block set and "
+                         "replay it when the same block set returns (default off: the block tables change "
+                         "from one NRG iteration to the next, so graphs are captured and never replayed, and "
+                         "capture costs more than launching; the bench measures the graph form in a leg of "
+                         "its own)")
     ap.add_argument("--tuning", choices=["cache", "off"], default="cache",
                     help="launch configuration of the tridiagonalisation from the tuner's cache (per block-size "
                          "histogram) or the defaults")
@@ -246,18 +250,17 @@ def workload_nk8000(gpu, models, Host, log):
 
 
 def copy_calibration(nbytes):
-    """A plain HBM copy of known size: under `rocprofv3 --pmc FETCH_SIZE` (or
-    WRITE_SIZE) its counter reading per dispatch, against nbytes, gives the
-    factor that turns the counters of the NRG kernels into bytes."""
-    import torch
-    n = nbytes // 8
-    src = torch.arange(n, dtype=torch.float64, device="cuda")
-    dst = torch.empty_like(src)
-    for _ in range(5):
-        dst.copy_(src)
-    torch.cuda.synchronize()
-    ok = bool(torch.equal(dst[-4:], src[-4:]))
-    print(json.dumps({"copy_bytes": 8 * n, "dispatches": 5, "ok": ok}))
+    """A plain HBM copy of known size (the library's k_stream_copy): under
+    `rocprofv3 --pmc FETCH_SIZE` (or WRITE_SIZE) its counter reading per
+    dispatch, against nbytes, gives the factor that turns the counters of the
+    NRG kernels into bytes."""
+    import ctypes
+    from nrgamd import gpu
+    gpu.init(0)
+    ok = ctypes.c_int(0)
+    gpu._check(gpu.load().nrg_gpu_copy_calibration(int(nbytes) // 16 * 16, 5, ctypes.byref(ok)))
+    print(json.dumps({"copy_bytes": int(nbytes) // 16 * 16, "dispatches": 5, "kernel": "k_stream_copy",
+                      "ok": bool(ok.value)}))
 
 
 def main():
@@ -294,7 +297,7 @@ def main():
         print(open(a.tuning_cache).read())
         return
     need = a.warmup + a.steps
-    legs = 3 * (int(a.plain) + int(a.split_leg > 0 or a.split > 0) + 1)
+    legs = 3 * (int(a.plain) + int(a.split_leg > 0 or a.split > 0) + 2)
     if need + legs > spec.niter:
         raise SystemExit(f"warmup+steps+legs={need + legs} exceeds the {spec.niter} iterations of {a.workload}")
     tmp = tempfile.mkdtemp(prefix="nrgbench_")
@@ -364,14 +367,14 @@ def main():
         cb["reference_run"] = reference_count_csv(a.workload)
         log(f"CPU baseline: {cb['seconds_per_iteration']:.1f} s per iteration")
 
-    def leg(split, batch_limit, label, plain_launch=True):
+    def leg(split, batch_limit, label, graphs=False, use_tuning=False):
         nonlocal nxt
         gpu.load().nrg_gpu_set_gemm_split(split)
         gpu.load().nrg_gpu_set_batch_limit(batch_limit)
-        if plain_launch:
-            # the comparison legs run without graphs and with the default
-            # launch configuration
-            gpu.load().nrg_gpu_set_graphs(0)
+        gpu.load().nrg_gpu_set_graphs(1 if graphs else 0)
+        if use_tuning and tuned:
+            gpu.load().nrg_gpu_set_tuning(2, a.tuning_cache.encode())
+        else:
             gpu.load().nrg_gpu_set_tuning(0, None)
         dts, itl = [], []
         try:
@@ -396,9 +399,16 @@ def main():
         return {"value": len(dts) / sum(dts), "unit": "iter/s", "iterations": itl, "seconds": dts}
 
     plain = batched_split = None
-    # the timed form without graph replay and tuned launch configuration
+    # the timed form with the default launch configuration instead of the
+    # tuned one, and with HIP graphs
     untuned = leg(0, 0, "batched FP64, no graphs, default launch configuration")
     untuned["form"] = "batched FP64 eigensolve and contractions, launches issued one by one, default configuration"
+    with_graphs = leg(0, 0, "batched FP64, HIP graphs, tuned launch configuration", graphs=True, use_tuning=True)
+    with_graphs["form"] = ("batched FP64 eigensolve and contractions, tridiagonalisation captured into HIP graphs "
+                           "(replayed when a block set returns), tuned configuration")
+    st_g = gpu.stage_times()
+    with_graphs["graph_captures"] = int(st_g["graph_captures"]) - gpu_graphs[0]
+    with_graphs["graph_replays"] = int(st_g["graph_replays"]) - gpu_graphs[1]
     if a.plain:
         # one block per eigensolver pass (the per-block schedule of the
         # reference's dsyev loop) with FP64 contraction products
@@ -461,6 +471,7 @@ def main():
         "launch_options": {"graphs": a.graphs, "tuning": ("cache " + os.path.relpath(a.tuning_cache, ROOT))
                            if tuned else "off", "graph_captures": gpu_graphs[0], "graph_replays": gpu_graphs[1]},
         "batched_no_graphs_default_config": untuned,
+        "batched_graphs_tuned": with_graphs,
     }
     if ks:
         fam = max(ks, key=lambda f: ks[f]["ms"])

@@ -284,6 +284,12 @@ int nrg_gpu_launch_log(int fam_mask);
  * counters after reading. */
 int nrg_gpu_work_counters(double* out, int nfam, int reset);
 int nrg_gpu_launch_log_get(double* out, int64_t cap, int64_t* count);
+/* Measurement helper with no counterpart in the reference: copies nbytes
+ * (a multiple of 16) device to device reps times with the library's own
+ * streaming kernel (k_stream_copy: every byte read once and written once),
+ * the known-size kernel that scales the rocprofv3 FETCH_SIZE / WRITE_SIZE
+ * counters of the NRG kernels to bytes; *ok = 1 when the copy arrived. */
+int nrg_gpu_copy_calibration(int64_t nbytes, int reps, int* ok);
 int nrg_gpu_launch_log_kernel(int kid, char* name, int cap);
 
 /* Scheduling option with no counterpart in the reference: solve the blocks

@@ -2,6 +2,7 @@
 #include "../../include/nrg_gpu.h"
 #include "assemble.h"
 #include "noalloc.h"
+#include "calib.h"
 #include "cfs.h"
 #include "context.h"
 #include <algorithm>
@@ -714,6 +715,14 @@ int nrg_gpu_work_counters(double* out, int nfam, int reset) {
       out[3 * f + 2] = P.w_flops[f];
     }
     if (reset) { P.w_launches = {}; P.w_bytes = {}; P.w_flops = {}; }
+  });
+}
+
+int nrg_gpu_copy_calibration(int64_t nbytes, int reps, int* ok) {
+  return guarded([&](Context& C) {
+    need_init(C);
+    const bool good = copy_calibration(nbytes, reps, C.stream);
+    if (ok) *ok = good ? 1 : 0;
   });
 }
 

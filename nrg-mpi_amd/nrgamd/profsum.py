@@ -196,7 +196,7 @@ def main(argv=None):
     t.add_argument("--copy-fetch", required=True)
     t.add_argument("--copy-write", required=True)
     t.add_argument("--copy-bytes", type=float, required=True)
-    t.add_argument("--copy-kernel", default="copyBuffer|copy")
+    t.add_argument("--copy-kernel", default="k_stream_copy")
     t.add_argument("--variant", nargs=4, action="append", required=True,
                    metavar=("REGEX", "FIRST", "FETCH_DIR", "WRITE_DIR"))
     t.add_argument("--title", default="HBM traffic per dispatch")
