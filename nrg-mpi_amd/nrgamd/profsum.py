@@ -181,6 +181,16 @@ def traffic(a):
     return "\n".join(lines)
 
 
+def first_timed(log, base, ints, skip=0):
+    """Per-kernel ordinal of the first dispatch in the timed steps of the
+    kernel `base<ints...>` (template integers as a prefix), plus `skip`."""
+    for r in read_log(log):
+        b, k = kernel_key(r["kernel"])
+        if b == base and k[:len(ints)] == tuple(ints) and r["phase"] == "timed":
+            return int(r["ordinal"]) + skip
+    raise SystemExit(f"no timed dispatch of {base}<{ints}> in {log}")
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -191,6 +201,11 @@ def main(argv=None):
     s.add_argument("--top", type=int, default=30)
     s.add_argument("--log", default=None)
     s.add_argument("--family", default=None)
+    f = sub.add_parser("first-timed")
+    f.add_argument("--log", required=True)
+    f.add_argument("--kernel", required=True, help="base name, e.g. k_tr_symv")
+    f.add_argument("--ints", default="", help="leading template integers, e.g. 2,16")
+    f.add_argument("--skip", type=int, default=0)
     t = sub.add_parser("traffic")
     t.add_argument("--log", required=True)
     t.add_argument("--copy-fetch", required=True)
@@ -204,6 +219,9 @@ def main(argv=None):
     a = ap.parse_args(argv)
     if a.cmd == "stats":
         sys.stdout.write(stats(a.dir, a.title, a.command, a.top, a.log, a.family))
+    elif a.cmd == "first-timed":
+        ints = [int(x) for x in a.ints.split(",") if x.strip()]
+        print(first_timed(a.log, a.kernel, ints, a.skip))
     else:
         sys.stdout.write(traffic(a))
 
