@@ -80,12 +80,22 @@ def parse():
                     help="slices of the split-precision contraction GEMMs in the timed steps (0: FP64 matrix "
                          "cores, the faster form on MI355X; the split form is measured in its own iteration)")
     ap.add_argument("--split-leg", type=int, default=8,
-                    help="slices of the three-iteration split-precision comparison (0: skip)")
+                    help="slices of the invariant-matrix products in the three-iteration split-precision "
+                         "comparison and in the split form of the Nkeep=8000 leg (0: skip)")
+    ap.add_argument("--split-spectral", type=int, default=6,
+                    help="slices of the spectral-side products (spectral invariant, CFS weights, rho) in the "
+                         "split forms: they feed only the binned spectrum, whose 1e-8 budget 6 slices meet")
+    ap.add_argument("--split-min-flops", type=float, default=4e9,
+                    help="batches below this many FP64-equivalent flops stay on FP64 matrix cores in the split "
+                         "forms")
     ap.add_argument("--no-spectrum-leg", dest="spectrum_leg", action="store_false",
                     help="skip the Nkeep=8000 full pipeline with spectra (forward + rho sweep + CFS, FP64 and "
                          "split contraction products)")
     ap.add_argument("--no-synthetic", dest="synthetic", action="store_false",
                     help="skip the synthetic 256-block batch (eigensolve, FP64 vs split-precision U^T O U)")
+    ap.add_argument("--no-graph-leg", dest="graph_leg", action="store_false",
+                    help="skip the three-iteration leg with HIP graphs (profiled runs skip it: under "
+                         "rocprofv3 --kernel-trace its first graph launch failed)")
     ap.add_argument("--no-plain", dest="plain", action="store_false",
                     help="skip the three-iteration plain (one block per pass) comparison")
     return ap.parse_args()
@@ -203,20 +213,24 @@ def synthetic_batch(log):
     return r
 
 
-def workload_nk8000(gpu, models, Host, log):
+def workload_nk8000(gpu, models, Host, log, split_cfg):
     """Two-orbital Kanamori model at Nkeep=8000 with spectra (45 iterations,
     4 spectral types): forward pass with stocks, rho sweep and CFS spikes,
-    with FP64 contraction products and with split-precision (8-slice)
-    products; the FP64 form is the oracle of the split one."""
+    with FP64 contraction products and with split-precision products
+    (split_cfg = invariant slices, spectral slices, minimum batch flops);
+    the FP64 form is the oracle of the split one."""
     import numpy as np
     out = {"config": "two-orbital Kanamori U=0.3 J=0.05 Gamma=0.01/orbital, Nkeep=8000, 45 iterations, "
                      "4 spectral types"}
     bins = {}
-    for split in (0, 8):
+    for split in (0, split_cfg[0]):
         tmp = tempfile.mkdtemp(prefix="nrg8000_")
         models.write_inputs(models.workload("two_orbital_nk8000_it45"), tmp)
         h = Host(tmp, device=0)
-        gpu.load().nrg_gpu_set_gemm_split(split)
+        if split:
+            gpu.load().nrg_gpu_set_gemm_split_families(split_cfg[0], split_cfg[1], split_cfg[2])
+        else:
+            gpu.load().nrg_gpu_set_gemm_split(0)
         try:
             t0 = time.perf_counter()
             steady = []
@@ -231,12 +245,15 @@ def workload_nk8000(gpu, models, Host, log):
             st = h.spectrum_times()
             neg, pos = h.bins()
             tr = h.rho_traces()
-            key = "split8" if split else "fp64"
+            key = "split" if split else "fp64"
             out[key] = {"forward_s": t1 - t0, "steady_state_iter_per_s": len(steady) / sum(steady),
                         "stock_total_s": st["stock"], "rho_sweep_s": st["rho"], "cfs_s": st["cfs"],
                         "pipeline_s": t2 - t0,
                         "sum_rule_max_dev": float(np.max(np.abs(neg.sum(1) + pos.sum(1) - 1.0))),
                         "trace_rho_max_dev": float(max(abs(v - 1.0) for v in tr.values()))}
+            if split:
+                out[key]["slices"] = {"invariant": split_cfg[0], "spectral": split_cfg[1],
+                                      "min_batch_flops": split_cfg[2]}
             bins[split] = (neg, pos)
             if split:
                 peak = max(np.abs(bins[0][0]).max(), np.abs(bins[0][1]).max())
@@ -297,7 +314,7 @@ def main():
         print(open(a.tuning_cache).read())
         return
     need = a.warmup + a.steps
-    legs = 3 * (int(a.plain) + int(a.split_leg > 0 or a.split > 0) + 2)
+    legs = 3 * (int(a.plain) + int(a.split_leg > 0 or a.split > 0) + 1 + int(a.graph_leg))
     if need + legs > spec.niter:
         raise SystemExit(f"warmup+steps+legs={need + legs} exceeds the {spec.niter} iterations of {a.workload}")
     tmp = tempfile.mkdtemp(prefix="nrgbench_")
@@ -369,7 +386,10 @@ def main():
 
     def leg(split, batch_limit, label, graphs=False, use_tuning=False):
         nonlocal nxt
-        gpu.load().nrg_gpu_set_gemm_split(split)
+        if split:
+            gpu.load().nrg_gpu_set_gemm_split_families(split, a.split_spectral, a.split_min_flops)
+        else:
+            gpu.load().nrg_gpu_set_gemm_split(0)
         gpu.load().nrg_gpu_set_batch_limit(batch_limit)
         gpu.load().nrg_gpu_set_graphs(1 if graphs else 0)
         if use_tuning and tuned:
@@ -403,12 +423,15 @@ def main():
     # tuned one, and with HIP graphs
     untuned = leg(0, 0, "batched FP64, no graphs, default launch configuration")
     untuned["form"] = "batched FP64 eigensolve and contractions, launches issued one by one, default configuration"
-    with_graphs = leg(0, 0, "batched FP64, HIP graphs, tuned launch configuration", graphs=True, use_tuning=True)
-    with_graphs["form"] = ("batched FP64 eigensolve and contractions, tridiagonalisation captured into HIP graphs "
-                           "(replayed when a block set returns), tuned configuration")
-    st_g = gpu.stage_times()
-    with_graphs["graph_captures"] = int(st_g["graph_captures"]) - gpu_graphs[0]
-    with_graphs["graph_replays"] = int(st_g["graph_replays"]) - gpu_graphs[1]
+    with_graphs = None
+    if a.graph_leg:
+        with_graphs = leg(0, 0, "batched FP64, HIP graphs, tuned launch configuration", graphs=True,
+                          use_tuning=True)
+        with_graphs["form"] = ("batched FP64 eigensolve and contractions, tridiagonalisation captured into HIP "
+                               "graphs (replayed when a block set returns), tuned configuration")
+        st_g = gpu.stage_times()
+        with_graphs["graph_captures"] = int(st_g["graph_captures"]) - gpu_graphs[0]
+        with_graphs["graph_replays"] = int(st_g["graph_replays"]) - gpu_graphs[1]
     if a.plain:
         # one block per eigensolver pass (the per-block schedule of the
         # reference's dsyev loop) with FP64 contraction products
@@ -417,7 +440,9 @@ def main():
     if a.split == 0 and a.split_leg > 0:
         batched_split = leg(a.split_leg, 0, "batched split")
         batched_split["form"] = (f"batched eigensolve (FP64) + split-precision contraction products "
-                                 f"({a.split_leg} slices)")
+                                 f"({a.split_leg} slices for the invariant matrices, {a.split_spectral} for "
+                                 f"the spectral-side families, batches under {a.split_min_flops:.0e} flops in "
+                                 f"FP64)")
     h.close()
 
     # executed work of the timed steps, per step
@@ -512,7 +537,8 @@ def main():
         out["synthetic_batch"] = synthetic_batch(log)
     if a.spectrum_leg:
         if time.perf_counter() - t_start < a.leg_budget:
-            out["workload_nk8000"] = workload_nk8000(gpu, models, Host, log)
+            out["workload_nk8000"] = workload_nk8000(gpu, models, Host, log,
+                                                     (a.split_leg or 8, a.split_spectral, a.split_min_flops))
         else:
             out["workload_nk8000"] = {"skipped": f"over the {a.leg_budget:.0f} s budget for starting this leg"}
     log("done")

@@ -221,6 +221,16 @@ int nrg_gpu_dgemm(int transa, int transb, int m, int n, int k, double alpha,
  * 0 (the default) keeps FP64 matrix cores; 3..8 slices.  The eigensolver
  * always stays FP64. */
 int nrg_gpu_set_gemm_split(int slices);
+/* The same option per product family (no counterpart in the reference):
+ * invariant_slices for the invariant-matrix products, whose results feed the
+ * next iteration's Hamiltonian and so its eigenvalues (1e-12 relative: 8
+ * slices), spectral_slices for the spectral invariant, CFS weight and
+ * reduced-density products, which feed only the binned spectrum (1e-8 of
+ * the peak bin: 6 slices, 41 bits per operand, are enough); a batch below
+ * min_batch_flops FP64-equivalent flops (2mnk summed over its products)
+ * stays on FP64 matrix cores.  nrg_gpu_set_gemm_split(s) is
+ * (s, s, 0). */
+int nrg_gpu_set_gemm_split_families(int invariant_slices, int spectral_slices, double min_batch_flops);
 
 /* Batched GEMM on host arrays (kernel-level interface for the synthetic
  * block batch): task i computes C_i = op(A_i) op(B_i), m x n x k = mnk[3i..],

@@ -653,6 +653,20 @@ int nrg_gpu_set_gemm_split(int slices) {
   return guarded([&](Context& C) {
     NRG_REQUIRE(slices == 0 || (slices >= 3 && slices <= 8), "set_gemm_split: slices must be 0 or 3..8");
     C.gemm.split = slices;
+    C.gemm.split_spec = slices;
+    C.gemm.split_min_flops = 0.0;
+  });
+}
+
+int nrg_gpu_set_gemm_split_families(int invariant_slices, int spectral_slices, double min_batch_flops) {
+  return guarded([&](Context& C) {
+    auto ok = [](int s) { return s == 0 || (s >= 3 && s <= 8); };
+    NRG_REQUIRE(ok(invariant_slices) && ok(spectral_slices),
+                "set_gemm_split_families: slices must be 0 or 3..8");
+    NRG_REQUIRE(min_batch_flops >= 0.0, "set_gemm_split_families: min_batch_flops must be >= 0");
+    C.gemm.split = invariant_slices;
+    C.gemm.split_spec = spectral_slices;
+    C.gemm.split_min_flops = min_batch_flops;
   });
 }
 

@@ -70,9 +70,14 @@ struct GemmRunner {
   void launch(const GemmBatch& b, const double* A, const double* B, double* C,
               const int32_t* gatherA, const int32_t* gatherB, const double* kscale,
               hipStream_t s, int family, int slot);
-  // split > 0: the contraction families (invariant, spectral invariant, CFS
-  // weights, rho) run as split-precision products with `split` slices
+  // split > 0: the invariant-matrix products run as split-precision
+  // products with `split` slices; split_spec > 0: the spectral-side families
+  // (spectral invariant, CFS weights, rho) with `split_spec` slices.  A batch
+  // of fewer than split_min_flops FP64-equivalent flops stays on the FP64
+  // kernel (the slicing passes would cost more than they save).
   int split = 0;
+  int split_spec = 0;
+  double split_min_flops = 0.0;
   void run(const GemmBatch& b, const double* A, const double* B, double* C,
            const int32_t* gatherA, const int32_t* gatherB, const double* kscale,
            hipStream_t s, int family = KF_OTHER);

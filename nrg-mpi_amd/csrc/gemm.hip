@@ -377,10 +377,17 @@ void GemmRunner::run(const GemmBatch& b, const double* A, const double* B, doubl
                      hipStream_t s, int family) {
   const int nt = (int)b.tasks.size();
   if (nt == 0) return;
-  if (split > 0 && (family == KF_GEMM_INV || family == KF_GEMM_SPEC || family == KF_GEMM_CFSW ||
-                    family == KF_GEMM_CFSX || family == KF_GEMM_RHO)) {
-    run_split(b, A, B, C, gatherA, gatherB, kscale, split, s, family);
-    return;
+  const int sl = (family == KF_GEMM_INV) ? split
+                 : (family == KF_GEMM_SPEC || family == KF_GEMM_CFSW || family == KF_GEMM_CFSX ||
+                    family == KF_GEMM_RHO) ? split_spec : 0;
+  if (sl > 0) {
+    double fl = 0.0;
+    if (split_min_flops > 0.0)
+      for (const GemmTask& t : b.tasks) fl += (t.lower ? 1.0 : 2.0) * t.m * t.n * (double)t.k;
+    if (fl >= split_min_flops) {
+      run_split(b, A, B, C, gatherA, gatherB, kscale, sl, s, family);
+      return;
+    }
   }
   // Products with a long K and few output tiles (the back-transform's
   // V^T V and V^T Z) are split along K into partial products written to a

@@ -157,7 +157,14 @@ __global__ __launch_bounds__(256) void k_oz_slice(OzOps o, const OzUnit* __restr
   }
 }
 
-template <int S>
+// Product kernel.  WIN: K spans more than one int32 window, so the diagonal
+// sums are flushed to FP64 every OZ_FLUSH chunks; otherwise they are
+// combined in FP64 once, in the epilogue (the same order of additions as
+// a single flush).  The loads run one step ahead of the matrix cores: the B
+// fragments of the next 64-deep K chunk while the current chunk's products
+// run, and the A fragments of the next slice (of the next chunk's slice 0
+// at a chunk's last slice).
+template <int S, bool WIN>
 __global__ __launch_bounds__(256) void k_oz_gemm(const GemmTask* __restrict__ tasks, const OzTask* __restrict__ oz,
                                                  const int32_t* __restrict__ tile_start, int ntasks,
                                                  const int8_t* __restrict__ planes, const int32_t* __restrict__ ex,
@@ -195,20 +202,21 @@ __global__ __launch_bounds__(256) void k_oz_gemm(const GemmTask* __restrict__ ta
   const int64_t sub = (int64_t)16 * z.kp;   // the second 16-row / 16-column sub-tile
 
   v4i acc[S][2][2];
-  double acc64[2][2][4];
 #pragma unroll
   for (int d = 0; d < S; ++d)
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
       for (int b = 0; b < 2; ++b) acc[d][a][b] = v4i{0, 0, 0, 0};
+  double acc64[WIN ? 2 : 1][2][4];
+  if constexpr (WIN) {
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+    for (int a = 0; a < 2; ++a)
 #pragma unroll
-    for (int b = 0; b < 2; ++b)
+      for (int b = 0; b < 2; ++b)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) acc64[a][b][r] = 0.0;
-
+        for (int r = 0; r < 4; ++r) acc64[a][b][r] = 0.0;
+  }
   auto flush = [&]() {
 #pragma unroll
     for (int d = 0; d < S; ++d) {
@@ -225,19 +233,29 @@ __global__ __launch_bounds__(256) void k_oz_gemm(const GemmTask* __restrict__ ta
   };
 
   const int nchunk = z.kp / OZ_T;
+  const int64_t klast = (int64_t)(nchunk - 1) * OZ_T;
+  v4i bv[S][2], av[2];
+#pragma unroll
+  for (int tt = 0; tt < S; ++tt)
+#pragma unroll
+    for (int ni = 0; ni < 2; ++ni) bv[tt][ni] = *reinterpret_cast<const v4i*>(Bp + tt * pb + ni * sub);
+#pragma unroll
+  for (int mi = 0; mi < 2; ++mi) av[mi] = *reinterpret_cast<const v4i*>(Ap + mi * sub);
   for (int c = 0; c < nchunk; ++c) {
+    // the next chunk (the last chunk again at the end: in bounds, unused)
+    const int64_t kn = min((int64_t)(c + 1) * OZ_T, klast);
     const int64_t ko = (int64_t)c * OZ_T;
-    v4i bv[S][2];
+    v4i bn[S][2];
 #pragma unroll
     for (int tt = 0; tt < S; ++tt)
 #pragma unroll
-      for (int ni = 0; ni < 2; ++ni)
-        bv[tt][ni] = *reinterpret_cast<const v4i*>(Bp + tt * pb + ni * sub + ko);
+      for (int ni = 0; ni < 2; ++ni) bn[tt][ni] = *reinterpret_cast<const v4i*>(Bp + tt * pb + ni * sub + kn);
 #pragma unroll
     for (int sa = 0; sa < S; ++sa) {
-      v4i av[2];
+      v4i an[2];
+      const int64_t ao = (sa + 1 < S) ? (int64_t)(sa + 1) * pa + ko : kn;
 #pragma unroll
-      for (int mi = 0; mi < 2; ++mi) av[mi] = *reinterpret_cast<const v4i*>(Ap + sa * pa + mi * sub + ko);
+      for (int mi = 0; mi < 2; ++mi) an[mi] = *reinterpret_cast<const v4i*>(Ap + ao + mi * sub);
 #pragma unroll
       for (int tt = 0; sa + tt < S; ++tt)
 #pragma unroll
@@ -246,10 +264,18 @@ __global__ __launch_bounds__(256) void k_oz_gemm(const GemmTask* __restrict__ ta
           for (int ni = 0; ni < 2; ++ni)
             acc[sa + tt][mi][ni] =
                 __builtin_amdgcn_mfma_i32_16x16x64_i8(av[mi], bv[tt][ni], acc[sa + tt][mi][ni], 0, 0, 0);
+#pragma unroll
+      for (int mi = 0; mi < 2; ++mi) av[mi] = an[mi];
     }
-    if ((c + 1) % OZ_FLUSH == 0) flush();
+    if constexpr (WIN) {
+      if ((c + 1) % OZ_FLUSH == 0) flush();
+    }
+#pragma unroll
+    for (int tt = 0; tt < S; ++tt)
+#pragma unroll
+      for (int ni = 0; ni < 2; ++ni) bv[tt][ni] = bn[tt][ni];
   }
-  flush();
+  if constexpr (WIN) flush();
 
   double* Cb = C + t.offC;
   const int32_t* ea = ex + z.ea;
@@ -260,11 +286,19 @@ __global__ __launch_bounds__(256) void k_oz_gemm(const GemmTask* __restrict__ ta
     for (int ni = 0; ni < 2; ++ni)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
+        double sum;
+        if constexpr (WIN) {
+          sum = acc64[mi][ni][r];
+        } else {
+          sum = 0.0;
+#pragma unroll
+          for (int d = 0; d < S; ++d) sum += ldexp(1.0, -OZ_BETA * (d + 2)) * (double)acc[d][mi][ni][r];
+        }
         const int gi = m0 + wm * 32 + mi * 16 + (lane >> 4) * 4 + r;
         const int gj = n0 + wn * 32 + ni * 16 + (lane & 15);
         if (gi < t.m && gj < t.n) {
           double* cp = Cb + (int64_t)gi + (int64_t)gj * t.ldc;
-          double v = t.alpha * ldexp(acc64[mi][ni][r], ea[gi] + eb[gj]);
+          double v = t.alpha * ldexp(sum, ea[gi] + eb[gj]);
           if (t.beta != 0.0) v += t.beta * (*cp);
           *cp = v;
         }
@@ -275,7 +309,7 @@ template <int S>
 static void oz_launch(int family, const std::vector<GemmTask>& tk, const std::vector<OzTask>& oz,
                       const std::vector<int32_t>& tstart, int64_t tiles, const std::vector<OzUnit>& eunits,
                       const std::vector<OzUnit>& sunits, OzOps o, GemmRunner& R, hipStream_t s, double flops,
-                      double bytes) {
+                      double bytes, int64_t kmax) {
   // tables in one upload: tasks, oz records, tile starts, exponent units, slice units
   auto al = [](size_t n) { return (n + 255) / 256 * 256; };
   const size_t b0 = al(tk.size() * sizeof(GemmTask)), b1 = al(oz.size() * sizeof(OzTask)),
@@ -299,8 +333,12 @@ static void oz_launch(int family, const std::vector<GemmTask>& tk, const std::ve
   for (const OzUnit& u : sunits) { (void)u; sl_bytes += 64.0 * 64.0 * (8.0 + S); }
   plaunch(KF_OTHER, 0, 0, k_oz_exponents, dim3((unsigned)eunits.size()), dim3(256), 0, s, o, deu);
   plaunch(KF_OTHER, sl_bytes, 0, k_oz_slice<S>, dim3((unsigned)sunits.size()), dim3(256), 0, s, o, dsu);
-  plaunch(family, bytes, flops, k_oz_gemm<S>, dim3((unsigned)tiles), dim3(256), 0, s, o.tasks, o.oz, dts,
-          (int)tk.size(), (const int8_t*)o.planes, (const int32_t*)o.ex, R.oz_C_);
+  if (kmax > OZ_FLUSH * OZ_T)
+    plaunch(family, bytes, flops, k_oz_gemm<S, true>, dim3((unsigned)tiles), dim3(256), 0, s, o.tasks, o.oz, dts,
+            (int)tk.size(), (const int8_t*)o.planes, (const int32_t*)o.ex, R.oz_C_);
+  else
+    plaunch(family, bytes, flops, k_oz_gemm<S, false>, dim3((unsigned)tiles), dim3(256), 0, s, o.tasks, o.oz, dts,
+            (int)tk.size(), (const int8_t*)o.planes, (const int32_t*)o.ex, R.oz_C_);
 }
 
 void GemmRunner::run_split(const GemmBatch& b, const double* A, const double* B, double* C,
@@ -318,7 +356,7 @@ void GemmRunner::run_split(const GemmBatch& b, const double* A, const double* B,
     std::vector<OzTask> oz;
     std::vector<int32_t> tstart;
     std::vector<OzUnit> eu, su;
-    int64_t pbytes = 0, nex = 0, tiles = 0;
+    int64_t pbytes = 0, nex = 0, tiles = 0, kmax = 0;
     double flops = 0, bytes = 0;
     int i = i0;
     for (; i < nt; ++i) {
@@ -327,6 +365,7 @@ void GemmRunner::run_split(const GemmBatch& b, const double* A, const double* B,
       NRG_REQUIRE(!t.lower || t.m == t.n, "gemm: a lower-triangle task must be square");
       OzTask z{};
       z.mp = pad(t.m); z.np = pad(t.n); z.kp = pad(t.k);
+      kmax = std::max<int64_t>(kmax, z.kp);
       const int64_t need = (int64_t)slices * ((int64_t)z.mp + z.np) * z.kp;
       if (i > i0 && pbytes + need > OZ_PLANE_BUDGET) break;
       z.pa = pbytes;
@@ -357,12 +396,12 @@ void GemmRunner::run_split(const GemmBatch& b, const double* A, const double* B,
     OzOps o{nullptr, nullptr, A, B, gatherA, gatherB, kscale, d_planes_.p, d_exp_.p};
     oz_C_ = C;
     switch (slices) {
-      case 3: oz_launch<3>(family, tk, oz, tstart, tiles, eu, su, o, *this, s, flops, bytes); break;
-      case 4: oz_launch<4>(family, tk, oz, tstart, tiles, eu, su, o, *this, s, flops, bytes); break;
-      case 5: oz_launch<5>(family, tk, oz, tstart, tiles, eu, su, o, *this, s, flops, bytes); break;
-      case 6: oz_launch<6>(family, tk, oz, tstart, tiles, eu, su, o, *this, s, flops, bytes); break;
-      case 7: oz_launch<7>(family, tk, oz, tstart, tiles, eu, su, o, *this, s, flops, bytes); break;
-      default: oz_launch<8>(family, tk, oz, tstart, tiles, eu, su, o, *this, s, flops, bytes); break;
+      case 3: oz_launch<3>(family, tk, oz, tstart, tiles, eu, su, o, *this, s, flops, bytes, kmax); break;
+      case 4: oz_launch<4>(family, tk, oz, tstart, tiles, eu, su, o, *this, s, flops, bytes, kmax); break;
+      case 5: oz_launch<5>(family, tk, oz, tstart, tiles, eu, su, o, *this, s, flops, bytes, kmax); break;
+      case 6: oz_launch<6>(family, tk, oz, tstart, tiles, eu, su, o, *this, s, flops, bytes, kmax); break;
+      case 7: oz_launch<7>(family, tk, oz, tstart, tiles, eu, su, o, *this, s, flops, bytes, kmax); break;
+      default: oz_launch<8>(family, tk, oz, tstart, tiles, eu, su, o, *this, s, flops, bytes, kmax); break;
     }
     // the chunk's tables and planes are reused by the next chunk
     if (i < nt) NRG_HIP(hipStreamSynchronize(s));

@@ -135,6 +135,12 @@ module nrg_gpu_api
        import :: c_int
        integer(c_int), value :: slices
      end function
+     integer(c_int) function nrg_gpu_set_gemm_split_families(invariant_slices, spectral_slices, min_batch_flops) &
+         bind(C, name="nrg_gpu_set_gemm_split_families")
+       import :: c_int, c_double
+       integer(c_int), value :: invariant_slices, spectral_slices
+       real(c_double), value :: min_batch_flops
+     end function
      integer(c_int) function nrg_gpu_set_batch_limit(max_blocks) bind(C, name="nrg_gpu_set_batch_limit")
        import :: c_int
        integer(c_int), value :: max_blocks

@@ -6,6 +6,12 @@
         with ``--log LOG --family F``: also the mean duration of the family's
         dispatches in the timed steps, located through the launch log that
         bench.py --launch-log wrote in the same run;
+    python -m nrgamd.profsum mfma DIR --kernel REGEX [--mops COUNTER --scale S --peak P]
+        matrix-core utilisation (MfmaUtil) and executed operations over the
+        dispatch durations of a ``rocprofv3 --kernel-trace --pmc MfmaUtil
+        SQ_INSTS_VALU_MFMA_MOPS_F64`` (or ..._I8) run;
+    python -m nrgamd.profsum first-timed --log LOG --kernel K --ints I,J
+        per-kernel ordinal of the first timed dispatch of a kernel variant;
     python -m nrgamd.profsum traffic --log LOG --copy-fetch DIR --copy-write DIR
         --copy-bytes B --variant REGEX FIRST FETCH_DIR WRITE_DIR [--variant ...]
         HBM bytes per dispatch of each kernel variant over the dispatches that
@@ -181,6 +187,46 @@ def traffic(a):
     return "\n".join(lines)
 
 
+def mfma(d, kernel_re, counters, scale, peak_tflops, title, command):
+    """Matrix-core figures of the dispatches of kernels matching kernel_re
+    in a ``rocprofv3 --kernel-trace --pmc MfmaUtil <MOPS counter>`` run:
+    per kernel name the dispatch count, mean MfmaUtil (percent of SIMD
+    cycles with the matrix core busy), executed operations (the MOPS counter
+    x `scale`) over the summed dispatch durations, against `peak_tflops`."""
+    rows = _rows(d, "counter_collection.csv")
+    trace = {int(_col(r, "Dispatch_Id", "Dispatch_ID")): r for r in _rows(d, "kernel_trace.csv")}
+    per = {}
+    for r in rows:
+        name = _col(r, "Kernel_Name")
+        if not re.search(kernel_re, name):
+            continue
+        did = int(_col(r, "Dispatch_Id", "Dispatch_ID"))
+        rec = per.setdefault(re.sub(r"\(.*$", "", name), {}).setdefault(did, {})
+        rec[_col(r, "Counter_Name")] = rec.get(_col(r, "Counter_Name"), 0.0) + float(_col(r, "Counter_Value"))
+    lines = [f"# {title}", "", "Commands (repo root as cwd):", ""] + ["    " + c for c in command] + [
+        "", f"Operations = {counters[1]} x {scale:g} per dispatch; time = the dispatch's kernel-trace "
+            f"duration; peak {peak_tflops:g} TFLOP/s (or TOP/s).", "",
+        "| kernel | dispatches | mean MfmaUtil % | executed T(FL)OP | summed duration ms | achieved T(FL)OP/s | "
+        "fraction of peak |", "|---|---:|---:|---:|---:|---:|---:|"]
+    for name, recs in sorted(per.items()):
+        ops = dur = util = 0.0
+        n = 0
+        for did, rec in recs.items():
+            if did not in trace or counters[1] not in rec:
+                continue
+            t = trace[did]
+            dur += (int(_col(t, "End_Timestamp")) - int(_col(t, "Start_Timestamp"))) * 1e-9
+            ops += rec[counters[1]] * scale
+            util += rec.get(counters[0], 0.0)
+            n += 1
+        if n == 0:
+            continue
+        ach = ops / dur / 1e12 if dur > 0 else 0.0
+        lines.append(f"| `{name}` | {n} | {util / n:.1f} | {ops / 1e12:.3f} | {dur * 1e3:.2f} | {ach:.2f} | "
+                     f"{ach / peak_tflops:.3f} |")
+    return "\n".join(lines) + "\n"
+
+
 def first_timed(log, base, ints, skip=0):
     """Per-kernel ordinal of the first dispatch in the timed steps of the
     kernel `base<ints...>` (template integers as a prefix), plus `skip`."""
@@ -206,6 +252,15 @@ def main(argv=None):
     f.add_argument("--kernel", required=True, help="base name, e.g. k_tr_symv")
     f.add_argument("--ints", default="", help="leading template integers, e.g. 2,16")
     f.add_argument("--skip", type=int, default=0)
+    mf = sub.add_parser("mfma")
+    mf.add_argument("dir")
+    mf.add_argument("--kernel", required=True, help="kernel name regex")
+    mf.add_argument("--util", default="MfmaUtil")
+    mf.add_argument("--mops", default="SQ_INSTS_VALU_MFMA_MOPS_F64")
+    mf.add_argument("--scale", type=float, default=512.0)
+    mf.add_argument("--peak", type=float, default=78.6)
+    mf.add_argument("--title", default="matrix-core counters")
+    mf.add_argument("--command", action="append", default=[])
     t = sub.add_parser("traffic")
     t.add_argument("--log", required=True)
     t.add_argument("--copy-fetch", required=True)
@@ -219,6 +274,8 @@ def main(argv=None):
     a = ap.parse_args(argv)
     if a.cmd == "stats":
         sys.stdout.write(stats(a.dir, a.title, a.command, a.top, a.log, a.family))
+    elif a.cmd == "mfma":
+        sys.stdout.write(mfma(a.dir, a.kernel, (a.util, a.mops), a.scale, a.peak, a.title, a.command))
     elif a.cmd == "first-timed":
         ints = [int(x) for x in a.ints.split(",") if x.strip()]
         print(first_timed(a.log, a.kernel, ints, a.skip))

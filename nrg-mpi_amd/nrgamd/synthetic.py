@@ -213,3 +213,33 @@ def run(batch: Batch, nops: int = 4, split: tuple = (8, 6), repeat: int = 3, sam
     out["contraction"] = res
     out["contraction_flops"] = flops
     return out
+
+
+def main(argv=None):
+    """python -m nrgamd.synthetic [--nblk N] [--median M] [--nops K] [--split 8,6]:
+    one batch on device 0, results as one JSON line (the kernel-profiling
+    workload of the split-precision products)."""
+    import argparse
+    import json
+    import sys
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--seed", type=int, default=20240)
+    ap.add_argument("--nblk", type=int, default=64)
+    ap.add_argument("--median", type=float, default=600.0)
+    ap.add_argument("--sigma", type=float, default=1.0)
+    ap.add_argument("--nops", type=int, default=1)
+    ap.add_argument("--split", default="8,6")
+    ap.add_argument("--repeat", type=int, default=3)
+    a = ap.parse_args(argv)
+    gpu.init(0)
+    b = make_batch(seed=a.seed, nblk=a.nblk, median=a.median, sigma=a.sigma)
+    split = tuple(int(x) for x in a.split.split(",") if x.strip())
+
+    def log(m):
+        print(m, file=sys.stderr, flush=True)
+    r = run(b, nops=a.nops, split=split, repeat=a.repeat, sample=4, log=log)
+    print(json.dumps(r))
+
+
+if __name__ == "__main__":
+    main()
