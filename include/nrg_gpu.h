@@ -302,6 +302,15 @@ int nrg_gpu_launch_log_get(double* out, int64_t cap, int64_t* count);
 int nrg_gpu_copy_calibration(int64_t nbytes, int reps, int* ok);
 int nrg_gpu_launch_log_kernel(int kid, char* name, int cap);
 
+/* Algorithm option with no counterpart in the reference (it replaces the
+ * tridiagonalisation inside dsyev, diagonalization.F90:266-269):
+ * two_stage = 0 reduces each block to tridiagonal form in one stage
+ * (Householder columns with a trailing SYMV, the default); 1 reduces it to a
+ * band of width 32 by panel QRs and matrix-core updates, then the band to
+ * tridiagonal form by bulge chasing, and back-transforms the requested
+ * eigenvector columns through both stages. */
+int nrg_gpu_set_tridiag_method(int two_stage);
+
 /* Scheduling option with no counterpart in the reference: solve the blocks
  * of an iteration in groups of at most max_blocks (0 = all in one batched
  * pass, the default; 1 = one block at a time, the per-block schedule of the

@@ -692,6 +692,13 @@ int nrg_gpu_set_tuning(int mode, const char* cache_path) {
   });
 }
 
+int nrg_gpu_set_tridiag_method(int two_stage) {
+  return guarded([&](Context& C) {
+    need_init(C);
+    C.eig.set_two_stage(two_stage != 0);
+  });
+}
+
 int nrg_gpu_set_tridiag_schedule(int align_end) {
   return guarded([&](Context& C) {
     need_init(C);

@@ -58,11 +58,17 @@ class BatchedEigensolver {
   // Capture the tridiagonalisation's launch sequence as a HIP graph and
   // replay it for a repeated block set (off by default).
   void set_graphs(bool on) { use_graphs_ = on; }
+  // Two-stage tridiagonalisation (dense -> band of 32 -> tridiagonal, sbr.hip)
+  // instead of the one-stage reduction.
+  void set_two_stage(bool on) { two_stage_ = on; }
+  bool two_stage() const { return two_stage_; }
   long long graph_captures() const { return graph_captures_; }
   long long graph_replays() const { return graph_replays_; }
 
  private:
   void tridiag(hipStream_t s);
+  void tridiag_two_stage(hipStream_t s);
+  void back_q2(const int* lo, const int* hi, hipStream_t s);
   void divide_conquer(hipStream_t s);
   void back_transform(const int* lo, const int* hi, hipStream_t s);
   void panel_factors(hipStream_t s);
@@ -131,6 +137,14 @@ class BatchedEigensolver {
   static constexpr size_t kMaxGraphs = 4;
   std::vector<TrGraph> graphs_;
   bool use_graphs_ = false;
+  // two-stage reduction: band, stage-2 reflectors and their offsets
+  bool two_stage_ = false;
+  bool solved_two_stage_ = false;   // how the current solve reduced (its back-transform)
+  int bt_off_ = 1;                  // row offset of a stage-1 reflector's unit element
+  DevBuf<double> d_tq_, d_band_, d_v2_, d_t2_;
+  DevBuf<int64_t> d_sboff_;
+  std::vector<int64_t> h_sboff_;
+  DevBuf<int32_t> d_sbch_, d_err_;
   long long graph_clock_ = 0, graph_captures_ = 0, graph_replays_ = 0;
 };
 

@@ -1293,22 +1293,22 @@ struct BtArgs {
 
 // Explicit reflector panel V (rows c0+1..n-1).  Grid: (chunks, panels);
 // panel i of the list is block pl[2i], panel pl[2i+1].
-__global__ void k_bt_buildv(BtArgs a, const int32_t* pl) {
+__global__ void k_bt_buildv(BtArgs a, const int32_t* pl, int off) {
   const int b = pl[2 * blockIdx.y], p = pl[2 * blockIdx.y + 1];
   const int n = a.n[b];
   const int c0 = p * NBB;
-  if (c0 >= n - 1) return;
-  const int nbp = min(NBB, n - 1 - c0);
-  const int mr = n - c0 - 1;
+  if (c0 >= n - off) return;
+  const int nbp = min(NBB, n - off - c0);
+  const int mr = n - c0 - off;
   const double* A = a.A + a.aoff[b];
   double* V = a.V + a.vpo[b] + (int64_t)p * n * NBB;
   for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < mr * NBB; q += gridDim.x * blockDim.x) {
     const int r = q % mr, i = q / mr;
-    const int row = c0 + 1 + r, c = c0 + i;
+    const int row = c0 + off + r, c = c0 + i;
     double v = 0.0;
     if (i < nbp) {
-      if (row == c + 1) v = 1.0;
-      else if (row > c + 1) v = A[row + (int64_t)c * n];
+      if (row == c + off) v = 1.0;
+      else if (row > c + off) v = A[row + (int64_t)c * n];
     }
     V[r + (int64_t)i * n] = v;
   }
@@ -1320,12 +1320,12 @@ __global__ void k_bt_buildv(BtArgs a, const int32_t* pl) {
 // Row j of the product is shared by the TQ = 4 adjacent lanes 4j..4j+3
 // (terms l = j + sub, j + sub + 4, ...), combined with two xor shuffles.
 constexpr int TQ = 4;
-__global__ __launch_bounds__(NBB * TQ) void k_bt_tmat(BtArgs a, const int32_t* pl) {
+__global__ __launch_bounds__(NBB * TQ) void k_bt_tmat(BtArgs a, const int32_t* pl, int off) {
   const int b = pl[2 * blockIdx.x], p = pl[2 * blockIdx.x + 1];
   const int n = a.n[b];
   const int c0 = p * NBB;
-  if (c0 >= n - 1) return;
-  const int nbp = min(NBB, n - 1 - c0);
+  if (c0 >= n - off) return;
+  const int nbp = min(NBB, n - off - c0);
   const double* G = a.T + (int64_t)(a.tpo[b] + p) * 2 * NBB * NBB;
   double* T = a.T + (int64_t)(a.tpo[b] + p) * 2 * NBB * NBB + NBB * NBB;
   const double* tau = a.tau + a.voff[b] + c0;
@@ -1432,7 +1432,10 @@ void BatchedEigensolver::solve(int nblk, const int* n, const int64_t* aoff, cons
   fill(d_e_.p, 0.0, tot_n_, s);
   fill(d_tau_.p, 0.0, tot_n_, s);
   double t0 = now_ms();
-  tridiag(s);
+  solved_two_stage_ = two_stage_;
+  bt_off_ = two_stage_ ? 32 : 1;
+  if (two_stage_) tridiag_two_stage(s);
+  else tridiag(s);
   if (st) { NRG_HIP(hipStreamSynchronize(s)); st->ms_tridiag = now_ms() - t0; t0 = now_ms(); }
   // the back-transform's panel factors overlap the divide and conquer
   if (sg_[1]) {
@@ -1941,7 +1944,7 @@ void BatchedEigensolver::panel_factors(hipStream_t s) {
   int64_t rows_max = 0;
   for (int b = 0; b < nblk; ++b) {
     const int n = n_[b];
-    const int np = (n - 1 + NBB - 1) / NBB;   // panels of reflectors 0..n-2
+    const int np = std::max(0, (n - bt_off_ + NBB - 1) / NBB);   // panels of reflectors 0..n-1-off
     h_vpo_[b] = vacc;
     h_tpo_[b] = tacc;
     vacc += (int64_t)np * n * NBB;
@@ -1961,20 +1964,20 @@ void BatchedEigensolver::panel_factors(hipStream_t s) {
   d_plist_.upload(h_plist_.data(), h_plist_.size(), s);
   BtArgs ba{d_n_.p, d_aoff_.p, d_voff_.p, A_, d_tau_.p, d_V_.p, d_T_.p, d_vpo_.p, d_tpo_.p};
   hipLaunchKernelGGL(k_bt_buildv, dim3((unsigned)std::max<int64_t>(1, ceil_div(rows_max * NBB, 256 * 4)), npl),
-                     dim3(256), 0, s, ba, (const int32_t*)d_plist_.p);
+                     dim3(256), 0, s, ba, (const int32_t*)d_plist_.p, bt_off_);
   // G = V^T V for every panel
   GemmBatch& bt = batch_bt_;
   bt.clear();
   for (int i = 0; i < npl; ++i) {
     const int b = h_plist_[2 * i], p = h_plist_[2 * i + 1];
     const int n = n_[b], c0 = p * NBB;
-    const int mr = n - c0 - 1;
-    const int nbp = std::min(NBB, n - 1 - c0);
+    const int mr = n - c0 - bt_off_;
+    const int nbp = std::min(NBB, n - bt_off_ - c0);
     const int64_t vo = h_vpo_[b] + (int64_t)p * n * NBB;
     bt.add(gemm_task(nbp, nbp, mr, 1, 0, vo, n, vo, n, (int64_t)(h_tpo_[b] + p) * 2 * NBB * NBB, NBB, 1.0, 0.0));
   }
   gemm_bt_.run(bt, d_V_.p, d_V_.p, d_T_.p, nullptr, nullptr, nullptr, s, KF_GEMM_BACK);
-  plaunch(KF_BT_SMALL, 0, 0, k_bt_tmat, dim3(npl), dim3(NBB * TQ), 0, s, ba, (const int32_t*)d_plist_.p);
+  plaunch(KF_BT_SMALL, 0, 0, k_bt_tmat, dim3(npl), dim3(NBB * TQ), 0, s, ba, (const int32_t*)d_plist_.p, bt_off_);
   NRG_HIP(hipGetLastError());
 }
 
@@ -1987,27 +1990,28 @@ void BatchedEigensolver::back_transform(const int* lo, const int* hi, hipStream_
   }
   d_X_.reserve(tot_n_ * NBB);
   d_X2_.reserve(tot_n_ * NBB);
-  const int npan = (nmax - 1 + NBB - 1) / NBB;
+  const int off = bt_off_;
+  const int npan = std::max(0, (nmax - off + NBB - 1) / NBB);
   // the panels' products are recorded and issued after one table upload
   gemm_.begin();
   for (int p = npan - 1; p >= 0; --p) {
     const int c0 = p * NBB;
-    // active: n - 1 > c0  <=>  n > c0 + 1
+    // active: n - off > c0
     int act = 0;
-    while (act < nblk && n_[order_[act]] > c0 + 1) ++act;
+    while (act < nblk && n_[order_[act]] > c0 + off) ++act;
     if (act == 0) continue;
     auto vo = [&](int b) { return h_vpo_[b] + (int64_t)p * n_[b] * NBB; };
     auto to = [&](int b) { return (int64_t)(h_tpo_[b] + p) * 2 * NBB * NBB; };
-    // X = V^T Z(c0+1:, :nc)
+    // X = V^T Z(c0+off:, :nc)
     batch_.clear();
     for (int q = 0; q < act; ++q) {
       const int b = order_[q];
       const int n = n_[b];
-      const int mr = n - c0 - 1;
-      const int nbp = std::min(NBB, n - 1 - c0);
+      const int mr = n - c0 - off;
+      const int nbp = std::min(NBB, n - off - c0);
       const int nc = hi[b] - lo[b];
       if (nc <= 0) continue;
-      batch_.add(gemm_task(nbp, nc, mr, 1, 0, vo(b), n, aoff_[b] + (int64_t)lo[b] * n + c0 + 1, n,
+      batch_.add(gemm_task(nbp, nc, mr, 1, 0, vo(b), n, aoff_[b] + (int64_t)lo[b] * n + c0 + off, n,
                            voff_[b] * NBB, NBB, 1.0, 0.0));
     }
     if (batch_.tasks.empty()) continue;
@@ -2017,24 +2021,24 @@ void BatchedEigensolver::back_transform(const int* lo, const int* hi, hipStream_
     for (int q = 0; q < act; ++q) {
       const int b = order_[q];
       const int n = n_[b];
-      const int nbp = std::min(NBB, n - 1 - c0);
+      const int nbp = std::min(NBB, n - off - c0);
       const int nc = hi[b] - lo[b];
       if (nc <= 0) continue;
       batch_.add(gemm_task(nbp, nc, nbp, 0, 0, to(b) + NBB * NBB, NBB, voff_[b] * NBB, NBB, voff_[b] * NBB, NBB,
                            1.0, 0.0));
     }
     gemm_.run(batch_, d_T_.p, d_X_.p, d_X2_.p, nullptr, nullptr, nullptr, s, KF_GEMM_BACK);
-    // Z(c0+1:, :nc) -= V X2
+    // Z(c0+off:, :nc) -= V X2
     batch_.clear();
     for (int q = 0; q < act; ++q) {
       const int b = order_[q];
       const int n = n_[b];
-      const int mr = n - c0 - 1;
-      const int nbp = std::min(NBB, n - 1 - c0);
+      const int mr = n - c0 - off;
+      const int nbp = std::min(NBB, n - off - c0);
       const int nc = hi[b] - lo[b];
       if (nc <= 0) continue;
       batch_.add(gemm_task(mr, nc, nbp, 0, 0, vo(b), n, voff_[b] * NBB, NBB,
-                           aoff_[b] + (int64_t)lo[b] * n + c0 + 1, n, -1.0, 1.0));
+                           aoff_[b] + (int64_t)lo[b] * n + c0 + off, n, -1.0, 1.0));
     }
     gemm_.run(batch_, d_V_.p, d_X2_.p, Z_, nullptr, nullptr, nullptr, s, KF_GEMM_BACK);
   }
@@ -2109,6 +2113,7 @@ void BatchedEigensolver::back_columns(const int* hi, hipStream_t s, double* ms) 
   if (!any) return;
   const double t0 = now_ms();
   materialize(h.data(), s);
+  if (solved_two_stage_) back_q2(lo.data(), h.data(), s);
   back_transform(lo.data(), h.data(), s);
   done_ = h;
   if (ms) { NRG_HIP(hipStreamSynchronize(s)); *ms += now_ms() - t0; }

@@ -163,9 +163,10 @@ __global__ __launch_bounds__(256) void k_oz_slice(OzOps o, const OzUnit* __restr
 // a single flush).  The loads run one step ahead of the matrix cores: the B
 // fragments of the next 64-deep K chunk while the current chunk's products
 // run, and the A fragments of the next slice (of the next chunk's slice 0
-// at a chunk's last slice).
-template <int S, bool WIN>
-__global__ __launch_bounds__(256) void k_oz_gemm(const GemmTask* __restrict__ tasks, const OzTask* __restrict__ oz,
+// at a chunk's last slice).  With S >= 7 the B fragments are loaded at the
+// start of their own chunk (one wave per SIMD: no room for a second set).
+template <int S, bool WIN, bool PF = (S <= 6)>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_oz_gemm(const GemmTask* __restrict__ tasks, const OzTask* __restrict__ oz,
                                                  const int32_t* __restrict__ tile_start, int ntasks,
                                                  const int8_t* __restrict__ planes, const int32_t* __restrict__ ex,
                                                  double* __restrict__ C) {
@@ -245,11 +246,21 @@ __global__ __launch_bounds__(256) void k_oz_gemm(const GemmTask* __restrict__ ta
     // the next chunk (the last chunk again at the end: in bounds, unused)
     const int64_t kn = min((int64_t)(c + 1) * OZ_T, klast);
     const int64_t ko = (int64_t)c * OZ_T;
-    v4i bn[S][2];
+    v4i bn[PF ? S : 1][2];
+    if constexpr (PF) {
 #pragma unroll
-    for (int tt = 0; tt < S; ++tt)
+      for (int tt = 0; tt < S; ++tt)
 #pragma unroll
-      for (int ni = 0; ni < 2; ++ni) bn[tt][ni] = *reinterpret_cast<const v4i*>(Bp + tt * pb + ni * sub + kn);
+        for (int ni = 0; ni < 2; ++ni) bn[tt][ni] = *reinterpret_cast<const v4i*>(Bp + tt * pb + ni * sub + kn);
+    } else if (c > 0) {
+      // S >= 7: one wave per SIMD already; the B fragments of this chunk
+      // are loaded here instead of a chunk ahead (the registers of a second
+      // set would not fit)
+#pragma unroll
+      for (int tt = 0; tt < S; ++tt)
+#pragma unroll
+        for (int ni = 0; ni < 2; ++ni) bv[tt][ni] = *reinterpret_cast<const v4i*>(Bp + tt * pb + ni * sub + ko);
+    }
 #pragma unroll
     for (int sa = 0; sa < S; ++sa) {
       v4i an[2];
@@ -270,10 +281,12 @@ __global__ __launch_bounds__(256) void k_oz_gemm(const GemmTask* __restrict__ ta
     if constexpr (WIN) {
       if ((c + 1) % OZ_FLUSH == 0) flush();
     }
+    if constexpr (PF) {
 #pragma unroll
-    for (int tt = 0; tt < S; ++tt)
+      for (int tt = 0; tt < S; ++tt)
 #pragma unroll
-      for (int ni = 0; ni < 2; ++ni) bv[tt][ni] = bn[tt][ni];
+        for (int ni = 0; ni < 2; ++ni) bv[tt][ni] = bn[tt][ni];
+    }
   }
   if constexpr (WIN) flush();
 

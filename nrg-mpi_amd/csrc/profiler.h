@@ -34,6 +34,8 @@ enum KFamily : int {
   KF_CFS_BIN,
   KF_GEMM_RHO,
   KF_RHO_GATHER,
+  KF_TR_BAND,
+  KF_BT_BAND,
   KF_OTHER,
   KF_COUNT
 };

@@ -47,6 +47,7 @@ SIGNATURES = {
     "nrg_gpu_profile_sample": (ctypes.c_int, [ctypes.c_int]),
     "nrg_gpu_launch_log": (ctypes.c_int, [ctypes.c_int]),
     "nrg_gpu_work_counters": (ctypes.c_int, [_f64p, ctypes.c_int, ctypes.c_int]),
+    "nrg_gpu_set_tridiag_method": (ctypes.c_int, [ctypes.c_int]),
     "nrg_gpu_set_gemm_split_families": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_double]),
     "nrg_gpu_copy_calibration": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
     "nrg_gpu_launch_log_get": (ctypes.c_int, [_f64p, ctypes.c_int64, _i64p]),
@@ -79,7 +80,7 @@ SIGNATURES = {
 KERNEL_FAMILIES = ["assemble", "tridiag_col", "tridiag_symv", "tridiag_gemm", "dc_leaf", "dc_prep",
                    "dc_secular", "dc_vectors", "dc_gemm", "dc_finish", "back_small", "back_gemm",
                    "invariant_gemm", "spectral_gemm", "cfs_products", "cfs_rho_weight", "cfs_binning",
-                   "rho_gemm", "rho_gather", "other"]
+                   "rho_gemm", "rho_gather", "tridiag_band", "back_band", "other"]
 
 
 class NrgGpuError(RuntimeError):
