@@ -513,11 +513,17 @@ __global__ __launch_bounds__(SB2_WAVES * 64) void k_sb2st(Sb2Args a) {
       if (lane == 0) a.tau2[to] = tau;
       if (h == 0) svp[c] = vc;
       tau_p = tau;
+      // the task's band stores must have completed before its progress word
+      // is published: the workgroup-scope release orders the LDS store after
+      // earlier LDS operations only, so the store counter is drained first
+      __builtin_amdgcn_s_waitcnt(0);
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
       if (lane == 0)
         __hip_atomic_store(&s_prog[wv], (j << 10) | (k + 1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
       __builtin_amdgcn_wave_barrier();
     }
+    __builtin_amdgcn_s_waitcnt(0);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     if (lane == 0)
       __hip_atomic_store(&s_prog[wv], (j << 10) | 1023, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
   }
