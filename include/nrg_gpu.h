@@ -310,6 +310,11 @@ int nrg_gpu_launch_log_kernel(int kid, char* name, int cap);
  * tridiagonal form by bulge chasing, and back-transforms the requested
  * eigenvector columns through both stages. */
 int nrg_gpu_set_tridiag_method(int two_stage);
+/* Launch option with no counterpart in the reference: the number of stream
+ * groups (1..4) the blocks of the tridiagonalisation are split over, in
+ * place of the tuner's choice; 0 returns the choice to the tuner (the
+ * default, 2 groups, with tuning off). */
+int nrg_gpu_set_tridiag_groups(int groups);
 
 /* Scheduling option with no counterpart in the reference: solve the blocks
  * of an iteration in groups of at most max_blocks (0 = all in one batched

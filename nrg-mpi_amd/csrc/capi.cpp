@@ -305,7 +305,7 @@ int nrg_gpu_diagonalize(int iteration, int nsub, const int32_t* subspace_info, i
         tkey = Tuner::key(C.bn.data(), nsub);
         cfg = C.tuner.choose(tkey, &explore);
       }
-      C.eig.set_groups(cfg.groups);
+      C.eig.set_groups(C.groups_override > 0 ? C.groups_override : cfg.groups);
       C.eig.set_symv_wide(cfg.symv_wide_wg, 64.0e6);
       C.eig.set_graphs(C.graphs_on && !explore);
       C.eig.solve(nsub, C.bn.data(), C.boff.data(), C.bvoff.data(), C.d_H.p, C.d_U.p, C.d_w.p, defer,
@@ -689,6 +689,15 @@ int nrg_gpu_set_tuning(int mode, const char* cache_path) {
     need_init(C);
     NRG_REQUIRE(mode >= 0 && mode <= 2, "set_tuning: mode must be 0, 1 or 2");
     C.tuner.set(mode, cache_path ? std::string(cache_path) : std::string());
+  });
+}
+
+int nrg_gpu_set_tridiag_groups(int groups) {
+  return guarded([&](Context& C) {
+    need_init(C);
+    NRG_REQUIRE(groups >= 0 && groups <= 4, "set_tridiag_groups: groups must be 0..4");
+    C.groups_override = groups;
+    C.eig.set_groups(groups > 0 ? groups : Tuner::candidates()[0].groups);
   });
 }
 

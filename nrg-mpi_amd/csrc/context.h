@@ -36,6 +36,7 @@ struct Context {
   // tridiagonalisation launch options: graphs (nrg_gpu_set_graphs) and the
   // histogram-keyed tuner (nrg_gpu_set_tuning)
   bool graphs_on = false;
+  int groups_override = 0;   // stream groups of the tridiagonalisation (0: tuner / default)
   Tuner tuner;
 
   // ---- model constants (set once)

@@ -391,7 +391,10 @@ __global__ __launch_bounds__(SB2_WAVES * 64) void k_sb2st(Sb2Args a) {
           }
           __builtin_amdgcn_s_sleep(1);
         }
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        // agent-scope acquire: invalidates this CU's L1, so that the band
+        // lines another wave of the workgroup stored are not served from a
+        // copy this wave cached before
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
       }
       if (__hip_atomic_load(&s_abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) break;
       const int r0 = j + 1 + k * NBS;
