@@ -19,7 +19,9 @@
 //   * kb + 1 column launches (kb = min(32, m) reflectors), each over the
 //     64-row tiles of every active block: launch jc completes reflector
 //     jc - 1 from the partial sums the previous launch recorded (norm of the
-//     column below its diagonal, its products with the other panel columns),
+//     column below its diagonal, its products with the other panel columns)
+//     and the copy of row r0 + jc - 1 it made (ping-pong buffers: tile 0
+//     rewrites that row during the launch that reads it),
 //     applies it to the panel columns to its right, stores v in place below
 //     the band, builds column jc - 1 of the compact-WY factor T, and records
 //     the partial sums of the updated column jc (one launch per column: the
@@ -162,7 +164,9 @@ __global__ __launch_bounds__(256) void k_sb_col(SbArgs a, int t, int jc, int act
       for (int q = 0; q < ntile; ++q) s += rc[(int64_t)q * SBREC + tid];
       s_sum[tid] = s;
     } else if (tid >= 64 && tid < 64 + NBS) {
-      s_row[tid - 64] = A[(r0 + j) + (int64_t)(c0 + tid - 64) * ld];   // row r0 + j of the panel
+      // row r0 + j of the panel as launch j left it, from the copy launch j
+      // made (tile 0 rewrites the row itself during this launch)
+      s_row[tid - 64] = a.rec[sl.pRec + 2 * (int64_t)sl.ntile * SBREC + (j & 1) * NBS + (tid - 64)];
     }
     __syncthreads();
     const double xn2 = s_sum[0];
@@ -221,6 +225,12 @@ __global__ __launch_bounds__(256) void k_sb_col(SbArgs a, int t, int jc, int act
   }
 
   if (jc < kb) {
+    // row r0 + jc as this launch leaves it, for the next launch
+    if (tile == 0 && lane == jc) {
+      double* rowc = a.rec + sl.pRec + 2 * (int64_t)sl.ntile * SBREC + (jc & 1) * NBS;
+#pragma unroll
+      for (int u = 0; u < 8; ++u) rowc[8 * wv + u] = x[u];
+    }
     // partial sums of the updated column jc over this tile's rows r > r0 + jc
     if (wv == (jc >> 3)) {
       double xr = 0.0;
@@ -598,7 +608,7 @@ void BatchedEigensolver::tridiag_two_stage(hipStream_t s) {
     const int n = n_[b];
     poff[b] = voff_[b] * PCOL;
     roff[b] = racc;
-    racc += 2 * (int64_t)((std::max(n - NBS, 1) + SBR - 1) / SBR) * SBREC;
+    racc += 2 * (int64_t)((std::max(n - NBS, 1) + SBR - 1) / SBR) * SBREC + 2 * NBS;   // + two row copies
     tqoff[b] = (int64_t)b * (64 * NBS + NBS * NBS);
     boff[b] = bacc;
     bacc += (int64_t)n * LB;
