@@ -99,6 +99,8 @@ def parse():
     ap.add_argument("--no-graph-leg", dest="graph_leg", action="store_false",
                     help="skip the three-iteration leg with HIP graphs (profiled runs skip it: under "
                          "rocprofv3 --kernel-trace its first graph launch failed)")
+    ap.add_argument("--no-two-stage-leg", dest="two_stage_leg", action="store_false",
+                    help="skip the three-iteration leg with the two-stage tridiagonalisation")
     ap.add_argument("--no-plain", dest="plain", action="store_false",
                     help="skip the three-iteration plain (one block per pass) comparison")
     return ap.parse_args()
@@ -317,7 +319,8 @@ def main():
         print(open(a.tuning_cache).read())
         return
     need = a.warmup + a.steps
-    legs = 3 * (int(a.plain) + int(a.split_leg > 0 or a.split > 0) + 1 + int(a.graph_leg))
+    legs = 3 * (int(a.plain) + int(a.split_leg > 0 or a.split > 0) + 1 + int(a.graph_leg) +
+                int(a.two_stage_leg and a.tridiag == "one-stage"))
     if need + legs > spec.niter:
         raise SystemExit(f"warmup+steps+legs={need + legs} exceeds the {spec.niter} iterations of {a.workload}")
     tmp = tempfile.mkdtemp(prefix="nrgbench_")
@@ -436,6 +439,15 @@ def main():
         st_g = gpu.stage_times()
         with_graphs["graph_captures"] = int(st_g["graph_captures"]) - gpu_graphs[0]
         with_graphs["graph_replays"] = int(st_g["graph_replays"]) - gpu_graphs[1]
+    two_stage = None
+    if a.two_stage_leg and a.tridiag == "one-stage":
+        gpu.load().nrg_gpu_set_tridiag_method(1)
+        try:
+            two_stage = leg(0, 0, "batched FP64, two-stage tridiagonalisation")
+        finally:
+            gpu.load().nrg_gpu_set_tridiag_method(0)
+        two_stage["form"] = ("batched FP64 eigensolve with the two-stage tridiagonalisation (band of 32 by panel QRs "
+                             "and matrix-core updates, bulge chasing one workgroup per block), FP64 contractions")
     if a.plain:
         # one block per eigensolver pass (the per-block schedule of the
         # reference's dsyev loop) with FP64 contraction products
@@ -501,7 +513,9 @@ def main():
                            if tuned else "off", "graph_captures": gpu_graphs[0], "graph_replays": gpu_graphs[1]},
         "batched_no_graphs_default_config": untuned,
         "batched_graphs_tuned": with_graphs,
+        "batched_two_stage": two_stage,
     }
+    ks = {f: v for f, v in ks.items() if v.get("launches") and v.get("ms", 0) > 0}
     if ks:
         fam = max(ks, key=lambda f: ks[f]["ms"])
         k = ks[fam]
