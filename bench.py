@@ -77,6 +77,9 @@ def parse():
     ap.add_argument("--tridiag", choices=["one-stage", "two-stage"], default="one-stage",
                     help="tridiagonalisation: one-stage Householder columns with a trailing SYMV, or two-stage "
                          "(band of 32 by panel QRs and matrix-core updates, then bulge chasing)")
+    ap.add_argument("--symv", choices=["atomic", "rows"], default="atomic",
+                    help="SYMV of the one-stage reduction: row parts through LDS atomics, or summed in registers and "
+                         "LDS row chunks in a fixed order")
     ap.add_argument("--tridiag-align", choices=["end", "start"], default="end",
                     help="column schedule of the batched tridiagonalisation (blocks end or start together)")
     ap.add_argument("--split", type=int, default=0,
@@ -328,6 +331,7 @@ def main():
     h = Host(tmp, device=0, sort_mode=0 if a.sort == "exact" else 1)
     gpu.load().nrg_gpu_set_tridiag_schedule(1 if a.tridiag_align == "end" else 0)
     gpu.load().nrg_gpu_set_tridiag_method(1 if a.tridiag == "two-stage" else 0)
+    gpu.load().nrg_gpu_set_tridiag_symv(1 if a.symv == "rows" else 0)
     gpu.load().nrg_gpu_set_gemm_split(a.split)
     tuned = a.tuning == "cache" and os.path.exists(a.tuning_cache)
     gpu.load().nrg_gpu_set_tuning(2 if tuned else 0, a.tuning_cache.encode() if tuned else None)
@@ -498,7 +502,7 @@ def main():
                             "Lambda=2, D=1, local dim 16, labels (Q,2Sz)",
                    "nkeep": spec.nkeep, "timed_iterations": [its[a.warmup], its[-1]],
                    "states_per_iteration": nfull, "blocks_per_iteration": nsub, "largest_block": nmax,
-                   "sort": a.sort, "tridiag_schedule": a.tridiag_align, "tridiag": a.tridiag,
+                   "sort": a.sort, "tridiag_schedule": a.tridiag_align, "tridiag": a.tridiag, "symv": a.symv,
                    "form": ("batched eigensolve (FP64) + split-precision contraction GEMMs "
                             f"({a.split} slices on int8 matrix cores)") if a.split > 0 else
                            "batched eigensolve (FP64) + FP64 contraction GEMMs"},

@@ -315,6 +315,11 @@ int nrg_gpu_set_tridiag_method(int two_stage);
  * place of the tuner's choice; 0 returns the choice to the tuner (the
  * default, 2 groups, with tuning off). */
 int nrg_gpu_set_tridiag_groups(int groups);
+/* Kernel option with no counterpart in the reference: the SYMV of the
+ * one-stage tridiagonalisation adds the row parts of its columns with LDS
+ * atomics (kind 0, the default) or sums them in registers and 256-row LDS
+ * chunks in a fixed order (kind 1, bit-reproducible). */
+int nrg_gpu_set_tridiag_symv(int kind);
 
 /* Scheduling option with no counterpart in the reference: solve the blocks
  * of an iteration in groups of at most max_blocks (0 = all in one batched

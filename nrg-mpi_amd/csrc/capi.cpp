@@ -692,6 +692,14 @@ int nrg_gpu_set_tuning(int mode, const char* cache_path) {
   });
 }
 
+int nrg_gpu_set_tridiag_symv(int kind) {
+  return guarded([&](Context& C) {
+    need_init(C);
+    NRG_REQUIRE(kind == 0 || kind == 1, "set_tridiag_symv: kind must be 0 or 1");
+    C.eig.set_symv_rows(kind == 1);
+  });
+}
+
 int nrg_gpu_set_tridiag_groups(int groups) {
   return guarded([&](Context& C) {
     need_init(C);

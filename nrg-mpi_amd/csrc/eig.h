@@ -61,6 +61,9 @@ class BatchedEigensolver {
   // Two-stage tridiagonalisation (dense -> band of 32 -> tridiagonal, sbr.hip)
   // instead of the one-stage reduction.
   void set_two_stage(bool on) { two_stage_ = on; }
+  // SYMV of the one-stage reduction: row parts added with LDS atomics
+  // (false) or in register/LDS row chunks in a fixed order (true)
+  void set_symv_rows(bool on) { symv_rows_ = on; }
   bool two_stage() const { return two_stage_; }
   long long graph_captures() const { return graph_captures_; }
   long long graph_replays() const { return graph_replays_; }
@@ -139,6 +142,7 @@ class BatchedEigensolver {
   bool use_graphs_ = false;
   // two-stage reduction: band, stage-2 reflectors and their offsets
   bool two_stage_ = false;
+  bool symv_rows_ = false;
   bool solved_two_stage_ = false;   // how the current solve reduced (its back-transform)
   int bt_off_ = 1;                  // row offset of a stage-1 reflector's unit element
   DevBuf<double> d_tq_, d_band_, d_v2_, d_t2_;

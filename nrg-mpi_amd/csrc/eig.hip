@@ -518,6 +518,188 @@ __global__ __launch_bounds__(NW * 64) void k_tr_symv(TrArgs a, int t, int jj, in
   }
 }
 
+// Row-chunk SYMV: the same product as k_tr_symv without LDS atomics.  The
+// workgroup walks the rows of its panel in chunks of SR = 256; in a chunk
+// each wave streams its CPW columns (4 rows per lane and column, the next
+// chunk's loads issued before this chunk's arithmetic), keeps the column
+// dots in registers and sums its row parts A(i,j) v_j over its columns in
+// registers; the 16 waves' row parts go to a double-buffered LDS array and
+// 256 threads add them in a fixed wave order (one barrier per chunk).  The
+// first chunk's rows wait in LDS for the column dots (they hold the panel's
+// own rows).  Results are the same from run to run bit for bit.
+constexpr int SR = 256;
+template <int CPW, int VL>
+__global__ __launch_bounds__(1024) void k_tr_symv_rows(TrArgs a, int t, int jj, int act, int panel_major) {
+  constexpr int NW = 16, PW = CPW * NW;
+  extern __shared__ double ax[];   // VL: v of rows [j0, m)
+  __shared__ double s_part[2][NW][SR];
+  __shared__ double s_first[SR];
+  __shared__ double s_cd[PW];
+  __shared__ double s_red[NW];
+  __shared__ double s_u[NW][64];
+  int qs, pan;
+  auto tiles = [&](int q) {
+    const int cq = t - a.slot[q].coff, mq = a.slot[q].n - cq - 1;
+    return (cq >= 0 && mq > 0) ? (mq + PW - 1) / PW : 0;
+  };
+  TrSlot sl;
+  if (panel_major) {
+    qs = (int)blockIdx.x % act;
+    pan = (int)blockIdx.x / act;
+    sl = a.slot[qs];
+    TR_PIN("s"(sl.n), "s"(sl.coff), "s"(sl.aoff), "s"(sl.voff), "s"(sl.pP), "s"(sl.pRec), "s"(sl.pPy),
+           "s"(a.A), "s"(a.P), "s"(a.part), "s"(a.py), "s"(a.tau), "s"(a.e));
+    const int cq = t - sl.coff, mq = sl.n - cq - 1;
+    if (pan >= ((cq >= 0 && mq > 0) ? (mq + PW - 1) / PW : 0)) return;
+  } else {
+    if (!flat_map((int)blockIdx.x, act, tiles, qs, pan)) return;
+    sl = a.slot[qs];
+  }
+  const int c = t - sl.coff;
+  const int n = sl.n;
+  const int m = n - c - 1;
+  const int j0 = pan * PW;
+  const int64_t ld = n;
+  const double* A = a.A + sl.aoff;
+  double* P = a.P + sl.pP;
+  const double* colrec = a.part + sl.pRec;
+  double* symrec = a.part + sl.pRec + colrec_size(n);
+  double* ubuf = symrec + symrec_size(n);
+  double* py = a.py + sl.pPy + (int64_t)pan * n;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wv = tid >> 6;
+
+  // reflector from the updated column c (identical in every workgroup)
+  const int ncol = (n - c + RA - 1) / RA;
+  double xn2;
+  {
+    double sl2 = 0.0;
+    for (int q = lane; q < ncol; q += 64) sl2 += colrec[(int64_t)q * USTRIDE + 2 * NB];
+    xn2 = wave_sum_full(sl2);
+  }
+  const double alpha = A[(c + 1) + c * ld];
+  double beta, tau, scal;
+  if (xn2 == 0.0) {
+    beta = alpha; tau = 0.0; scal = 0.0;
+  } else {
+    beta = -copysign(sqrt(alpha * alpha + xn2), alpha);
+    tau = (beta - alpha) / beta;
+    scal = 1.0 / (alpha - beta);
+  }
+  const double* vcol = A + (c + 1) + (int64_t)c * ld;
+  auto vv = [&](int i) { return (i == 0) ? 1.0 : vcol[i] * scal; };
+  if (pan == 0) {
+    if (tid == 0) {
+      a.e[sl.voff + c] = beta;
+      a.tau[sl.voff + c] = tau;
+    }
+    double s = 0.0;
+    if (lane < 2 * jj) {
+      const int slot = (lane < jj) ? lane : NB + (lane - jj);
+      for (int q = wv; q < ncol; q += NW) s += colrec[(int64_t)q * USTRIDE + slot];
+    }
+    s_u[wv][lane] = s;
+  }
+  if (VL)
+    for (int i = j0 + tid; i < m; i += NW * 64) ax[i - j0] = vv(i);
+  auto vr = [&](int i) { return VL ? ax[i - j0] : vv(i); };
+  __syncthreads();
+  if (pan == 0 && tid < 2 * jj) {
+    const int col = (tid < jj) ? tid : NB + (tid - jj);
+    double s = 0.0;
+    for (int w = 0; w < NW; ++w) s += s_u[w][tid];
+    ubuf[col] = P[(c + 1) + (int64_t)col * ld] + scal * s;
+  }
+
+  const double* At = A + (c + 1) + (int64_t)(c + 1) * ld;   // At[i + j*ld] = A(c+1+i, c+1+j)
+  int jq[CPW];
+  double vjq[CPW], cd[CPW];
+#pragma unroll
+  for (int q = 0; q < CPW; ++q) {
+    jq[q] = j0 + wv * CPW + q;
+    vjq[q] = (jq[q] < m) ? vr(jq[q]) : 0.0;
+    cd[q] = 0.0;
+  }
+  const int nch = (m - j0 + SR - 1) / SR;
+  double xn[CPW][4];
+  auto load = [&](int ch) {
+    const int base = j0 + ch * SR;
+#pragma unroll
+    for (int q = 0; q < CPW; ++q)
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int i = base + lane + 64 * u;
+        xn[q][u] = (jq[q] < m && i >= jq[q] && i < m) ? At[i + (int64_t)jq[q] * ld] : 0.0;
+      }
+  };
+  load(0);
+  double yvp = 0.0;
+  for (int ch = 0; ch < nch; ++ch) {
+    double x[CPW][4];
+#pragma unroll
+    for (int q = 0; q < CPW; ++q)
+#pragma unroll
+      for (int u = 0; u < 4; ++u) x[q][u] = xn[q][u];
+    if (ch + 1 < nch) load(ch + 1);
+    const int base = j0 + ch * SR;
+    double (*Pp)[SR] = s_part[ch & 1];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int i = base + lane + 64 * u;
+      const double vi = (i < m) ? vr(i) : 0.0;
+      double ra = 0.0;
+#pragma unroll
+      for (int q = 0; q < CPW; ++q) {
+        cd[q] += x[q][u] * vi;
+        if (i > jq[q]) ra += x[q][u] * vjq[q];
+      }
+      Pp[wv][lane + 64 * u] = ra;
+    }
+    __syncthreads();
+    if (tid < SR) {
+      double s = 0.0;
+#pragma unroll
+      for (int w = 0; w < NW; ++w) s += Pp[w][tid];
+      const int i = base + tid;
+      if (ch == 0) s_first[tid] = s;
+      else if (i < m) {
+        py[i] = s;
+        yvp += s * vr(i);
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < CPW; ++q) {
+    const double tq = wave_sum_full(cd[q]);
+    if (lane == 0) s_cd[wv * CPW + q] = tq;
+  }
+  __syncthreads();
+  if (tid < SR) {
+    const int i = j0 + tid;
+    if (i < m) {
+      const double s = s_first[tid] + ((tid < PW) ? s_cd[tid] : 0.0);
+      py[i] = s;
+      yvp += s * vr(i);
+    }
+  }
+  if (tid < PW && j0 + tid < m) {
+    const int i = j0 + tid;
+    const int r = c + 1 + i;
+    const double vi = vr(i);
+    P[r + jj * ld] = vi;
+    P[r + (3 * NB + jj) * ld] = vi;
+  }
+  yvp = wave_sum_full(yvp);
+  if (lane == 0) s_red[wv] = yvp;
+  __syncthreads();
+  if (tid == 0) {
+    double t2 = 0.0;
+    for (int w = 0; w < NW; ++w) t2 += s_red[w];
+    symrec[pan] = t2;
+  }
+}
+
 // ===========================================================================
 // 2. divide and conquer
 // ===========================================================================
@@ -1483,6 +1665,17 @@ static size_t symv_static_lds() {
 static bool symv_lds_fits(int m, int per_row) {
   return (size_t)per_row * (size_t)std::max(m, 1) * sizeof(double) + symv_static_lds() <= 160 * 1024;
 }
+// the same for the row-chunk SYMV, whose only dynamic LDS is v (VL = 1)
+static bool symv_rows_v_fits(int m) {
+  static size_t st = 0;
+  if (!st) {
+    hipFuncAttributes fa{}, fb{};
+    NRG_HIP(hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(k_tr_symv_rows<2, 1>)));
+    NRG_HIP(hipFuncGetAttributes(&fb, reinterpret_cast<const void*>(k_tr_symv_rows<1, 1>)));
+    st = std::max(fa.sharedSizeBytes, fb.sharedSizeBytes);
+  }
+  return (size_t)std::max(m, 1) * sizeof(double) + st <= 160 * 1024;
+}
 
 void BatchedEigensolver::tridiag(hipStream_t s) {
   const int nblk = nblk_;
@@ -1507,7 +1700,7 @@ void BatchedEigensolver::tridiag(hipStream_t s) {
   // every slot sees the same panel phase t mod NB.
   std::vector<int32_t> coff(nblk);
   for (int q = 0; q < nblk; ++q) coff[q] = align_end_ ? ((nmax - n_[order_[q]]) / NB) * NB : 0;
-  NRG_REQUIRE(symv_lds_fits(nmax - 1, 1),
+  NRG_REQUIRE(symv_rows_ || symv_lds_fits(nmax - 1, 1),
               "eigensolver: block too large for the SYMV stage (its row-part vector exceeds the LDS)");
   int T = 0;
   for (int q = 0; q < nblk; ++q) T = std::max(T, coff[q] + n_[order_[q]]);
@@ -1712,9 +1905,11 @@ void BatchedEigensolver::tridiag(hipStream_t s) {
       if (m > 0) tmax = std::max(tmax, ceil_div(m, pw));
     }
     if (gB > 0) {
-      // LDS: row parts and (VL) v of rows [j0, m) of the longest trailing block
-      const bool vl = symv_lds_fits(mB, 2);
-      const uint32_t shB = (uint32_t)((vl ? 2 : 1) * mB * sizeof(double));
+      // LDS: row parts and (VL) v of rows [j0, m) of the longest trailing
+      // block (the row-chunk SYMV keeps only v there)
+      const bool vl = symv_rows_ ? symv_rows_v_fits(mB) : symv_lds_fits(mB, 2);
+      const uint32_t shB = symv_rows_ ? (uint32_t)(vl ? mB * sizeof(double) : 0)
+                                      : (uint32_t)((vl ? 2 : 1) * mB * sizeof(double));
       // panel-major order when the panel counts are nearly uniform
       const int pm = (align_end_ && (int64_t)act * tmax <= gB + gB / 8) ? 1 : 0;
       const int g = pm ? act * tmax : gB;
@@ -1730,7 +1925,16 @@ void BatchedEigensolver::tridiag(hipStream_t s) {
             NRG_HIP(hipEventRecord(evf_[h], grp[h].s));
             NRG_HIP(hipStreamWaitEvent(G.s, evf_[h], 0));
           }
-      if (pw == 2 * CG && vl)
+      if (symv_rows_) {
+        if (pw == 2 * CG && vl)
+          plaunch(KF_TR_SYMV, bytes, flops, k_tr_symv_rows<2, 1>, dim3(g), dim3(1024), shB, G.s, ta, t, jj, act, pm);
+        else if (vl)
+          plaunch(KF_TR_SYMV, bytes, flops, k_tr_symv_rows<1, 1>, dim3(g), dim3(1024), shB, G.s, ta, t, jj, act, pm);
+        else if (pw == 2 * CG)
+          plaunch(KF_TR_SYMV, bytes, flops, k_tr_symv_rows<2, 0>, dim3(g), dim3(1024), shB, G.s, ta, t, jj, act, pm);
+        else
+          plaunch(KF_TR_SYMV, bytes, flops, k_tr_symv_rows<1, 0>, dim3(g), dim3(1024), shB, G.s, ta, t, jj, act, pm);
+      } else if (pw == 2 * CG && vl)
         plaunch(KF_TR_SYMV, bytes, flops, k_tr_symv<2, 16, SYMV_U, 1>, dim3(g), dim3(1024), shB, G.s, ta, t, jj, act, pm);
       else if (vl)
         plaunch(KF_TR_SYMV, bytes, flops, k_tr_symv<1, 16, SYMV_U, 1>, dim3(g), dim3(1024), shB, G.s, ta, t, jj, act, pm);

@@ -49,6 +49,7 @@ SIGNATURES = {
     "nrg_gpu_work_counters": (ctypes.c_int, [_f64p, ctypes.c_int, ctypes.c_int]),
     "nrg_gpu_set_tridiag_method": (ctypes.c_int, [ctypes.c_int]),
     "nrg_gpu_set_tridiag_groups": (ctypes.c_int, [ctypes.c_int]),
+    "nrg_gpu_set_tridiag_symv": (ctypes.c_int, [ctypes.c_int]),
     "nrg_gpu_set_gemm_split_families": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_double]),
     "nrg_gpu_copy_calibration": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
     "nrg_gpu_launch_log_get": (ctypes.c_int, [_f64p, ctypes.c_int64, _i64p]),
