@@ -56,6 +56,7 @@ namespace nrg {
 constexpr int NBS = 32;            // band width
 constexpr int SBR = 64;            // rows per workgroup of the panel kernel
 constexpr int SBREC = NBS + 1;     // record: norm, 32 column sums
+constexpr int SBG = 6;             // thread groups that sum the records
 constexpr int PCOL = 5 * NBS;      // panel buffer columns
 constexpr int LB = 2 * NBS;        // band storage rows per column
 constexpr int SB2_WAVES = 8;
@@ -146,6 +147,7 @@ __global__ __launch_bounds__(256) void k_sb_col(SbArgs a, int t, int jc, int act
   const int r = r0 + tile * SBR + lane;
   const bool rv = r < n;
   __shared__ double s_sum[SBREC];
+  __shared__ double s_grp[SBG][SBREC];
   __shared__ double s_row[NBS];
   __shared__ double s_x[SBR];
   __shared__ double s_red[4][8];
@@ -158,15 +160,31 @@ __global__ __launch_bounds__(256) void k_sb_col(SbArgs a, int t, int jc, int act
   if (jc >= 1) {
     const int j = jc - 1;
     // complete reflector j from the records of launch j (fixed tile order)
+    // (SBG groups of 33 threads each sum every SBG-th tile, then the groups
+    // are added in a fixed order: the loads of one thread are few and issued
+    // together)
     const double* rc = a.rec + sl.pRec + (int64_t)(j & 1) * sl.ntile * SBREC;
-    if (tid < SBREC) {
-      double s = 0.0;
-      for (int q = 0; q < ntile; ++q) s += rc[(int64_t)q * SBREC + tid];
-      s_sum[tid] = s;
-    } else if (tid >= 64 && tid < 64 + NBS) {
+    if (tid < SBG * SBREC) {
+      const int g = tid / SBREC, comp = tid - g * SBREC;
+      double s0 = 0.0, s1 = 0.0;
+      int q = g;
+      for (; q + SBG < ntile; q += 2 * SBG) {
+        s0 += rc[(int64_t)q * SBREC + comp];
+        s1 += rc[(int64_t)(q + SBG) * SBREC + comp];
+      }
+      if (q < ntile) s0 += rc[(int64_t)q * SBREC + comp];
+      s_grp[g][comp] = s0 + s1;
+    } else if (tid >= 224 && tid < 224 + NBS) {
       // row r0 + j of the panel as launch j left it, from the copy launch j
       // made (tile 0 rewrites the row itself during this launch)
-      s_row[tid - 64] = a.rec[sl.pRec + 2 * (int64_t)sl.ntile * SBREC + (j & 1) * NBS + (tid - 64)];
+      s_row[tid - 224] = a.rec[sl.pRec + 2 * (int64_t)sl.ntile * SBREC + (j & 1) * NBS + (tid - 224)];
+    }
+    __syncthreads();
+    if (tid < SBREC) {
+      double s = 0.0;
+#pragma unroll
+      for (int g = 0; g < SBG; ++g) s += s_grp[g][tid];
+      s_sum[tid] = s;
     }
     __syncthreads();
     const double xn2 = s_sum[0];
@@ -566,6 +584,7 @@ struct Sb2Back {
 
 constexpr int SB2B_LDS = 152 * 1024;
 
+constexpr int SB2B_K = 4;   // reflectors of one sweep per half-wave and round
 __global__ __launch_bounds__(1024) void k_sb2st_back(Sb2Back a) {
   extern __shared__ double zs[];
   const int b = a.chunks[3 * blockIdx.x], col0 = a.chunks[3 * blockIdx.x + 1], nc = a.chunks[3 * blockIdx.x + 2];
@@ -574,21 +593,48 @@ __global__ __launch_bounds__(1024) void k_sb2st_back(Sb2Back a) {
   const int tid = threadIdx.x;
   for (int e = tid; e < n * nc; e += 1024) zs[e] = Z[(e % nc) * (int64_t)n + e / nc];   // zs[row * nc + col]
   __syncthreads();
-  const int half = tid >> 5, hl = tid & 31;   // 32 half-waves of 32 lanes
-  for (int j = n - 3; j >= 0; --j) {
+  // half-wave h applies the reflectors k = h + 32 i of each sweep to all nc
+  // columns, SB2B_K of them per round; the first round of the next sweep is
+  // loaded while the current sweep is applied (for n <= 4097 a sweep has
+  // one round)
+  const int half = tid >> 5, hl = tid & 31;
+  const double* v2 = a.v2 + a.v2off[b];
+  const double* t2 = a.tau2 + a.t2off[b];
+  auto load = [&](int j, int round, double (&vv)[SB2B_K], double (&tt)[SB2B_K]) {
     const int nst = (n - 1 - j + NBS - 1) / NBS;
     const int64_t t0 = sb_task0(n, j);
-    const int items = nst * nc;
-    for (int it = half; it < items; it += 32) {
-      const int k = it / nc, cc = it % nc;
+#pragma unroll
+    for (int i = 0; i < SB2B_K; ++i) {
+      const int k = half + 32 * (SB2B_K * round + i);
       const int r0 = j + 1 + k * NBS;
-      const int L = min(NBS, n - r0);
-      const double tau = a.tau2[a.t2off[b] + t0 + k];
-      const double v = (hl < L) ? a.v2[a.v2off[b] + (t0 + k) * NBS + hl] : 0.0;
-      const double z = (hl < L) ? zs[(r0 + hl) * nc + cc] : 0.0;
-      double s = v * z;
-      for (int o = 16; o > 0; o >>= 1) s += __shfl_xor(s, o, 32);
-      if (hl < L) zs[(r0 + hl) * nc + cc] = z - tau * s * v;
+      vv[i] = (k < nst && hl < n - r0) ? v2[(t0 + k) * NBS + hl] : 0.0;
+      tt[i] = (k < nst) ? t2[t0 + k] : 0.0;
+    }
+  };
+  double vn[SB2B_K], tn[SB2B_K];
+  if (n >= 3) load(n - 3, 0, vn, tn);
+  for (int j = n - 3; j >= 0; --j) {
+    const int nst = (n - 1 - j + NBS - 1) / NBS;
+    const int rounds = (nst + 32 * SB2B_K - 1) / (32 * SB2B_K);
+    double vc[SB2B_K], tc[SB2B_K];
+#pragma unroll
+    for (int i = 0; i < SB2B_K; ++i) { vc[i] = vn[i]; tc[i] = tn[i]; }
+    if (j > 0) load(j - 1, 0, vn, tn);
+    for (int round = 0; round < rounds; ++round) {
+      if (round > 0) load(j, round, vc, tc);
+#pragma unroll
+      for (int i = 0; i < SB2B_K; ++i) {
+        const int k = half + 32 * (SB2B_K * round + i);
+        if (k >= nst) break;
+        const int r0 = j + 1 + k * NBS;
+        const bool in = hl < n - r0;
+        for (int cc = 0; cc < nc; ++cc) {
+          const double z = in ? zs[(r0 + hl) * nc + cc] : 0.0;
+          double sdot = vc[i] * z;
+          for (int o = 16; o > 0; o >>= 1) sdot += __shfl_xor(sdot, o, 32);
+          if (in) zs[(r0 + hl) * nc + cc] = z - tc[i] * sdot * vc[i];
+        }
+      }
     }
     __syncthreads();
   }
