@@ -194,7 +194,9 @@ int nrg_gpu_history_clear(void);
  * t[6] spectral invariant, t[7] rho sweep, t[8] CFS.  Also returns the
  * algorithmic FP64 flops of the last eigensolve (sum 10/3 n^3) in t[9] and,
  * for n >= 12, the tridiagonalisation graphs captured (t[10]) and replayed
- * (t[11]) since init (nrg_gpu_set_graphs). */
+ * (t[11]) since init (nrg_gpu_set_graphs); for n >= 13, the host time spent
+ * issuing the last tridiagonalisation's launches (t[12], <= t[2]: equal when
+ * the host, not the device, sets its pace). */
 int nrg_gpu_stage_times(double* t, int n);
 
 /* ---- batched eigensolver on host arrays (kernel-level interface) -------

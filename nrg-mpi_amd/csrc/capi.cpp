@@ -329,6 +329,7 @@ int nrg_gpu_diagonalize(int iteration, int nsub, const int32_t* subspace_info, i
         C.eig.solve(nb, C.bn.data() + b0, ao.data(), vo.data(), C.d_H.p + C.boff[b0], C.d_U.p + C.boff[b0],
                     C.d_w.p + C.bvoff[b0], false, C.stream, &st);
         sum.ms_tridiag += st.ms_tridiag; sum.ms_dc += st.ms_dc; sum.ms_back += st.ms_back;
+        sum.ms_tridiag_issue += st.ms_tridiag_issue;
         sum.flops_alg += st.flops_alg;
         sum.max_n = std::max(sum.max_n, st.max_n);
         sum.nblk += st.nblk;
@@ -540,10 +541,11 @@ int nrg_gpu_stage_times(double* t, int n) {
     double back = C.timers.back;
     if (C.back_timed) back += span(C.ev_back);
     const double inv = C.inv_timed ? span(C.ev_inv) : 0.0;
-    const double v[12] = {C.timers.assemble, C.timers.eig, C.timers.tridiag, C.timers.dc, back,
+    const double v[13] = {C.timers.assemble, C.timers.eig, C.timers.tridiag, C.timers.dc, back,
                           inv, C.timers.spectral, C.timers.rho, C.timers.cfs,
-                          C.eig_stats.flops_alg, (double)C.eig.graph_captures(), (double)C.eig.graph_replays()};
-    for (int i = 0; i < n && i < 12; ++i) t[i] = v[i];
+                          C.eig_stats.flops_alg, (double)C.eig.graph_captures(), (double)C.eig.graph_replays(),
+                          C.eig_stats.ms_tridiag_issue};
+    for (int i = 0; i < n && i < 13; ++i) t[i] = v[i];
   });
 }
 
@@ -696,7 +698,7 @@ int nrg_gpu_set_tridiag_symv(int kind) {
   return guarded([&](Context& C) {
     need_init(C);
     NRG_REQUIRE(kind == 0 || kind == 1, "set_tridiag_symv: kind must be 0 or 1");
-    C.eig.set_symv_rows(kind == 1);
+    C.eig.set_symv_kind(kind);
   });
 }
 

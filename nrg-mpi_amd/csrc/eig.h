@@ -22,6 +22,7 @@ namespace nrg {
 
 struct EigStats {
   double ms_tridiag = 0, ms_dc = 0, ms_back = 0;
+  double ms_tridiag_issue = 0;   // host time spent issuing the tridiagonalisation's launches
   int max_n = 0, nblk = 0, dc_levels = 0;
   double flops_alg = 0;   // sum (10/3) n^3 over blocks
 };
@@ -61,9 +62,9 @@ class BatchedEigensolver {
   // Two-stage tridiagonalisation (dense -> band of 32 -> tridiagonal, sbr.hip)
   // instead of the one-stage reduction.
   void set_two_stage(bool on) { two_stage_ = on; }
-  // SYMV of the one-stage reduction: row parts added with LDS atomics
-  // (false) or in register/LDS row chunks in a fixed order (true)
-  void set_symv_rows(bool on) { symv_rows_ = on; }
+  // SYMV of the one-stage reduction: row parts added with LDS atomics (0)
+  // or in register/LDS row chunks in a fixed order (1)
+  void set_symv_kind(int kind) { symv_kind_ = kind; }
   bool two_stage() const { return two_stage_; }
   long long graph_captures() const { return graph_captures_; }
   long long graph_replays() const { return graph_replays_; }
@@ -142,7 +143,7 @@ class BatchedEigensolver {
   bool use_graphs_ = false;
   // two-stage reduction: band, stage-2 reflectors and their offsets
   bool two_stage_ = false;
-  bool symv_rows_ = false;
+  int symv_kind_ = 0;
   bool solved_two_stage_ = false;   // how the current solve reduced (its back-transform)
   int bt_off_ = 1;                  // row offset of a stage-1 reflector's unit element
   DevBuf<double> d_tq_, d_band_, d_v2_, d_t2_;

@@ -227,42 +227,75 @@ __global__ __launch_bounds__(1024) void k_tr_col(TrArgs a, int t, int jp, int jj
     pv[u] = (col < nvc && rr < n) ? P[rr + (int64_t)col * ld] : 0.0;
     pwc[u] = (col < nwc && rr < n) ? P[rr + (int64_t)(NB + col) * ld] : 0.0;
   }
-  if (tid < nvc) sVc[tid] = P[c + (int64_t)tid * ld];
-  else if (tid >= 64 && tid - 64 < nwc) sWc[tid - 64] = P[c + (int64_t)(NB + tid - 64) * ld];
+  // row c of V and W, and u1 = V^T v, u2 = W^T v of column c-1 (completed by
+  // the SYMV kernel): one value per thread, kept in a register until every
+  // read is issued
+  int rslot = -1;
+  double rv = 0.0;
+  if (tid < nvc) {
+    rslot = 0; rv = P[c + (int64_t)tid * ld];
+  } else if (tid >= 64 && tid - 64 < nwc) {
+    rslot = 1; rv = P[c + (int64_t)(NB + tid - 64) * ld];
+  } else if (jp >= 0 && tid >= 128 && tid - 128 < jp) {
+    rslot = 2; rv = ubuf[tid - 128];
+  } else if (jp >= 0 && tid >= 192 && tid - 192 < jp) {
+    rslot = 3; rv = ubuf[NB + tid - 192];
+  }
   const bool copy = (c >= 1 && c - 1 <= n - 2 && r >= c + 1 && r < n);
   double a_rc = 0.0, vsl = 0.0, y_c = 0.0, tp = 0.0;
   if (wv == 0) {
     if (jj >= 0 && r < n) a_rc = A[r + c * ld];
     if (copy) vsl = P[r + vslot_prev * ld];
     if (jp >= 0) {
-      y_c = py[0];
       tp = a.tau[sl.voff + c - 1];
+      y_c = py[0];   // row c has one partial
     }
   }
+  // The first round of the SYMV partials below is straight-line loads from
+  // clamped (always valid) addresses, selected afterwards: no loop and no
+  // branch, so that it is issued together with every read above; loops
+  // handle the rare remainders after the first values are consumed.
+  double ty[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  double yvp = 0.0;
+  int npan = 0, nq = 0;
+  const int rr = min(r, n - 1);          // a valid row for every lane
+  const double* ycol = py + (rr - c);
   if (jp >= 0) {
-    const int npan = (n - c + pw - 1) / pw;   // SYMV workgroups of column c-1 (m = n - c)
-    // y of row r: the 16 waves split the partial vectors py[q][r-c], q <= (r-c)/pw
-    double t[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};   // 8 loads in flight per lane
-    if (r < n) {
-      const int i = r - c;
-      const int nq = i / pw + 1;
-      const double* col = py + i;
-      int q = wv;
-      for (; q + 112 < nq; q += 128) {
+    // y.v: one partial per SYMV workgroup of column c-1 (m = n - c >= 1)
+    npan = (n - c + pw - 1) / pw;
+    yvp = symrec[min(tid, npan - 1)];
+    if (tid >= npan) yvp = 0.0;
+    // y of row r: the 16 waves split the partial vectors py[q][r-c],
+    // q <= (r-c)/pw; 8 loads per lane cover the first 128 partials of a row
+    const int nqr = (rr - c) / pw + 1;
+    nq = (r < n) ? nqr : 0;
 #pragma unroll
-        for (int u = 0; u < 8; ++u) t[u] += col[(int64_t)(q + 16 * u) * n];
+    for (int u = 0; u < 8; ++u) ty[u] = ycol[(int64_t)min(wv + 16 * u, nqr - 1) * n];
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (wv + 16 * u >= nq) ty[u] = 0.0;
+  }
+  if (jp >= 0) {
+    for (int q = tid + 1024; q < npan; q += 1024) yvp += symrec[q];
+    for (int q0 = wv + 128; q0 < nq; q0 += 128) {
+      double x[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int q = q0 + 16 * u;
+        x[u] = (q < nq) ? ycol[(int64_t)q * n] : 0.0;
       }
-      for (; q < nq; q += 16) t[0] += col[(int64_t)q * n];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) ty[u] += x[u];
     }
-    s_y[wv][lane] = ((t[0] + t[1]) + (t[2] + t[3])) + ((t[4] + t[5]) + (t[6] + t[7]));
-    // y.v: one partial per SYMV workgroup
-    double s = 0.0;
-    for (int q = tid; q < npan; q += 1024) s += symrec[q];
-    s = wave_sum_full(s);
+  }
+  if (rslot == 0) sVc[tid] = rv;
+  else if (rslot == 1) sWc[tid - 64] = rv;
+  else if (rslot == 2) su1[tid - 128] = rv;
+  else if (rslot == 3) su2[tid - 192] = rv;
+  if (jp >= 0) {
+    s_y[wv][lane] = ((ty[0] + ty[1]) + (ty[2] + ty[3])) + ((ty[4] + ty[5]) + (ty[6] + ty[7]));
+    const double s = wave_sum_full(yvp);
     if (lane == 0) s_red[wv] = s;
-    // u1 = V^T v, u2 = W^T v of column c-1 (completed by the SYMV kernel)
-    if (tid >= 128 && tid - 128 < jp) su1[tid - 128] = ubuf[tid - 128];
-    else if (tid >= 192 && tid - 192 < jp) su2[tid - 192] = ubuf[NB + tid - 192];
   }
 #pragma unroll
   for (int u = 0; u < 2; ++u) {
@@ -362,7 +395,7 @@ __global__ __launch_bounds__(1024) void k_tr_col(TrArgs a, int t, int jp, int jj
 // VL = 0 reads v from the updated column in global memory (1 double per
 // row), for trailing blocks whose two LDS vectors would not fit.
 template <int CPW, int NW, int U, int VL>
-__global__ __launch_bounds__(NW * 64) void k_tr_symv(TrArgs a, int t, int jj, int act, int panel_major) {
+__global__ __launch_bounds__(NW * 64, 8) void k_tr_symv(TrArgs a, int t, int jj, int act, int panel_major) {
   constexpr int PW = CPW * NW;
   extern __shared__ double ax[];   // row parts, rows [j0, m), then (VL) v of those rows
   int qs, pan;
@@ -403,15 +436,40 @@ __global__ __launch_bounds__(NW * 64) void k_tr_symv(TrArgs a, int t, int jj, in
   __shared__ double s_red[NW];
   __shared__ double s_u[NW][64];
 
-  // reflector from the updated column c (identical in every workgroup)
+  // ---- one round of reads before anything waits: the column norms and
+  // alpha (the reflector), the raw column c of the rows this workgroup
+  // stages (v = scal x, once scal is known), the u records (panel 0) and
+  // the first chunk of the wave's first column.  Clamped, always valid
+  // addresses; out-of-range values are discarded where they are used.
   const int ncol = (n - c + RA - 1) / RA;   // column-kernel records of column c
+  const double* vcol = A + (c + 1) + (int64_t)c * ld;
+  const double* At = A + (c + 1) + (int64_t)(c + 1) * ld;   // At[i + j*ld] = A(c+1+i, c+1+j)
+  double nrm = colrec[(int64_t)min(lane, ncol - 1) * USTRIDE + 2 * NB];
+  const double alpha = A[(c + 1) + c * ld];
+  constexpr int VS = 2;   // staged rows per thread in the first round
+  double vraw[VS];
+#pragma unroll
+  for (int u = 0; u < VS; ++u) vraw[u] = VL ? vcol[min(j0 + tid + NW * 64 * u, m - 1)] : 0.0;
+  // record slot of u1 (lanes < jj) and u2 (lanes jj..2jj-1); a valid
+  // dummy slot for the other lanes, so that the loads need no lane branch
+  const int uslot = (lane >= 2 * jj) ? 0 : (lane < jj) ? lane : NB + (lane - jj);
+  double urec[2] = {0.0, 0.0};
+  if (pan == 0) {
+#pragma unroll
+    for (int u = 0; u < 2; ++u) urec[u] = colrec[(int64_t)min(wv + NW * u, ncol - 1) * USTRIDE + uslot];
+  }
+  const int jf = min(j0 + wv * CPW, m - 1);   // the wave's first column (clamped)
+  double xf[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) xf[u] = At[min(jf + lane + 64 * u, m - 1) + (int64_t)jf * ld];
+
+  // reflector from the updated column c (identical in every workgroup)
   double xn2;
   {
-    double sl = 0.0;
-    for (int q = lane; q < ncol; q += 64) sl += colrec[(int64_t)q * USTRIDE + 2 * NB];
-    xn2 = wave_sum_full(sl);   // every wave forms the same sum
+    if (lane >= ncol) nrm = 0.0;
+    for (int q = lane + 64; q < ncol; q += 64) nrm += colrec[(int64_t)q * USTRIDE + 2 * NB];
+    xn2 = wave_sum_full(nrm);   // every wave forms the same sum
   }
-  const double alpha = A[(c + 1) + c * ld];
   double beta, tau, scal;
   if (xn2 == 0.0) {
     beta = alpha; tau = 0.0; scal = 0.0;
@@ -421,7 +479,6 @@ __global__ __launch_bounds__(NW * 64) void k_tr_symv(TrArgs a, int t, int jj, in
     scal = 1.0 / (alpha - beta);
   }
   // v_i = 1 (i = 0), A(c+1+i, c) * scal (i > 0): read from the column
-  const double* vcol = A + (c + 1) + (int64_t)c * ld;
   auto vv = [&](int i) { return (i == 0) ? 1.0 : vcol[i] * scal; };
   if (pan == 0) {
     if (tid == 0) {
@@ -432,13 +489,23 @@ __global__ __launch_bounds__(NW * 64) void k_tr_symv(TrArgs a, int t, int jj, in
     // waves split the records, lane = slot, fixed order
     double s = 0.0;
     if (lane < 2 * jj) {
-      const int slot = (lane < jj) ? lane : NB + (lane - jj);
-      for (int q = wv; q < ncol; q += NW) s += colrec[(int64_t)q * USTRIDE + slot];
+#pragma unroll
+      for (int u = 0; u < 2; ++u)
+        if (wv + NW * u < ncol) s += urec[u];
+      for (int q = wv + 2 * NW; q < ncol; q += NW) s += colrec[(int64_t)q * USTRIDE + uslot];
     }
     s_u[wv][lane] = s;
   }
   double* sv = ax + (m - j0);   // VL: v_i of rows [j0, m), staged once per workgroup
-  for (int i = j0 + tid; i < m; i += NW * 64) {
+#pragma unroll
+  for (int u = 0; u < VS; ++u) {
+    const int i = j0 + tid + NW * 64 * u;
+    if (i < m) {
+      ax[i - j0] = 0.0;
+      if (VL) sv[i - j0] = (i == 0) ? 1.0 : vraw[u] * scal;
+    }
+  }
+  for (int i = j0 + tid + NW * 64 * VS; i < m; i += NW * 64) {
     ax[i - j0] = 0.0;
     if (VL) sv[i - j0] = vv(i);
   }
@@ -452,7 +519,6 @@ __global__ __launch_bounds__(NW * 64) void k_tr_symv(TrArgs a, int t, int jj, in
     ubuf[col] = P[(c + 1) + (int64_t)col * ld] + scal * s;
   }
 
-  const double* At = A + (c + 1) + (int64_t)(c + 1) * ld;   // At[i + j*ld] = A(c+1+i, c+1+j)
   for (int q = 0; q < CPW; ++q) {
     const int j = j0 + wv * CPW + q;
     if (j >= m) break;
@@ -461,7 +527,24 @@ __global__ __launch_bounds__(NW * 64) void k_tr_symv(TrArgs a, int t, int jj, in
     // U loads in flight per lane, all issued before the first use; two
     // accumulators keep the kernel within 64 VGPRs (two workgroups per CU)
     double s0 = 0.0, s1 = 0.0;
+    // rows i + 64 u of column j, those below m
+    auto use_guarded = [&](const double (&x)[U], int i) {
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int r = i + 64 * u;
+        if (r < m) {
+          if (u & 1) s1 += x[u] * vr(r);
+          else s0 += x[u] * vr(r);
+          if (r > j) atomicAdd(&ax[r - j0], x[u] * vj);
+        }
+      }
+    };
     int i = j + lane;
+    if (q == 0) {
+      // the chunk read before the reflector
+      use_guarded(xf, i);
+      i += 64 * U;
+    }
     for (; i + 64 * (U - 1) < m; i += 64 * U) {
       double x[U];
 #pragma unroll
@@ -481,14 +564,7 @@ __global__ __launch_bounds__(NW * 64) void k_tr_symv(TrArgs a, int t, int jj, in
         x[u] = 0.0;
         if (i + 64 * u < m) x[u] = col[i + 64 * u];
       }
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        if (i + 64 * u < m) {
-          if (u & 1) s1 += x[u] * vr(i + 64 * u);
-          else s0 += x[u] * vr(i + 64 * u);
-          if (u > 0 || i > j) atomicAdd(&ax[i + 64 * u - j0], x[u] * vj);
-        }
-      }
+      use_guarded(x, i);
     }
     const double t = wave_sum_full(s0 + s1);
     if (lane == 0) atomicAdd(&ax[j - j0], t);
@@ -1618,7 +1694,12 @@ void BatchedEigensolver::solve(int nblk, const int* n, const int64_t* aoff, cons
   bt_off_ = two_stage_ ? 32 : 1;
   if (two_stage_) tridiag_two_stage(s);
   else tridiag(s);
-  if (st) { NRG_HIP(hipStreamSynchronize(s)); st->ms_tridiag = now_ms() - t0; t0 = now_ms(); }
+  if (st) {
+    st->ms_tridiag_issue = now_ms() - t0;
+    NRG_HIP(hipStreamSynchronize(s));
+    st->ms_tridiag = now_ms() - t0;
+    t0 = now_ms();
+  }
   // the back-transform's panel factors overlap the divide and conquer
   if (sg_[1]) {
     NRG_HIP(hipEventRecord(ev_fork_, s));
@@ -1700,7 +1781,7 @@ void BatchedEigensolver::tridiag(hipStream_t s) {
   // every slot sees the same panel phase t mod NB.
   std::vector<int32_t> coff(nblk);
   for (int q = 0; q < nblk; ++q) coff[q] = align_end_ ? ((nmax - n_[order_[q]]) / NB) * NB : 0;
-  NRG_REQUIRE(symv_rows_ || symv_lds_fits(nmax - 1, 1),
+  NRG_REQUIRE(symv_kind_ != 0 || symv_lds_fits(nmax - 1, 1),
               "eigensolver: block too large for the SYMV stage (its row-part vector exceeds the LDS)");
   int T = 0;
   for (int q = 0; q < nblk; ++q) T = std::max(T, coff[q] + n_[order_[q]]);
@@ -1795,6 +1876,7 @@ void BatchedEigensolver::tridiag(hipStream_t s) {
     key.push_back(align_end_);
     key.push_back(ng);
     key.push_back(symv_wide_wg_);
+    key.push_back(symv_kind_);
     key.push_back((int64_t)symv_wide_bytes_);
     for (int b = 0; b < nblk; ++b) { key.push_back(n_[b]); key.push_back(aoff_[b]); key.push_back(voff_[b]); }
     for (const void* p : {(const void*)A_, (const void*)d_d_.p, (const void*)d_e_.p, (const void*)d_tau_.p,
@@ -1907,9 +1989,9 @@ void BatchedEigensolver::tridiag(hipStream_t s) {
     if (gB > 0) {
       // LDS: row parts and (VL) v of rows [j0, m) of the longest trailing
       // block (the row-chunk SYMV keeps only v there)
-      const bool vl = symv_rows_ ? symv_rows_v_fits(mB) : symv_lds_fits(mB, 2);
-      const uint32_t shB = symv_rows_ ? (uint32_t)(vl ? mB * sizeof(double) : 0)
-                                      : (uint32_t)((vl ? 2 : 1) * mB * sizeof(double));
+      const bool vl = symv_kind_ == 1 ? symv_rows_v_fits(mB) : symv_lds_fits(mB, 2);
+      const uint32_t shB = symv_kind_ == 1 ? (uint32_t)(vl ? mB * sizeof(double) : 0)
+                                           : (uint32_t)((vl ? 2 : 1) * mB * sizeof(double));
       // panel-major order when the panel counts are nearly uniform
       const int pm = (align_end_ && (int64_t)act * tmax <= gB + gB / 8) ? 1 : 0;
       const int g = pm ? act * tmax : gB;
@@ -1925,7 +2007,7 @@ void BatchedEigensolver::tridiag(hipStream_t s) {
             NRG_HIP(hipEventRecord(evf_[h], grp[h].s));
             NRG_HIP(hipStreamWaitEvent(G.s, evf_[h], 0));
           }
-      if (symv_rows_) {
+      if (symv_kind_ == 1) {
         if (pw == 2 * CG && vl)
           plaunch(KF_TR_SYMV, bytes, flops, k_tr_symv_rows<2, 1>, dim3(g), dim3(1024), shB, G.s, ta, t, jj, act, pm);
         else if (vl)

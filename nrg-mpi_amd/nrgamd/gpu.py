@@ -274,8 +274,8 @@ def kernel_stats():
 
 
 def stage_times():
-    t = np.zeros(12)
-    _check(load().nrg_gpu_stage_times(_p(t, _f64p), 12))
+    t = np.zeros(13)
+    _check(load().nrg_gpu_stage_times(_p(t, _f64p), 13))
     keys = ["assemble", "eig", "tridiag", "dc", "back", "invariant", "spectral", "rho", "cfs", "eig_flops",
-            "graph_captures", "graph_replays"]
+            "graph_captures", "graph_replays", "tridiag_issue"]
     return dict(zip(keys, t.tolist()))

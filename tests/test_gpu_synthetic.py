@@ -16,6 +16,8 @@ pytestmark = pytest.mark.gpu
 EIG_TOL_OVER_N = 8 * 2.0 ** -52
 
 
+# the numpy checks of the split products cover every block (minutes of CPU)
+@pytest.mark.timeout(600)
 def test_synthetic_batch_256_blocks(gpu):
     from nrgamd import synthetic as S
     print("generating the 256-block batch", flush=True)
