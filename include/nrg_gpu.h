@@ -346,8 +346,8 @@ int nrg_gpu_set_graphs(int on);
  * 32-column panels -- chosen per block-size histogram (block count, largest
  * block, sum n^3).  mode 0: defaults (2 groups, 512 workgroups); 1: time the
  * candidate configurations on successive iterations with the same
- * histogram, then keep the fastest, updating the cache file; 2: use the
- * cache file's fastest only.  cache_path: a text file of measurements (NULL
+ * histogram (three times each), then keep the one with the lowest mean
+ * time, updating the cache file; 2: use the cache file's choice only.  cache_path: a text file of measurements (NULL
  * or "" keeps them in memory). */
 int nrg_gpu_set_tuning(int mode, const char* cache_path);
 

@@ -37,18 +37,20 @@ TuneConfig Tuner::choose(const std::string& k, bool* exploring) {
   if (mode_ == 0) return cand[0];
   auto it = t_.find(k);
   if (mode_ == 1) {
-    for (const TuneConfig& c : cand)
-      if (it == t_.end() || !it->second.count({c.groups, c.symv_wide_wg})) {
+    for (const TuneConfig& c : cand) {
+      const bool seen = it != t_.end() && it->second.count({c.groups, c.symv_wide_wg});
+      if (!seen || it->second.at({c.groups, c.symv_wide_wg}).samples < kMinSamples) {
         *exploring = true;
         return c;
       }
+    }
   }
   if (it == t_.end() || it->second.empty()) return cand[0];
   TuneConfig best = cand[0];
   double bt = 1e300;
   for (const auto& kv : it->second)
-    if (kv.second.samples > 0 && kv.second.best < bt) {
-      bt = kv.second.best;
+    if (kv.second.samples > 0 && kv.second.mean() < bt) {
+      bt = kv.second.mean();
       best = TuneConfig{kv.first.first, kv.first.second};
     }
   return best;
@@ -57,7 +59,7 @@ TuneConfig Tuner::choose(const std::string& k, bool* exploring) {
 void Tuner::record(const std::string& k, const TuneConfig& c, double ms) {
   if (mode_ != 1 || !(ms > 0)) return;
   Stat& s = t_[k][{c.groups, c.symv_wide_wg}];
-  s.best = s.samples ? std::min(s.best, ms) : ms;
+  s.sum += ms;
   s.samples += 1;
   if (!path_.empty()) save();
 }
@@ -72,7 +74,7 @@ void Tuner::load() {
     double ms;
     if (!(is >> k >> g >> wg >> ms >> ns) || k.empty() || k[0] == '#') continue;
     Stat& s = t_[k][{g, wg}];
-    s.best = ms;
+    s.sum = ms * ns;
     s.samples = ns;
   }
 }
@@ -81,10 +83,10 @@ void Tuner::save() const {
   const std::string tmp = path_ + ".part";
   {
     std::ofstream o(tmp);
-    o << "# key groups symv_wide_wg best_ms samples (batched tridiagonalisation, MI355X)\n";
+    o << "# key groups symv_wide_wg mean_ms samples (batched tridiagonalisation, MI355X)\n";
     for (const auto& kv : t_)
       for (const auto& c : kv.second)
-        o << kv.first << " " << c.first.first << " " << c.first.second << " " << c.second.best << " "
+        o << kv.first << " " << c.first.first << " " << c.first.second << " " << c.second.mean() << " "
           << c.second.samples << "\n";
   }
   std::rename(tmp.c_str(), path_.c_str());

@@ -7,9 +7,12 @@
 // launch size on the SYMV uses 32-column panels.  An NRG run meets a few
 // block sets again and again (the steady state alternates between two block
 // tables), so the tuner times each candidate configuration on successive
-// iterations with the same histogram key, then keeps the fastest.  The
-// measurements persist in a text cache file, one line per (key, candidate):
-//     <key> <groups> <symv_wide_wg> <best_ms> <samples>
+// iterations with the same histogram key (kMinSamples times each), then
+// keeps the one with the lowest mean time: the mean, not the minimum, so
+// that a configuration timed more often is not favoured by its luckiest
+// sample.  The measurements persist in a text cache file, one line per
+// (key, candidate):
+//     <key> <groups> <symv_wide_wg> <mean_ms> <samples>
 // Modes: 0 off (the defaults), 1 explore unmeasured candidates and update the
 // cache, 2 use the cache only (defaults for keys it does not hold).
 #pragma once
@@ -41,7 +44,12 @@ class Tuner {
   size_t entries() const { return t_.size(); }
 
  private:
-  struct Stat { double best = 0; int samples = 0; };
+  static constexpr int kMinSamples = 3;
+  struct Stat {
+    double sum = 0;
+    int samples = 0;
+    double mean() const { return samples ? sum / samples : 0.0; }
+  };
   void load();
   void save() const;
   int mode_ = 0;
