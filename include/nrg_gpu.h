@@ -189,6 +189,9 @@ int nrg_gpu_load_restart(const char* path, int* iteration, int nstate, int32_t* 
 int nrg_gpu_history_clear(void);
 
 /* ---- per-stage timing of the last calls (milliseconds) -----------------
+ * Never waits for the device: t[4] (its deferred part) and t[5] time work
+ * that returns before the device has done it, and report the last such
+ * span that has completed.
  * t[0] H assembly, t[1] eigensolver total, t[2] tridiagonalisation,
  * t[3] divide and conquer, t[4] back-transform, t[5] invariant,
  * t[6] spectral invariant, t[7] rho sweep, t[8] CFS.  Also returns the

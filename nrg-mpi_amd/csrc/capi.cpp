@@ -93,6 +93,7 @@ int nrg_gpu_init(int device) {
     NRG_HIP(hipStreamCreateWithFlags(&C.stream, hipStreamNonBlocking));
     NRG_HIP(hipEventCreate(&C.ev0));
     NRG_HIP(hipEventCreate(&C.ev1));
+    for (hipEvent_t& e : C.ev_dz) NRG_HIP(hipEventCreate(&e));
     for (int i = 0; i < 2; ++i) {
       NRG_HIP(hipEventCreate(&C.ev_back[i]));
       NRG_HIP(hipEventCreate(&C.ev_inv[i]));
@@ -117,6 +118,10 @@ int nrg_gpu_finalize(void) {
     C.d_tmp.release(); C.d_gidx.release(); C.d_gscale.release();
     (void)hipEventDestroy(C.ev0);
     (void)hipEventDestroy(C.ev1);
+    for (hipEvent_t& e : C.ev_dz) {
+      (void)hipEventDestroy(e);
+      e = nullptr;
+    }
     for (int i = 0; i < 2; ++i) {
       (void)hipEventDestroy(C.ev_back[i]);
       (void)hipEventDestroy(C.ev_inv[i]);
@@ -280,17 +285,20 @@ int nrg_gpu_diagonalize(int iteration, int nsub, const int32_t* subspace_info, i
     C.d_H.reserve(n2);
     C.d_U.reserve(n2);
     C.d_w.reserve(nfull);
-    double t0 = wall_ms();
+    // stage times from events, read once the eigenvalues have been waited
+    // for: no synchronisation between the stages, so the host prepares the
+    // eigensolver's tables while the device assembles
+    NRG_HIP(hipEventRecord(C.ev_dz[0], C.stream));
     double cbcs[4] = {0, 0, 0, 0};   // chain_BCS(iteration+1, ch) (diagonalization.F90:242)
     if (C.bcs_rows) {
       NRG_REQUIRE(iteration + 1 >= 0 && iteration + 1 < C.bcs_rows, "diagonalize: iteration outside chain_BCS");
       for (int ch = 0; ch < C.nchain && ch < 4; ++ch) cbcs[ch] = C.chain_bcs[(size_t)ch * C.bcs_rows + iteration + 1];
     }
     assemble(C, chain_diag, chain_nondiag, sqrt_lambda, C.bcs_rows ? cbcs : nullptr);
-    NRG_HIP(hipStreamSynchronize(C.stream));
-    double t1 = wall_ms();
-    C.timers.assemble = t1 - t0;
+    NRG_HIP(hipEventRecord(C.ev_dz[1], C.stream));
     C.u_lazy = false;
+    std::string tune_key;
+    TuneConfig tune_cfg;
     if (C.batch_limit <= 0 || C.batch_limit >= nsub) {
       // without an eigenvector output the back-transform waits for the
       // columns the invariant / stock calls read (ensure_columns)
@@ -310,7 +318,10 @@ int nrg_gpu_diagonalize(int iteration, int nsub, const int32_t* subspace_info, i
       C.eig.set_graphs(C.graphs_on && !explore);
       C.eig.solve(nsub, C.bn.data(), C.boff.data(), C.bvoff.data(), C.d_H.p, C.d_U.p, C.d_w.p, defer,
                   C.stream, &C.eig_stats);
-      if (C.tuner.mode() == 1) C.tuner.record(tkey, cfg, C.eig_stats.ms_tridiag);
+      // stage times and the tuner's measurement once the stream has been
+      // waited for below
+      tune_key = tkey;
+      tune_cfg = cfg;
       C.u_lazy = defer;
     } else {
       // groups of at most batch_limit blocks, one solve each (batch_limit 1:
@@ -328,6 +339,7 @@ int nrg_gpu_diagonalize(int iteration, int nsub, const int32_t* subspace_info, i
         EigStats st;
         C.eig.solve(nb, C.bn.data() + b0, ao.data(), vo.data(), C.d_H.p + C.boff[b0], C.d_U.p + C.boff[b0],
                     C.d_w.p + C.bvoff[b0], false, C.stream, &st);
+        C.eig.stats(&st);   // waits for this group's solve
         sum.ms_tridiag += st.ms_tridiag; sum.ms_dc += st.ms_dc; sum.ms_back += st.ms_back;
         sum.ms_tridiag_issue += st.ms_tridiag_issue;
         sum.flops_alg += st.flops_alg;
@@ -336,15 +348,25 @@ int nrg_gpu_diagonalize(int iteration, int nsub, const int32_t* subspace_info, i
       }
       C.eig_stats = sum;
     }
+    NRG_HIP(hipEventRecord(C.ev_dz[2], C.stream));
+    C.d_w.download(eigenvalue, nfull, C.stream);
+    if (eigenvector) C.d_U.download(eigenvector, n2, C.stream);
     NRG_HIP(hipStreamSynchronize(C.stream));
-    C.timers.eig = wall_ms() - t1;
+    {
+      float a = 0.f, e = 0.f;
+      NRG_HIP(hipEventElapsedTime(&a, C.ev_dz[0], C.ev_dz[1]));
+      NRG_HIP(hipEventElapsedTime(&e, C.ev_dz[1], C.ev_dz[2]));
+      C.timers.assemble = a;
+      C.timers.eig = e;
+    }
+    if (C.batch_limit <= 0 || C.batch_limit >= nsub) {
+      C.eig.stats(&C.eig_stats);
+      if (C.tuner.mode() == 1) C.tuner.record(tune_key, tune_cfg, C.eig_stats.ms_tridiag);
+    }
     C.timers.tridiag = C.eig_stats.ms_tridiag;
     C.timers.dc = C.eig_stats.ms_dc;
     C.timers.back = C.eig_stats.ms_back;
     C.back_timed = C.inv_timed = false;
-    C.d_w.download(eigenvalue, nfull, C.stream);
-    if (eigenvector) C.d_U.download(eigenvector, n2, C.stream);
-    NRG_HIP(hipStreamSynchronize(C.stream));
     // global shift and chop (diagonalization.F90:351-356)
     double mn = eigenvalue[0];
     for (int i = 1; i < nfull; ++i) mn = std::min(mn, eigenvalue[i]);
@@ -532,15 +554,25 @@ int nrg_gpu_invariant_spectrum(const int32_t* kept_per_block, double* out) {
 
 int nrg_gpu_stage_times(double* t, int n) {
   return guarded([&](Context& C) {
-    auto span = [&](hipEvent_t* e) {
-      NRG_HIP(hipEventSynchronize(e[1]));
-      float ms = 0.f;
-      NRG_HIP(hipEventElapsedTime(&ms, e[0], e[1]));
-      return (double)ms;
+    // spans of work that returned without waiting for the device (the
+    // deferred back-transform, the invariant products): read once they have
+    // completed, never waited for here, so that asking for the times does
+    // not drain the device between iterations; until a span completes, the
+    // last completed one is reported
+    auto span = [&](hipEvent_t* e, double& last) {
+      const hipError_t q = hipEventQuery(e[1]);
+      if (q == hipSuccess) {
+        float ms = 0.f;
+        NRG_HIP(hipEventElapsedTime(&ms, e[0], e[1]));
+        last = ms;
+      } else if (q != hipErrorNotReady) {
+        NRG_HIP(q);
+      }
+      return last;
     };
     double back = C.timers.back;
-    if (C.back_timed) back += span(C.ev_back);
-    const double inv = C.inv_timed ? span(C.ev_inv) : 0.0;
+    if (C.back_timed) back += span(C.ev_back, C.last_back_ms);
+    const double inv = C.inv_timed ? span(C.ev_inv, C.last_inv_ms) : 0.0;
     const double v[13] = {C.timers.assemble, C.timers.eig, C.timers.tridiag, C.timers.dc, back,
                           inv, C.timers.spectral, C.timers.rho, C.timers.cfs,
                           C.eig_stats.flops_alg, (double)C.eig.graph_captures(), (double)C.eig.graph_replays(),
@@ -570,6 +602,11 @@ int nrg_gpu_syev_batched(int nblk, const int32_t* n, const double* H, double* w,
     dw.download(w, v, C.stream);
     dZ.download(U, a, C.stream);
     NRG_HIP(hipStreamSynchronize(C.stream));
+    C.eig.stats(&C.eig_stats);
+    C.timers.tridiag = C.eig_stats.ms_tridiag;
+    C.timers.dc = C.eig_stats.ms_dc;
+    C.timers.back = C.eig_stats.ms_back;
+    C.back_timed = false;
     dA.release(); dZ.release(); dw.release();
   });
 }

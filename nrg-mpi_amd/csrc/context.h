@@ -30,6 +30,10 @@ struct Context {
   // pairs, read when the stage times are asked for
   hipEvent_t ev_back[2] = {nullptr, nullptr}, ev_inv[2] = {nullptr, nullptr};
   bool back_timed = false, inv_timed = false;
+  double last_back_ms = 0, last_inv_ms = 0;   // the last completed spans
+  // diagonalisation stage boundaries: before and after the assembly, after
+  // the eigensolve (read after the eigenvalue download has been waited for)
+  hipEvent_t ev_dz[3] = {nullptr, nullptr, nullptr};
   // this device's copy of the no-allocation entry point's scratch array
   // (hipGetSymbolAddress at init)
   char* scratch = nullptr;

@@ -43,6 +43,10 @@ class BatchedEigensolver {
   // (hi == nullptr: all columns); columns already transformed are skipped.
   // *ms (if not null) accumulates the time.
   void back_columns(const int* hi, hipStream_t s, double* ms = nullptr);
+  // Stage times (tridiagonalisation, divide and conquer, back-transform) of
+  // the last solve that was given an EigStats, from its events; waits for
+  // the last of them (callers read them after waiting for the device).
+  void stats(EigStats* st);
   // Columns of block b that hold eigenvectors of the full matrix.
   int columns_done(int b) const { return done_[b]; }
   void release();
@@ -125,6 +129,7 @@ class BatchedEigensolver {
   hipEvent_t evg_[MAX_TR_GROUPS] = {};
   hipEvent_t ev_fork_ = nullptr;
   hipEvent_t evf_[MAX_TR_GROUPS] = {};   // fences around timed SYMV launches
+  hipEvent_t ev_st_[4] = {};             // stage boundaries of the last timed solve (stats())
   DevBuf<char> d_slotg_[MAX_TR_GROUPS];
   GemmRunner gemmg_[MAX_TR_GROUPS];
   GemmBatch batchg_[MAX_TR_GROUPS];

@@ -1633,6 +1633,18 @@ static double now_ms() {
   return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
+void BatchedEigensolver::stats(EigStats* st) {
+  if (!st || !ev_st_[3]) return;
+  NRG_HIP(hipEventSynchronize(ev_st_[3]));
+  float a = 0.f, b = 0.f, c = 0.f;
+  NRG_HIP(hipEventElapsedTime(&a, ev_st_[0], ev_st_[1]));
+  NRG_HIP(hipEventElapsedTime(&b, ev_st_[1], ev_st_[2]));
+  NRG_HIP(hipEventElapsedTime(&c, ev_st_[2], ev_st_[3]));
+  st->ms_tridiag = a;
+  st->ms_dc = b;
+  st->ms_back = c;
+}
+
 void BatchedEigensolver::release() {
   for (TrGraph& e : graphs_) (void)hipGraphExecDestroy(e.exec);
   graphs_.clear();
@@ -1653,7 +1665,10 @@ void BatchedEigensolver::release() {
   bt_wait_ = false;
   d_vpo_.release(); d_tpo_.release(); d_plist_.release(); gemm_bt_.release();
   for (int g = 0; g < MAX_TR_GROUPS; ++g)
-    if (evf_[g]) { (void)hipEventDestroy(evf_[g]); evf_[g] = nullptr; } d_aoff_.release(); d_voff_.release();
+    if (evf_[g]) { (void)hipEventDestroy(evf_[g]); evf_[g] = nullptr; }
+  for (hipEvent_t& e : ev_st_)
+    if (e) { (void)hipEventDestroy(e); e = nullptr; }
+  d_aoff_.release(); d_voff_.release();
   d_d_.release(); d_e_.release(); d_tau_.release(); d_y_.release(); d_P_.release(); d_part_.release();
   d_ws_.release(); d_iws_.release(); d_tasks_.release(); d_k_.release(); d_usel_.release(); d_mtask_.release();
   d_V_.release(); d_T_.release(); d_X_.release(); d_X2_.release(); d_rng_.release(); gemm_.release();
@@ -1689,16 +1704,23 @@ void BatchedEigensolver::solve(int nblk, const int* n, const int64_t* aoff, cons
   d_tau_.reserve(tot_n_);
   fill(d_e_.p, 0.0, tot_n_, s);
   fill(d_tau_.p, 0.0, tot_n_, s);
-  double t0 = now_ms();
+  // Stage times come from events on the caller's stream, read by
+  // stats() once the caller has waited for the device: timing adds no
+  // synchronisation, so the host issues the next stage's launches while the
+  // device still runs the previous one.
+  if (st) {
+    for (hipEvent_t& e : ev_st_)
+      if (!e) NRG_HIP(hipEventCreate(&e));
+    NRG_HIP(hipEventRecord(ev_st_[0], s));
+  }
+  const double t0 = now_ms();
   solved_two_stage_ = two_stage_;
   bt_off_ = two_stage_ ? 32 : 1;
   if (two_stage_) tridiag_two_stage(s);
   else tridiag(s);
   if (st) {
     st->ms_tridiag_issue = now_ms() - t0;
-    NRG_HIP(hipStreamSynchronize(s));
-    st->ms_tridiag = now_ms() - t0;
-    t0 = now_ms();
+    NRG_HIP(hipEventRecord(ev_st_[1], s));
   }
   // the back-transform's panel factors overlap the divide and conquer
   if (sg_[1]) {
@@ -1713,12 +1735,11 @@ void BatchedEigensolver::solve(int nblk, const int* n, const int64_t* aoff, cons
     bt_wait_ = false;
   }
   divide_conquer(s);
-  if (st) { NRG_HIP(hipStreamSynchronize(s)); st->ms_dc = now_ms() - t0; t0 = now_ms(); }
+  if (st) NRG_HIP(hipEventRecord(ev_st_[2], s));
   done_.assign(nblk, 0);
   if (!defer_back) back_columns(nullptr, s, nullptr);
   if (st) {
-    NRG_HIP(hipStreamSynchronize(s));
-    st->ms_back = now_ms() - t0;
+    NRG_HIP(hipEventRecord(ev_st_[3], s));
     st->max_n = nmax;
     st->nblk = nblk;
     double f = 0;
