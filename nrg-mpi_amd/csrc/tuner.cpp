@@ -83,6 +83,7 @@ void Tuner::save() const {
   const std::string tmp = path_ + ".part";
   {
     std::ofstream o(tmp);
+    o.precision(10);   // means keep accumulating samples across runs
     o << "# key groups symv_wide_wg mean_ms samples (batched tridiagonalisation, MI355X)\n";
     for (const auto& kv : t_)
       for (const auto& c : kv.second)
