@@ -157,19 +157,37 @@ __global__ __launch_bounds__(256) void k_oz_slice(OzOps o, const OzUnit* __restr
   }
 }
 
-// Product kernel.  WIN: K spans more than one int32 window, so the diagonal
-// sums are flushed to FP64 every OZ_FLUSH chunks; otherwise they are
-// combined in FP64 once, in the epilogue (the same order of additions as
-// a single flush).  The loads run one step ahead of the matrix cores: the B
-// fragments of the next 64-deep K chunk while the current chunk's products
-// run, and the A fragments of the next slice (of the next chunk's slice 0
-// at a chunk's last slice).  With S >= 7 the B fragments are loaded at the
-// start of their own chunk (one wave per SIMD: no room for a second set).
-template <int S, bool WIN, bool PF = (S <= 6)>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_oz_gemm(const GemmTask* __restrict__ tasks, const OzTask* __restrict__ oz,
-                                                 const int32_t* __restrict__ tile_start, int ntasks,
-                                                 const int8_t* __restrict__ planes, const int32_t* __restrict__ ex,
-                                                 double* __restrict__ C) {
+// Product kernel: one workgroup of 4 waves per 64 x 64 output tile, each
+// wave a 32 x 32 quadrant (2 x 2 tiles of v_mfma_i32_16x16x64_i8, one int32
+// accumulator set per diagonal d < S).  The digit planes of a 64-deep K
+// chunk (64 rows of every A plane, 64 columns of every B plane: 8S KB) are
+// copied into LDS once per workgroup by direct global-to-LDS loads
+// (global_load_lds_dwordx4: each wave instruction fills 1 KB, 16 plane rows
+// of 64 bytes, in lane order), and the four waves read their fragments from
+// LDS (16 rows x 64 bytes per ds_read_b128, contiguous, no bank conflicts):
+// every fragment crosses L2 once per workgroup instead of once per wave.
+// WIN: K spans more than one int32 window, so the diagonal sums are flushed
+// to FP64 every OZ_FLUSH chunks; otherwise they are combined in FP64 once,
+// in the epilogue (the same order of additions as a single flush).
+typedef __attribute__((address_space(3))) void oz_lds_t;
+typedef __attribute__((address_space(1))) void oz_glb_t;
+constexpr int OZ_PL = OZ_T * OZ_T;   // bytes of one plane's 64 x 64 chunk
+
+// Buffering by slice count (measured on the synthetic batch): with S <= 6
+// one buffer per workgroup and two workgroups per CU (one computes while the
+// other's chunk streams in) is faster; with S >= 7 the larger footprint
+// favours one workgroup per CU with a double buffer.
+template <int S>
+constexpr int oz_nbuf() { return S <= 6 ? 1 : 2; }
+template <int S>
+constexpr size_t oz_lds_bytes() { return (size_t)oz_nbuf<S>() * 2 * S * OZ_PL; }
+
+template <int S, bool WIN, int NBUF = oz_nbuf<S>()>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3 - NBUF, 3 - NBUF))) void k_oz_gemm_lds(
+    const GemmTask* __restrict__ tasks, const OzTask* __restrict__ oz, const int32_t* __restrict__ tile_start,
+    int ntasks, const int8_t* __restrict__ planes, const int32_t* __restrict__ ex, double* __restrict__ C) {
+  extern __shared__ __attribute__((aligned(16))) int8_t s_pl[];   // [2 buffers][A planes, B planes][64][64]
+  constexpr int BUF = 2 * S * OZ_PL;
   const int bid = blockIdx.x;
   int lo = 0, hi = ntasks - 1;
   while (lo < hi) {
@@ -194,13 +212,25 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int wm = wave & 1, wn = wave >> 1;
-  // lane layout of v_mfma_i32_16x16x64_i8: A row (lane & 15), k 16*(lane>>4)..+15;
-  // B column (lane & 15), same k; D row 4*(lane>>4)+r, column lane & 15
   const int li = lane & 15, kq = (lane >> 4) * 16;
   const int64_t pa = (int64_t)z.mp * z.kp, pb = (int64_t)z.np * z.kp;
-  const int8_t* Ap = planes + z.pa + (int64_t)(m0 + wm * 32 + li) * z.kp + kq;
-  const int8_t* Bp = planes + z.pb + (int64_t)(n0 + wn * 32 + li) * z.kp + kq;
-  const int64_t sub = (int64_t)16 * z.kp;   // the second 16-row / 16-column sub-tile
+  // staging of chunk c into buffer `buf`: wave w copies rows 16w..16w+15 of
+  // every A and B plane (lane l: bytes 16 (l & 3).. of row 16w + (l >> 2)),
+  // one 1 KB instruction per plane; byte offsets into the planes fit 32
+  // bits (the plane workspace is bounded by OZ_PLANE_BUDGET)
+  const int row = 16 * wave + (lane >> 2);
+  const uint32_t oa = (uint32_t)(z.pa + (int64_t)(m0 + row) * z.kp) + 16 * (lane & 3);
+  const uint32_t ob = (uint32_t)(z.pb + (int64_t)(n0 + row) * z.kp) + 16 * (lane & 3);
+  auto stage = [&](int c, int buf) {
+    const uint32_t ko = (uint32_t)c * OZ_T;
+    int8_t* l0 = s_pl + buf * BUF + wave * 1024;
+#pragma unroll
+    for (int u = 0; u < 2 * S; ++u) {
+      const int opnd = u / S, pl = u % S;
+      const uint32_t o = (opnd ? ob + (uint32_t)(pl * pb) : oa + (uint32_t)(pl * pa)) + ko;
+      __builtin_amdgcn_global_load_lds((oz_glb_t*)(planes + o), (oz_lds_t*)(l0 + u * OZ_PL), 16, 0, 0);
+    }
+  };
 
   v4i acc[S][2][2];
 #pragma unroll
@@ -234,39 +264,36 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
   };
 
   const int nchunk = z.kp / OZ_T;
-  const int64_t klast = (int64_t)(nchunk - 1) * OZ_T;
-  v4i bv[S][2], av[2];
-#pragma unroll
-  for (int tt = 0; tt < S; ++tt)
-#pragma unroll
-    for (int ni = 0; ni < 2; ++ni) bv[tt][ni] = *reinterpret_cast<const v4i*>(Bp + tt * pb + ni * sub);
-#pragma unroll
-  for (int mi = 0; mi < 2; ++mi) av[mi] = *reinterpret_cast<const v4i*>(Ap + mi * sub);
+  // fragment offsets inside a buffer: A plane s, sub-tile mi; B plane t, sub-tile ni
+  const int aoff = (wm * 32 + li) * OZ_T + kq;
+  const int boff = S * OZ_PL + (wn * 32 + li) * OZ_T + kq;
+  if constexpr (NBUF == 2) {
+    stage(0, 0);
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+  }
   for (int c = 0; c < nchunk; ++c) {
-    // the next chunk (the last chunk again at the end: in bounds, unused)
-    const int64_t kn = min((int64_t)(c + 1) * OZ_T, klast);
-    const int64_t ko = (int64_t)c * OZ_T;
-    v4i bn[PF ? S : 1][2];
-    if constexpr (PF) {
-#pragma unroll
-      for (int tt = 0; tt < S; ++tt)
-#pragma unroll
-        for (int ni = 0; ni < 2; ++ni) bn[tt][ni] = *reinterpret_cast<const v4i*>(Bp + tt * pb + ni * sub + kn);
-    } else if (c > 0) {
-      // S >= 7: one wave per SIMD already; the B fragments of this chunk
-      // are loaded here instead of a chunk ahead (the registers of a second
-      // set would not fit)
-#pragma unroll
-      for (int tt = 0; tt < S; ++tt)
-#pragma unroll
-        for (int ni = 0; ni < 2; ++ni) bv[tt][ni] = *reinterpret_cast<const v4i*>(Bp + tt * pb + ni * sub + ko);
+    int buf = 0;
+    if constexpr (NBUF == 2) {
+      buf = c & 1;
+      if (c + 1 < nchunk) stage(c + 1, buf ^ 1);
+    } else {
+      stage(c, 0);
+      __builtin_amdgcn_s_waitcnt(0);
+      __syncthreads();
     }
+    const int8_t* sb = s_pl + buf * BUF;
+    v4i bv[S][2];
+#pragma unroll
+    for (int tt = 0; tt < S; ++tt)
+#pragma unroll
+      for (int ni = 0; ni < 2; ++ni)
+        bv[tt][ni] = *reinterpret_cast<const v4i*>(sb + boff + tt * OZ_PL + ni * 16 * OZ_T);
 #pragma unroll
     for (int sa = 0; sa < S; ++sa) {
-      v4i an[2];
-      const int64_t ao = (sa + 1 < S) ? (int64_t)(sa + 1) * pa + ko : kn;
+      v4i av[2];
 #pragma unroll
-      for (int mi = 0; mi < 2; ++mi) an[mi] = *reinterpret_cast<const v4i*>(Ap + ao + mi * sub);
+      for (int mi = 0; mi < 2; ++mi) av[mi] = *reinterpret_cast<const v4i*>(sb + aoff + sa * OZ_PL + mi * 16 * OZ_T);
 #pragma unroll
       for (int tt = 0; sa + tt < S; ++tt)
 #pragma unroll
@@ -275,24 +302,43 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
           for (int ni = 0; ni < 2; ++ni)
             acc[sa + tt][mi][ni] =
                 __builtin_amdgcn_mfma_i32_16x16x64_i8(av[mi], bv[tt][ni], acc[sa + tt][mi][ni], 0, 0, 0);
-#pragma unroll
-      for (int mi = 0; mi < 2; ++mi) av[mi] = an[mi];
     }
     if constexpr (WIN) {
       if ((c + 1) % OZ_FLUSH == 0) flush();
     }
-    if constexpr (PF) {
-#pragma unroll
-      for (int tt = 0; tt < S; ++tt)
-#pragma unroll
-        for (int ni = 0; ni < 2; ++ni) bv[tt][ni] = bn[tt][ni];
-    }
+    // this wave's copies of chunk c+1 have landed; after the barrier every
+    // wave's have, and no wave reads buffer `buf` any more
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
   }
   if constexpr (WIN) flush();
 
+  // epilogue: the exponents of this lane's 8 rows and 2 columns (padded
+  // rows and columns have them too) and, with beta != 0, the 16 old C
+  // values (clamped addresses) are read before any is used
   double* Cb = C + t.offC;
   const int32_t* ea = ex + z.ea;
   const int32_t* eb = ex + z.eb;
+  const int gi0 = m0 + wm * 32 + (lane >> 4) * 4, gj0 = n0 + wn * 32 + (lane & 15);
+  int er[2][4], ec[2];
+#pragma unroll
+  for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) er[mi][r] = ea[gi0 + mi * 16 + r];
+#pragma unroll
+  for (int ni = 0; ni < 2; ++ni) ec[ni] = eb[gj0 + ni * 16];
+  double cold[2][2][4] = {};
+  if (t.beta != 0.0) {
+#pragma unroll
+    for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+      for (int ni = 0; ni < 2; ++ni)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int gi = min(gi0 + mi * 16 + r, t.m - 1), gj = min(gj0 + ni * 16, t.n - 1);
+          cold[mi][ni][r] = Cb[(int64_t)gi + (int64_t)gj * t.ldc];
+        }
+  }
 #pragma unroll
   for (int mi = 0; mi < 2; ++mi)
 #pragma unroll
@@ -307,13 +353,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 #pragma unroll
           for (int d = 0; d < S; ++d) sum += ldexp(1.0, -OZ_BETA * (d + 2)) * (double)acc[d][mi][ni][r];
         }
-        const int gi = m0 + wm * 32 + mi * 16 + (lane >> 4) * 4 + r;
-        const int gj = n0 + wn * 32 + ni * 16 + (lane & 15);
+        const int gi = gi0 + mi * 16 + r, gj = gj0 + ni * 16;
         if (gi < t.m && gj < t.n) {
-          double* cp = Cb + (int64_t)gi + (int64_t)gj * t.ldc;
-          double v = t.alpha * ldexp(sum, ea[gi] + eb[gj]);
-          if (t.beta != 0.0) v += t.beta * (*cp);
-          *cp = v;
+          double v = t.alpha * ldexp(sum, er[mi][r] + ec[ni]);
+          if (t.beta != 0.0) v += t.beta * cold[mi][ni][r];
+          Cb[(int64_t)gi + (int64_t)gj * t.ldc] = v;
         }
       }
 }
@@ -346,12 +390,13 @@ static void oz_launch(int family, const std::vector<GemmTask>& tk, const std::ve
   for (const OzUnit& u : sunits) { (void)u; sl_bytes += 64.0 * 64.0 * (8.0 + S); }
   plaunch(KF_OTHER, 0, 0, k_oz_exponents, dim3((unsigned)eunits.size()), dim3(256), 0, s, o, deu);
   plaunch(KF_OTHER, sl_bytes, 0, k_oz_slice<S>, dim3((unsigned)sunits.size()), dim3(256), 0, s, o, dsu);
+  const uint32_t shm = (uint32_t)oz_lds_bytes<S>();
   if (kmax > OZ_FLUSH * OZ_T)
-    plaunch(family, bytes, flops, k_oz_gemm<S, true>, dim3((unsigned)tiles), dim3(256), 0, s, o.tasks, o.oz, dts,
-            (int)tk.size(), (const int8_t*)o.planes, (const int32_t*)o.ex, R.oz_C_);
+    plaunch(family, bytes, flops, k_oz_gemm_lds<S, true>, dim3((unsigned)tiles), dim3(256), shm, s, o.tasks, o.oz,
+            dts, (int)tk.size(), (const int8_t*)o.planes, (const int32_t*)o.ex, R.oz_C_);
   else
-    plaunch(family, bytes, flops, k_oz_gemm<S, false>, dim3((unsigned)tiles), dim3(256), 0, s, o.tasks, o.oz, dts,
-            (int)tk.size(), (const int8_t*)o.planes, (const int32_t*)o.ex, R.oz_C_);
+    plaunch(family, bytes, flops, k_oz_gemm_lds<S, false>, dim3((unsigned)tiles), dim3(256), shm, s, o.tasks, o.oz,
+            dts, (int)tk.size(), (const int8_t*)o.planes, (const int32_t*)o.ex, R.oz_C_);
 }
 
 void GemmRunner::run_split(const GemmBatch& b, const double* A, const double* B, double* C,
