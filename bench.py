@@ -68,8 +68,11 @@ def parse():
                          "histogram) or the defaults")
     ap.add_argument("--tuning-cache", default=os.path.join(ROOT, "nrg-mpi_amd", "tuning", "tridiag_mi355x.txt"))
     ap.add_argument("--tune", action="store_true",
-                    help="only run the tuner over the workload (two passes, exploring and updating the cache "
-                         "file) and exit")
+                    help="only run the tuner over the workload (--tune-passes passes, exploring and updating the "
+                         "cache file) and exit")
+    ap.add_argument("--tune-passes", type=int, default=2,
+                    help="passes over the workload with --tune; a block set seen once per pass needs 15 passes "
+                         "for every candidate to be timed three times")
     ap.add_argument("--leg-budget", type=float, default=240.0,
                     help="start the Nkeep=8000 leg only if the run so far took less than this many seconds")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -311,7 +314,7 @@ def main():
     spec = models.workload(a.workload)
     if a.tune:
         os.makedirs(os.path.dirname(a.tuning_cache), exist_ok=True)
-        for p in range(2):
+        for p in range(a.tune_passes):
             tmp = tempfile.mkdtemp(prefix="nrgtune_")
             models.write_inputs(spec, tmp)
             h = Host(tmp, device=0)
