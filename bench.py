@@ -486,8 +486,9 @@ def main():
                 "(lower-triangle SYR2K halved) on FP64 matrix cores, the tridiagonal SYMV's 2m^2 per column "
                 "on the vector units; eigensolver rate = (SYMV + SYR2K + D&C merge + back-transform flops) "
                 "over the tridiagonalisation + D&C + back-transform time",
-        "per_family": {f: {"launches": v["launches"], "flops": v["flops"] / a.steps,
+        "per_family": {f: {"launches": v["launches"] / a.steps, "flops": v["flops"] / a.steps,
                            "bytes": v["bytes"] / a.steps} for f, v in work.items() if v["launches"]},
+        "per_family_unit": "launches, flops and bytes per timed step",
     }
     out = {
         "metric": METRIC,

@@ -106,7 +106,7 @@ class BatchedEigensolver {
   DevBuf<double> d_usel_;
   DevBuf<int32_t> d_mtask_;
   // back-transform
-  DevBuf<double> d_V_, d_T_, d_X_, d_X2_;
+  DevBuf<double> d_V_, d_T_, d_W_, d_X2_;   // panel V, [G | T], W = V T^T, T V^T Z
   DevBuf<int64_t> d_rng_;
   // reflector panels: V at vpo[b] + p n NBB, [G | T] at index tpo[b] + p
   std::vector<int64_t> h_vpo_;

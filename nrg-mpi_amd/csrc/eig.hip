@@ -1671,7 +1671,7 @@ void BatchedEigensolver::release() {
   d_aoff_.release(); d_voff_.release();
   d_d_.release(); d_e_.release(); d_tau_.release(); d_y_.release(); d_P_.release(); d_part_.release();
   d_ws_.release(); d_iws_.release(); d_tasks_.release(); d_k_.release(); d_usel_.release(); d_mtask_.release();
-  d_V_.release(); d_T_.release(); d_X_.release(); d_X2_.release(); d_rng_.release(); gemm_.release();
+  d_V_.release(); d_T_.release(); d_W_.release(); d_X2_.release(); d_rng_.release(); gemm_.release();
   nblk_ = 0;
 }
 
@@ -2236,11 +2236,11 @@ void BatchedEigensolver::divide_conquer(hipStream_t s) {
   NRG_HIP(hipGetLastError());
 }
 
-// V, G = V^T V and T of every reflector panel of every block.  They depend
-// only on the tridiagonalisation (A below the subdiagonal, tau), not on the
-// columns transformed, so they are built once per solve, in three launches
-// over all panels, on stream s (issued while the divide and conquer runs on
-// the caller's stream when s is the second stream).
+// V, G = V^T V, T and W = V T^T of every reflector panel of every block.
+// They depend only on the tridiagonalisation (A below the subdiagonal, tau),
+// not on the columns transformed, so they are built once per solve, in four
+// launches over all panels, on stream s (issued while the divide and conquer
+// runs on the caller's stream when s is the second stream).
 void BatchedEigensolver::panel_factors(hipStream_t s) {
   const int nblk = nblk_;
   h_vpo_.assign(nblk, 0);
@@ -2285,6 +2285,20 @@ void BatchedEigensolver::panel_factors(hipStream_t s) {
   }
   gemm_bt_.run(bt, d_V_.p, d_V_.p, d_T_.p, nullptr, nullptr, nullptr, s, KF_GEMM_BACK);
   plaunch(KF_BT_SMALL, 0, 0, k_bt_tmat, dim3(npl), dim3(NBB * TQ), 0, s, ba, (const int32_t*)d_plist_.p, bt_off_);
+  // W = V T^T for every panel (layout of V): the back-transform then forms
+  // T V^T Z as one product W^T Z per panel
+  d_W_.reserve(std::max<int64_t>(vacc, 1));
+  bt.clear();
+  for (int i = 0; i < npl; ++i) {
+    const int b = h_plist_[2 * i], p = h_plist_[2 * i + 1];
+    const int n = n_[b], c0 = p * NBB;
+    const int mr = n - c0 - bt_off_;
+    const int nbp = std::min(NBB, n - bt_off_ - c0);
+    const int64_t vo = h_vpo_[b] + (int64_t)p * n * NBB;
+    bt.add(gemm_task(mr, nbp, nbp, 0, 1, vo, n, (int64_t)(h_tpo_[b] + p) * 2 * NBB * NBB + NBB * NBB, NBB, vo, n,
+                     1.0, 0.0));
+  }
+  gemm_bt_.run(bt, d_V_.p, d_T_.p, d_W_.p, nullptr, nullptr, nullptr, s, KF_GEMM_BACK);
   NRG_HIP(hipGetLastError());
 }
 
@@ -2295,7 +2309,6 @@ void BatchedEigensolver::back_transform(const int* lo, const int* hi, hipStream_
     NRG_HIP(hipStreamWaitEvent(s, ev_bt_, 0));
     bt_wait_ = false;
   }
-  d_X_.reserve(tot_n_ * NBB);
   d_X2_.reserve(tot_n_ * NBB);
   const int off = bt_off_;
   const int npan = std::max(0, (nmax - off + NBB - 1) / NBB);
@@ -2308,8 +2321,7 @@ void BatchedEigensolver::back_transform(const int* lo, const int* hi, hipStream_
     while (act < nblk && n_[order_[act]] > c0 + off) ++act;
     if (act == 0) continue;
     auto vo = [&](int b) { return h_vpo_[b] + (int64_t)p * n_[b] * NBB; };
-    auto to = [&](int b) { return (int64_t)(h_tpo_[b] + p) * 2 * NBB * NBB; };
-    // X = V^T Z(c0+off:, :nc)
+    // X2 = T V^T Z(c0+off:, :nc) = W^T Z(c0+off:, :nc)
     batch_.clear();
     for (int q = 0; q < act; ++q) {
       const int b = order_[q];
@@ -2322,19 +2334,7 @@ void BatchedEigensolver::back_transform(const int* lo, const int* hi, hipStream_
                            voff_[b] * NBB, NBB, 1.0, 0.0));
     }
     if (batch_.tasks.empty()) continue;
-    gemm_.run(batch_, d_V_.p, Z_, d_X_.p, nullptr, nullptr, nullptr, s, KF_GEMM_BACK);
-    // X2 = T X
-    batch_.clear();
-    for (int q = 0; q < act; ++q) {
-      const int b = order_[q];
-      const int n = n_[b];
-      const int nbp = std::min(NBB, n - off - c0);
-      const int nc = hi[b] - lo[b];
-      if (nc <= 0) continue;
-      batch_.add(gemm_task(nbp, nc, nbp, 0, 0, to(b) + NBB * NBB, NBB, voff_[b] * NBB, NBB, voff_[b] * NBB, NBB,
-                           1.0, 0.0));
-    }
-    gemm_.run(batch_, d_T_.p, d_X_.p, d_X2_.p, nullptr, nullptr, nullptr, s, KF_GEMM_BACK);
+    gemm_.run(batch_, d_W_.p, Z_, d_X2_.p, nullptr, nullptr, nullptr, s, KF_GEMM_BACK);
     // Z(c0+off:, :nc) -= V X2
     batch_.clear();
     for (int q = 0; q < act; ++q) {

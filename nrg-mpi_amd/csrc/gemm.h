@@ -97,7 +97,7 @@ struct GemmRunner {
   // caller issues many small batched products in a row.
   struct RLaunch {
     int kind;                  // 0: GEMM, 1: split-K reduction
-    int fast, tv, family;
+    int fast, tv, g, family;   // g: gather/scale bits of the fast kernel
     unsigned grid, gx;
     int nt;
     size_t tab, tst;           // byte offsets of the task table / tile starts (or reduction records)

@@ -131,10 +131,14 @@ __global__ __launch_bounds__(256) void k_gemm_f64(
 }
 
 // ---------------------------------------------------------------------------
-// Fast path for products without gathers or k-scales (the eigensolver's
-// trailing updates, merges and back-transform): the task table is searched
-// in LDS, the K loop runs in 32-deep slices and the next slice's operands
-// are fetched into registers while the matrix cores work on the current one.
+// Fast path: the task table is searched in LDS, the K loop runs in 32-deep
+// slices and the next slice's operands are fetched into registers while the
+// matrix cores work on the current one.  G selects gathers and the k-scale
+// along K (bit 0: opA = A^T with its stored rows gathered, bit 1: opB = B
+// with its stored rows gathered, bit 2: opB scaled per k), the forms of the
+// invariant-matrix, spectral and reduced-density contractions: K is the
+// lane-contiguous index of those loads, so each lane needs one gather index
+// (and one scale) per slice, fetched one slice ahead of the operands.
 constexpr int TK2 = 32;
 constexpr int MAXT_LDS = 4096;   // tasks whose tile starts are searched in LDS
 
@@ -157,10 +161,13 @@ __device__ inline int find_task(const int32_t* __restrict__ tile_start, int ntas
   return lo;
 }
 
-template <int TA, int TB>
+template <int TA, int TB, int G>
 __global__ __launch_bounds__(256, 3) void k_gemm_f64_fast(
     const GemmTask* __restrict__ tasks, const int32_t* __restrict__ tile_start, int ntasks,
-    const double* __restrict__ A, const double* __restrict__ B, double* __restrict__ C) {
+    const double* __restrict__ A, const double* __restrict__ B, double* __restrict__ C,
+    const int32_t* __restrict__ gA, const int32_t* __restrict__ gB, const double* __restrict__ ks) {
+  static_assert(!(G & 1) || TA == 1, "A is gathered along K only when stored K-major");
+  static_assert(!(G & 6) || TB == 0, "B is gathered and scaled along K only when stored K-major");
   // A and B slices; after the K loop the same space holds the C tile
   // (64 columns of LDS_STRIDE rows) for the column-wise epilogue
   static_assert(2 * TK2 == TN, "the C tile fits in the A/B slices");
@@ -203,26 +210,46 @@ __global__ __launch_bounds__(256, 3) void k_gemm_f64_fast(
   const uint32_t b_off = (uint32_t)(b_s * t.ldb + b_c) * 8u;
   // strides of u along the strided index: 4 (64-wide contiguous) or 8
   constexpr int a_su = !TA ? 4 : 8, b_su = TB ? 4 : 8;
+  // gathers and k-scale of this task (identity where the task has none):
+  // stored row of slice element k0 + (tid & 31), and its scale, for the
+  // slice being fetched (_c) and the one after it (_n)
+  const int32_t* gat = ((G & 1) && t.offGA >= 0) ? gA + t.offGA : nullptr;
+  const int32_t* gbt = ((G & 2) && t.offGB >= 0) ? gB + t.offGB : nullptr;
+  const double* kst = ((G & 4) && t.offS >= 0) ? ks + t.offS : nullptr;
+  int ga_c = 0, gb_c = 0, ga_n = 0, gb_n = 0;
+  double ks_c = 1.0, ks_n = 1.0;
+  auto load_idx = [&](int k0) {
+    const int kk = min(k0 + (tid & 31), max(t.k - 1, 0));
+    if (G & 1) ga_n = gat ? gat[kk] : kk;
+    if (G & 2) gb_n = gbt ? gbt[kk] : kk;
+    if (G & 4) ks_n = kst ? kst[kk] : 1.0;
+  };
   auto fetch = [&](int k0) {
+    if (G) { ga_c = ga_n; gb_c = gb_n; ks_c = ks_n; }
+    // gathered operands: row bases without k0, the stored row in the lane offset
+    const uint32_t a_lo = (G & 1) ? (uint32_t)(a_s * t.lda + ga_c) * 8u : a_off;
+    const uint32_t b_lo = (G & 2) ? (uint32_t)(b_s * t.ldb + gb_c) * 8u : b_off;
+    const int ka = (G & 1) ? 0 : k0, kb = (G & 2) ? 0 : k0;
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
       // opA: TA = 0 -> row i = m0 + a_c, k = k0 + a_s + 4u; TA = 1 -> k = k0 + a_c, row = m0 + a_s + 8u
       const int sa = a_s + a_su * u;
       const int gi = !TA ? m0 + a_c : m0 + sa, gk = !TA ? k0 + sa : k0 + a_c;
       const char* pa = (const char*)(Ab + (!TA ? (int64_t)(k0 + a_su * u) * t.lda + m0
-                                               : (int64_t)(m0 + a_su * u) * t.lda + k0));
+                                               : (int64_t)(m0 + a_su * u) * t.lda + ka));
       double v = 0.0;
-      if (gi < t.m && gk < t.k) v = *(const double*)(pa + a_off);
+      if (gi < t.m && gk < t.k) v = *(const double*)(pa + a_lo);
       ra[u] = v;
       // opB: TB = 1 -> col j = n0 + b_c, k = k0 + b_s + 4u; TB = 0 -> k = k0 + b_c, col = n0 + b_s + 8u
       const int sb = b_s + b_su * u;
       const int gj = TB ? n0 + b_c : n0 + sb, gk2 = TB ? k0 + sb : k0 + b_c;
       const char* pb = (const char*)(Bb + (TB ? (int64_t)(k0 + b_su * u) * t.ldb + n0
-                                              : (int64_t)(n0 + b_su * u) * t.ldb + k0));
+                                              : (int64_t)(n0 + b_su * u) * t.ldb + kb));
       double w = 0.0;
-      if (gj < t.n && gk2 < t.k) w = *(const double*)(pb + b_off);
+      if (gj < t.n && gk2 < t.k) w = *(const double*)(pb + b_lo);
       rb[u] = w;
     }
+    if (G) load_idx(k0 + TK2);   // the next slice's rows, in flight with these operands
   };
   auto store = [&]() {
 #pragma unroll
@@ -233,7 +260,7 @@ __global__ __launch_bounds__(256, 3) void k_gemm_f64_fast(
       As[kk * LDS_STRIDE + i] = ra[u];
       int j, kb;
       if (!TB) { kb = e & 31; j = e >> 5; } else { j = e & 63; kb = e >> 6; }
-      Bs[kb * LDS_STRIDE + j] = rb[u];
+      Bs[kb * LDS_STRIDE + j] = (G & 4) ? rb[u] * ks_c : rb[u];
     }
   };
 
@@ -259,6 +286,7 @@ __global__ __launch_bounds__(256, 3) void k_gemm_f64_fast(
     for (int b = 0; b < 2; ++b) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
 
   if (t.k > 0) {
+    if (G) load_idx(0);
     fetch(0);
     store();
     __syncthreads();
@@ -318,6 +346,33 @@ __global__ __launch_bounds__(256, 3) void k_gemm_f64_fast(
 
 
 
+// Fast-kernel instance for transposition code tv = 2 transA + transB and
+// gather/scale bits g; nullptr where the fast kernel has no such form (a
+// gather along M or N, a scale along a strided index): the general kernel
+// runs those.
+using FastGemm = void (*)(const GemmTask*, const int32_t*, int, const double*, const double*, double*,
+                          const int32_t*, const int32_t*, const double*);
+static FastGemm fast_kernel(int tv, int g) {
+  switch (tv * 8 + g) {
+    case 0: return k_gemm_f64_fast<0, 0, 0>;
+    case 2: return k_gemm_f64_fast<0, 0, 2>;
+    case 4: return k_gemm_f64_fast<0, 0, 4>;
+    case 6: return k_gemm_f64_fast<0, 0, 6>;
+    case 8: return k_gemm_f64_fast<0, 1, 0>;
+    case 16: return k_gemm_f64_fast<1, 0, 0>;
+    case 17: return k_gemm_f64_fast<1, 0, 1>;
+    case 18: return k_gemm_f64_fast<1, 0, 2>;
+    case 19: return k_gemm_f64_fast<1, 0, 3>;
+    case 20: return k_gemm_f64_fast<1, 0, 4>;
+    case 21: return k_gemm_f64_fast<1, 0, 5>;
+    case 22: return k_gemm_f64_fast<1, 0, 6>;
+    case 23: return k_gemm_f64_fast<1, 0, 7>;
+    case 24: return k_gemm_f64_fast<1, 1, 0>;
+    case 25: return k_gemm_f64_fast<1, 1, 1>;
+    default: return nullptr;
+  }
+}
+
 // Deterministic split-K reduction: C = alpha sum_c W_c + beta C, partials
 // summed in chunk order.  Grid: (element chunks, records).
 __global__ void k_splitk_reduce(const SplitRed* __restrict__ recs, const double* __restrict__ W, double* C) {
@@ -338,12 +393,17 @@ int64_t GemmRunner::decompose(const GemmBatch& b) {
   plain_.clear();
   part_.clear();
   h_red_.clear();
-  int64_t wsz = 0;
+  // A launch of fewer than 512 tiles leaves most of the 256 CUs idle while
+  // each tile runs its whole K loop: then every task with K >= 256 is split
+  // into parts of 128-255, and the unsplit rest joins the split launch.
+  int64_t tot = 0;
   for (const GemmTask& t : b.tasks) {
-    const int64_t tiles = (int64_t)ceil_div(t.m, TM) * ceil_div(t.n, TN);
-    int parts = 1;
-    if (!t.lower && t.k >= 1024 && tiles <= 16) parts = std::min(16, t.k / 512);
-    if (parts < 2) { plain_.add(t); continue; }
+    const int64_t tm = ceil_div(t.m, TM);
+    tot += t.lower ? tm * (tm + 1) / 2 : tm * ceil_div(t.n, TN);
+  }
+  const bool thin = tot < 512;
+  int64_t wsz = 0;
+  auto split = [&](const GemmTask& t, int parts) {
     const int chunk = (int)(ceil_div(t.k, parts * TK2) * TK2);
     parts = ceil_div(t.k, chunk);
     SplitRed r;
@@ -368,6 +428,22 @@ int64_t GemmRunner::decompose(const GemmBatch& b) {
       part_.add(u);
     }
     wsz += (int64_t)parts * t.m * t.n;
+  };
+  for (const GemmTask& t : b.tasks) {
+    const int64_t tiles = (int64_t)ceil_div(t.m, TM) * ceil_div(t.n, TN);
+    int parts = 1;
+    if (!t.lower && t.k >= 1024 && tiles <= 16) parts = std::min(16, t.k / 512);
+    if (thin && !t.lower && t.k >= 256) parts = std::max(parts, std::min(32, t.k / 128));
+    if (parts < 2) plain_.add(t);
+    else split(t, parts);
+  }
+  if (thin && !part_.tasks.empty() && !plain_.tasks.empty()) {
+    const GemmBatch rest = plain_;
+    plain_.clear();
+    for (const GemmTask& t : rest.tasks) {
+      if (t.lower || t.k == 0) plain_.add(t);
+      else split(t, 1);
+    }
   }
   return wsz;
 }
@@ -438,13 +514,19 @@ void GemmRunner::launch(const GemmBatch& b, const double* A, const double* B, do
   }
   NRG_REQUIRE(total < (1LL << 31), "gemm: too many tiles");
   bool uniform = true;
-  for (int i = 1; i < nt && uniform; ++i)
-    uniform = (b.tasks[i].transA == b.tasks[0].transA && b.tasks[i].transB == b.tasks[0].transB);
+  int gbits = 0;
+  for (int i = 0; i < nt; ++i) {
+    const GemmTask& t = b.tasks[i];
+    uniform = uniform && t.transA == b.tasks[0].transA && t.transB == b.tasks[0].transB;
+    if (gatherA && t.offGA >= 0) gbits |= 1;
+    if (gatherB && t.offGB >= 0) gbits |= 2;
+    if (kscale && t.offS >= 0) gbits |= 4;
+  }
   const int tv = b.tasks[0].transA * 2 + b.tasks[0].transB;
-  const bool fast = (!gatherA && !gatherB && !kscale && uniform);
+  const FastGemm fk = uniform ? fast_kernel(tv, gbits) : nullptr;
   if (rec_) {
     RLaunch r{};
-    r.kind = 0; r.fast = fast ? 1 : 0; r.tv = tv; r.family = family;
+    r.kind = 0; r.fast = fk ? 1 : 0; r.tv = tv; r.g = gbits; r.family = family;
     r.grid = (unsigned)total; r.nt = nt;
     r.tab = r_append(b.tasks.data(), (size_t)nt * sizeof(GemmTask));
     r.tst = r_append(h_tile_start.data(), (size_t)nt * sizeof(int32_t));
@@ -465,11 +547,8 @@ void GemmRunner::launch(const GemmBatch& b, const double* A, const double* B, do
     upload_tables(b, s, tk, ts, slot);
   }
   const dim3 g((unsigned)total), blk(256);
-  if (fast) {
-    if (tv == 0) plaunch(family, bytes, flops, k_gemm_f64_fast<0, 0>, g, blk, 0, s, tk, ts, nt, A, B, C);
-    else if (tv == 1) plaunch(family, bytes, flops, k_gemm_f64_fast<0, 1>, g, blk, 0, s, tk, ts, nt, A, B, C);
-    else if (tv == 2) plaunch(family, bytes, flops, k_gemm_f64_fast<1, 0>, g, blk, 0, s, tk, ts, nt, A, B, C);
-    else plaunch(family, bytes, flops, k_gemm_f64_fast<1, 1>, g, blk, 0, s, tk, ts, nt, A, B, C);
+  if (fk) {
+    plaunch(family, bytes, flops, fk, g, blk, 0, s, tk, ts, nt, A, B, C, gatherA, gatherB, kscale);
   } else {
     plaunch(family, bytes, flops, k_gemm_f64, g, blk, 0, s, tk, ts, nt, A, B, C, gatherA, gatherB, kscale);
   }
@@ -565,10 +644,8 @@ void GemmRunner::issue(size_t i0, size_t i1, hipStream_t s) {
     double* C = r.C ? r.C : d_splitw_.p;
     const dim3 g(r.grid);
     if (r.fast) {
-      if (r.tv == 0) plaunch(r.family, r.bytes, r.flops, k_gemm_f64_fast<0, 0>, g, blk, 0, s, tk, ts, r.nt, r.A, r.B, C);
-      else if (r.tv == 1) plaunch(r.family, r.bytes, r.flops, k_gemm_f64_fast<0, 1>, g, blk, 0, s, tk, ts, r.nt, r.A, r.B, C);
-      else if (r.tv == 2) plaunch(r.family, r.bytes, r.flops, k_gemm_f64_fast<1, 0>, g, blk, 0, s, tk, ts, r.nt, r.A, r.B, C);
-      else plaunch(r.family, r.bytes, r.flops, k_gemm_f64_fast<1, 1>, g, blk, 0, s, tk, ts, r.nt, r.A, r.B, C);
+      plaunch(r.family, r.bytes, r.flops, fast_kernel(r.tv, r.g), g, blk, 0, s, tk, ts, r.nt, r.A, r.B, C, r.gA,
+              r.gB, r.ks);
     } else {
       plaunch(r.family, r.bytes, r.flops, k_gemm_f64, g, blk, 0, s, tk, ts, r.nt, r.A, r.B, C, r.gA, r.gB, r.ks);
     }
