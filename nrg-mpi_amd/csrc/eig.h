@@ -100,7 +100,9 @@ class BatchedEigensolver {
   DevBuf<int32_t> d_k_;
   std::vector<int32_t> h_k_;
   // deferred top merge of each block (lev 0: the block is a leaf)
-  struct TopMerge { int lev = 0, m = 0, k = 0; int64_t wo = 0, io = 0; };
+  // a block's deferred top merge: size m (first half m1), k non-deflated
+  // columns grouped as k1 first-half, k3 both-halves, k - k1 - k3 second-half
+  struct TopMerge { int lev = 0, m = 0, m1 = 0, k = 0, k1 = 0, k3 = 0; int64_t wo = 0, io = 0; };
   std::vector<TopMerge> top_;
   std::vector<int> mat_;                    // columns of Z holding tridiagonal eigenvectors
   DevBuf<double> d_usel_;

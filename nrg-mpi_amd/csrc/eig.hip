@@ -907,8 +907,21 @@ __global__ __launch_bounds__(64) void k_dc_leaf(DcArgs a, const int32_t* tasks, 
 
 // merge workspace layout (doubles, per merge, m = e - s):
 //   ds[m] zs[m] dl[m] zl[m] dv[m] zh[m] tr[m] rc[m] rs[m] | Qcd[m*m] | U[m*m] | Qt[m*m]
-// ints: col[m] nd[m] df[m] org[m] rp[2m] | sorted-deflated order[m]
+// ints: col[m] nd[m] df[m] org[m] rp[2m] | sorted-deflated order[m] | nrot, pad |
+//   ndt[m] perm[m] inv[m] half[m] (at 8m + 8: support of each non-deflated
+//   column, its position in the grouped order and back, the half of each
+//   sorted pole)
 __device__ inline int dws_vec(int m) { return 9 * m; }
+__host__ __device__ inline int64_t dws_int(int m) { return 12 * (int64_t)m + 16; }
+
+// The non-deflated columns of Qcd and the rows of U are stored grouped by
+// support (LAPACK dlaed2's column types): first the columns with nonzeros
+// only in the rows of the first half (bit 1), then those with both halves
+// (bits 1|2: a deflation rotation mixed two columns from different halves),
+// then those of the second half only (bit 2), each group in ascending pole
+// order.  Qcd U then splits into two products on the halves' rows, over the
+// columns [0, k1 + k3) and [k1, k): about half the flops of the dense one.
+// kout holds k per merge at [t] and (k1, k3) at [ntask + 2t].
 
 __global__ __launch_bounds__(256) void k_dc_merge_prep(DcArgs a, const int32_t* tasks, int ntask) {
   const int t = blockIdx.x;
@@ -926,11 +939,14 @@ __global__ __launch_bounds__(256) void k_dc_merge_prep(DcArgs a, const int32_t* 
   double *rc = W + 7 * m, *rsn = W + 8 * m;
   double* Qcd = W + dws_vec(m);
   int32_t *col = I, *nd = I + m, *df = I + 2 * m, *rp = I + 4 * m;
+  int32_t *ndt = I + 8 * m + 8, *perm = I + 9 * m + 8, *inv = I + 10 * m + 8, *hf = I + 11 * m + 8;
   const int tid = threadIdx.x;
   extern __shared__ double dc_shm[];
   __shared__ double s_red[4];
   __shared__ int s_k, s_nrot;
   __shared__ double s_rho;
+  __shared__ int s_cnt[3][256];
+  __shared__ int s_tot[3];
 
   const double beta = a.e[vo + mid - 1];
   const double rho = fabs(beta);
@@ -960,6 +976,7 @@ __global__ __launch_bounds__(256) void k_dc_merge_prep(DcArgs a, const int32_t* 
       ds[rank] = v;
       zs[rank] = z * 0.7071067811865476;  // z / sqrt(2): |z| = 1
       col[rank] = cidx;
+      hf[rank] = (i < m1) ? 1 : 2;
     }
   };
   if (m <= DC_LDS_M) {
@@ -975,16 +992,17 @@ __global__ __launch_bounds__(256) void k_dc_merge_prep(DcArgs a, const int32_t* 
   for (int i = tid; i < m; i += 256) mx = fmax(mx, fmax(fabs(ds[i]), fabs(zs[i])));
   for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o, 64));
   if ((tid & 63) == 0) s_red[tid >> 6] = mx;
-  // stage the sorted poles and weights in LDS for the serial scan
+  // stage the sorted poles, weights and halves in LDS for the serial scan
   const bool use_lds = (m <= DC_LDS_M);
+  unsigned char* shf = reinterpret_cast<unsigned char*>(dc_shm + 2 * m);
   if (use_lds)
-    for (int i = tid; i < m; i += 256) { dc_shm[i] = ds[i]; dc_shm[m + i] = zs[i]; }
+    for (int i = tid; i < m; i += 256) { dc_shm[i] = ds[i]; dc_shm[m + i] = zs[i]; shf[i] = (unsigned char)hf[i]; }
   __syncthreads();
   // The scan and the copies after it are written once and instantiated for
   // LDS and for global poles/weights: with one pointer that may be either,
   // the compiler emits flat accesses, and a flat load waits for every global
   // store before it (the scan's index stores), one memory latency per step.
-  auto scan = [&](auto* sds, auto* szs) {
+  auto scan = [&](auto* sds, auto* szs, const auto* shh) {
     const double rp2 = 2.0 * rho;
     const double tol = 8.0 * 2.220446049250313e-16 * fmax(fmax(s_red[0], s_red[1]), fmax(s_red[2], s_red[3]));
     s_rho = rp2;
@@ -992,14 +1010,17 @@ __global__ __launch_bounds__(256) void k_dc_merge_prep(DcArgs a, const int32_t* 
     // the running column pj are carried in registers (written back when pj
     // is settled) and those of the next position are fetched one step
     // ahead, so the loop-carried chain is arithmetic only.
-    int k = 0, nd_def = 0, nrot = 0, pj = -1;
+    // tp: halves holding the running column's nonzeros
+    int k = 0, nd_def = 0, nrot = 0, pj = -1, tp = 0;
     double zp = 0.0, dp = 0.0;
     double zn = szs[0], dn = sds[0];
+    int hn = shh[0];
     for (int j = 0; j < m; ++j) {
       const double zj = zn, dj = dn;
-      if (j + 1 < m) { zn = szs[j + 1]; dn = sds[j + 1]; }
+      const int hj = hn;
+      if (j + 1 < m) { zn = szs[j + 1]; dn = sds[j + 1]; hn = shh[j + 1]; }
       if (rp2 * fabs(zj) <= tol) { df[nd_def++] = j; continue; }
-      if (pj < 0) { pj = j; zp = zj; dp = dj; continue; }
+      if (pj < 0) { pj = j; zp = zj; dp = dj; tp = hj; continue; }
       const double dt = dj - dp;
       const double r2 = zj * zj + zp * zp;   // |z| <= 1: no overflow
       // |dt c s| <= tol with c = zj / r, s = -zp / r, r^2 = zj^2 + zp^2,
@@ -1020,18 +1041,22 @@ __global__ __launch_bounds__(256) void k_dc_merge_prep(DcArgs a, const int32_t* 
         pj = j;
         zp = tt;
         dp = dp * sv * sv + dj * cv * cv;
+        tp |= hj;   // column j now carries the rotated-out column's rows too
       } else {
         szs[pj] = zp;
         sds[pj] = dp;
+        ndt[k] = tp;
         nd[k++] = pj;
         pj = j;
         zp = zj;
         dp = dj;
+        tp = hj;
       }
     }
     if (pj >= 0) {
       szs[pj] = zp;
       sds[pj] = dp;
+      ndt[k] = tp;
       nd[k++] = pj;
     }
     s_k = k;
@@ -1048,15 +1073,56 @@ __global__ __launch_bounds__(256) void k_dc_merge_prep(DcArgs a, const int32_t* 
     for (int i = tid; i < m - k; i += 256) dv[i] = sds[df[i]];
   };
   if (use_lds) {
-    if (tid == 0) scan(dc_shm, dc_shm + m);
+    if (tid == 0) scan(dc_shm, dc_shm + m, shf);
     __syncthreads();
     gather(dc_shm, dc_shm + m);
   } else {
-    if (tid == 0) scan(ds, zs);
+    if (tid == 0) scan(ds, zs, hf);
     __syncthreads();
     gather(ds, zs);
   }
   if (tid == 0) W[5 * m] = s_rho;  // zh[0] slot keeps rho until the z-hat pass
+  // grouped order of the non-deflated columns: a stable partition of
+  // [0, k) by support (first half, both, second half); thread tid takes a
+  // contiguous range, the per-thread counts are scanned by one thread per group
+  {
+    const int k = s_k;
+    const int per = (k + 255) / 256;
+    const int i0 = min(k, tid * per), i1 = min(k, i0 + per);
+    int c1 = 0, c3 = 0, c2 = 0;
+    for (int i = i0; i < i1; ++i) {
+      const int ty = ndt[i];
+      c1 += (ty == 1);
+      c3 += (ty == 3);
+      c2 += (ty == 2);
+    }
+    s_cnt[0][tid] = c1;
+    s_cnt[1][tid] = c3;
+    s_cnt[2][tid] = c2;
+    __syncthreads();
+    if (tid < 3) {
+      int acc = 0;
+      for (int q = 0; q < 256; ++q) {
+        const int v = s_cnt[tid][q];
+        s_cnt[tid][q] = acc;
+        acc += v;
+      }
+      s_tot[tid] = acc;
+    }
+    __syncthreads();
+    const int k1 = s_tot[0], k3 = s_tot[1];
+    int p1 = s_cnt[0][tid], p3 = k1 + s_cnt[1][tid], p2 = k1 + k3 + s_cnt[2][tid];
+    for (int i = i0; i < i1; ++i) {
+      const int ty = ndt[i];
+      const int pos = (ty == 1) ? p1++ : (ty == 3) ? p3++ : p2++;
+      perm[i] = pos;
+      inv[pos] = i;
+    }
+    if (tid == 0) {
+      a.kout[ntask + 2 * t] = k1;
+      a.kout[ntask + 2 * t + 1] = k3;
+    }
+  }
   (void)Qcd;
 }
 
@@ -1125,8 +1191,8 @@ __global__ __launch_bounds__(256) void k_dc_rotate(DcArgs a, const int32_t* task
   if (row < m && cc >= 0) Zr[(int64_t)cc * n] = cvl;
 }
 
-// Gather the rotated merge columns as [non-deflated | deflated] into Qcd
-// (m x m, ld m).  Grid: (row chunks of 64, column chunks of DC_CCH,
+// Gather the rotated merge columns as [non-deflated (grouped order) |
+// deflated] into Qcd (m x m, ld m).  Grid: (row chunks of 64, column chunks of DC_CCH,
 // merges); each workgroup owns 64 rows and DC_CCH columns of one merge, its
 // 4 waves take every 4th column (a top-level merge is thousands of
 // workgroups, enough to stream the m^2 copy at bandwidth).
@@ -1145,7 +1211,7 @@ __global__ __launch_bounds__(256) void k_dc_rotgather(DcArgs a, const int32_t* t
   double* Z = a.Z + a.aoff[b];
   double* W = a.ws + join64(tk[4], tk[5]);
   const int32_t* I = a.iws + join64(tk[6], tk[7]);
-  const int32_t *col = I, *nd = I + m, *df = I + 2 * m;
+  const int32_t *col = I, *nd = I + m, *df = I + 2 * m, *inv = I + 10 * m + 8;
   double* Qcd = W + dws_vec(m);
   const int tid = threadIdx.x;
   const int rl = tid & 63, g = tid >> 6;
@@ -1153,7 +1219,7 @@ __global__ __launch_bounds__(256) void k_dc_rotgather(DcArgs a, const int32_t* t
   if (row >= m) return;
 #pragma unroll 8
   for (int c = c0 + g; c < c1; c += 4) {
-    const int srcc = (c < k) ? col[nd[c]] : col[df[c - k]];
+    const int srcc = (c < k) ? col[nd[inv[c]]] : col[df[c - k]];
     Qcd[row + (int64_t)c * m] = Z[(int64_t)(s + row) + (int64_t)srcc * n];
   }
 }
@@ -1334,7 +1400,8 @@ __global__ __launch_bounds__(256) void k_dc_zhat(DcArgs a, const int32_t* tasks,
   if (lane == 0) zh[j] = copysign(sqrt(fabs(p)), zl[j]);
 }
 
-// eigenvectors of the rank-one problem: one wave per column.
+// eigenvectors of the rank-one problem: one wave per column; the rows
+// follow the grouped column order of Qcd.
 __global__ __launch_bounds__(64) void k_dc_vectors(DcArgs a, const int32_t* tasks, int ntask) {
   const int t = blockIdx.y;
   const int i = blockIdx.x;
@@ -1344,18 +1411,19 @@ __global__ __launch_bounds__(64) void k_dc_vectors(DcArgs a, const int32_t* task
   const int m = tk[3] - tk[1];
   double* W = a.ws + join64(tk[4], tk[5]);
   const int32_t* org = a.iws + join64(tk[6], tk[7]) + 3 * m;
+  const int32_t* perm = a.iws + join64(tk[6], tk[7]) + 9 * m + 8;
   const double *dl = W + 2 * m, *tr = W + 6 * m, *zh = W + 7 * m;
   double* U = W + dws_vec(m) + (int64_t)m * m;
   const double dorg = dl[org[i]], ti = tr[i];
   double ss = 0.0;
   for (int j = threadIdx.x; j < k; j += 64) {
     const double v = zh[j] / ((dl[j] - dorg) - ti);
-    U[j + (int64_t)i * k] = v;
+    U[perm[j] + (int64_t)i * k] = v;   // row of pole j in the grouped order
     ss += v * v;
   }
   ss = wave_sum(ss);
   const double inv = 1.0 / sqrt(ss);
-  for (int j = threadIdx.x; j < k; j += 64) U[j + (int64_t)i * k] *= inv;
+  for (int j = threadIdx.x; j < k; j += 64) U[perm[j] + (int64_t)i * k] *= inv;
 }
 
 constexpr int FIN_T = 1024;   // threads of the final-ordering kernel (its bitonic sort)
@@ -2138,7 +2206,7 @@ void BatchedEigensolver::divide_conquer(hipStream_t s) {
         rec.push_back((int32_t)(wacc & 0xffffffff)); rec.push_back((int32_t)(wacc >> 32));
         rec.push_back((int32_t)(iacc & 0xffffffff)); rec.push_back((int32_t)(iacc >> 32));
         wacc += 9 * (int64_t)m + 3 * (int64_t)m * m + 8;
-        iacc += 8 * (int64_t)m + 8;
+        iacc += dws_int(m);
       }
     }
     wsmax = std::max(wsmax, wacc);
@@ -2156,9 +2224,9 @@ void BatchedEigensolver::divide_conquer(hipStream_t s) {
       for (int q = 0; q < 4; ++q) rec.push_back(tr[i + q]);
       rec.push_back((int32_t)(wtop & 0xffffffff)); rec.push_back((int32_t)(wtop >> 32));
       rec.push_back((int32_t)(itop & 0xffffffff)); rec.push_back((int32_t)(itop >> 32));
-      top_[b].lev = lev; top_[b].m = m; top_[b].wo = wtop; top_[b].io = itop;
+      top_[b].lev = lev; top_[b].m = m; top_[b].m1 = tr[i + 2] - tr[i + 1]; top_[b].wo = wtop; top_[b].io = itop;
       wtop += 9 * (int64_t)m + 3 * (int64_t)m * m + 8;
-      itop += 8 * (int64_t)m + 8;
+      itop += dws_int(m);
     }
   }
   d_ws_.reserve(std::max<int64_t>(wtop, 1));
@@ -2185,20 +2253,22 @@ void BatchedEigensolver::divide_conquer(hipStream_t s) {
     const int nt = (int)rec.size() / 8;
     if (nt == 0) continue;
     d_tasks_.upload(rec.data(), rec.size(), s);
-    d_k_.reserve(nt);
+    d_k_.reserve(3 * nt);
     da.kout = d_k_.p;
     int mmax = 0;
     for (int i = 0; i < nt; ++i) mmax = std::max(mmax, rec[8 * i + 3] - rec[8 * i + 1]);
     {
-      const size_t shm = 2 * sizeof(double) * (size_t)std::min(mmax, DC_LDS_M);
+      // poles and weights (2 doubles) and the half (1 byte) of each sorted pole
+      const size_t ml = (size_t)std::min(mmax, DC_LDS_M);
+      const size_t shm = 2 * sizeof(double) * ml + (ml + 7) / 8 * 8;
       const int32_t* tk = d_tasks_.p;
       plaunch(KF_DC_PREP, 0, 0, k_dc_merge_prep, dim3(nt), dim3(256), (uint32_t)shm, s, da, tk, nt);
       plaunch(KF_DC_PREP, 0, 0, k_dc_rotate, dim3(ceil_div(mmax, 256), nt), dim3(256), 0, s, da, tk, nt);
       plaunch(KF_DC_PREP, 0, 0, k_dc_rotgather, dim3(ceil_div(mmax, 64), ceil_div(mmax, DC_CCH), nt), dim3(256), 0,
               s, da, tk, nt);
     }
-    h_k_.resize(nt);
-    d_k_.download(h_k_.data(), nt, s);
+    h_k_.resize(3 * nt);
+    d_k_.download(h_k_.data(), 3 * nt, s);
     NRG_HIP(hipStreamSynchronize(s));
     int kmax = 0;
     for (int i = 0; i < nt; ++i) kmax = std::max(kmax, h_k_[i]);
@@ -2208,19 +2278,34 @@ void BatchedEigensolver::divide_conquer(hipStream_t s) {
               (uint32_t)(2 * sizeof(double) * (size_t)std::min(kmax, SEC_LDS_M)), s, da, tk, nt);
       plaunch(KF_DC_VECTORS, 0, 0, k_dc_zhat, dim3(ceil_div(kmax, 4), nt), dim3(256), 0, s, da, tk, nt);
       plaunch(KF_DC_VECTORS, 0, 0, k_dc_vectors, dim3(kmax, nt), dim3(64), 0, s, da, tk, nt);
+      // Qt = Qcd U on the rows of each half, over the grouped columns with
+      // support there: [0, k1 + k3) for the first half, [k1, k) for the
+      // second.  Merges of at most 64 rows (one output tile) stay one dense
+      // product: their launches are task-bound, not flop-bound.
       batch_.clear();
       for (int i = 0; i < nreg[lev - 1]; ++i) {
         const int k = h_k_[i];
         if (k == 0) continue;
-        const int m = rec[8 * i + 3] - rec[8 * i + 1];
+        const int k1 = h_k_[nt + 2 * i], k3 = h_k_[nt + 2 * i + 1];
+        const int m = rec[8 * i + 3] - rec[8 * i + 1], m1 = rec[8 * i + 2] - rec[8 * i + 1];
         const int64_t wo = (int64_t)(uint32_t)rec[8 * i + 4] | ((int64_t)rec[8 * i + 5] << 32);
-        const int64_t base = wo + 9 * (int64_t)m;
-        batch_.add(gemm_task(m, k, k, 0, 0, base, m, base + (int64_t)m * m, k,
-                             base + 2 * (int64_t)m * m, m, 1.0, 0.0));
+        const int64_t base = wo + 9 * (int64_t)m, ub = base + (int64_t)m * m, qt = base + 2 * (int64_t)m * m;
+        if (m <= 64) {
+          batch_.add(gemm_task(m, k, k, 0, 0, base, m, ub, k, qt, m, 1.0, 0.0));
+          continue;
+        }
+        batch_.add(gemm_task(m1, k, k1 + k3, 0, 0, base, m, ub, k, qt, m, 1.0, 0.0));
+        batch_.add(gemm_task(m - m1, k, k - k1, 0, 0, base + m1 + (int64_t)k1 * m, m, ub + k1, k, qt + m1, m,
+                             1.0, 0.0));
       }
       gemm_.run(batch_, d_ws_.p, d_ws_.p, d_ws_.p, nullptr, nullptr, nullptr, s, KF_GEMM_DC);
     }
-    for (int i = nreg[lev - 1]; i < nt; ++i) top_[rec[8 * i]].k = h_k_[i];
+    for (int i = nreg[lev - 1]; i < nt; ++i) {
+      TopMerge& T = top_[rec[8 * i]];
+      T.k = h_k_[i];
+      T.k1 = h_k_[nt + 2 * i];
+      T.k3 = h_k_[nt + 2 * i + 1];
+    }
     const int32_t* tk = d_tasks_.p;
     // LDS: (value, index) pairs of the deflated values, padded to a power of two
     int P = 1;
@@ -2385,9 +2470,14 @@ void BatchedEigensolver::materialize(const int* hi, hipStream_t s) {
                       (int32_t)(T.io & 0xffffffff), (int32_t)(T.io >> 32), (int32_t)(uacc & 0xffffffff),
                       (int32_t)(uacc >> 32)})
       tk.push_back(v);
-    if (k > 0)
-      batch_.add(gemm_task(m, nc, k, 0, 0, T.wo + 9 * (int64_t)m, m, uacc, k, aoff_[b] + (int64_t)lo * n_[b], n_[b],
+    if (k > 0) {
+      // rows of each half over the grouped columns with support there
+      const int m1 = T.m1, k1 = T.k1, k3 = T.k3;
+      const int64_t q = T.wo + 9 * (int64_t)m, z0 = aoff_[b] + (int64_t)lo * n_[b];
+      batch_.add(gemm_task(m1, nc, k1 + k3, 0, 0, q, m, uacc, k, z0, n_[b], 1.0, 0.0));
+      batch_.add(gemm_task(m - m1, nc, k - k1, 0, 0, q + m1 + (int64_t)k1 * m, m, uacc + k1, k, z0 + m1, n_[b],
                            1.0, 0.0));
+    }
     uacc += (int64_t)k * nc;
     ncmax = std::max(ncmax, nc);
     kmax = std::max(kmax, k);

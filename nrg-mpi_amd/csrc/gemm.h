@@ -60,9 +60,9 @@ struct GemmRunner {
   void upload_tables(const GemmBatch& b, hipStream_t s, const GemmTask*& tk, const int32_t*& ts, int slot = 0) {
     const size_t nt = b.tasks.size();
     const size_t tb = (nt * sizeof(GemmTask) + 255) / 256 * 256;
-    h_pack.resize(tb + nt * sizeof(int32_t));
+    h_pack.resize(tb + h_tile_start.size() * sizeof(int32_t));
     std::memcpy(h_pack.data(), b.tasks.data(), nt * sizeof(GemmTask));
-    std::memcpy(h_pack.data() + tb, h_tile_start.data(), nt * sizeof(int32_t));
+    std::memcpy(h_pack.data() + tb, h_tile_start.data(), h_tile_start.size() * sizeof(int32_t));
     d_pack[slot].upload(h_pack.data(), h_pack.size(), s);
     tk = reinterpret_cast<const GemmTask*>(d_pack[slot].p);
     ts = reinterpret_cast<const int32_t*>(d_pack[slot].p + tb);

@@ -3,9 +3,10 @@
 // One workgroup (256 lanes = 4 wave64) computes a 64x64 tile of C; each wave
 // owns a 32x32 quadrant held as 2x2 MFMA 16x16 accumulators.  The K loop
 // stages 16-deep slices of opA (64x16) and opB (16x64) through LDS.  Tiles of
-// all tasks are flattened into one grid; a workgroup finds its task by binary
-// search over the tile prefix sums, so blocks of very different sizes share
-// one launch.
+// all tasks are flattened into one grid, so blocks of very different sizes
+// share one launch; the tile table holds the first tile of every task
+// followed by the task of every tile, so a workgroup finds its task with one
+// load whatever the number of tasks.
 #include "gemm.h"
 
 namespace nrg {
@@ -23,12 +24,7 @@ __global__ __launch_bounds__(256) void k_gemm_f64(
   __shared__ double As[TK * LDS_STRIDE];
   __shared__ double Bs[TK * LDS_STRIDE];
   const int bid = blockIdx.x;
-  // binary search: last task with tile_start <= bid
-  int lo = 0, hi = ntasks - 1;
-  while (lo < hi) {
-    int mid = (lo + hi + 1) >> 1;
-    if (tile_start[mid] <= bid) lo = mid; else hi = mid - 1;
-  }
+  const int lo = tile_start[ntasks + bid];   // task of this tile
   const GemmTask t = tasks[lo];
   const int tiles_m = (t.m + TM - 1) / TM;
   const int local = bid - tile_start[lo];
@@ -140,26 +136,6 @@ __global__ __launch_bounds__(256) void k_gemm_f64(
 // lane-contiguous index of those loads, so each lane needs one gather index
 // (and one scale) per slice, fetched one slice ahead of the operands.
 constexpr int TK2 = 32;
-constexpr int MAXT_LDS = 4096;   // tasks whose tile starts are searched in LDS
-
-__device__ inline int find_task(const int32_t* __restrict__ tile_start, int ntasks, int bid, int32_t* s_ts) {
-  if (ntasks <= MAXT_LDS) {
-    for (int i = threadIdx.x; i < ntasks; i += 256) s_ts[i] = tile_start[i];
-    __syncthreads();
-    int lo = 0, hi = ntasks - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (s_ts[mid] <= bid) lo = mid; else hi = mid - 1;
-    }
-    return lo;
-  }
-  int lo = 0, hi = ntasks - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (tile_start[mid] <= bid) lo = mid; else hi = mid - 1;
-  }
-  return lo;
-}
 
 template <int TA, int TB, int G>
 __global__ __launch_bounds__(256, 3) void k_gemm_f64_fast(
@@ -174,9 +150,8 @@ __global__ __launch_bounds__(256, 3) void k_gemm_f64_fast(
   __shared__ double ABs[2 * TK2 * LDS_STRIDE];
   double* As = ABs;
   double* Bs = ABs + TK2 * LDS_STRIDE;
-  __shared__ int32_t s_ts[MAXT_LDS];
   const int bid = blockIdx.x;
-  const int ti = find_task(tile_start, ntasks, bid, s_ts);
+  const int ti = tile_start[ntasks + bid];   // task of this tile
   const GemmTask t = tasks[ti];
   const int tiles_m = (t.m + TM - 1) / TM;
   const int local = bid - tile_start[ti];
@@ -513,6 +488,12 @@ void GemmRunner::launch(const GemmBatch& b, const double* A, const double* B, do
     total += t.lower ? tm * (tm + 1) / 2 : tm * ceil_div(t.n, TN);
   }
   NRG_REQUIRE(total < (1LL << 31), "gemm: too many tiles");
+  // after the tile starts, the task of every tile (a workgroup's lookup)
+  h_tile_start.resize((size_t)nt + (size_t)total);
+  for (int i = 0; i < nt; ++i) {
+    const int64_t e = (i + 1 < nt) ? h_tile_start[i + 1] : total;
+    std::fill(h_tile_start.begin() + nt + h_tile_start[i], h_tile_start.begin() + nt + e, (int32_t)i);
+  }
   bool uniform = true;
   int gbits = 0;
   for (int i = 0; i < nt; ++i) {
@@ -529,7 +510,7 @@ void GemmRunner::launch(const GemmBatch& b, const double* A, const double* B, do
     r.kind = 0; r.fast = fk ? 1 : 0; r.tv = tv; r.g = gbits; r.family = family;
     r.grid = (unsigned)total; r.nt = nt;
     r.tab = r_append(b.tasks.data(), (size_t)nt * sizeof(GemmTask));
-    r.tst = r_append(h_tile_start.data(), (size_t)nt * sizeof(int32_t));
+    r.tst = r_append(h_tile_start.data(), h_tile_start.size() * sizeof(int32_t));
     r.bytes = bytes; r.flops = flops;
     r.A = A; r.B = B; r.C = (slot == 1) ? nullptr : C;
     r.gA = gatherA; r.gB = gatherB; r.ks = kscale;
@@ -540,7 +521,7 @@ void GemmRunner::launch(const GemmBatch& b, const double* A, const double* B, do
   const int32_t* ts;
   if (ws_) {
     // tables in the caller's workspace (run_into)
-    const size_t tb = (size_t)nt * sizeof(GemmTask), sb = (size_t)nt * sizeof(int32_t);
+    const size_t tb = (size_t)nt * sizeof(GemmTask), sb = h_tile_start.size() * sizeof(int32_t);
     tk = reinterpret_cast<const GemmTask*>(ws_stage(b.tasks.data(), tb));
     ts = reinterpret_cast<const int32_t*>(ws_stage(h_tile_start.data(), sb));
   } else {
@@ -572,8 +553,16 @@ size_t GemmRunner::workspace_bytes(const GemmBatch& b) {
   const int64_t wsz = decompose(b);
   auto al = [](size_t n) { return (n + 255) / 256 * 256; };
   size_t n = 0;
-  for (const GemmBatch* x : {&plain_, &part_})
-    if (!x->tasks.empty()) n += al(x->tasks.size() * sizeof(GemmTask)) + al(x->tasks.size() * sizeof(int32_t));
+  for (const GemmBatch* x : {&plain_, &part_}) {
+    if (x->tasks.empty()) continue;
+    // task table, tile starts and the task of every tile
+    size_t tiles = 0;
+    for (const GemmTask& t : x->tasks) {
+      const size_t tm = ceil_div(t.m, TM);
+      tiles += t.lower ? tm * (tm + 1) / 2 : tm * ceil_div(t.n, TN);
+    }
+    n += al(x->tasks.size() * sizeof(GemmTask)) + al((x->tasks.size() + tiles) * sizeof(int32_t));
+  }
   if (!part_.tasks.empty()) n += al(h_red_.size() * sizeof(SplitRed)) + al((size_t)wsz * sizeof(double));
   return n;
 }
