@@ -1129,11 +1129,13 @@ __global__ __launch_bounds__(256) void k_dc_merge_prep(DcArgs a, const int32_t* 
 // Apply the deflation rotations to the merged columns, one lane per row.
 // The scan makes chains: a rotation (pj, j) finishes column pj and hands j on
 // as the next pj, so the running column stays in a register and is stored
-// when its chain ends.  The second columns of the next 8 rotations are
-// fetched together: each is a fresh scan position that no earlier rotation
-// has written.  Grid: (row chunks of 256, merges).
+// when its chain ends.  Both columns of the next 16 rotations are fetched
+// together (degenerate clusters make many short chains, whose first
+// columns would otherwise each cost a load round trip).  Grid: (row chunks
+// of 256, merges).
 __global__ __launch_bounds__(256) void k_dc_rotate(DcArgs a, const int32_t* tasks, int ntask) {
   constexpr int CH = 1024;   // rotations staged in LDS per round
+  constexpr int RQ = 16;     // rotations whose columns are loaded together
   __shared__ int32_t s_c[2 * CH];
   __shared__ double s_r[2 * CH];
   const int t = blockIdx.y;
@@ -1163,14 +1165,22 @@ __global__ __launch_bounds__(256) void k_dc_rotate(DcArgs a, const int32_t* task
     }
     __syncthreads();
     if (row >= m) continue;
-    for (int q0 = 0; q0 < nch; q0 += 8) {
-      const int nq = min(8, nch - q0);
-      double y[8];
+    for (int q0 = 0; q0 < nch; q0 += RQ) {
+      const int nq = min(RQ, nch - q0);
+      // both columns of the next RQ rotations in one round of loads: a
+      // rotation's first column is the running column of its chain (then
+      // the register value replaces the loaded one) or a column that no
+      // rotation has written yet, and its second column is a fresh scan
+      // position, so the loads read current values
+      double y[RQ], xp[RQ];
 #pragma unroll
-      for (int u = 0; u < 8; ++u)
-        if (u < nq) y[u] = Zr[(int64_t)s_c[2 * (q0 + u) + 1] * n];
+      for (int u = 0; u < RQ; ++u)
+        if (u < nq) {
+          y[u] = Zr[(int64_t)s_c[2 * (q0 + u) + 1] * n];
+          xp[u] = Zr[(int64_t)s_c[2 * (q0 + u)] * n];
+        }
 #pragma unroll
-      for (int u = 0; u < 8; ++u)
+      for (int u = 0; u < RQ; ++u)
         if (u < nq) {
           const int q = q0 + u;
           const int c0 = s_c[2 * q];
@@ -1179,7 +1189,7 @@ __global__ __launch_bounds__(256) void k_dc_rotate(DcArgs a, const int32_t* task
             x = cvl;
           } else {
             if (cc >= 0) Zr[(int64_t)cc * n] = cvl;
-            x = Zr[(int64_t)c0 * n];
+            x = xp[u];
           }
           const double r = s_r[2 * q], sn = s_r[2 * q + 1];
           Zr[(int64_t)c0 * n] = r * x + sn * y[u];
