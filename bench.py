@@ -46,10 +46,13 @@ def parse():
     ap.add_argument("--sort", choices=["exact", "canonical"], default="exact")
     ap.add_argument("--no-profile", action="store_true", help="skip the HIP-event kernel timing")
     ap.add_argument("--profile-every", type=int, default=307,
-                    help="time every n-th launch of the profiled families (307: prime, so coprime to the "
-                         "32-column panel; a timed tridiagonal SYMV launch runs with the device to itself, "
-                         "so sampling more often also slows the multi-stream schedule: every 61st costs ~3%% "
-                         "of the iteration rate, every 307th ~0.5%%)")
+                    help="time every n-th launch of the profiled families in the timed steps (307: prime, so "
+                         "coprime to the 32-column panel); the launches are timed in place, sharing the device "
+                         "with the other stream groups")
+    ap.add_argument("--isolated-every", type=int, default=61,
+                    help="sampling stride of the isolated-launch leg after the timed steps (two iterations whose "
+                         "timed tridiagonal SYMV launches are fenced against the other stream groups, so that "
+                         "their event interval is the kernel alone; 0 skips the leg)")
     ap.add_argument("--profile-families", default="tridiag_symv",
                     help="comma-separated kernel families whose launches carry HIP timing events in the "
                          "timed steps (default: the dominant one, tridiag_symv; 'all' for every family)")
@@ -327,7 +330,7 @@ def main():
         return
     need = a.warmup + a.steps
     legs = 3 * (int(a.plain) + int(a.split_leg > 0 or a.split > 0) + 1 + int(a.graph_leg) +
-                int(a.two_stage_leg and a.tridiag == "one-stage"))
+                int(a.two_stage_leg and a.tridiag == "one-stage")) + (2 if a.isolated_every > 0 else 0)
     if need + legs > spec.niter:
         raise SystemExit(f"warmup+steps+legs={need + legs} exceeds the {spec.niter} iterations of {a.workload}")
     tmp = tempfile.mkdtemp(prefix="nrgbench_")
@@ -349,7 +352,7 @@ def main():
         log(f"warmup iteration {it}: nfull={h.i['nfull']} nsub={h.i['nsub']} kept={h.i['nkept']}")
     if not a.no_profile:
         fams = None if a.profile_families == "all" else [f for f in a.profile_families.split(",") if f]
-        gpu.profile(True, families=fams, every=a.profile_every)
+        gpu.profile(True, families=fams, every=a.profile_every, exclusive=False)
     stages = {}
     n_warm_log = len(gpu.launch_log_get()) if a.launch_log else 0
     gpu.work_counters(reset=True)
@@ -439,6 +442,30 @@ def main():
     # tuned one, and with HIP graphs
     untuned = leg(0, 0, "batched FP64, no graphs, default launch configuration")
     untuned["form"] = "batched FP64 eigensolve and contractions, launches issued one by one, default configuration"
+    # the dominant kernel with the device to itself: two more iterations in
+    # the timed form whose sampled SYMV launches are fenced against the
+    # other stream groups (not part of the timed value)
+    isolated = None
+    if a.isolated_every > 0 and not a.no_profile:
+        gpu.load().nrg_gpu_set_tuning(2 if tuned else 0, a.tuning_cache.encode() if tuned else None)
+        gpu.profile(True, families=["tridiag_symv"], every=a.isolated_every, exclusive=True)
+        itl = []
+        for _ in range(2):
+            h.iterate(nxt)
+            itl.append(nxt)
+            nxt += 1
+        ki = gpu.kernel_stats().get("tridiag_symv", {})
+        gpu.profile(False)
+        if ki.get("launches") and ki.get("ms", 0) > 0:
+            ach = ki["bytes"] / (ki["ms"] * 1e-3) / 1e9
+            isolated = {"achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+                        "launches_timed": ki["launches"], "avg_launch_us": 1e3 * ki["ms"] / ki["launches"],
+                        "alg_bytes_per_launch": ki["bytes"] / ki["launches"], "iterations": itl,
+                        "timing": f"HIP events on every {a.isolated_every}-th tridiagonal SYMV launch of "
+                                  f"iterations {itl}, each fenced against the other stream groups (the kernel "
+                                  f"with the device to itself)"}
+            log(f"isolated SYMV launches (iterations {itl}): {isolated['avg_launch_us']:.2f} us, "
+                f"{ach / 1e3:.2f} TB/s")
     with_graphs = None
     if a.graph_leg:
         with_graphs = leg(0, 0, "batched FP64, HIP graphs, tuned launch configuration", graphs=True,
@@ -540,8 +567,9 @@ def main():
         roof["kernel_family"] = fam
         roof["launches_timed"] = k["launches"]
         roof["timing"] = (f"HIP events on every {a.profile_every}-th launch of the family over the timed steps, "
-                          f"each such tridiagonal SYMV launch with the device to itself (achieved = algorithmic "
-                          f"bytes or flops / event time of those launches)")
+                          f"each timed in place (sharing the device with the other stream groups' kernels, as "
+                          f"every launch of the family does; achieved = algorithmic bytes or flops / event time "
+                          f"of those launches)")
         roof["avg_launch_us"] = 1e3 * k["ms"] / max(k["launches"], 1)
         roof["alg_bytes_per_launch"] = k["bytes"] / max(k["launches"], 1)
         roof["alg_flops_per_launch"] = k["flops"] / max(k["launches"], 1)
@@ -551,6 +579,24 @@ def main():
             roof["traffic_sample"] = {"alg_bytes_per_launch": tr.get("alg_bytes_per_launch"),
                                       "traffic_over_algorithmic": tr.get("traffic_over_algorithmic"),
                                       "source": tr["source"]}
+        if fam == "tridiag_symv":
+            # the same launches with the device to themselves (the isolated
+            # leg), and the whole family's algorithmic bytes over the time of
+            # the tridiagonalisations in the timed steps (the stream groups'
+            # launches overlap, so one launch's event interval under-states
+            # the bandwidth the family draws)
+            if isolated:
+                roof["isolated"] = isolated
+            tt = stages.get("gpu_tridiag", 0.0)
+            if tt > 0:
+                agg = work["tridiag_symv"]["bytes"] / tt / 1e9
+                roof["family_over_tridiag"] = {
+                    "achieved": agg, "frac": agg / HBM_PEAK_GBS, "unit": "GB/s",
+                    "bytes_per_step": work["tridiag_symv"]["bytes"] / a.steps,
+                    "tridiag_ms_per_step": 1e3 * tt / a.steps,
+                    "timing": "algorithmic bytes of every tridiagonal SYMV launch in the timed steps / the "
+                              "event-timed span of the tridiagonalisations (which also runs the column kernels "
+                              "and trailing updates)"}
         out["roofline"] = roof
         # sampled launches scaled to all launches of the family: kernel time
         # summed over launches (launches of different streams overlap in the

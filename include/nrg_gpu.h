@@ -282,6 +282,14 @@ int nrg_gpu_kernel_stats(double* out, int nfam);
 /* Time only every `every`-th launch of each timed family (default 1: all);
  * the statistics then cover the sampled launches. */
 int nrg_gpu_profile_sample(int every);
+/* Measurement option with no counterpart in the reference: on != 0 (the
+ * default) fences a timed SYMV launch of the multi-stream tridiagonalisation
+ * against the other stream groups, so that its event interval is its
+ * execution with the device to itself (each fence waits for the work the
+ * host has already queued on the other streams, which slows the schedule);
+ * 0 times it in place, sharing the device with the other groups' kernels as
+ * an untimed launch does. */
+int nrg_gpu_profile_exclusive(int on);
 /* Launch log for matching profiler counters to algorithmic work: with
  * fam_mask != 0 every launch of a family in the mask is recorded (the call
  * clears the log; 0 stops).  nrg_gpu_launch_log_get writes up to cap records
@@ -316,9 +324,10 @@ int nrg_gpu_launch_log_kernel(int kid, char* name, int cap);
  * eigenvector columns through both stages. */
 int nrg_gpu_set_tridiag_method(int two_stage);
 /* Launch option with no counterpart in the reference: the number of stream
- * groups (1..4) the blocks of the tridiagonalisation are split over, in
+ * groups (1..4) the blocks of the tridiagonalisation are split over (each
+ * block, in size order, to the group with the least sum n^3 so far), in
  * place of the tuner's choice; 0 returns the choice to the tuner (the
- * default, 2 groups, with tuning off). */
+ * default, 4 groups, with tuning off). */
 int nrg_gpu_set_tridiag_groups(int groups);
 /* Kernel option with no counterpart in the reference: the SYMV of the
  * one-stage tridiagonalisation adds the row parts of its columns with LDS
@@ -334,7 +343,7 @@ int nrg_gpu_set_batch_limit(int max_blocks);
 
 /* Launch option with no counterpart in the reference: on != 0 captures the
  * launch sequence of each batched tridiagonalisation (the column kernels,
- * SYMVs and trailing updates of all blocks on both stream groups) as a HIP
+ * SYMVs and trailing updates of all blocks on all stream groups) as a HIP
  * graph, keyed on the block set (sizes, offsets, schedule, buffers); a
  * later iteration with the same block set -- the NRG steady state repeats
  * its block tables -- replays the graph instead of issuing the launches
@@ -347,7 +356,7 @@ int nrg_gpu_set_graphs(int on);
  * configuration of the batched tridiagonalisation -- the number of stream
  * groups sharing the device and the launch size from which the SYMV uses
  * 32-column panels -- chosen per block-size histogram (block count, largest
- * block, sum n^3).  mode 0: defaults (2 groups, 512 workgroups); 1: time the
+ * block, sum n^3).  mode 0: defaults (4 groups, 512 workgroups); 1: time the
  * candidate configurations on successive iterations with the same
  * histogram (three times each), then keep the one with the lowest mean
  * time, updating the cache file; 2: use the cache file's choice only.  cache_path: a text file of measurements (NULL

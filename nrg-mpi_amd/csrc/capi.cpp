@@ -783,6 +783,10 @@ int nrg_gpu_profile_sample(int every) {
   });
 }
 
+int nrg_gpu_profile_exclusive(int on) {
+  return guarded([&](Context&) { profiler().exclusive = (on != 0); });
+}
+
 int nrg_gpu_work_counters(double* out, int nfam, int reset) {
   return guarded([&](Context&) {
     Profiler& P = profiler();

@@ -1911,9 +1911,23 @@ void BatchedEigensolver::tridiag(hipStream_t s) {
   if (ng > 1 && !ev_fork_) NRG_HIP(hipEventCreateWithFlags(&ev_fork_, hipEventDisableTiming));
   for (int g = 0; g < ng && ng > 1; ++g)
     if (!evf_[g]) NRG_HIP(hipEventCreateWithFlags(&evf_[g], hipEventDisableTiming));
+  {
+    // Blocks in size order go to the group with the least n^3 so far (the
+    // SYMV bytes of a block are ~(4/3) n^3): the groups run independently
+    // and share the bandwidth, so the heaviest sets the end.  Each group's
+    // slots stay in size order (start steps non-decreasing).
+    std::vector<double> load(ng, 0.0);
+    for (int q = 0; q < nblk; ++q) {
+      int g = 0;
+      for (int h = 1; h < ng; ++h)
+        if (load[h] < load[g]) g = h;
+      const double nq = n_[order_[q]];
+      load[g] += nq * nq * nq;
+      grp[g].q.push_back(q);
+    }
+  }
   for (int g = 0; g < ng; ++g) {
     Group& G = grp[g];
-    for (int q = g; q < nblk; q += ng) G.q.push_back(q);
     std::vector<TrSlot> slots(G.q.size());
     for (size_t i = 0; i < G.q.size(); ++i) {
       const int q = G.q[i], b = order_[q];
@@ -2099,7 +2113,7 @@ void BatchedEigensolver::tridiag(hipStream_t s) {
       // event interval is its execution and not also time spent waiting for
       // workgroup slots held by another stream's kernel.
       const int gi = (int)(&G - grp);
-      const bool excl = (ng > 1) && profiler().peek(KF_TR_SYMV);
+      const bool excl = (ng > 1) && profiler().exclusive && profiler().peek(KF_TR_SYMV);
       if (excl)
         for (int h = 0; h < ng; ++h)
           if (h != gi) {

@@ -49,6 +49,10 @@ struct Profiler {
   bool on = false;
   uint32_t mask = 0xffffffffu;   // families timed while on
   uint32_t every = 1;            // time every `every`-th launch of a family
+  // a timed SYMV launch of the multi-stream tridiagonalisation runs with
+  // the device to itself (fenced against the other stream groups) when set;
+  // otherwise it is timed in place, sharing the device as it does untimed
+  bool exclusive = true;
   std::array<uint64_t, KF_COUNT> seen{};
   // launch log: one record per launch of the logged families, with the
   // launch's ordinal among launches of the same kernel function (the

@@ -9,7 +9,7 @@
 namespace nrg {
 
 std::vector<TuneConfig> Tuner::candidates() {
-  return {TuneConfig{2, 512}, TuneConfig{1, 512}, TuneConfig{3, 512}, TuneConfig{2, 256}, TuneConfig{2, 1024}};
+  return {TuneConfig{4, 512}, TuneConfig{1, 512}, TuneConfig{2, 512}, TuneConfig{3, 512}, TuneConfig{4, 256}};
 }
 
 void Tuner::set(int mode, const std::string& path) {

@@ -45,6 +45,7 @@ SIGNATURES = {
     "nrg_gpu_profile": (ctypes.c_int, [ctypes.c_int]),
     "nrg_gpu_kernel_stats": (ctypes.c_int, [_f64p, ctypes.c_int]),
     "nrg_gpu_profile_sample": (ctypes.c_int, [ctypes.c_int]),
+    "nrg_gpu_profile_exclusive": (ctypes.c_int, [ctypes.c_int]),
     "nrg_gpu_launch_log": (ctypes.c_int, [ctypes.c_int]),
     "nrg_gpu_work_counters": (ctypes.c_int, [_f64p, ctypes.c_int, ctypes.c_int]),
     "nrg_gpu_set_tridiag_method": (ctypes.c_int, [ctypes.c_int]),
@@ -215,11 +216,14 @@ def rho_get(iteration: int, kept_per_block):
     return out
 
 
-def profile(enable=True, families=None, every=1):
+def profile(enable=True, families=None, every=1, exclusive=True):
     """Start (clearing) or stop the per-family kernel timing; `families`
     restricts it to the named families (see KERNEL_FAMILIES); `every` times
-    only every n-th launch of a family."""
+    only every n-th launch of a family; `exclusive` fences a timed SYMV
+    launch of the tridiagonalisation against the other stream groups
+    (False: timed in place)."""
     _check(load().nrg_gpu_profile_sample(int(every)))
+    _check(load().nrg_gpu_profile_exclusive(1 if exclusive else 0))
     if not enable:
         code = 0
     elif families is None:

@@ -29,7 +29,7 @@ int main(int argc, char** argv) {
   std::printf("key %s\n", k.c_str());
   Tuner t;
   t.set(1, path);
-  // synthetic times: groups 3 is fastest on average; (2, 256) gets one
+  // synthetic times: groups 3 is fastest on average; (4, 256) gets one
   // lucky sample below everyone's mean but is slower on average
   auto ms = [](const TuneConfig& c, int i) {
     if (c.groups == 3) return 100.0 + i;          // mean 101
@@ -75,14 +75,14 @@ def test_tuner_explores_then_keeps_the_lowest_mean(tmp_path):
     # candidates in order (the default first), each three times
     explore = [tuple(int(x) for x in l.split()[1:]) for l in out if l.startswith("explore")]
     assert [e[1:] for e in explore] == [(c, s) for c in range(5) for s in range(3)]
-    # lowest mean (groups 3), not the lowest single sample (2, 256)
+    # lowest mean (groups 3), not the lowest single sample (4, 256)
     assert "chosen 3 512" in out
     assert "exploring 0" in out
     # the cache file carries means and sample counts; mode 2 reads it back
     lines = [l.split() for l in cache.read_text().splitlines() if l and not l.startswith("#")]
     by = {(int(l[1]), int(l[2])): (float(l[3]), int(l[4])) for l in lines}
     assert by[(3, 512)] == (101.0, 3)
-    assert abs(by[(2, 256)][0] - 350.0 / 3.0) < 1e-6
+    assert abs(by[(4, 256)][0] - 350.0 / 3.0) < 1e-6
     assert "cached 3 512" in out
     # a block set the cache has not seen gets the default configuration
-    assert "absent 2 512" in out
+    assert "absent 4 512" in out
