@@ -87,13 +87,15 @@ def _ordinals(rows, pat):
     return out
 
 
-def family_window(d, log, family, events_only=False):
-    """Mean duration (us) of the family's dispatches in the timed steps
-    (events_only: only those that carried the bench's timing events)."""
+def family_window(d, log, family, events_only=False, phase="timed"):
+    """Mean duration (us) of the family's dispatches in one phase of the
+    bench's launch log ("timed": the timed steps, "isolated": the fenced
+    leg after them; events_only: only those that carried the bench's
+    timing events)."""
     rows = _rows(d, "kernel_trace.csv")
     want = {}
     for r in read_log(log):
-        if r["family"] == family and r["phase"] == "timed":
+        if r["family"] == family and r["phase"] == phase:
             if events_only and r.get("events", "1") != "1":
                 continue
             want.setdefault(kernel_key(r["kernel"]), set()).add(int(r["ordinal"]))
@@ -122,12 +124,16 @@ def stats(d, title, command, top, log=None, family=None):
         lines.append(f"| `{short}` | {calls} | {tot / 1e3:.0f} | {avg / 1e3:.2f} | {pct:.1f} |")
     if log and family:
         us_all, n_all = family_window(d, log, family)
-        us, n = family_window(d, log, family, events_only=True)
-        lines += ["", f"Family `{family}` in the timed steps (dispatches located through the launch log of the "
-                      f"same run): {n_all} dispatches, mean duration {us_all:.2f} us; the {n} of them that "
-                      f"carried the bench's timing events (run with the device to themselves): mean duration "
+        us_in, n_in = family_window(d, log, family, events_only=True)
+        us, n = family_window(d, log, family, events_only=True, phase="isolated")
+        lines += ["", f"Family `{family}` (dispatches located through the launch log of the same run): in the "
+                      f"timed steps {n_all} dispatches, mean duration {us_all:.2f} us (the {n_in} of them that "
+                      f"carried the bench's in-place timing events: {us_in:.2f} us); in the isolated leg after "
+                      f"the timed steps, the {n} dispatches that carried the bench's events (each fenced against "
+                      f"the other stream groups, the kernel with the device to itself): mean duration "
                       f"{us:.2f} us.", "",
-                  f"timed_family_avg_us = {us:.3f}", f"timed_family_launches = {n}",
+                  f"isolated_family_avg_us = {us:.3f}", f"isolated_family_launches = {n}",
+                  f"timed_family_events_avg_us = {us_in:.3f}", f"timed_family_events_launches = {n_in}",
                   f"timed_family_all_avg_us = {us_all:.3f}", f"timed_family_all_launches = {n_all}"]
     return "\n".join(lines) + "\n"
 
