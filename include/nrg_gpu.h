@@ -329,6 +329,12 @@ int nrg_gpu_set_tridiag_method(int two_stage);
  * place of the tuner's choice; 0 returns the choice to the tuner (the
  * default, 4 groups, with tuning off). */
 int nrg_gpu_set_tridiag_groups(int groups);
+/* Kernel option with no counterpart in the reference: the one-stage
+ * tridiagonalisation stops its column steps at the panel boundary from which
+ * every block has at most `rows` rows left (0..192, default 192) and reduces
+ * those last columns of all blocks in one launch, one workgroup per block,
+ * with the trailing block in LDS; 0 runs the column steps to the end. */
+int nrg_gpu_set_tridiag_finish(int rows);
 /* Kernel option with no counterpart in the reference: the SYMV of the
  * one-stage tridiagonalisation adds the row parts of its columns with LDS
  * atomics (kind 0, the default) or sums them in registers and 256-row LDS

@@ -748,6 +748,14 @@ int nrg_gpu_set_tridiag_groups(int groups) {
   });
 }
 
+int nrg_gpu_set_tridiag_finish(int rows) {
+  return guarded([&](Context& C) {
+    need_init(C);
+    NRG_REQUIRE(rows >= 0 && rows <= 192, "set_tridiag_finish: rows must be 0..192");
+    C.eig.set_finish(rows);
+  });
+}
+
 int nrg_gpu_set_tridiag_method(int two_stage) {
   return guarded([&](Context& C) {
     need_init(C);

@@ -69,6 +69,9 @@ class BatchedEigensolver {
   // SYMV of the one-stage reduction: row parts added with LDS atomics (0)
   // or in register/LDS row chunks in a fixed order (1)
   void set_symv_kind(int kind) { symv_kind_ = kind; }
+  // trailing size from which the last columns of every block are reduced in
+  // LDS by one launch (0: column steps to the end; at most 192)
+  void set_finish(int m) { finish_m_ = std::max(0, std::min(m, 192)); }
   bool two_stage() const { return two_stage_; }
   long long graph_captures() const { return graph_captures_; }
   long long graph_replays() const { return graph_replays_; }
@@ -151,6 +154,8 @@ class BatchedEigensolver {
   // two-stage reduction: band, stage-2 reflectors and their offsets
   bool two_stage_ = false;
   int symv_kind_ = 0;
+  int finish_m_ = 192;
+  DevBuf<char> d_fin_;   // tail finisher tasks
   bool solved_two_stage_ = false;   // how the current solve reduced (its back-transform)
   int bt_off_ = 1;                  // row offset of a stage-1 reflector's unit element
   DevBuf<double> d_tq_, d_band_, d_v2_, d_t2_;

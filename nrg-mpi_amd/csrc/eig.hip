@@ -594,6 +594,124 @@ __global__ __launch_bounds__(NW * 64, 8) void k_tr_symv(TrArgs a, int t, int jj,
   }
 }
 
+// Tail finisher: the last columns of every block (a trailing block of at
+// most FIN_MX rows, whose lower triangle fits one workgroup's LDS packed by
+// columns: 148 KB) reduced in one launch, one workgroup per block, by the
+// unblocked two-sided Householder reduction (dsytd2, lower): per column the
+// reflector from the column below the diagonal, p = tau A22 v (4 lanes per
+// row), w = p - (tau/2)(p.v) v, A22 -= v w^T + w v^T.  The reflectors,
+// d, e and tau are stored as the blocked steps store them.  Starts at a
+// panel boundary, after that panel's trailing update: the trailing block is
+// then the fully updated matrix.  Replaces the last ~FIN_MX column steps,
+// which are latency-bound launches (two kernels per step and group).
+constexpr int FIN_MX = 192;
+struct FinTask {
+  int32_t n, c0;      // block size, first column left to reduce
+  int64_t aoff, voff;
+};
+
+__global__ __launch_bounds__(1024) void k_tr_finish(const FinTask* __restrict__ tasks, double* A, double* d,
+                                                    double* e, double* tau) {
+  __shared__ double L[FIN_MX * (FIN_MX + 1) / 2];
+  __shared__ double sv[FIN_MX], sw[FIN_MX];
+  __shared__ double s_par[4];
+  const FinTask tk = tasks[blockIdx.x];
+  const int n = tk.n, c0 = tk.c0, m = n - c0;
+  if (m < 1) return;
+  double* Ab = A + tk.aoff;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  // column j of the packed lower triangle starts at j m - j (j - 1) / 2
+  auto pk = [&](int i, int j) { return j * m - ((j * (j - 1)) >> 1) + (i - j); };
+  // the trailing lower triangle into LDS: wave wv takes columns wv + 16 q,
+  // lanes the rows; four columns' rows are in flight together
+  for (int j0 = wv; j0 < m; j0 += 64) {
+    double x[4][3];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int r = 0; r < 3; ++r) {
+        const int j = j0 + 16 * q, i = j + lane + 64 * r;
+        x[q][r] = (j < m && i < m) ? Ab[(c0 + i) + (int64_t)(c0 + j) * n] : 0.0;
+      }
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int r = 0; r < 3; ++r) {
+        const int j = j0 + 16 * q, i = j + lane + 64 * r;
+        if (j < m && i < m) L[pk(i, j)] = x[q][r];
+      }
+  }
+  __syncthreads();
+  for (int k = 0; k < m; ++k) {
+    const int c = c0 + k;
+    const double* colk = &L[pk(k, k)];   // colk[r] = A(k + r, k)
+    if (tid == 0) d[tk.voff + c] = colk[0];
+    if (k == m - 1) break;
+    const int len = m - k - 1;           // rows k+1 .. m-1
+    if (wv == 0) {
+      double s2 = 0.0;
+      for (int r = 2 + lane; r <= len; r += 64) {
+        const double x = colk[r];
+        s2 += x * x;
+      }
+      s2 = wave_sum_full(s2);
+      if (lane == 0) {
+        const double alpha = colk[1];
+        double beta, tq, scal;
+        if (s2 == 0.0) {
+          beta = alpha; tq = 0.0; scal = 0.0;
+        } else {
+          beta = -copysign(sqrt(alpha * alpha + s2), alpha);
+          tq = (beta - alpha) / beta;
+          scal = 1.0 / (alpha - beta);
+        }
+        s_par[0] = beta; s_par[1] = tq; s_par[2] = scal;
+        e[tk.voff + c] = beta;
+        tau[tk.voff + c] = tq;
+      }
+    }
+    __syncthreads();
+    const double tq = s_par[1], scal = s_par[2];
+    // v (v_{k+1} = 1) into LDS and below the subdiagonal of column c
+    for (int r = tid; r < len; r += 1024) {
+      const int i = k + 1 + r;
+      const double vi = (r == 0) ? 1.0 : colk[1 + r] * scal;
+      sv[i] = vi;
+      if (r > 0) Ab[(c0 + i) + (int64_t)c * n] = vi;
+    }
+    __syncthreads();
+    if (tq == 0.0) continue;   // H = I (uniform over the workgroup)
+    // p = tau A22 v: row k+1+tid/4, the 4 lanes of a row take every 4th column
+    {
+      const int row = k + 1 + (tid >> 2), q = tid & 3;
+      double acc = 0.0;
+      if (row < m)
+        for (int j = k + 1 + q; j < m; j += 4) acc += ((row >= j) ? L[pk(row, j)] : L[pk(j, row)]) * sv[j];
+      acc += __shfl_xor(acc, 1, 64);
+      acc += __shfl_xor(acc, 2, 64);
+      if (row < m && q == 0) sw[row] = tq * acc;
+    }
+    __syncthreads();
+    if (wv == 0) {
+      double sdot = 0.0;
+      for (int i = k + 1 + lane; i < m; i += 64) sdot += sw[i] * sv[i];
+      sdot = wave_sum_full(sdot);
+      if (lane == 0) s_par[3] = -0.5 * tq * sdot;
+    }
+    __syncthreads();
+    const double a2 = s_par[3];
+    for (int i = k + 1 + tid; i < m; i += 1024) sw[i] += a2 * sv[i];
+    __syncthreads();
+    // A22 -= v w^T + w v^T on the lower triangle: columns per wave, rows per lane
+    for (int j = k + 1 + wv; j < m; j += 16) {
+      const double vj = sv[j], wj = sw[j];
+      double* col = &L[pk(j, j)];
+      for (int i = j + lane; i < m; i += 64) col[i - j] -= sv[i] * wj + sw[i] * vj;
+    }
+    __syncthreads();
+  }
+}
+
 // Row-chunk SYMV: the same product as k_tr_symv without LDS atomics.  The
 // workgroup walks the rows of its panel in chunks of SR = 256; in a chunk
 // each wave streams its CPW columns (4 rows per lane and column, the next
@@ -1729,6 +1847,7 @@ void BatchedEigensolver::release() {
   d_n_.release(); d_order_.release();
   for (int g = 0; g < MAX_TR_GROUPS; ++g) {
     d_slotg_[g].release();
+    d_fin_.release();
     gemmg_[g].release();
     if (sg_[g]) {
       (void)hipStreamSynchronize(sg_[g]);
@@ -1884,9 +2003,13 @@ void BatchedEigensolver::tridiag(hipStream_t s) {
               "eigensolver: block too large for the SYMV stage (its row-part vector exceeds the LDS)");
   int T = 0;
   for (int q = 0; q < nblk; ++q) T = std::max(T, coff[q] + n_[order_[q]]);
+  // The column steps stop at the panel boundary t0 from which every block's
+  // remaining trailing block has at most finish_m_ rows; the tail finisher
+  // reduces those in LDS (finish_m_ = 0: column steps to the end).
+  const int t0 = (finish_m_ > 0) ? std::max(0, ceil_div(T - finish_m_, NB) * NB) : T;
 
-  // Groups of blocks (slot q in group q mod ng of the size order) follow the
-  // same schedule on their own streams.  Each step of a block is a
+  // Groups of blocks (balanced by sum n^3, below) follow the same schedule
+  // on their own streams.  Each step of a block is a
   // latency-bound column kernel followed by a bandwidth-bound SYMV; with the
   // groups on separate streams the column kernel of one group runs while the
   // SYMV of another streams its trailing matrices.
@@ -1968,12 +2091,26 @@ void BatchedEigensolver::tridiag(hipStream_t s) {
     Group& G = grp[g];
     G.gemm->begin();
     G.syr2k.assign(1, 0);
-    for (int t = NB; t < T; t += NB) {
+    for (int t = NB; t < T && t <= t0; t += NB) {
       syr2k_batch(G, t);
       G.gemm->run(*G.batch, d_P_.p, d_P_.p, A_, nullptr, nullptr, nullptr, G.s, KF_GEMM_TRIDIAG);
       G.syr2k.push_back(G.gemm->recorded());
     }
     G.gemm->upload_recorded(s);   // on the caller's stream, before the fork
+  }
+
+  // tail finisher tasks: every block with columns left at step t0 (blocks
+  // not started by then are reduced whole)
+  std::vector<FinTask> fin;
+  if (t0 < T) {
+    for (int q = 0; q < nblk; ++q) {
+      const int b = order_[q];
+      const int c0 = std::max(0, t0 - coff[q]);
+      if (c0 >= n_[b]) continue;
+      NRG_REQUIRE(n_[b] - c0 <= FIN_MX, "eigensolver: tail finisher block above its LDS size");
+      fin.push_back(FinTask{n_[b], c0, aoff_[b], voff_[b]});
+    }
+    d_fin_.upload(reinterpret_cast<const char*>(fin.data()), fin.size() * sizeof(FinTask), s);
   }
 
   // From here on the work is launches only (every table is on the device):
@@ -1990,11 +2127,13 @@ void BatchedEigensolver::tridiag(hipStream_t s) {
     key.push_back(ng);
     key.push_back(symv_wide_wg_);
     key.push_back(symv_kind_);
+    key.push_back(finish_m_);
     key.push_back((int64_t)symv_wide_bytes_);
     for (int b = 0; b < nblk; ++b) { key.push_back(n_[b]); key.push_back(aoff_[b]); key.push_back(voff_[b]); }
     for (const void* p : {(const void*)A_, (const void*)d_d_.p, (const void*)d_e_.p, (const void*)d_tau_.p,
                           (const void*)d_y_.p, (const void*)d_P_.p, (const void*)d_part_.p})
       key.push_back((int64_t)(intptr_t)p);
+    key.push_back((int64_t)(intptr_t)d_fin_.p);
     for (int g = 0; g < ng; ++g) {
       key.push_back((int64_t)(intptr_t)d_slotg_[g].p);
       key.push_back((int64_t)(intptr_t)gemmg_[g].r_dev_.p);
@@ -2023,7 +2162,9 @@ void BatchedEigensolver::tridiag(hipStream_t s) {
     for (int g = 1; g < ng; ++g) NRG_HIP(hipStreamWaitEvent(sg_[g], ev_fork_, 0));
   }
 
-  auto step = [&](Group& G, int t) {
+  // step t of group G; close_only: only close the previous panel (its last
+  // w, its trailing update), as at the finisher's start step
+  auto step = [&](Group& G, int t, bool close_only) {
     const int ns = (int)G.q.size();
     // slots of the group started by step t (coff is non-decreasing)
     int act;
@@ -2062,6 +2203,7 @@ void BatchedEigensolver::tridiag(hipStream_t s) {
               G.pw_prev, act, tmA);
       const int pi = t / NB - 1;
       G.gemm->issue(G.syr2k[pi], G.syr2k[pi + 1], G.s);
+      if (close_only) return;
       plaunch(KF_TR_COL, colbytes(t), 0, k_tr_col, dim3(gAl), dim3(1024), 0, G.s, ta, t, -1, 0, NB - 1,
               G.pw_prev, act, tmA);
     } else {
@@ -2145,12 +2287,20 @@ void BatchedEigensolver::tridiag(hipStream_t s) {
       G.pw_prev = pw;
     }
   };
-  for (int t = 0; t < T; ++t)
-    for (int g = 0; g < ng; ++g) step(grp[g], t);
+  for (int t = 0; t < t0; ++t)
+    for (int g = 0; g < ng; ++g) step(grp[g], t, false);
+  if (t0 > 0 && t0 < T)
+    for (int g = 0; g < ng; ++g) step(grp[g], t0, true);
   for (int g = 0; g < ng; ++g) grp[g].gemm->end_replay();
   for (int g = 1; g < ng; ++g) {
     NRG_HIP(hipEventRecord(evg_[g], sg_[g]));
     NRG_HIP(hipStreamWaitEvent(s, evg_[g], 0));
+  }
+  if (!fin.empty()) {
+    double fb = 0;
+    for (const FinTask& f : fin) fb += 8.0 * (double)(f.n - f.c0) * (f.n - f.c0 + 1);
+    plaunch(KF_TR_COL, fb, 0, k_tr_finish, dim3((unsigned)fin.size()), dim3(1024), 0, s,
+            reinterpret_cast<const FinTask*>(d_fin_.p), A_, d_d_.p, d_e_.p, d_tau_.p);
   }
   if (graph) {
     hipGraph_t gr = nullptr;

@@ -46,6 +46,7 @@ SIGNATURES = {
     "nrg_gpu_kernel_stats": (ctypes.c_int, [_f64p, ctypes.c_int]),
     "nrg_gpu_profile_sample": (ctypes.c_int, [ctypes.c_int]),
     "nrg_gpu_profile_exclusive": (ctypes.c_int, [ctypes.c_int]),
+    "nrg_gpu_set_tridiag_finish": (ctypes.c_int, [ctypes.c_int]),
     "nrg_gpu_launch_log": (ctypes.c_int, [ctypes.c_int]),
     "nrg_gpu_work_counters": (ctypes.c_int, [_f64p, ctypes.c_int, ctypes.c_int]),
     "nrg_gpu_set_tridiag_method": (ctypes.c_int, [ctypes.c_int]),

@@ -139,3 +139,31 @@ def test_syev_blocks_above_lds_limits(gpu, n, ladder):
     # eigenvalue error ~ n eps |H|, residual and orthogonality ~ n eps
     res, orth = _check_planted(H, ws[0], Us[0], lam, rng, tol=2e-16 * n * 8)
     print("residual", res, "orthogonality", orth)
+
+
+@pytest.mark.parametrize("finish", [192, 96, 0])
+def test_syev_tail_finisher(gpu, finish):
+    """The last columns of every block reduced in LDS by one launch (trailing
+    blocks of at most `finish` rows) against the column steps run to the end
+    (finish 0) and against LAPACK: random blocks around the 192-row boundary
+    and larger ones, with degenerate (deflating) and geometric spectra."""
+    rng = np.random.default_rng(29)
+    blocks = []
+    for n in [1, 2, 31, 96, 97, 150, 191, 192, 193, 226, 450, 700]:
+        a = rng.standard_normal((n, n))
+        blocks.append(a + a.T)
+    for n in [180, 500]:
+        lam = np.repeat(np.arange(n // 4, dtype=float), 4)[:n]
+        Q = _haar(rng, n)
+        blocks.append((Q * lam) @ Q.T)
+    k = rng.integers(0, 61, size=260)
+    Q = _haar(rng, 260)
+    blocks.append((Q * 2.0 ** (-k / 2.0)) @ Q.T)
+    lib = gpu.load()
+    try:
+        gpu._check(lib.nrg_gpu_set_tridiag_finish(finish))
+        ws, Us = gpu.syev_batched(blocks)
+    finally:
+        gpu._check(lib.nrg_gpu_set_tridiag_finish(192))
+    for H, w, U in zip(blocks, ws, Us):
+        _check_eig(H, w, U)
