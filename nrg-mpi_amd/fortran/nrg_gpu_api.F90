@@ -149,6 +149,43 @@ module nrg_gpu_api
        import :: c_int
        integer(c_int), value :: align_end
      end function
+     integer(c_int) function nrg_gpu_set_tridiag_method(two_stage) bind(C, name="nrg_gpu_set_tridiag_method")
+       import :: c_int
+       integer(c_int), value :: two_stage
+     end function
+     integer(c_int) function nrg_gpu_set_tridiag_groups(groups) bind(C, name="nrg_gpu_set_tridiag_groups")
+       import :: c_int
+       integer(c_int), value :: groups
+     end function
+     integer(c_int) function nrg_gpu_set_tridiag_finish(rows) bind(C, name="nrg_gpu_set_tridiag_finish")
+       import :: c_int
+       integer(c_int), value :: rows
+     end function
+     integer(c_int) function nrg_gpu_set_tridiag_symv(kind) bind(C, name="nrg_gpu_set_tridiag_symv")
+       import :: c_int
+       integer(c_int), value :: kind
+     end function
+     integer(c_int) function nrg_gpu_set_graphs(on) bind(C, name="nrg_gpu_set_graphs")
+       import :: c_int
+       integer(c_int), value :: on
+     end function
+     integer(c_int) function nrg_gpu_set_tuning(mode, cache_path) bind(C, name="nrg_gpu_set_tuning")
+       import :: c_int, c_char
+       integer(c_int), value :: mode
+       character(kind=c_char), intent(in) :: cache_path(*)
+     end function
+     integer(c_int) function nrg_gpu_profile(enable) bind(C, name="nrg_gpu_profile")
+       import :: c_int
+       integer(c_int), value :: enable
+     end function
+     integer(c_int) function nrg_gpu_profile_sample(every) bind(C, name="nrg_gpu_profile_sample")
+       import :: c_int
+       integer(c_int), value :: every
+     end function
+     integer(c_int) function nrg_gpu_profile_exclusive(on) bind(C, name="nrg_gpu_profile_exclusive")
+       import :: c_int
+       integer(c_int), value :: on
+     end function
   end interface
 
 contains
