@@ -928,9 +928,13 @@ __global__ void k_dc_tear(const int32_t* bnd, int nb_, const int32_t* n, const i
   d[o + mid] -= r;
 }
 
-// QL with implicit shifts on a leaf (m <= LEAF), one wave per leaf.  Every
-// lane runs the identical scalar recurrence on the shared dd/ee arrays and
-// applies each plane rotation to its own row of the leaf eigenvector matrix.
+// QL with implicit shifts on a leaf (m <= LEAF), one wave per leaf.  The
+// diagonal and off-diagonal live in registers (lane j holds d_j and e_j);
+// the scalar recurrence reads them with readlane, so every lane runs it with
+// the same values, and the lane that owns an entry takes its update.  Each
+// lane applies the plane rotations to its own row of the leaf eigenvector
+// matrix in LDS, carrying in a register the entry that the next rotation
+// of the sweep reuses.  No LDS traffic or barrier on the recurrence.
 __global__ __launch_bounds__(64) void k_dc_leaf(DcArgs a, const int32_t* tasks, int ntask) {
   const int t = blockIdx.x;
   if (t >= ntask) return;
@@ -938,86 +942,89 @@ __global__ __launch_bounds__(64) void k_dc_leaf(DcArgs a, const int32_t* tasks, 
   const int lane = threadIdx.x;
   const int n = a.n[b];
   const int64_t vo = a.voff[b];
-  __shared__ double dd[LEAF + 1], ee[LEAF + 1];
-  __shared__ double zl[LEAF][LEAF + 1];
+  __shared__ double zl[LEAF][LEAF + 1];   // row `lane` is touched by that lane only
+  double dr = 0.0, er = 0.0;
   if (lane < m) {
-    dd[lane] = a.d[vo + s + lane];
-    ee[lane] = (lane < m - 1) ? a.e[vo + s + lane] : 0.0;
+    dr = a.d[vo + s + lane];
+    er = (lane < m - 1) ? a.e[vo + s + lane] : 0.0;
     for (int j = 0; j < m; ++j) zl[lane][j] = (j == lane) ? 1.0 : 0.0;
   }
-  __syncthreads();
   const double eps = 2.220446049250313e-16;
   for (int l = 0; l < m; ++l) {
     int iter = 0;
     while (true) {
-      int mm;
-      for (mm = l; mm < m - 1; ++mm) {
-        const double dsum = fabs(dd[mm]) + fabs(dd[mm + 1]);
-        if (fabs(ee[mm]) <= eps * dsum) break;
-      }
+      // first mm >= l with a negligible e_mm relative to its two diagonal
+      // neighbours (m - 1 when there is none)
+      const double dn = __shfl_down(dr, 1, 64);
+      const bool neg = lane >= l && lane < m - 1 && fabs(er) <= eps * (fabs(dr) + fabs(dn));
+      const unsigned long long hit = __ballot(neg);
+      const int mm = hit ? __ffsll(hit) - 1 : m - 1;
       if (mm == l) break;
       if (++iter > 90) break;
-      double g = (dd[l + 1] - dd[l]) / (2.0 * ee[l]);
+      const double dl0 = lane_value(dr, l), el = lane_value(er, l);
+      double g = (lane_value(dr, l + 1) - dl0) / (2.0 * el);
       double r = hypot(g, 1.0);
-      g = dd[mm] - dd[l] + ee[l] / (g + copysign(r, g));
+      g = lane_value(dr, mm) - dl0 + el / (g + copysign(r, g));
       double sn = 1.0, cs = 1.0, p = 0.0;
-      int i;
+      double zc = (lane < m) ? zl[lane][mm] : 0.0;   // entry i + 1 of the rotation at i
+      int last = -1;
       bool early = false;
-      for (i = mm - 1; i >= l; --i) {
-        double f = sn * ee[i];
-        double bb = cs * ee[i];
+      for (int i = mm - 1; i >= l; --i) {
+        const double ei = lane_value(er, i);
+        const double f = sn * ei;
+        const double bb = cs * ei;
         r = hypot(f, g);
-        __syncthreads();
-        if (lane == 0) ee[i + 1] = r;
-        __syncthreads();
+        if (lane == i + 1) er = r;
         if (r == 0.0) {
-          __syncthreads();
-          if (lane == 0) { dd[i + 1] -= p; ee[mm] = 0.0; }
-          __syncthreads();
+          if (lane == i + 1) dr -= p;
+          if (lane == mm) er = 0.0;
           early = true;
           break;
         }
         sn = f / r;
         cs = g / r;
-        g = dd[i + 1] - p;
-        r = (dd[i] - g) * sn + 2.0 * cs * bb;
+        g = lane_value(dr, i + 1) - p;
+        r = (lane_value(dr, i) - g) * sn + 2.0 * cs * bb;
         p = sn * r;
-        __syncthreads();
-        if (lane == 0) dd[i + 1] = g + p;
-        __syncthreads();
+        if (lane == i + 1) dr = g + p;
         g = cs * r - bb;
         if (lane < m) {
-          const double zf = zl[lane][i + 1];
           const double zi = zl[lane][i];
-          zl[lane][i + 1] = sn * zi + cs * zf;
-          zl[lane][i] = cs * zi - sn * zf;
+          zl[lane][i + 1] = sn * zi + cs * zc;
+          zc = cs * zi - sn * zc;
         }
+        last = i;
       }
+      if (last >= 0 && lane < m) zl[lane][last] = zc;
       if (early) continue;
-      __syncthreads();
-      if (lane == 0) { dd[l] -= p; ee[l] = g; ee[mm] = 0.0; }
-      __syncthreads();
+      if (lane == l) { dr -= p; er = g; }
+      if (lane == mm) er = 0.0;
     }
   }
-  // selection sort ascending, columns follow
+  // selection sort ascending, columns follow: the minimum of d over
+  // [i, m), first index on ties, by a butterfly over the wave
   for (int i = 0; i < m - 1; ++i) {
-    int kmin = i;
-    double vmin = dd[i];
-    for (int j = i + 1; j < m; ++j)
-      if (dd[j] < vmin) { vmin = dd[j]; kmin = j; }
-    __syncthreads();
+    double v = (lane >= i && lane < m) ? dr : __builtin_inf();
+    int k = lane;
+    for (int o = 32; o > 0; o >>= 1) {
+      const double v2 = __shfl_xor(v, o, 64);
+      const int k2 = __shfl_xor(k, o, 64);
+      if (v2 < v || (v2 == v && k2 < k)) { v = v2; k = k2; }
+    }
+    const int kmin = __builtin_amdgcn_readfirstlane(k);
     if (kmin != i) {
-      if (lane == 0) { dd[kmin] = dd[i]; dd[i] = vmin; }
+      const double di = lane_value(dr, i);
+      if (lane == i) dr = v;
+      else if (lane == kmin) dr = di;
       if (lane < m) {
         const double tz = zl[lane][i];
         zl[lane][i] = zl[lane][kmin];
         zl[lane][kmin] = tz;
       }
     }
-    __syncthreads();
   }
   if (lane < m) {
-    a.w[vo + s + lane] = dd[lane];
+    a.w[vo + s + lane] = dr;
     double* Z = a.Z + a.aoff[b];
     for (int j = 0; j < m; ++j) Z[(int64_t)(s + lane) + (int64_t)(s + j) * n] = zl[lane][j];
   }
