@@ -68,6 +68,7 @@ contains
     if (allocated(basis_out)) deallocate(basis_out)
     if (allocated(eigval)) deallocate(eigval)
     allocate(basis_out(nv + 3, n), eigval(n))
+    bo_partial = .false.
     do i = 1, n
        basis_out(:, i) = basis((i - 1) * (nv + 3) + 1:i * (nv + 3))
        eigval(i) = e(i)
@@ -183,6 +184,7 @@ contains
        m = min(int(n), size(basis_in))
        out(1:m) = reshape(basis_in, [size(basis_in)])
     case (3)
+       call basis_out_complete()
        m = min(int(n), size(basis_out))
        out(1:m) = reshape(basis_out, [size(basis_out)])
     case (4)

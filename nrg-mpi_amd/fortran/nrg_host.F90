@@ -51,6 +51,12 @@ module nrg_host
   double precision, allocatable, target :: eigval(:), eigval_old(:)
   double precision :: trunc_margin = -1.0d0
   double precision :: thermo_x, thermo_s, thermo_c
+  ! After a truncation basis_out holds the kept prefix; the discarded
+  ! states (their indices in truncation order, bo_rest, with the block table
+  ! they index, bo_sub) fill the rest when the whole array is read
+  ! (basis_out_complete).
+  integer, allocatable :: bo_rest(:), bo_sub(:, :)
+  logical :: bo_partial = .false.
 
   ! ---- options
   integer :: sort_mode = 0          ! 0: exact replay of the reference sort, 1: canonical
@@ -183,6 +189,7 @@ contains
        read(u, *) basis_out(1:nv, i), basis_out(nv + 1, i), eigval(i)
        basis_out(nv + 2:nv + 3, i) = 0
     end do
+    bo_partial = .false.
     read(u, *) hdr
     do c = 1, ncm
        do j = 1, nb0
@@ -673,7 +680,7 @@ contains
     integer, intent(out) :: ierr
     double precision :: cd(nchain), cnd(nchain)
     integer(8) :: n2
-    integer :: b, j, s0
+    integer :: b
     type(c_ptr) :: pvec
     do b = 1, nchain
        cd(b) = chain_diag(it + 1, b)
@@ -696,43 +703,40 @@ contains
     end if
     ierr = nrg_gpu_diagonalize(int(it, c_int), int(nsub, c_int), subspace, int(nfull, c_int), basis_in, &
          int(nold, c_int), eigval_old, cd, cnd, sqrt(lambda), eigval, pvec, int(n2, c_int64_t))
-    if (ierr /= 0) return
     ! basis_output <- (block labels, column index) (diagonalization.F90:395-407)
-    if (allocated(basis_out)) deallocate(basis_out)
-    allocate(basis_out(nv + 3, nfull))
-    do b = 1, nsub
-       s0 = subspace(nv + 3, b)
-       do j = 1, subspace(nv + 1, b)
-          basis_out(1:nv, s0 + j - 1) = subspace(1:nv, b)
-          basis_out(nv + 1, s0 + j - 1) = j
-          basis_out(nv + 2:nv + 3, s0 + j - 1) = 0
-       end do
-    end do
+    ! is formed in its post-truncation order by truncate: state s0 + j - 1 of
+    ! block b is (subspace(1:nv, b), j, 0, 0)
   end subroutine diagonalization
 
   ! =====================================================================
   ! thermodynamics (module.F90:1931-1982), before truncation
   ! =====================================================================
   subroutine calc_thermo()
-    integer :: i, j
+    integer :: i, j, b, jj
     double precision :: be, ex, zn, en, sz2, sz, cn, rs
     integer :: szt
     double precision, allocatable :: exv(:)
     allocate(exv(nfull))
     zn = 0; en = 0; sz2 = 0; sz = 0; cn = 0
-    do i = 1, nfull
-       be = betabar * eigval(i)
-       ex = exp(-be)
-       exv(i) = ex
-       zn = zn + ex
-       en = en + be * ex
-       szt = 0
-       do j = cons_block(2, 1), cons_block(2, 2)
-          szt = szt + basis_out(j, i)
+    ! states in index order: block by block (the blocks are contiguous and
+    ! ascending), the labels of state i those of its block
+    i = 0
+    do b = 1, nsub
+       do jj = 1, subspace(nv + 1, b)
+          i = i + 1
+          be = betabar * eigval(i)
+          ex = exp(-be)
+          exv(i) = ex
+          zn = zn + ex
+          en = en + be * ex
+          szt = 0
+          do j = cons_block(2, 1), cons_block(2, 2)
+             szt = szt + state_label(subspace, j, b, jj)
+          end do
+          rs = dble(szt) * 0.5d0
+          sz2 = sz2 + ex * rs * rs
+          sz = sz + ex * rs
        end do
-       rs = dble(szt) * 0.5d0
-       sz2 = sz2 + ex * rs * rs
-       sz = sz + ex * rs
     end do
     do i = 1, nfull
        be = betabar * eigval(i)
@@ -743,15 +747,60 @@ contains
     thermo_x = sz2 / zn - (sz / zn) * (sz / zn)
   end subroutine calc_thermo
 
+  ! Entry c of basis_output for state jj of block b of the table sub
+  ! (labels 1..nv, the column index, then two zeros).
+  pure integer function state_label(sub, c, b, jj)
+    integer, intent(in) :: sub(:, :), c, b, jj
+    if (c <= nv) then
+       state_label = sub(c, b)
+    else if (c == nv + 1) then
+       state_label = jj
+    else
+       state_label = 0
+    end if
+  end function state_label
+
+  subroutine set_state(i, sub, b, jj)
+    integer, intent(in) :: i, sub(:, :), b, jj
+    integer :: c
+    do c = 1, nv
+       basis_out(c, i) = sub(c, b)
+    end do
+    basis_out(nv + 1, i) = jj
+    basis_out(nv + 2, i) = 0
+    basis_out(nv + 3, i) = 0
+  end subroutine set_state
+
+  ! The discarded part of basis_output (states in the truncation's energy
+  ! order), for readers of the whole array.
+  subroutine basis_out_complete()
+    integer, allocatable :: blk(:)
+    integer :: b, j, i, s, nb
+    if (.not. bo_partial) return
+    nb = size(basis_out, 2) - size(bo_rest)
+    allocate(blk(size(basis_out, 2)))
+    do b = 1, size(bo_sub, 2)
+       do j = 1, bo_sub(nv + 1, b)
+          blk(bo_sub(nv + 3, b) + j - 1) = b
+       end do
+    end do
+    do i = 1, size(bo_rest)
+       s = bo_rest(i)
+       b = blk(s)
+       call set_state(nb + i, bo_sub, b, s - bo_sub(nv + 3, b) + 1)
+    end do
+    bo_partial = .false.
+  end subroutine basis_out_complete
+
   ! =====================================================================
   ! postprocess (postprocess.F90:71-207): sort, truncate, reorder
   ! =====================================================================
   subroutine truncate(it)
     integer, intent(in) :: it
-    integer, allocatable :: idx(:), wp(:), kidx(:), order(:), tmp(:, :)
+    integer, allocatable :: idx(:), wp(:), kidx(:), order(:)
     double precision, allocatable :: es(:), e2(:)
     integer(8), allocatable :: kk(:), wk(:)
-    integer :: i, limit, cutoff, nb
+    integer :: i, limit, cutoff, nb, b
     double precision :: ecut
     allocate(idx(nfull), wp(nfull), es(nfull))
     do i = 1, nfull
@@ -813,11 +862,22 @@ contains
     allocate(e2(nfull))
     e2 = eigval(order)
     eigval = e2
-    if (nb < nfull) then
-       allocate(tmp(nv + 3, nfull))
-       tmp = basis_out(:, order)
-       call move_alloc(tmp, basis_out)
-    end if
+    ! basis_output in this order: the kept prefix now (in index order, so its
+    ! blocks follow the table), the discarded states when a reader asks
+    if (allocated(basis_out)) deallocate(basis_out)
+    allocate(basis_out(nv + 3, nfull))
+    b = 1
+    do i = 1, nb
+       do while (order(i) >= subspace(nv + 3, b) + subspace(nv + 1, b))
+          b = b + 1
+       end do
+       call set_state(i, subspace, b, order(i) - subspace(nv + 3, b) + 1)
+    end do
+    if (allocated(bo_rest)) deallocate(bo_rest)
+    allocate(bo_rest(nfull - nb))
+    bo_rest = order(nb + 1:nfull)
+    bo_sub = subspace(:, 1:nsub)
+    bo_partial = nb < nfull
     nkept = nb
     if (write_files) then
        ! kept states in index order with their labels (postprocess.F90:163-167)
