@@ -69,6 +69,7 @@ contains
     if (allocated(eigval)) deallocate(eigval)
     allocate(basis_out(nv + 3, n), eigval(n))
     bo_partial = .false.
+    ev_partial = .false.
     do i = 1, n
        basis_out(:, i) = basis((i - 1) * (nv + 3) + 1:i * (nv + 3))
        eigval(i) = e(i)
@@ -85,6 +86,7 @@ contains
     integer :: b, j, s0
     if (allocated(eigval)) deallocate(eigval)
     allocate(eigval(n))
+    ev_partial = .false.
     eigval = e(1:n)
     nfull = n
     if (allocated(basis_out)) deallocate(basis_out)
@@ -165,6 +167,7 @@ contains
     real(c_double), intent(out) :: out(*)
     integer(c_int), value :: n
     integer :: m
+    call eigval_complete()
     m = min(int(n), size(eigval))
     out(1:m) = eigval(1:m)
     nrg_host_get_eigenvalues = m

@@ -51,12 +51,18 @@ module nrg_host
   double precision, allocatable, target :: eigval(:), eigval_old(:)
   double precision :: trunc_margin = -1.0d0
   double precision :: thermo_x, thermo_s, thermo_c
-  ! After a truncation basis_out holds the kept prefix; the discarded
-  ! states (their indices in truncation order, bo_rest, with the block table
-  ! they index, bo_sub) fill the rest when the whole array is read
-  ! (basis_out_complete).
-  integer, allocatable :: bo_rest(:), bo_sub(:, :)
-  logical :: bo_partial = .false.
+  ! After a truncation eigval and basis_out hold the kept prefix; the
+  ! discarded states follow in ascending order of energy, formed when a
+  ! reader asks for the whole arrays (eigval_complete, basis_out_complete)
+  ! from the eigenvalues before the truncation (ev_pre), the kept count
+  ! (tr_nb) and the block table they index (bo_sub).
+  double precision, allocatable :: ev_pre(:)
+  integer, allocatable :: bo_sub(:, :)
+  integer :: tr_nb = 0
+  logical :: bo_partial = .false., ev_partial = .false.
+  ! k-way merge of ascending runs (kw_*): run starts, current heads, heap
+  integer, allocatable :: kw_rs(:), kw_hd(:), kw_hp(:)
+  integer :: kw_nr = 0, kw_nh = 0
 
   ! ---- options
   integer :: sort_mode = 0          ! 0: exact replay of the reference sort, 1: canonical
@@ -190,6 +196,7 @@ contains
        basis_out(nv + 2:nv + 3, i) = 0
     end do
     bo_partial = .false.
+    ev_partial = .false.
     read(u, *) hdr
     do c = 1, ncm
        do j = 1, nb0
@@ -695,6 +702,7 @@ contains
     eigval_old = eigval(1:nold)
     if (allocated(eigval)) deallocate(eigval)
     allocate(eigval(nfull))
+    ev_partial = .false.
     pvec = c_null_ptr
     if (keep_vectors) then
        if (allocated(eigvec)) deallocate(eigvec)
@@ -771,43 +779,150 @@ contains
     basis_out(nv + 3, i) = 0
   end subroutine set_state
 
-  ! The discarded part of basis_output (states in the truncation's energy
-  ! order), for readers of the whole array.
+  ! Indices (before the truncation) of the discarded states in ascending
+  ! order of energy, ties in index order: the stable sort of ev_pre past
+  ! its first tr_nb entries.
+  subroutine discarded_order(rest)
+    integer, allocatable, intent(out) :: rest(:)
+    integer, allocatable :: idx(:)
+    double precision, allocatable :: es(:)
+    integer :: i, n
+    n = size(ev_pre)
+    allocate(idx(n), es(n))
+    do i = 1, n
+       idx(i) = i
+    end do
+    es = ev_pre
+    call run_merge_sort_r8(n, es, idx)
+    rest = idx(tr_nb + 1:n)
+  end subroutine discarded_order
+
+  ! The discarded part of basis_output, for readers of the whole array.
   subroutine basis_out_complete()
-    integer, allocatable :: blk(:)
-    integer :: b, j, i, s, nb
+    integer, allocatable :: blk(:), rest(:)
+    integer :: b, j, i, s
     if (.not. bo_partial) return
-    nb = size(basis_out, 2) - size(bo_rest)
-    allocate(blk(size(basis_out, 2)))
+    call discarded_order(rest)
+    allocate(blk(size(ev_pre)))
     do b = 1, size(bo_sub, 2)
        do j = 1, bo_sub(nv + 1, b)
           blk(bo_sub(nv + 3, b) + j - 1) = b
        end do
     end do
-    do i = 1, size(bo_rest)
-       s = bo_rest(i)
+    do i = 1, size(rest)
+       s = rest(i)
        b = blk(s)
-       call set_state(nb + i, bo_sub, b, s - bo_sub(nv + 3, b) + 1)
+       call set_state(tr_nb + i, bo_sub, b, s - bo_sub(nv + 3, b) + 1)
     end do
     bo_partial = .false.
   end subroutine basis_out_complete
+
+  ! The discarded part of eigval, for readers of the whole array.
+  subroutine eigval_complete()
+    integer, allocatable :: rest(:)
+    if (.not. ev_partial) return
+    call discarded_order(rest)
+    eigval(tr_nb + 1:tr_nb + size(rest)) = ev_pre(rest)
+    ev_partial = .false.
+  end subroutine eigval_complete
+
+  ! k-way merge of the ascending runs of v(1:n) (a new run wherever a value
+  ! is below its predecessor; the eigenvalues arrive as one run per block):
+  ! a binary heap of run heads ordered by (value, run), so equal values
+  ! leave in index order and kw_next yields the stable ascending order,
+  ! one element per call, without sorting what is never asked for.
+  subroutine kw_init(n, v)
+    integer, intent(in) :: n
+    double precision, intent(in) :: v(n)
+    integer :: i
+    if (allocated(kw_rs)) deallocate(kw_rs, kw_hd, kw_hp)
+    kw_nr = 1
+    do i = 2, n
+       if (v(i) < v(i - 1)) kw_nr = kw_nr + 1
+    end do
+    allocate(kw_rs(kw_nr + 1))
+    kw_nr = 1
+    kw_rs(1) = 1
+    do i = 2, n
+       if (v(i) < v(i - 1)) then
+          kw_nr = kw_nr + 1
+          kw_rs(kw_nr) = i
+       end if
+    end do
+    kw_rs(kw_nr + 1) = n + 1
+    allocate(kw_hd(kw_nr), kw_hp(kw_nr))
+    kw_nh = 0
+    do i = 1, kw_nr
+       kw_hd(i) = kw_rs(i)
+       if (kw_hd(i) < kw_rs(i + 1)) then
+          kw_nh = kw_nh + 1
+          kw_hp(kw_nh) = i
+       end if
+    end do
+    do i = kw_nh / 2, 1, -1
+       call kw_down(v, i)
+    end do
+  end subroutine kw_init
+
+  logical function kw_less(v, a, b)
+    double precision, intent(in) :: v(:)
+    integer, intent(in) :: a, b
+    kw_less = v(kw_hd(a)) < v(kw_hd(b)) .or. (v(kw_hd(a)) == v(kw_hd(b)) .and. a < b)
+  end function kw_less
+
+  subroutine kw_down(v, i0)
+    double precision, intent(in) :: v(:)
+    integer, intent(in) :: i0
+    integer :: i, c, t
+    i = i0
+    do
+       c = 2 * i
+       if (c > kw_nh) exit
+       if (c < kw_nh) then
+          if (kw_less(v, kw_hp(c + 1), kw_hp(c))) c = c + 1
+       end if
+       if (.not. kw_less(v, kw_hp(c), kw_hp(i))) exit
+       t = kw_hp(i); kw_hp(i) = kw_hp(c); kw_hp(c) = t
+       i = c
+    end do
+  end subroutine kw_down
+
+  ! next element of the merge: its value and index; .false. when exhausted
+  logical function kw_next(v, val, pos)
+    double precision, intent(in) :: v(:)
+    double precision, intent(out) :: val
+    integer, intent(out) :: pos
+    integer :: r
+    kw_next = kw_nh > 0
+    if (.not. kw_next) return
+    r = kw_hp(1)
+    pos = kw_hd(r)
+    val = v(pos)
+    kw_hd(r) = pos + 1
+    if (kw_hd(r) >= kw_rs(r + 1)) then
+       kw_hp(1) = kw_hp(kw_nh)
+       kw_nh = kw_nh - 1
+    end if
+    if (kw_nh > 0) call kw_down(v, 1)
+  end function kw_next
 
   ! =====================================================================
   ! postprocess (postprocess.F90:71-207): sort, truncate, reorder
   ! =====================================================================
   subroutine truncate(it)
     integer, intent(in) :: it
-    integer, allocatable :: idx(:), wp(:), kidx(:), order(:)
-    double precision, allocatable :: es(:), e2(:)
+    integer, allocatable :: idx(:), wp(:), kidx(:)
+    double precision, allocatable :: es(:)
     integer(8), allocatable :: kk(:), wk(:)
-    integer :: i, limit, cutoff, nb, b
-    double precision :: ecut
-    allocate(idx(nfull), wp(nfull), es(nfull))
-    do i = 1, nfull
-       idx(i) = i
-    end do
-    es = eigval
-    call run_merge_sort_r8(nfull, es, idx)
+    integer :: i, limit, cutoff, nb, b, ne, pos
+    double precision :: ecut, val
+    logical :: more
+    ! the stable ascending order of the eigenvalues (postprocess.F90:71-77),
+    ! es(1:ne) with their indices idx(1:ne): as far as the cutoff needs it
+    ! when truncating (k-way merge of the per-block runs), whole when every
+    ! state is kept and the kept energies are written, not at all otherwise
+    allocate(idx(nfull), es(nfull))
+    ne = 0
     nb = nfull
     trunc_margin = -1.0d0
     if (it /= i_first .and. it /= n_iter .and. nfull > nkeep) then
@@ -815,17 +930,26 @@ contains
           start_trunc = it
           flag_trunc = .true.
        end if
+       call kw_init(nfull, eigval)
+       do while (ne < nkeep)
+          more = kw_next(eigval, val, pos)
+          ne = ne + 1
+          es(ne) = val
+          idx(ne) = pos
+       end do
        limit = int(nkeep * truncation_limit_scale)
        ecut = es(nkeep) + truncation_threshold
        cutoff = nkeep
-       i = nkeep + 1
-       do while (i <= nfull)
-          if (es(i) <= ecut) then
+       do while (ne < nfull)
+          more = kw_next(eigval, val, pos)
+          ne = ne + 1
+          es(ne) = val
+          idx(ne) = pos
+          if (val <= ecut) then
              cutoff = cutoff + 1
           else
              exit
           end if
-          i = i + 1
        end do
        do while (cutoff > limit)
           ecut = ecut - 0.001d0
@@ -838,7 +962,15 @@ contains
           end do
        end do
        nb = cutoff
-       trunc_margin = minval(abs(es - ecut))
+       ! the same set of values as the sorted array: the same minimum
+       trunc_margin = minval(abs(eigval - ecut))
+    else if (write_files) then
+       do i = 1, nfull
+          idx(i) = i
+       end do
+       es = eigval
+       call run_merge_sort_r8(nfull, es, idx)
+       ne = nfull
     end if
     if (write_files) then
        ! kept energies in ascending order (postprocess.F90:128-137)
@@ -850,34 +982,39 @@ contains
           end if
        end do
     end if
-    ! kept prefix back to index order (sortByIndex on 1..nb)
-    allocate(kidx(nb), order(nfull), kk(nb), wk(nb))
-    do i = 1, nb
-       kk(i) = idx(i)
-       kidx(i) = idx(i)
-    end do
-    if (nb > 0) call merge_sort_i8(nb, kk, kidx, wk, wp(1:nb))
-    order(1:nb) = kidx
-    order(nb + 1:nfull) = idx(nb + 1:nfull)
-    allocate(e2(nfull))
-    e2 = eigval(order)
-    eigval = e2
-    ! basis_output in this order: the kept prefix now (in index order, so its
-    ! blocks follow the table), the discarded states when a reader asks
+    ! kept prefix back to index order (sortByIndex on 1..nb); every state
+    ! kept: index order as it stands
+    allocate(kidx(nb))
+    if (nb < nfull) then
+       allocate(kk(nb), wk(nb), wp(nb))
+       do i = 1, nb
+          kk(i) = idx(i)
+          kidx(i) = idx(i)
+       end do
+       if (nb > 0) call merge_sort_i8(nb, kk, kidx, wk, wp)
+    else
+       do i = 1, nb
+          kidx(i) = i
+       end do
+    end if
+    ! eigval and basis_output in the new order: the kept prefix now, the
+    ! discarded states (ascending energy) when a reader asks
+    call move_alloc(eigval, ev_pre)
+    allocate(eigval(nfull))
+    eigval(1:nb) = ev_pre(kidx)
     if (allocated(basis_out)) deallocate(basis_out)
     allocate(basis_out(nv + 3, nfull))
     b = 1
     do i = 1, nb
-       do while (order(i) >= subspace(nv + 3, b) + subspace(nv + 1, b))
+       do while (kidx(i) >= subspace(nv + 3, b) + subspace(nv + 1, b))
           b = b + 1
        end do
-       call set_state(i, subspace, b, order(i) - subspace(nv + 3, b) + 1)
+       call set_state(i, subspace, b, kidx(i) - subspace(nv + 3, b) + 1)
     end do
-    if (allocated(bo_rest)) deallocate(bo_rest)
-    allocate(bo_rest(nfull - nb))
-    bo_rest = order(nb + 1:nfull)
     bo_sub = subspace(:, 1:nsub)
+    tr_nb = nb
     bo_partial = nb < nfull
+    ev_partial = nb < nfull
     nkept = nb
     if (write_files) then
        ! kept states in index order with their labels (postprocess.F90:163-167)
