@@ -147,3 +147,48 @@ def test_eigenvalue_sort_is_stable(seed):
         ref = np.argsort(v, kind="stable")
         np.testing.assert_array_equal(p, ref)
         np.testing.assert_array_equal(vs, v[ref])
+
+
+@pytest.mark.parametrize("grid", [8.0, 1.0e6])
+def test_truncation_with_ties_matches_oracle(tmp_path, grid):
+    """The host's truncation (k-way merge of the per-block runs as far as the
+    cutoff window reaches, discarded part formed on request) against the
+    oracle's truncation (a full stable sort, pinned by the reference
+    fixtures) on synthetic block eigenvalues: on a coarse grid (many exact
+    ties within and across blocks) and a fine one.  Kept count, the whole
+    eigenvalue array, the whole basis_output and the kept count per block."""
+    from types import SimpleNamespace
+    from nrgamd.host import Host
+    import nrg_oracle as orc
+    name = "siam_nk256_it40"
+    fx = np.load(os.path.join(GOLD, f"ref_{name}.npz"))
+    h = Host(deck(tmp_path, name), device=None)
+    nv = h.i["nv"]
+    it = 12
+    bo = kept_basis(fx[f"it{it - 1}_subspace"], fx[f"it{it - 1}_kept_blk"], nv)
+    h.set_basis_output(bo, np.zeros(len(bo)))
+    h.prepare(0)
+    sub = h.subspace()
+    rng = np.random.default_rng(int(grid))
+    eb = np.concatenate([np.sort(np.round(rng.random(int(r[nv])) * 3.0 * grid) / grid) for r in sub])
+    nfull = len(eb)
+    assert nfull > 256
+    nk = h.truncate(it, eb)
+    # the oracle's state for the same iteration: eigenvalues in block order,
+    # basis_output (block labels, column index, 0, 0) in index order
+    rows = []
+    for r in sub:
+        for j in range(int(r[nv])):
+            rows.append(list(r[:nv]) + [j + 1, 0, 0])
+    o = orc.NRG.__new__(orc.NRG)
+    o.m = SimpleNamespace(nkeep=256, i0=int(h.i["i_first"]), N=int(h.i["n_iter"]), nv=nv)
+    o.eigenvalue = eb.copy()
+    o.basis_output = np.array(rows, dtype=np.int64)
+    o.subspace = sub.astype(np.int64)
+    o.start_trunc = 10 ** 9
+    o.flag_trunc = False
+    nb = o.truncate(it)
+    assert nk == nb and nb < nfull
+    np.testing.assert_array_equal(h.eigenvalues(), o.eigenvalue)
+    np.testing.assert_array_equal(h.basis_output(), o.basis_output)
+    np.testing.assert_array_equal(h.kept_per_block(), o.kept_per_block())
