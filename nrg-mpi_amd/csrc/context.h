@@ -79,6 +79,10 @@ struct Context {
   DevBuf<double> d_tmp;
   DevBuf<int32_t> d_gidx;
   DevBuf<double> d_gscale;
+  // host staging of the gather lists and scales, kept between calls so
+  // their pages stay mapped and they do not regrow every iteration
+  std::vector<int32_t> h_gidx;
+  std::vector<double> h_gscale;
 };
 
 Context& ctx();

@@ -45,8 +45,10 @@ void invariant(Context& C, const int32_t* kept, double* out_host) {
   for (int b = 0; b < nsub; ++b) rows[b] = block_rows(C, b);
   const int cs0 = C.cons[0] - 1, cs1 = C.cons[1], ss0 = C.cons[2] - 1, ss1 = C.cons[3];
   const bool sz_only = (C.cons[0] == C.cons[1] && C.cons[0] == 0);
-  std::vector<int32_t> gidx;
-  std::vector<double> gsc;
+  std::vector<int32_t>& gidx = C.h_gidx;
+  std::vector<double>& gsc = C.h_gscale;
+  gidx.clear();
+  gsc.clear();
   GemmBatch batch;
   // table rows of each type, in table order
   std::vector<std::vector<int>> qt(C.ncm);
